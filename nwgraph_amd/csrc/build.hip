@@ -1,0 +1,210 @@
+// build.hip -- on-device edge list -> CSC / CSR construction.
+//
+// Pipeline (reference semantics in brackets, pnnl/NWGraph):
+//   1. pack key = (row << b) | entry, b = bits(n-1); with clean=1 drop u == v
+//      [remove_self_loops, build.hpp:404-410] -- dropping before the sort gives the
+//      same edge set as the reference's sort -> uniq -> remove order (tc.cpp:66-68)
+//   2. stable LSD radix sort of the 2b-bit keys [lexical_sort_by, build.hpp:57-70]
+//   3. with clean=1 drop adjacent equal keys [uniq, build.hpp:392-402]
+//   4. row pointers from the sorted keys and entries = low bits
+//      [fill_directed<idx> + sort_to_be_indexed, build.hpp:209-232,
+//       compressed.hpp:491-502: each row ascending makes the layout unique]
+//   5. out-degree = histogram of sources [build_degrees, bench/gapbs/common.hpp:110-121]
+#include "device_utils.hpp"
+#include "nwg_internal.hpp"
+
+namespace nwg {
+
+namespace {
+
+inline int bits_for(uint32_t n) {
+  int b = 1;
+  while (b < 32 && (1ull << b) < (uint64_t)n) ++b;
+  return b;
+}
+
+// flag[i] = keep edge i; also records an id out of range.
+__global__ __launch_bounds__(256) void edge_flag_kernel(const uint32_t* __restrict__ src,
+                                                        const uint32_t* __restrict__ dst, uint64_t ne, uint32_t n,
+                                                        int clean, uint32_t* __restrict__ flag,
+                                                        uint32_t* __restrict__ bad) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= ne) return;
+  const uint32_t u = src[i], v = dst[i];
+  const bool self = (u == v);
+  uint32_t keep = 1;
+  if (clean && self) keep = 0;
+  if (keep && (u >= n || v >= n)) {
+    atomicOr(bad, 1u);
+    keep = 0;
+  }
+  flag[i] = keep;
+}
+
+// keys[pos[i]] = (row << b) | entry for kept edges (row = dst for idx=1, src for idx=0)
+__global__ __launch_bounds__(256) void pack_kernel(const uint32_t* __restrict__ src, const uint32_t* __restrict__ dst,
+                                                   uint64_t ne, int b, int idx, const uint32_t* __restrict__ flag,
+                                                   const uint32_t* __restrict__ pos, uint64_t* __restrict__ keys) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= ne || !flag[i]) return;
+  const uint64_t r = idx ? dst[i] : src[i];
+  const uint64_t e = idx ? src[i] : dst[i];
+  keys[pos[i]] = (r << b) | e;
+}
+
+__global__ __launch_bounds__(256) void uniq_flag_kernel(const uint64_t* __restrict__ keys, uint64_t m,
+                                                        uint32_t* __restrict__ flag) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= m) return;
+  flag[i] = (i == 0 || keys[i] != keys[i - 1]) ? 1u : 0u;
+}
+
+__global__ __launch_bounds__(256) void compact_keys_kernel(const uint64_t* __restrict__ in, uint64_t m,
+                                                           const uint32_t* __restrict__ flag,
+                                                           const uint32_t* __restrict__ pos,
+                                                           uint64_t* __restrict__ out) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < m && flag[i]) out[pos[i]] = in[i];
+}
+
+// Row pointers: every key i fills ptr[r] for rows (row(i-1), row(i)]; the last key
+// also fills rows (row(m-1), n].  entries = low b bits.
+__global__ __launch_bounds__(256) void csx_kernel(const uint64_t* __restrict__ keys, uint64_t m, uint32_t n, int b,
+                                                  uint32_t* __restrict__ ptr, uint32_t* __restrict__ ent) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= m) return;
+  const uint64_t k = keys[i];
+  const uint64_t r = k >> b;
+  ent[i] = (uint32_t)(k & ((1ull << b) - 1));
+  const int64_t prev = i ? (int64_t)(keys[i - 1] >> b) : -1;
+  for (int64_t rr = prev + 1; rr <= (int64_t)r; ++rr) ptr[rr] = (uint32_t)i;
+  if (i == m - 1)
+    for (uint64_t rr = r + 1; rr <= n; ++rr) ptr[rr] = (uint32_t)m;
+}
+
+__global__ __launch_bounds__(256) void fill_u32_kernel(uint32_t* __restrict__ p, uint64_t n, uint32_t v) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+__global__ __launch_bounds__(256) void degree_kernel(const uint32_t* __restrict__ col, uint64_t nnz,
+                                                     uint32_t* __restrict__ deg) {
+  const uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p < nnz) atomicAdd(&deg[col[p]], 1u);
+}
+
+// CSR keys from a CSC: (source << b) | target, payload = CSC position; one thread per CSC row.
+__global__ __launch_bounds__(256) void transpose_keys_kernel(uint32_t n, const uint32_t* __restrict__ ptr,
+                                                             const uint32_t* __restrict__ col, int b,
+                                                             uint64_t* __restrict__ keys,
+                                                             uint32_t* __restrict__ pos) {
+  const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  for (uint32_t p = ptr[r]; p < ptr[r + 1]; ++p) {
+    keys[p] = ((uint64_t)col[p] << b) | r;
+    pos[p] = p;
+  }
+}
+
+inline uint32_t grid_for(uint64_t n) { return (uint32_t)ceil_div(n ? n : 1, 256); }
+
+uint32_t read_back_u32(hipStream_t s, const uint32_t* d) {
+  uint32_t h = 0;
+  NWG_HIP(hipMemcpyAsync(&h, d, sizeof h, hipMemcpyDeviceToHost, s));
+  NWG_HIP(hipStreamSynchronize(s));
+  return h;
+}
+
+}  // namespace
+
+void out_degrees_device(hipStream_t s, uint32_t n, uint64_t nnz, const uint32_t* col, uint32_t* deg) {
+  fill_u32_kernel<<<grid_for(n), 256, 0, s>>>(deg, n, 0u);
+  NWG_LAUNCH_CHECK();
+  if (nnz) {
+    degree_kernel<<<grid_for(nnz), 256, 0, s>>>(col, nnz, deg);
+    NWG_LAUNCH_CHECK();
+  }
+}
+
+// Sorted compressed structure from packed keys (consumes keys/tmp).
+static void keys_to_compressed(hipStream_t s, uint32_t n, int b, DevBuf<uint64_t>& keys, DevBuf<uint64_t>& tmp,
+                               uint64_t m, bool clean, DevBuf<uint32_t>& ptr, DevBuf<uint32_t>& ent,
+                               uint64_t* nnz_out) {
+  radix_sort_pairs(s, keys.p, tmp.p, nullptr, nullptr, m, 2 * b);
+  if (clean && m) {
+    DevBuf<uint32_t> flag(m), pos(m + 1);
+    uniq_flag_kernel<<<grid_for(m), 256, 0, s>>>(keys.p, m, flag.p);
+    NWG_LAUNCH_CHECK();
+    exclusive_scan_u32(s, flag.p, pos.p, m);
+    const uint32_t mu = read_back_u32(s, pos.p + m);
+    compact_keys_kernel<<<grid_for(m), 256, 0, s>>>(keys.p, m, flag.p, pos.p, tmp.p);
+    NWG_LAUNCH_CHECK();
+    std::swap(keys, tmp);
+    m = mu;
+    NWG_HIP(hipStreamSynchronize(s));
+  }
+  ptr.alloc((uint64_t)n + 1);
+  ent.alloc(m ? m : 1);
+  if (m) {
+    csx_kernel<<<grid_for(m), 256, 0, s>>>(keys.p, m, n, b, ptr.p, ent.p);
+  } else {
+    fill_u32_kernel<<<grid_for((uint64_t)n + 1), 256, 0, s>>>(ptr.p, (uint64_t)n + 1, 0u);
+  }
+  NWG_LAUNCH_CHECK();
+  *nnz_out = m;
+}
+
+void build_csc_device(hipStream_t s, uint32_t n, uint64_t ne, uint32_t* src, uint32_t* dst, bool clean,
+                      nwg_graph* g) {
+  NWG_REQUIRE(n >= 1, "graph needs at least one vertex");
+  NWG_REQUIRE(ne < (1ull << 32), "edge count must be below 2^32 (uint32 indices)");
+  const int b = bits_for(n);
+  uint64_t m = 0;
+  DevBuf<uint64_t> keys(ne ? ne : 1), tmp(ne ? ne : 1);
+  if (ne) {
+    DevBuf<uint32_t> flag(ne), pos(ne + 1), bad(1);
+    fill_u32_kernel<<<1, 256, 0, s>>>(bad.p, 1, 0u);
+    edge_flag_kernel<<<grid_for(ne), 256, 0, s>>>(src, dst, ne, n, clean ? 1 : 0, flag.p, bad.p);
+    NWG_LAUNCH_CHECK();
+    exclusive_scan_u32(s, flag.p, pos.p, ne);
+    if (read_back_u32(s, bad.p)) throw Error(NWG_EINVAL, "edge list holds a vertex id >= n");
+    m = read_back_u32(s, pos.p + ne);
+    pack_kernel<<<grid_for(ne), 256, 0, s>>>(src, dst, ne, b, 1, flag.p, pos.p, keys.p);
+    NWG_LAUNCH_CHECK();
+    NWG_HIP(hipStreamSynchronize(s));
+  }
+  uint64_t nnz = 0;
+  keys_to_compressed(s, n, b, keys, tmp, m, clean, g->csc_ptr, g->csc_idx, &nnz);
+  g->n = n;
+  g->nnz = nnz;
+  g->outdeg.alloc(n);
+  out_degrees_device(s, n, nnz, g->csc_idx.p, g->outdeg.p);
+  NWG_HIP(hipStreamSynchronize(s));
+}
+
+void build_csr_device(hipStream_t s, nwg_graph* g) {
+  if (g->has_csr) return;
+  const uint32_t n = g->n;
+  const uint64_t nnz = g->nnz;
+  const int b = bits_for(n);
+  DevBuf<uint64_t> keys(nnz ? nnz : 1), tmp(nnz ? nnz : 1);
+  DevBuf<uint32_t> pos(nnz ? nnz : 1), pos_tmp(nnz ? nnz : 1);
+  if (nnz) {
+    transpose_keys_kernel<<<grid_for(n), 256, 0, s>>>(n, g->csc_ptr.p, g->csc_idx.p, b, keys.p, pos.p);
+    NWG_LAUNCH_CHECK();
+    radix_sort_pairs(s, keys.p, tmp.p, pos.p, pos_tmp.p, nnz, 2 * b);
+  }
+  g->csr_ptr.alloc((uint64_t)n + 1);
+  g->csr_idx.alloc(nnz ? nnz : 1);
+  if (nnz) {
+    csx_kernel<<<grid_for(nnz), 256, 0, s>>>(keys.p, nnz, n, b, g->csr_ptr.p, g->csr_idx.p);
+  } else {
+    fill_u32_kernel<<<grid_for((uint64_t)n + 1), 256, 0, s>>>(g->csr_ptr.p, (uint64_t)n + 1, 0u);
+  }
+  NWG_LAUNCH_CHECK();
+  g->csr_to_csc = std::move(pos);
+  NWG_HIP(hipStreamSynchronize(s));
+  g->has_csr = true;
+}
+
+}  // namespace nwg

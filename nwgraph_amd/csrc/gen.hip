@@ -1,0 +1,125 @@
+// gen.hip -- synthetic graph generators on the device (RMAT, 2D grid) and edge weights.
+//
+// Every output element is a pure function of (index, seed) through Philox4x32-10,
+// so any subrange can be regenerated and the host restatement produces the same
+// edges bit for bit.
+//
+// RMAT edge e: level l (most significant bit first) takes word l%4 of
+// Philox({e.lo, e.hi, l/4, 'RMAT'}, seed) and picks quadrant A=(0,0) if r < tA,
+// B=(0,1) if r < tAB, C=(1,0) if r < tABC, else D=(1,1), as (source bit, target bit).
+// No noise and no vertex permutation.
+#include "device_utils.hpp"
+#include "nwg_internal.hpp"
+#include "philox.hpp"
+
+namespace nwg {
+
+namespace {
+
+__global__ __launch_bounds__(256) void rmat_kernel(uint32_t scale, uint64_t tA, uint64_t tAB, uint64_t tABC,
+                                                   uint64_t seed, uint64_t first, uint64_t count,
+                                                   uint32_t* __restrict__ src, uint32_t* __restrict__ dst) {
+  const uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= count) return;
+  const uint64_t e = first + k;
+  uint32_t s = 0, d = 0;
+  for (uint32_t l0 = 0; l0 < scale; l0 += 4) {
+    const U32x4 r = philox_idx(e, l0 >> 2, kTagRmat, seed);
+#pragma unroll
+    for (uint32_t q = 0; q < 4; ++q) {
+      const uint32_t l = l0 + q;
+      if (l < scale) {
+        const uint64_t x = r.v[q];
+        const uint32_t sb = x >= tAB ? 1u : 0u;
+        const uint32_t db = x < tA ? 0u : (x < tAB ? 1u : (x < tABC ? 0u : 1u));
+        s |= sb << (scale - 1 - l);
+        d |= db << (scale - 1 - l);
+      }
+    }
+  }
+  src[k] = s;
+  dst[k] = d;
+}
+
+// Lattice edge k -> slots 2k, 2k+1; long-range edge of vertex v -> slot 2S + v.
+__global__ __launch_bounds__(256) void grid_kernel(uint64_t W, uint64_t tk, uint64_t tl, uint64_t seed,
+                                                   uint32_t* __restrict__ src, uint32_t* __restrict__ dst) {
+  const uint64_t H = W * (W - 1), S = 2 * H, N = W * W;
+  const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < S) {
+    const uint64_t k = t;
+    uint64_t a, b;
+    if (k < H) {
+      const uint64_t row = k / (W - 1), col = k % (W - 1);
+      a = row * W + col;
+      b = a + 1;
+    } else {
+      const uint64_t kk = k - H, row = kk / W, col = kk % W;
+      a = row * W + col;
+      b = a + W;
+    }
+    const U32x4 r = philox_idx(k, 0, kTagGrid, seed);
+    if (r.v[0] < tk) {
+      src[2 * k] = (uint32_t)a;
+      dst[2 * k] = (uint32_t)b;
+      src[2 * k + 1] = (uint32_t)b;
+      dst[2 * k + 1] = (uint32_t)a;
+    } else {
+      src[2 * k] = dst[2 * k] = src[2 * k + 1] = dst[2 * k + 1] = 0xFFFFFFFFu;
+    }
+  } else if (t < S + N) {
+    const uint64_t v = t - S;
+    const U32x4 r = philox_idx(v, 1, kTagGrid, seed);
+    if (r.v[0] < tl) {
+      const uint64_t off = ((uint64_t)r.v[1] * (N - 1)) >> 32;
+      src[2 * S + v] = (uint32_t)v;
+      dst[2 * S + v] = (uint32_t)((v + 1 + off) % N);
+    } else {
+      src[2 * S + v] = dst[2 * S + v] = 0xFFFFFFFFu;
+    }
+  }
+}
+
+// w(u, v) = fma(0.9, t, 0.1), t = (Philox({u, v, 3, 'WGHT'}, seed)[0] >> 8) * 2^-24, one thread per row.
+__global__ __launch_bounds__(256) void weight_kernel(uint32_t n, const uint32_t* __restrict__ ptr,
+                                                     const uint32_t* __restrict__ idx, uint64_t seed,
+                                                     float* __restrict__ w) {
+  const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= n) return;
+  for (uint32_t p = ptr[v]; p < ptr[v + 1]; ++p) {
+    const uint32_t u = idx[p];
+    const U32x4 r = philox4x32_10(u, v, 3u, kTagWght, (uint32_t)seed, (uint32_t)(seed >> 32));
+    const float t = (float)(r.v[0] >> 8) * 0x1p-24f;
+    w[p] = __builtin_fmaf(0.9f, t, 0.1f);
+  }
+}
+
+}  // namespace
+
+void rmat_edges_device(hipStream_t s, uint32_t scale, uint32_t a, uint32_t b, uint32_t c, uint64_t seed,
+                       uint64_t first, uint64_t count, uint32_t* src, uint32_t* dst) {
+  NWG_REQUIRE(scale >= 1 && scale <= 31, "rmat scale must be in [1, 31]");
+  NWG_REQUIRE((uint64_t)a + b + c <= 10000, "rmat a + b + c must not exceed 10000");
+  if (!count) return;
+  const uint64_t tA = per10000_threshold(a), tAB = per10000_threshold(a + b), tABC = per10000_threshold(a + b + c);
+  rmat_kernel<<<(uint32_t)ceil_div(count, 256), 256, 0, s>>>(scale, tA, tAB, tABC, seed, first, count, src, dst);
+  NWG_LAUNCH_CHECK();
+}
+
+void grid_edges_device(hipStream_t s, uint32_t width, uint32_t keep, uint32_t lng, uint64_t seed, uint32_t* src,
+                       uint32_t* dst) {
+  NWG_REQUIRE(width >= 2 && (uint64_t)width * width < (1ull << 32), "grid width must be in [2, 65535]");
+  const uint64_t W = width, total = 2 * W * (W - 1) + W * W;
+  grid_kernel<<<(uint32_t)ceil_div(total, 256), 256, 0, s>>>(W, per10000_threshold(keep), per10000_threshold(lng),
+                                                              seed, src, dst);
+  NWG_LAUNCH_CHECK();
+}
+
+void edge_weights_device(hipStream_t s, uint32_t n, const uint32_t* csc_ptr, const uint32_t* csc_idx, uint64_t seed,
+                         float* w) {
+  if (!n) return;
+  weight_kernel<<<(uint32_t)ceil_div(n, 256), 256, 0, s>>>(n, csc_ptr, csc_idx, seed, w);
+  NWG_LAUNCH_CHECK();
+}
+
+}  // namespace nwg

@@ -1,0 +1,150 @@
+// io.cpp -- graph input formats of the reference's PageRank driver (load_graph,
+// bench/gapbs/common.hpp:82-101): Matrix Market text and the NW binary edge list.
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <fstream>
+#include <sstream>
+#include <string>
+#include <vector>
+
+#include "../../include/nwgraph_hip.h"
+
+namespace {
+
+struct Edges {
+  std::vector<uint32_t> s, d;
+};
+
+int export_edges(Edges& e, uint32_t* ne_n, uint64_t* ne, uint32_t** src, uint32_t** dst, uint64_t n) {
+  (void)ne_n;
+  const size_t m = e.s.size();
+  uint32_t* a = static_cast<uint32_t*>(std::malloc((m ? m : 1) * sizeof(uint32_t)));
+  uint32_t* b = static_cast<uint32_t*>(std::malloc((m ? m : 1) * sizeof(uint32_t)));
+  if (!a || !b) {
+    std::free(a);
+    std::free(b);
+    return NWG_ENOMEM;
+  }
+  if (m) {
+    std::memcpy(a, e.s.data(), m * sizeof(uint32_t));
+    std::memcpy(b, e.d.data(), m * sizeof(uint32_t));
+  }
+  *ne = m;
+  *src = a;
+  *dst = b;
+  (void)n;
+  return NWG_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+// read_mm<directed> (io/mmio.hpp:171-208) + mm_fill (mmio.hpp:86-103): header
+// "%%MatrixMarket matrix coordinate <field> <symmetric|general>", '%' comment lines,
+// "n0 n1 nnz", then nnz lines "i j [value]" (1-based).  A symmetric file also adds
+// (j, i) for i != j.  n = max(n0, max id + 1) (edge_list.hpp:188-199).
+int nwg_read_mm(const char* path, uint32_t* n, uint64_t* ne, uint32_t** src, uint32_t** dst) {
+  if (!path || !n || !ne || !src || !dst) return NWG_EINVAL;
+  std::ifstream in(path);
+  if (!in) return NWG_EINVAL;
+  std::string line;
+  if (!std::getline(in, line)) return NWG_EFORMAT;
+  std::stringstream h(line);
+  std::string hdr[5];
+  for (auto& t : hdr) h >> t;
+  if (hdr[0] != "%%MatrixMarket") return NWG_EFORMAT;
+  bool sym;
+  if (hdr[4] == "symmetric") sym = true;
+  else if (hdr[4] == "general") sym = false;
+  else return NWG_EFORMAT;
+  while (std::getline(in, line))
+    if (line.empty() || line[0] != '%') break;
+  unsigned long long n0 = 0, n1 = 0, nnz = 0;
+  if (std::sscanf(line.c_str(), "%llu %llu %llu", &n0, &n1, &nnz) != 3) return NWG_EFORMAT;
+  Edges e;
+  e.s.reserve((sym ? 2 : 1) * nnz);
+  e.d.reserve((sym ? 2 : 1) * nnz);
+  uint64_t N = n0;
+  for (unsigned long long k = 0; k < nnz; ++k) {
+    if (!std::getline(in, line)) return NWG_EFORMAT;
+    unsigned long long a = 0, b = 0;
+    if (std::sscanf(line.c_str(), "%llu %llu", &a, &b) != 2 || a == 0 || b == 0) return NWG_EFORMAT;
+    if (a > 0xFFFFFFFFull || b > 0xFFFFFFFFull) return NWG_EFORMAT;
+    e.s.push_back((uint32_t)(a - 1));
+    e.d.push_back((uint32_t)(b - 1));
+    if (sym && a != b) {
+      e.s.push_back((uint32_t)(b - 1));
+      e.d.push_back((uint32_t)(a - 1));
+    }
+    const uint64_t m = a > b ? a : b;
+    if (m > N) N = m;
+  }
+  if (N > 0xFFFFFFFFull) return NWG_EFORMAT;
+  *n = (uint32_t)N;
+  return export_edges(e, nullptr, ne, src, dst, N);
+}
+
+// NW binary edge list: char magic[27] "NW Graph index_edge_list", int directedness,
+// size_t N, bool is_open, then struct_of_arrays: size_t nedges, size_t narrays and per
+// array size_t count, size_t elsize, raw data (edge_list.hpp:237-280,
+// graph_base.hpp:72-87, containers/soa.hpp:332-375).
+static const char kMagic[27] = "NW Graph index_edge_list";
+
+int nwg_read_nw(const char* path, uint32_t* n, uint64_t* ne, uint32_t** src, uint32_t** dst) {
+  if (!path || !n || !ne || !src || !dst) return NWG_EINVAL;
+  std::ifstream in(path, std::ios::binary);
+  if (!in) return NWG_EINVAL;
+  char magic[27];
+  int32_t dirn = 0;
+  uint64_t N = 0, st = 0, na = 0;
+  char open_flag = 0;
+  in.read(magic, 27);
+  if (!in || std::memcmp(magic, kMagic, 27) != 0) return NWG_EFORMAT;
+  in.read(reinterpret_cast<char*>(&dirn), 4);
+  in.read(reinterpret_cast<char*>(&N), 8);
+  in.read(&open_flag, 1);
+  in.read(reinterpret_cast<char*>(&st), 8);
+  in.read(reinterpret_cast<char*>(&na), 8);
+  if (!in || na < 2 || N > 0xFFFFFFFFull) return NWG_EFORMAT;
+  Edges e;
+  std::vector<uint32_t>* cols[2] = {&e.s, &e.d};
+  for (int a = 0; a < 2; ++a) {
+    uint64_t cnt = 0, es = 0;
+    in.read(reinterpret_cast<char*>(&cnt), 8);
+    in.read(reinterpret_cast<char*>(&es), 8);
+    if (!in || cnt != st || es != 4) return NWG_EFORMAT;
+    cols[a]->resize(cnt);
+    in.read(reinterpret_cast<char*>(cols[a]->data()), (std::streamsize)(cnt * 4));
+    if (!in) return NWG_EFORMAT;
+  }
+  *n = (uint32_t)N;
+  return export_edges(e, nullptr, ne, src, dst, N);
+}
+
+int nwg_write_nw(const char* path, uint32_t n, uint64_t ne, const uint32_t* src, const uint32_t* dst) {
+  if (!path || ((!src || !dst) && ne)) return NWG_EINVAL;
+  std::ofstream out(path, std::ios::binary);
+  if (!out) return NWG_EINVAL;
+  const int32_t dirn = 1;  // directedness::directed
+  const uint64_t N = n, narr = 2, es = 4;
+  const char open_flag = 0;
+  out.write(kMagic, 27);
+  out.write(reinterpret_cast<const char*>(&dirn), 4);
+  out.write(reinterpret_cast<const char*>(&N), 8);
+  out.write(&open_flag, 1);
+  out.write(reinterpret_cast<const char*>(&ne), 8);
+  out.write(reinterpret_cast<const char*>(&narr), 8);
+  const uint32_t* cols[2] = {src, dst};
+  for (int a = 0; a < 2; ++a) {
+    out.write(reinterpret_cast<const char*>(&ne), 8);
+    out.write(reinterpret_cast<const char*>(&es), 8);
+    if (ne) out.write(reinterpret_cast<const char*>(cols[a]), (std::streamsize)(ne * 4));
+  }
+  return out ? NWG_OK : NWG_EFORMAT;
+}
+
+void nwg_free(void* p) { std::free(p); }
+
+}  // extern "C"

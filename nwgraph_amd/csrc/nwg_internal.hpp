@@ -1,0 +1,198 @@
+// nwg_internal.hpp -- shared host/device definitions of libnwgraph_hip.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/nwgraph_hip.h"
+
+namespace nwg {
+
+constexpr int kWave = 64;  // CDNA wavefront width
+
+// Error carried from the implementation to the C boundary, where it becomes a status.
+struct Error : std::runtime_error {
+  int code;
+  Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+#define NWG_HIP(call)                                                                      \
+  do {                                                                                     \
+    hipError_t e_ = (call);                                                                \
+    if (e_ != hipSuccess)                                                                  \
+      throw ::nwg::Error(e_ == hipErrorOutOfMemory ? NWG_ENOMEM : NWG_EHIP,                \
+                         std::string(#call) + " failed: " + hipGetErrorString(e_));        \
+  } while (0)
+
+#define NWG_REQUIRE(cond, msg)                                                             \
+  do {                                                                                     \
+    if (!(cond)) throw ::nwg::Error(NWG_EINVAL, msg);                                      \
+  } while (0)
+
+#define NWG_LAUNCH_CHECK() NWG_HIP(hipGetLastError())
+
+inline uint64_t ceil_div(uint64_t a, uint64_t b) { return (a + b - 1) / b; }
+
+// Owning device allocation.
+template <class T>
+struct DevBuf {
+  T* p = nullptr;
+  size_t n = 0;
+  DevBuf() = default;
+  explicit DevBuf(size_t count) { alloc(count); }
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  DevBuf(DevBuf&& o) noexcept : p(o.p), n(o.n) { o.p = nullptr; o.n = 0; }
+  DevBuf& operator=(DevBuf&& o) noexcept {
+    if (this != &o) { release(); p = o.p; n = o.n; o.p = nullptr; o.n = 0; }
+    return *this;
+  }
+  ~DevBuf() { release(); }
+  void alloc(size_t count) {
+    release();
+    if (count) NWG_HIP(hipMalloc(reinterpret_cast<void**>(&p), count * sizeof(T)));
+    n = count;
+  }
+  // grow-only reuse
+  void ensure(size_t count) {
+    if (count > n) alloc(count);
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  T* get() const { return p; }
+};
+
+
+// Scheduling plan of the pull sweep over one CSC.
+// Row classes by in-degree d (rows of a CSC are destinations):
+//   d <= short_max             one lane per row, inside 256-row tiles covering all rows
+//   short_max < d <= group_max a group of 4 / 8 / 16 / 32 lanes per row (by d)
+//   group_max < d < hub_min    one wavefront (64 lanes) per row
+//   d >= hub_min               hub: hub_segment nonzeros per workgroup, partials combined
+//                              in a second kernel in segment order
+constexpr int kGroupClasses = 5;  // 4, 8, 16, 32, 64 lanes
+struct Plan {
+  nwg_plan_params params{};
+  bool built = false;
+  uint32_t n_tiles = 0;
+  uint32_t cls_count[kGroupClasses] = {};
+  uint32_t cls_off[kGroupClasses + 1] = {};  // offsets of each class in group_rows
+  DevBuf<uint32_t> group_rows;               // row ids, ascending inside each class
+  uint32_t n_hub_rows = 0, n_hub_segs = 0;
+  DevBuf<uint32_t> hub_rows;     // ascending row ids
+  DevBuf<uint32_t> hub_seg_ptr;  // n_hub_rows + 1: first segment of each hub row
+  DevBuf<uint32_t> hub_seg_row;  // segment -> index into hub_rows
+};
+
+// Identity of a captured sweep loop: re-capture when any of it changes.
+struct CaptureKey {
+  const void* ptrs[8] = {};
+  uint64_t plan_version = 0;
+  uint64_t iters = 0;
+  float damping = 0.f, threshold = 0.f;
+  uint32_t extra = 0;
+  bool operator==(const CaptureKey& o) const {
+    for (int i = 0; i < 8; ++i)
+      if (ptrs[i] != o.ptrs[i]) return false;
+    return plan_version == o.plan_version && iters == o.iters && damping == o.damping &&
+           threshold == o.threshold && extra == o.extra;
+  }
+};
+
+// A captured and instantiated HIP graph.
+struct Captured {
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  CaptureKey key;
+  bool valid = false;
+  void reset() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    exec = nullptr;
+    graph = nullptr;
+    valid = false;
+  }
+  ~Captured() { reset(); }
+};
+
+// Cached state of PageRank solves on one graph (buffers + captured HIP graph).
+struct PrState {
+  DevBuf<float> x, contrib0, contrib1, base;
+  DevBuf<unsigned long long> acc;  // per-sweep L1 change, fixed point (see device_utils.hpp)
+  DevBuf<float> hub_part;          // per hub segment partial sums
+  Captured cap;
+};
+
+}  // namespace nwg
+
+struct nwg_ctx {
+  int device = 0;
+  hipStream_t own_stream = nullptr;
+  hipStream_t stream = nullptr;
+  std::string last_error;
+  std::mutex mu;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+};
+
+struct nwg_graph {
+  nwg_ctx* ctx = nullptr;
+  int device = 0;
+  uint32_t n = 0;
+  uint64_t nnz = 0;
+  nwg::DevBuf<uint32_t> csc_ptr, csc_idx, outdeg;
+  // optional transposed structure for the adjoint
+  bool has_csr = false;
+  nwg::DevBuf<uint32_t> csr_ptr, csr_idx, csr_to_csc;
+  // optional weights (CSC order, normalised) and raw weights
+  bool has_weights = false;
+  nwg::DevBuf<float> w_csc, w_raw, w_csr, w_rowsum;
+  nwg::Plan plan;
+  uint64_t plan_version = 1;
+  std::unique_ptr<nwg::PrState> pr;
+  std::unique_ptr<nwg::PrState> wpr;
+};
+
+namespace nwg {
+
+// ---- builder (build.hip) ----
+// Sort-based CSC/CSR construction on the device.  src/dst are device arrays of ne
+// raw edges (they are consumed: contents are undefined afterwards).
+void build_csc_device(hipStream_t s, uint32_t n, uint64_t ne, uint32_t* src, uint32_t* dst,
+                      bool clean, nwg_graph* g);
+void build_csr_device(hipStream_t s, nwg_graph* g);
+void out_degrees_device(hipStream_t s, uint32_t n, uint64_t nnz, const uint32_t* col, uint32_t* deg);
+
+// ---- primitives (scan.hip / sort.hip) ----
+// exclusive scan of n values; out has n+1 entries (out[n] = total).
+void exclusive_scan_u32(hipStream_t s, const uint32_t* in, uint32_t* out, uint64_t n);
+// stable LSD radix sort of 64-bit keys over bits [0, key_bits), optional 32-bit payload.
+// Sorted result ends in keys/vals (tmp buffers must have n entries).
+void radix_sort_pairs(hipStream_t s, uint64_t* keys, uint64_t* keys_tmp, uint32_t* vals,
+                      uint32_t* vals_tmp, uint64_t n, int key_bits);
+
+// ---- generators (gen.hip) ----
+void rmat_edges_device(hipStream_t s, uint32_t scale, uint32_t a, uint32_t b, uint32_t c, uint64_t seed,
+                       uint64_t first, uint64_t count, uint32_t* src, uint32_t* dst);
+void grid_edges_device(hipStream_t s, uint32_t width, uint32_t keep, uint32_t lng, uint64_t seed,
+                       uint32_t* src, uint32_t* dst);
+void edge_weights_device(hipStream_t s, uint32_t n, const uint32_t* csc_ptr, const uint32_t* csc_idx,
+                         uint64_t seed, float* w);
+
+// ---- PageRank (pagerank.hip) ----
+// (re)build the sweep plan of g with the given parameters; g is unchanged on failure
+void plan_build(hipStream_t s, nwg_graph* g, const nwg_plan_params& pp);
+nwg_plan_params default_plan_params();
+void pagerank_run(nwg_ctx* ctx, nwg_graph* g, const float* teleport_dev, float damping, float threshold,
+                  uint64_t max_iters, nwg_pr_info* info, float* scores_dev);
+
+}  // namespace nwg

@@ -1,0 +1,467 @@
+// pagerank.hip -- pull-direction PageRank sweep for gfx950.
+//
+// Replaces the sweep of nw::graph::page_rank (include/nwgraph/algorithms/page_rank.hpp:121-147)
+// with the Jacobi schedule of its twin page_rank_v10
+// (include/nwgraph/experimental/algorithms/page_rank.hpp:397-430):
+//   z_i  = sum_{u in in(i)} contrib[u]                 (gather over CSC row i)
+//   x_i' = fma(d, z_i, base)                           (page_rank.hpp:133)
+//   err += |x_i' - x_i|  (float difference, double sum) (page_rank.hpp:134)
+//   contrib'_i = x_i' / (float)outdeg_i                 (page_rank.hpp:116,135)
+// contrib is double-buffered (read contrib[k&1], write contrib[(k+1)&1]); x is
+// updated in place (each row owns its entry).  One sweep is one launch of
+// pr_sweep_kernel, whose workgroups are of four kinds (degree-binned schedule):
+//   tiles      256 consecutive rows, one lane per row with in-degree <= short_max
+//   groups     rows listed per class, 4/8/16/32/64 lanes per row, shuffle reduction
+//   hub segs   hub_segment nonzeros of one hub row per workgroup -> partial sum
+// followed, when hubs exist, by pr_hub_finish_kernel, which adds each hub row's
+// segment partials in order and updates the row.
+//
+// Convergence stays on the device: the L1 change of sweep k is accumulated in
+// fixed point in acc[k] (integer atomics: independent of block order); every
+// workgroup of sweep k+1 first reads acc[k] and exits if it is below threshold
+// (page_rank.hpp:144-146), so a captured loop of max_iters sweeps stops doing work
+// after convergence without a host round trip.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+#include "device_utils.hpp"
+#include "nwg_internal.hpp"
+
+namespace nwg {
+
+namespace {
+
+constexpr int kThreads = 256;
+
+struct SweepParams {
+  const uint32_t* rp;
+  const uint32_t* col;
+  const uint32_t* deg;
+  const float* cin;
+  float* cout;
+  float* x;
+  const float* basev;  // personalised base per row, or null
+  float base;
+  float damp;
+  uint32_t n;
+  uint32_t short_max;
+  uint32_t n_tiles;
+  uint32_t cls_blk[kGroupClasses + 1];  // first workgroup of each group class (relative to n_tiles)
+  uint32_t cls_off[kGroupClasses + 1];
+  const uint32_t* group_rows;
+  uint32_t hub_blk;  // first hub-segment workgroup
+  uint32_t n_hub_segs;
+  uint32_t n_hub_rows;
+  uint32_t hub_segment;
+  const uint32_t* hub_rows;
+  const uint32_t* seg_ptr;
+  const uint32_t* seg_row;
+  float* hub_part;
+  unsigned long long* acc;
+  uint32_t iter;
+  double threshold;
+};
+
+__device__ __forceinline__ bool converged_before(const SweepParams& P) {
+  return P.iter > 0 && fixed_to_err(P.acc[P.iter - 1]) < P.threshold;
+}
+
+__device__ __forceinline__ double update_row(const SweepParams& P, uint32_t i, float z) {
+  const float b = P.basev ? P.basev[i] : P.base;
+  const float xo = P.x[i];
+  const float xn = __builtin_fmaf(P.damp, z, b);
+  P.x[i] = xn;
+  P.cout[i] = xn / (float)P.deg[i];
+  return (double)fabsf(xn - xo);
+}
+
+template <int G>
+__device__ __forceinline__ double group_rows_part(const SweepParams& P, uint32_t local_block, uint32_t cls) {
+  constexpr uint32_t kRowsPerBlock = kThreads / G;
+  const uint32_t slot = local_block * kRowsPerBlock + threadIdx.x / G;
+  const uint32_t gl = threadIdx.x % G;
+  const uint32_t cnt = P.cls_off[cls + 1] - P.cls_off[cls];
+  const bool active = slot < cnt;
+  float z = 0.f;
+  uint32_t i = 0;
+  if (active) {
+    i = P.group_rows[P.cls_off[cls] + slot];
+    const uint32_t p1 = P.rp[i + 1];
+    for (uint32_t p = P.rp[i] + gl; p < p1; p += G) z += P.cin[P.col[p]];
+  }
+  z = group_sum<G>(z);
+  return (active && gl == 0) ? update_row(P, i, z) : 0.0;
+}
+
+__global__ __launch_bounds__(kThreads) void pr_sweep_kernel(SweepParams P) {
+  if (converged_before(P)) return;
+  __shared__ double red_d[kThreads / 64];
+  __shared__ float red_f[kThreads / 64];
+  const uint32_t b = blockIdx.x;
+  double e = 0.0;
+  if (b < P.n_tiles) {
+    // one lane per short row, rows in order inside the lane
+    const uint32_t i = b * kThreads + threadIdx.x;
+    if (i < P.n) {
+      const uint32_t p0 = P.rp[i], p1 = P.rp[i + 1];
+      if (p1 - p0 <= P.short_max) {
+        float z = 0.f;
+        for (uint32_t p = p0; p < p1; ++p) z += P.cin[P.col[p]];
+        e = update_row(P, i, z);
+      }
+    }
+  } else if (b < P.hub_blk) {
+    const uint32_t gb = b - P.n_tiles;
+    if (gb < P.cls_blk[1]) e = group_rows_part<4>(P, gb - P.cls_blk[0], 0);
+    else if (gb < P.cls_blk[2]) e = group_rows_part<8>(P, gb - P.cls_blk[1], 1);
+    else if (gb < P.cls_blk[3]) e = group_rows_part<16>(P, gb - P.cls_blk[2], 2);
+    else if (gb < P.cls_blk[4]) e = group_rows_part<32>(P, gb - P.cls_blk[3], 3);
+    else e = group_rows_part<64>(P, gb - P.cls_blk[4], 4);
+  } else {
+    // one segment of a hub row: partial sum only, finished by pr_hub_finish_kernel
+    const uint32_t s = b - P.hub_blk;
+    const uint32_t h = P.seg_row[s];
+    const uint32_t i = P.hub_rows[h];
+    const uint32_t p0 = P.rp[i] + (s - P.seg_ptr[h]) * P.hub_segment;
+    const uint32_t pe = P.rp[i + 1];
+    const uint32_t p1 = (pe - p0 > P.hub_segment) ? p0 + P.hub_segment : pe;
+    float z = 0.f;
+    for (uint32_t p = p0 + threadIdx.x; p < p1; p += kThreads) z += P.cin[P.col[p]];
+    z = block_sum<kThreads>(z, red_f);
+    if (threadIdx.x == 0) P.hub_part[s] = z;
+    return;
+  }
+  e = block_sum<kThreads>(e, red_d);
+  if (threadIdx.x == 0 && e > 0.0) atomicAdd(P.acc + P.iter, err_to_fixed(e));
+}
+
+__global__ __launch_bounds__(kThreads) void pr_hub_finish_kernel(SweepParams P) {
+  if (converged_before(P)) return;
+  __shared__ double red_d[kThreads / 64];
+  const uint32_t h = blockIdx.x * kThreads + threadIdx.x;
+  double e = 0.0;
+  if (h < P.n_hub_rows) {
+    float z = 0.f;
+    const uint32_t s1 = P.seg_ptr[h + 1];
+    for (uint32_t s = P.seg_ptr[h]; s < s1; ++s) z += P.hub_part[s];
+    e = update_row(P, P.hub_rows[h], z);
+  }
+  e = block_sum<kThreads>(e, red_d);
+  if (threadIdx.x == 0 && e > 0.0) atomicAdd(P.acc + P.iter, err_to_fixed(e));
+}
+
+// x = init, contrib = init / outdeg (page_rank.hpp:100-104,114-118); acc cleared.
+__global__ __launch_bounds__(kThreads) void pr_init_kernel(uint32_t n, const uint32_t* __restrict__ deg, float init,
+                                                           float* __restrict__ x, float* __restrict__ c0,
+                                                           unsigned long long* __restrict__ acc, uint64_t iters) {
+  const uint64_t i = (uint64_t)blockIdx.x * kThreads + threadIdx.x;
+  if (i < n) {
+    x[i] = init;
+    c0[i] = init / (float)deg[i];
+  }
+  if (i < iters) acc[i] = 0ull;
+}
+
+// personalised base (1-d) * v_i rounded once from double
+__global__ __launch_bounds__(kThreads) void pr_base_kernel(uint32_t n, const float* __restrict__ v, double omd,
+                                                           float* __restrict__ base) {
+  const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
+  if (i < n) base[i] = (float)(omd * (double)v[i]);
+}
+
+// ---------------------------------------------------------------------------
+// plan construction
+// ---------------------------------------------------------------------------
+// class id of a row: 0 tile, 1..5 group classes (4..64 lanes), 6 hub
+__device__ __forceinline__ uint32_t row_class(uint32_t d, const nwg_plan_params& pp) {
+  if (d <= pp.short_max) return 0;
+  if (d <= pp.group_max) return 1u + (d > 16) + (d > 32) + (d > 64);
+  if (d < pp.hub_min) return 5;
+  return 6;
+}
+
+__global__ __launch_bounds__(kThreads) void class_flag_kernel(uint32_t n, const uint32_t* __restrict__ rp,
+                                                              nwg_plan_params pp, uint32_t cls,
+                                                              uint32_t* __restrict__ flag) {
+  const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
+  if (i < n) flag[i] = row_class(rp[i + 1] - rp[i], pp) == cls ? 1u : 0u;
+}
+
+__global__ __launch_bounds__(kThreads) void class_scatter_kernel(uint32_t n, const uint32_t* __restrict__ flag,
+                                                                 const uint32_t* __restrict__ pos,
+                                                                 uint32_t* __restrict__ out) {
+  const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
+  if (i < n && flag[i]) out[pos[i]] = i;
+}
+
+__global__ __launch_bounds__(kThreads) void hub_nseg_kernel(uint32_t nh, const uint32_t* __restrict__ rows,
+                                                            const uint32_t* __restrict__ rp, uint32_t seg,
+                                                            uint32_t* __restrict__ nseg) {
+  const uint32_t h = blockIdx.x * kThreads + threadIdx.x;
+  if (h < nh) {
+    const uint32_t i = rows[h];
+    const uint32_t d = rp[i + 1] - rp[i];
+    nseg[h] = (d + seg - 1) / seg;
+  }
+}
+
+__global__ __launch_bounds__(kThreads) void hub_segrow_kernel(uint32_t nh, const uint32_t* __restrict__ seg_ptr,
+                                                              uint32_t* __restrict__ seg_row) {
+  const uint32_t h = blockIdx.x * kThreads + threadIdx.x;
+  if (h < nh)
+    for (uint32_t s = seg_ptr[h]; s < seg_ptr[h + 1]; ++s) seg_row[s] = h;
+}
+
+inline uint32_t blocks_for(uint64_t n) { return (uint32_t)ceil_div(n ? n : 1, kThreads); }
+
+uint32_t d2h_u32(hipStream_t s, const uint32_t* p) {
+  uint32_t v = 0;
+  NWG_HIP(hipMemcpyAsync(&v, p, sizeof v, hipMemcpyDeviceToHost, s));
+  NWG_HIP(hipStreamSynchronize(s));
+  return v;
+}
+
+// rows of class `cls` in ascending order -> out (allocated); returns count
+uint32_t select_class(hipStream_t s, const nwg_graph* g, const nwg_plan_params& pp, uint32_t cls,
+                      DevBuf<uint32_t>& flag, DevBuf<uint32_t>& pos, uint32_t* out /*may be null to count*/) {
+  const uint32_t n = g->n;
+  class_flag_kernel<<<blocks_for(n), kThreads, 0, s>>>(n, g->csc_ptr.p, pp, cls, flag.p);
+  NWG_LAUNCH_CHECK();
+  exclusive_scan_u32(s, flag.p, pos.p, n);
+  const uint32_t cnt = d2h_u32(s, pos.p + n);
+  if (out && cnt) {
+    class_scatter_kernel<<<blocks_for(n), kThreads, 0, s>>>(n, flag.p, pos.p, out);
+    NWG_LAUNCH_CHECK();
+  }
+  return cnt;
+}
+
+}  // namespace
+
+nwg_plan_params default_plan_params() {
+  nwg_plan_params p{};
+  p.short_max = 8;
+  p.group_max = 128;
+  p.hub_min = 16384;
+  p.hub_segment = 4096;
+  p.lds_hub_prefix = 0;
+  return p;
+}
+
+static void validate_params(const nwg_plan_params& p) {
+  NWG_REQUIRE(p.short_max <= p.group_max, "plan: short_max must not exceed group_max");
+  NWG_REQUIRE(p.group_max < p.hub_min, "plan: group_max must be below hub_min");
+  NWG_REQUIRE(p.hub_segment >= 256 && p.hub_segment % 256 == 0, "plan: hub_segment must be a multiple of 256");
+  NWG_REQUIRE(p.lds_hub_prefix == 0, "plan: lds_hub_prefix is not supported by this sweep");
+}
+
+void plan_build(hipStream_t s, nwg_graph* g, const nwg_plan_params& pp) {
+  validate_params(pp);
+  const uint32_t n = g->n;
+  Plan pl;
+  pl.params = pp;
+  DevBuf<uint32_t> flag(n), pos((uint64_t)n + 1);
+  uint32_t counts[kGroupClasses];
+  uint32_t total = 0;
+  for (int c = 0; c < kGroupClasses; ++c) {
+    counts[c] = select_class(s, g, pp, 1 + c, flag, pos, nullptr);
+    total += counts[c];
+  }
+  pl.group_rows.alloc(total ? total : 1);
+  pl.cls_off[0] = 0;
+  for (int c = 0; c < kGroupClasses; ++c) {
+    pl.cls_count[c] = counts[c];
+    pl.cls_off[c + 1] = pl.cls_off[c] + counts[c];
+    if (counts[c]) select_class(s, g, pp, 1 + c, flag, pos, pl.group_rows.p + pl.cls_off[c]);
+  }
+  const uint32_t nh = select_class(s, g, pp, 6, flag, pos, nullptr);
+  pl.n_hub_rows = nh;
+  pl.hub_rows.alloc(nh ? nh : 1);
+  pl.hub_seg_ptr.alloc((uint64_t)nh + 1);
+  pl.n_hub_segs = 0;
+  if (nh) {
+    select_class(s, g, pp, 6, flag, pos, pl.hub_rows.p);
+    hub_nseg_kernel<<<blocks_for(nh), kThreads, 0, s>>>(nh, pl.hub_rows.p, g->csc_ptr.p, pp.hub_segment, flag.p);
+    NWG_LAUNCH_CHECK();
+    exclusive_scan_u32(s, flag.p, pl.hub_seg_ptr.p, nh);
+    pl.n_hub_segs = d2h_u32(s, pl.hub_seg_ptr.p + nh);
+    pl.hub_seg_row.alloc(pl.n_hub_segs);
+    hub_segrow_kernel<<<blocks_for(nh), kThreads, 0, s>>>(nh, pl.hub_seg_ptr.p, pl.hub_seg_row.p);
+    NWG_LAUNCH_CHECK();
+  } else {
+    pl.hub_seg_row.alloc(1);
+  }
+  pl.n_tiles = (uint32_t)ceil_div(n, kThreads);
+  NWG_HIP(hipStreamSynchronize(s));
+  pl.built = true;
+  g->plan = std::move(pl);
+  g->plan_version++;
+}
+
+static SweepParams make_params(nwg_graph* g, PrState& st, bool personalised, float damping, float threshold) {
+  const Plan& pl = g->plan;
+  SweepParams P{};
+  P.rp = g->csc_ptr.p;
+  P.col = g->csc_idx.p;
+  P.deg = g->outdeg.p;
+  P.x = st.x.p;
+  P.basev = personalised ? st.base.p : nullptr;
+  P.base = (float)((1.0 - (double)damping) / g->n);  // page_rank.hpp:94
+  P.damp = damping;
+  P.n = g->n;
+  P.short_max = pl.params.short_max;
+  P.n_tiles = pl.n_tiles;
+  static constexpr uint32_t lanes[kGroupClasses] = {4, 8, 16, 32, 64};
+  P.cls_blk[0] = 0;
+  for (int c = 0; c < kGroupClasses; ++c) {
+    const uint32_t rows_per_block = kThreads / lanes[c];
+    P.cls_blk[c + 1] = P.cls_blk[c] + (uint32_t)ceil_div(pl.cls_count[c], rows_per_block);
+    P.cls_off[c] = pl.cls_off[c];
+  }
+  P.cls_off[kGroupClasses] = pl.cls_off[kGroupClasses];
+  P.group_rows = pl.group_rows.p;
+  P.hub_blk = P.n_tiles + P.cls_blk[kGroupClasses];
+  P.n_hub_segs = pl.n_hub_segs;
+  P.n_hub_rows = pl.n_hub_rows;
+  P.hub_segment = pl.params.hub_segment;
+  P.hub_rows = pl.hub_rows.p;
+  P.seg_ptr = pl.hub_seg_ptr.p;
+  P.seg_row = pl.hub_seg_row.p;
+  P.hub_part = st.hub_part.p;
+  P.acc = st.acc.p;
+  P.threshold = (double)threshold;  // page_rank.hpp:144 compares double error with (double)threshold
+  return P;
+}
+
+// Enqueue init + max_iters sweeps on stream s.  With ev (2*iters events) every sweep
+// kernel is bracketed by an event pair.
+static void enqueue_solve(hipStream_t s, nwg_graph* g, PrState& st, SweepParams P, uint64_t iters,
+                          hipEvent_t* ev) {
+  const uint32_t n = g->n;
+  const float init = (float)(1.0 / n);  // page_rank.hpp:93
+  pr_init_kernel<<<blocks_for(std::max<uint64_t>(n, iters)), kThreads, 0, s>>>(n, g->outdeg.p, init, st.x.p,
+                                                                               st.contrib0.p, st.acc.p, iters);
+  NWG_LAUNCH_CHECK();
+  const uint32_t grid = P.hub_blk + P.n_hub_segs;
+  for (uint64_t k = 0; k < iters; ++k) {
+    P.iter = (uint32_t)k;
+    P.cin = (k & 1) ? st.contrib1.p : st.contrib0.p;
+    P.cout = (k & 1) ? st.contrib0.p : st.contrib1.p;
+    if (ev) NWG_HIP(hipEventRecord(ev[2 * k], s));
+    pr_sweep_kernel<<<grid, kThreads, 0, s>>>(P);
+    NWG_LAUNCH_CHECK();
+    if (ev) NWG_HIP(hipEventRecord(ev[2 * k + 1], s));
+    if (P.n_hub_rows) {
+      pr_hub_finish_kernel<<<blocks_for(P.n_hub_rows), kThreads, 0, s>>>(P);
+      NWG_LAUNCH_CHECK();
+    }
+  }
+}
+
+void pagerank_run(nwg_ctx* ctx, nwg_graph* g, const float* teleport_dev, float damping, float threshold,
+                  uint64_t max_iters, nwg_pr_info* info, float* scores_dev) {
+  hipStream_t s = ctx->stream;
+  NWG_REQUIRE(max_iters < (1ull << 31), "max_iters too large");
+  if (!g->plan.built) plan_build(s, g, default_plan_params());
+  if (!g->pr) g->pr.reset(new PrState());
+  PrState& st = *g->pr;
+  const uint32_t n = g->n;
+  st.x.ensure(n);
+  st.contrib0.ensure(n);
+  st.contrib1.ensure(n);
+  st.acc.ensure(std::max<uint64_t>(max_iters, 1));
+  st.hub_part.ensure(std::max<uint32_t>(g->plan.n_hub_segs, 1));
+  const bool personalised = teleport_dev != nullptr;
+  if (personalised) {
+    st.base.ensure(n);
+    pr_base_kernel<<<blocks_for(n), kThreads, 0, s>>>(n, teleport_dev, 1.0 - (double)damping, st.base.p);
+    NWG_LAUNCH_CHECK();
+  }
+  const SweepParams P = make_params(g, st, personalised, damping, threshold);
+  const bool timing = info && (info->flags & NWG_INFO_KERNEL_TIMING);
+  float ms_total = 0.f;
+  double sweep_ms = 0.0;
+  std::vector<hipEvent_t> evs;
+  if (!timing) {
+    CaptureKey key;
+    key.ptrs[0] = st.x.p;
+    key.ptrs[1] = st.contrib0.p;
+    key.ptrs[2] = st.contrib1.p;
+    key.ptrs[3] = st.acc.p;
+    key.ptrs[4] = st.hub_part.p;
+    key.ptrs[5] = personalised ? st.base.p : nullptr;
+    key.ptrs[6] = g->csc_ptr.p;
+    key.plan_version = g->plan_version;
+    key.iters = max_iters;
+    key.damping = damping;
+    key.threshold = threshold;
+    if (!st.cap.valid || !(st.cap.key == key)) {
+      st.cap.reset();
+      NWG_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+      try {
+        enqueue_solve(s, g, st, P, max_iters, nullptr);
+      } catch (...) {
+        hipGraph_t tmp = nullptr;
+        (void)hipStreamEndCapture(s, &tmp);
+        if (tmp) (void)hipGraphDestroy(tmp);
+        throw;
+      }
+      NWG_HIP(hipStreamEndCapture(s, &st.cap.graph));
+      NWG_HIP(hipGraphInstantiate(&st.cap.exec, st.cap.graph, nullptr, nullptr, 0));
+      st.cap.key = key;
+      st.cap.valid = true;
+    }
+    NWG_HIP(hipEventRecord(ctx->ev0, s));
+    NWG_HIP(hipGraphLaunch(st.cap.exec, s));
+    NWG_HIP(hipEventRecord(ctx->ev1, s));
+  } else {
+    evs.resize(2 * max_iters);
+    for (auto& e : evs) NWG_HIP(hipEventCreate(&e));
+    NWG_HIP(hipEventRecord(ctx->ev0, s));
+    try {
+      enqueue_solve(s, g, st, P, max_iters, evs.data());
+    } catch (...) {
+      for (auto& e : evs) (void)hipEventDestroy(e);
+      throw;
+    }
+    NWG_HIP(hipEventRecord(ctx->ev1, s));
+  }
+  std::vector<unsigned long long> acc(std::max<uint64_t>(max_iters, 1));
+  NWG_HIP(hipMemcpyAsync(acc.data(), st.acc.p, max_iters * sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
+  if (scores_dev) NWG_HIP(hipMemcpyAsync(scores_dev, st.x.p, (size_t)n * sizeof(float), hipMemcpyDeviceToDevice, s));
+  NWG_HIP(hipStreamSynchronize(s));
+  NWG_HIP(hipEventElapsedTime(&ms_total, ctx->ev0, ctx->ev1));
+  uint64_t iters = max_iters;
+  double last = 0.0;
+  for (uint64_t k = 0; k < max_iters; ++k) {
+    const double e = fixed_to_err(acc[k]);
+    last = e;
+    if (e < (double)threshold) {
+      iters = k + 1;
+      break;
+    }
+  }
+  if (timing) {
+    for (uint64_t k = 0; k < iters; ++k) {
+      float t = 0.f;
+      NWG_HIP(hipEventElapsedTime(&t, evs[2 * k], evs[2 * k + 1]));
+      sweep_ms += t;
+    }
+    for (auto& e : evs) (void)hipEventDestroy(e);
+    sweep_ms = iters ? sweep_ms / (double)iters : 0.0;
+  }
+  if (info) {
+    info->iterations = iters;
+    info->final_error = max_iters ? last : 0.0;
+    info->ms_total = ms_total;
+    info->ms_per_iter = iters ? ms_total / (double)iters : 0.0;
+    info->sweep_ms_avg = sweep_ms;
+    if (info->error_trace)
+      for (uint64_t k = 0; k < iters && k < info->error_trace_len; ++k) info->error_trace[k] = fixed_to_err(acc[k]);
+  }
+}
+
+}  // namespace nwg

@@ -1,0 +1,239 @@
+"""GPU parity of the generators, the on-device builder and the PageRank sweep, all
+called through the C ABI, against the CPU oracle on the same seeded inputs."""
+import os
+
+import numpy as np
+import pytest
+
+import pyoracle as O
+from conftest import GOLDEN, golden
+
+pytestmark = pytest.mark.gpu
+
+D = 0.85
+
+
+def csc_equal(g, n, ind, col, deg):
+    ptr, idx, od = g.csc()
+    assert g.n == n and g.nnz == col.size
+    assert np.array_equal(ptr, ind), "row pointers differ"
+    assert np.array_equal(idx, col), "column ids differ"
+    assert np.array_equal(od, deg), "out-degrees differ"
+
+
+# ---------------------------------------------------------------- generators
+@pytest.mark.parametrize("scale,first,count", [(12, 0, 65536), (16, 123456, 300000), (25, 10 ** 9, 4096)])
+def test_rmat_edges_bit_exact(ctx, scale, first, count):
+    s, d = ctx.rmat_edges(scale, first, count, seed=7)
+    so, do = O.rmat_edges(scale, first, count, seed=7)
+    assert np.array_equal(s, so) and np.array_equal(d, do)
+
+
+def test_grid_edges_bit_exact(ctx):
+    s, d = ctx.grid_edges(96, seed=11)
+    so, do = O.grid_edges(96, seed=11)
+    assert np.array_equal(s, so) and np.array_equal(d, do)
+
+
+# ---------------------------------------------------------------- builder
+@pytest.mark.parametrize("scale,ef", [(8, 4), (12, 16), (16, 16)])
+def test_rmat_build_bit_exact(ctx, scale, ef):
+    g = ctx.rmat(scale, ef, seed=3)
+    csc_equal(g, *O.rmat_csc(scale, ef, seed=3))
+
+
+def test_grid_build_bit_exact(ctx):
+    g = ctx.grid(128, seed=5)
+    csc_equal(g, *O.grid_csc(128, seed=5))
+
+
+def test_builder_kats_on_device(ctx):
+    for case in golden("builder_kat.json")["cases"]:
+        e = np.array(case["edges"], np.uint32)
+        n = case["n"]
+        op = case.get("op")
+        if op == "clean":
+            g = ctx.graph_from_edges(n, e[:, 0], e[:, 1], clean=True)
+            assert g.nnz == case["clean_count"], case["name"]
+        elif op == "lexical_sort":
+            # CSR of the raw edges = lexical (u, v) order
+            g = ctx.graph_from_edges(n, e[:, 1], e[:, 0], clean=False)
+            ptr, idx, _ = g.csc()
+            rows = np.repeat(np.arange(n), np.diff(ptr.astype(np.int64)))
+            pairs = list(zip(rows.tolist(), idx.tolist()))
+            assert pairs[: len(case["sorted_prefix"])] == [tuple(p) for p in case["sorted_prefix"]]
+        else:
+            s, d = (e[:, 0], e[:, 1]) if case["idx"] == 1 else (e[:, 1], e[:, 0])
+            g = ctx.graph_from_edges(n, s, d, clean=False)
+            ptr, _, _ = g.csc()
+            assert np.diff(ptr.astype(np.int64)).tolist() == case["row_sizes"], case["name"]
+
+
+def test_build_uncleaned_keeps_duplicates_and_loops(ctx):
+    rng = np.random.default_rng(4)
+    n = 500
+    s = rng.integers(0, n, 20000).astype(np.uint32)
+    d = rng.integers(0, n, 20000).astype(np.uint32)
+    g = ctx.graph_from_edges(n, s, d, clean=False)
+    ind, col = O.csc(n, s, d)
+    csc_equal(g, n, ind, col, O.out_degrees(n, ind, col))
+    gc = ctx.graph_from_edges(n, s, d, clean=True)
+    cs, cd = O.clean_edges(s, d)
+    ind2, col2 = O.csc(n, cs, cd)
+    csc_equal(gc, n, ind2, col2, O.out_degrees(n, ind2, col2))
+
+
+def test_build_rejects_out_of_range_ids(ctx):
+    from nwgraph_amd import NwgError
+
+    with pytest.raises(NwgError):
+        ctx.graph_from_edges(4, np.array([0, 9], np.uint32), np.array([1, 2], np.uint32))
+
+
+def test_csr_bit_exact(ctx):
+    g = ctx.rmat(12, 16, seed=9)
+    n, ind, col, deg = O.rmat_csc(12, 16, seed=9)
+    rows = np.repeat(np.arange(n, dtype=np.uint32), np.diff(ind.astype(np.int64)))
+    rind, rcol = O.fill_sorted(0, n, col, rows)
+    ptr, idx, pos = g.csr()
+    assert np.array_equal(ptr, rind) and np.array_equal(idx, rcol)
+    # csr_to_csc maps every CSR entry (u -> v) to the CSC position of the same edge
+    assert np.array_equal(col[pos], np.repeat(np.arange(n), np.diff(rind.astype(np.int64))))
+    assert np.array_equal(rows[pos], rcol)
+
+
+# ---------------------------------------------------------------- PageRank
+def check_against_oracle(g, n, ind, col, deg, threshold=1e-4, max_iters=20, teleport=None, tol=1e-6):
+    x, info = g.pagerank(D, threshold, max_iters, teleport=teleport)
+    ref, it, tr, _ = O.pr_v10(n, ind, col, deg, D, threshold, max_iters, teleport=teleport, nthreads=8)
+    assert info["iterations"] == it
+    assert np.max(np.abs(x.astype(np.float64) - ref)) <= tol
+    assert np.allclose(info["error_trace"], tr, rtol=1e-3, atol=1e-12)
+    return x, info, ref
+
+
+@pytest.mark.parametrize("scale", [12, 16, 18])
+def test_pagerank_rmat_matches_v10(ctx, scale):
+    g = ctx.rmat(scale, 16, seed=1)
+    n, ind, col, deg = O.rmat_csc(scale, 16, seed=1)
+    x, info, ref = check_against_oracle(g, n, ind, col, deg)
+    # relative agreement far inside the 1e-6 absolute gate (FP32 summation order only)
+    assert np.max(np.abs(x - ref) / np.maximum(ref, 1e-30)) < 1e-4
+    # report-only gap to the sequential in-place (v11, one thread) schedule and to FP64
+    v11, it11, _ = O.pr_v11(n, ind, col, deg)
+    assert np.max(np.abs(x - v11)) < 5e-5
+    x64, _ = O.fp64_jacobi(n, ind, col, deg)
+    assert np.max(np.abs(x - x64)) < 1e-3
+
+
+@pytest.mark.parametrize(
+    "plan",
+    [
+        dict(short_max=0, group_max=0, hub_min=1, hub_segment=256),  # every row a hub
+        dict(short_max=0, group_max=64, hub_min=65, hub_segment=256),
+        dict(short_max=32, group_max=32, hub_min=33, hub_segment=512),
+        dict(short_max=4, group_max=1024, hub_min=100000, hub_segment=4096),  # no hubs
+        dict(short_max=100000, group_max=100000, hub_min=100001, hub_segment=4096),  # all short
+    ],
+)
+def test_pagerank_plans_agree(ctx, plan):
+    g = ctx.rmat(14, 16, seed=2)
+    g.set_plan(**plan)
+    assert {k: g.get_plan()[k] for k in plan} == plan
+    n, ind, col, deg = O.rmat_csc(14, 16, seed=2)
+    check_against_oracle(g, n, ind, col, deg)
+
+
+def test_pagerank_grid_converges_like_oracle(ctx):
+    g = ctx.grid(256, seed=5)
+    n, ind, col, deg = O.grid_csc(256, seed=5)
+    x, info, ref = check_against_oracle(g, n, ind, col, deg, max_iters=100)
+    assert info["iterations"] < 100  # converges before the cap
+
+
+def test_pagerank_karate_golden(ctx):
+    import nwgraph_amd
+
+    kg = golden("karate_pagerank.json")
+    el = nwgraph_amd.read_mm(os.path.join(GOLDEN, "karate.mtx"))
+    adj = nwgraph_amd.build_adjacency(el, ctx=ctx)
+    deg = nwgraph_amd.build_degrees(adj, ctx=ctx)
+    scores = np.empty(adj.size(), np.float32)
+    info = {}
+    nwgraph_amd.page_rank(adj, deg, scores, 0.85, 1e-7, 100, 1, ctx=ctx, info=info)
+    ans = np.array(kg["pagerank"])
+    assert np.all(np.abs(scores - ans) <= kg["rel_tolerance"] * ans)
+    ref, it, _, _ = O.pr_v10(34, adj.indices_, adj.to_be_indexed_, deg, 0.85, 1e-7, 100)
+    assert info["iterations"] == it and np.max(np.abs(scores - ref)) < 1e-7
+
+
+def test_pagerank_personalised_teleport(ctx):
+    g = ctx.rmat(13, 16, seed=4)
+    n, ind, col, deg = O.rmat_csc(13, 16, seed=4)
+    ids = O.ppr_sources(n, 1, 8, seed=4)[0]
+    v = np.zeros(n, np.float32)
+    v[ids] = 1.0 / 8
+    check_against_oracle(g, n, ind, col, deg, teleport=v)
+
+
+def test_edge_cases(ctx):
+    # no edges: every score becomes base after the first sweep
+    g = ctx.graph_from_edges(5, np.zeros(0, np.uint32), np.zeros(0, np.uint32))
+    x, info = g.pagerank(D, 1e-4, 20)
+    base = np.float32((1.0 - np.float64(np.float32(D))) / 5)
+    assert np.all(x == base) and info["iterations"] == 2
+    # one vertex with a self loop removed by cleaning
+    g1 = ctx.graph_from_edges(1, np.array([0], np.uint32), np.array([0], np.uint32))
+    x1, _ = g1.pagerank(D, 1e-4, 20)
+    assert g1.nnz == 0 and x1[0] == np.float32((1.0 - np.float64(np.float32(D))) / 1)
+    # max_iters = 0 leaves the initial scores (page_rank.hpp:100-104)
+    g2 = ctx.rmat(10, 8, seed=1)
+    x2, info2 = g2.pagerank(D, 1e-4, 0)
+    assert info2["iterations"] == 0 and np.all(x2 == np.float32(1.0 / 1024))
+    # a huge threshold stops after one sweep
+    n, ind, col, deg = O.rmat_csc(10, 8, seed=1)
+    x3, info3 = g2.pagerank(D, 10.0, 20)
+    ref, it, _, _ = O.pr_v10(n, ind, col, deg, D, 10.0, 20)
+    assert info3["iterations"] == it == 1 and np.max(np.abs(x3 - ref)) < 1e-9
+    # dangling vertices lose mass: sum of scores stays below 1 (no redistribution)
+    assert x3.sum() < 1.0
+
+
+def test_repeat_calls_bit_identical_and_timing_mode(ctx):
+    g = ctx.rmat(15, 16, seed=6)
+    a, ia = g.pagerank(D, 1e-4, 20)
+    b, ib = g.pagerank(D, 1e-4, 20)
+    c, ic = g.pagerank(D, 1e-4, 20, kernel_timing=True)
+    assert np.array_equal(a, b) and np.array_equal(a, c)
+    assert ia["iterations"] == ib["iterations"] == ic["iterations"]
+    assert ic["sweep_ms_avg"] > 0
+
+
+def test_full_size_properties_rmat22(ctx):
+    """Size-independent checks at RMAT-22: one extra sweep equals the oracle's update
+    of the previous iterate (sampled rows, FP64) and mass balance over all edges."""
+    g = ctx.rmat(22, 16, seed=1)
+    ptr, idx, deg = g.csc()
+    n = g.n
+    # builder invariants: sorted unique rows, no self loops, degree sum
+    assert ptr[-1] == g.nnz == idx.size and int(deg.sum(dtype=np.uint64)) == g.nnz
+    rows = np.repeat(np.arange(n, dtype=np.int64), np.diff(ptr.astype(np.int64)))
+    keys = rows * n + idx.astype(np.int64)
+    assert np.all(np.diff(keys) > 0) and not np.any(rows == idx)
+    x19, i19 = g.pagerank(D, 0.0, 19)
+    x20, i20 = g.pagerank(D, 0.0, 20)
+    assert i19["iterations"] == 19 and i20["iterations"] == 20
+    c = x19.astype(np.float64) / np.where(deg > 0, deg, 1)
+    base = np.float64(np.float32((1.0 - np.float64(np.float32(D))) / n))
+    rng = np.random.default_rng(0)
+    sample = np.concatenate([rng.integers(0, n, 2000), np.argsort(np.diff(ptr.astype(np.int64)))[-50:]])
+    for i in sample:
+        z = c[idx[ptr[i]:ptr[i + 1]]].sum()
+        want = base + np.float64(np.float32(D)) * z
+        assert abs(x20[i] - want) <= 1e-6 * max(want, 1e-12) + 1e-12
+    # mass balance: sum x20 = N base + d * sum_{deg>0} x19
+    lhs = x20.astype(np.float64).sum()
+    rhs = n * base + np.float64(np.float32(D)) * x19.astype(np.float64)[deg > 0].sum()
+    assert abs(lhs - rhs) < 1e-5 * rhs
+    # L1 trace equals the host L1 between the two iterates
+    assert abs(i20["error_trace"][-1] - np.abs(x20.astype(np.float64) - x19).sum()) < 1e-9
