@@ -66,7 +66,9 @@ typedef struct nwg_plan_params {
   uint32_t hub_min;      /* rows with in-degree >= hub_min: split over several workgroups */
   uint32_t hub_segment;  /* nonzeros per workgroup segment of a hub row */
   uint32_t lds_hub_prefix; /* contributions of vertices [0, lds_hub_prefix) staged in LDS (0 = off) */
-  uint32_t reserved[3];
+  uint32_t stream_hints; /* 1: non-temporal loads/stores for the streamed arrays (ids, pointers, scores) */
+  uint32_t ppr_tile;     /* k-tile of the batched solve: 8, 16, 32 or 64 vectors per lane group (0 = widest <= k) */
+  uint32_t reserved;
 } nwg_plan_params;
 
 /* ---------------------------------------------------------------- context */
@@ -167,6 +169,10 @@ int nwg_graph_autotune(nwg_ctx* ctx, nwg_graph* g, nwg_plan_params* chosen /*nul
 int nwg_ppr_batched(nwg_ctx* ctx, nwg_graph* g, uint32_t k, const uint32_t* seed_ptr,
                     const uint32_t* seeds, float* scores, float damping, float threshold,
                     uint64_t max_iters, int mem_kind, uint64_t* iterations, nwg_pr_info* info);
+/* Time the batched sweep with every k-tile (8, 16, 32, 64; at most k rounded up) on this
+ * graph and teleport set, keep the fastest in the graph's plan (ppr_tile) and return it. */
+int nwg_ppr_autotune(nwg_ctx* ctx, nwg_graph* g, uint32_t k, const uint32_t* seed_ptr,
+                     const uint32_t* seeds, uint32_t* tile);
 
 /* --------------------------------------------------- many graphs, one launch */
 /* Concatenate graphs into one block-diagonal device graph. */

@@ -52,11 +52,14 @@ class PlanParams(ctypes.Structure):
         ("hub_min", c_u32),
         ("hub_segment", c_u32),
         ("lds_hub_prefix", c_u32),
-        ("reserved", c_u32 * 3),
+        ("stream_hints", c_u32),
+        ("reserved", c_u32 * 2),
     ]
 
+    FIELDS = ("short_max", "group_max", "hub_min", "hub_segment", "lds_hub_prefix", "stream_hints")
+
     def as_dict(self):
-        return {k: int(getattr(self, k)) for k in ("short_max", "group_max", "hub_min", "hub_segment", "lds_hub_prefix")}
+        return {k: int(getattr(self, k)) for k in self.FIELDS}
 
 
 # name -> (restype, argtypes)
@@ -91,7 +94,8 @@ _SIGNATURES = {
     "nwg_graph_autotune": (c_int, [c_void_p, c_void_p, ctypes.POINTER(PlanParams)]),
     "nwg_ppr_batched": (c_int, [c_void_p, c_void_p, c_u32, c_void_p, c_void_p, c_void_p, c_float, c_float, c_u64,
                                 c_int, c_void_p, ctypes.POINTER(PrInfo)]),
-    "nwg_multigraph_create": (c_int, [c_void_p, c_u32, ctypes.POINTER(c_void_p), ctypes.POINTER(c_void_p)]),
+    "nwg_ppr_autotune": (c_int, [c_void_p, c_void_p, c_u32, c_void_p, c_void_p, P_u32]),
+    "nwg_multigraph_create":(c_int, [c_void_p, c_u32, ctypes.POINTER(c_void_p), ctypes.POINTER(c_void_p)]),
     "nwg_multigraph_destroy": (None, [c_void_p]),
     "nwg_multigraph_info": (c_int, [c_void_p, P_u32, P_u64, P_u64]),
     "nwg_multigraph_pagerank": (c_int, [c_void_p, c_void_p, c_void_p, c_float, c_float, c_u64, c_int, c_void_p,

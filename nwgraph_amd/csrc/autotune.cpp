@@ -1,0 +1,160 @@
+// autotune.cpp -- picks the sweep plan (degree-bin cutoffs, hub split, cache hints)
+// of a graph by timing candidate plans on the device with HIP events.
+//
+// Search: coordinate descent from the default plan, one parameter at a time, each
+// candidate timed over a few sweeps (launched one by one, each bracketed by events).
+// Results are cached per (log2 in-degree histogram, n, nnz, device) for the process
+// and, if NWG_TUNE_CACHE names a file, appended there and reloaded by later processes.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstdlib>
+#include <fstream>
+#include <map>
+#include <mutex>
+#include <sstream>
+#include <string>
+#include <vector>
+
+#include "nwg_internal.hpp"
+
+namespace nwg {
+
+namespace {
+
+std::mutex g_cache_mu;
+std::map<std::string, nwg_plan_params> g_cache;
+bool g_file_loaded = false;
+
+std::string params_to_string(const nwg_plan_params& p) {
+  std::ostringstream o;
+  o << p.short_max << ' ' << p.group_max << ' ' << p.hub_min << ' ' << p.hub_segment << ' ' << p.lds_hub_prefix
+    << ' ' << p.stream_hints;
+  return o.str();
+}
+
+bool params_from_stream(std::istream& in, nwg_plan_params* p) {
+  nwg_plan_params q{};
+  if (!(in >> q.short_max >> q.group_max >> q.hub_min >> q.hub_segment >> q.lds_hub_prefix >> q.stream_hints))
+    return false;
+  *p = q;
+  return true;
+}
+
+void load_file_cache() {
+  if (g_file_loaded) return;
+  g_file_loaded = true;
+  const char* path = std::getenv("NWG_TUNE_CACHE");
+  if (!path) return;
+  std::ifstream in(path);
+  std::string line;
+  while (std::getline(in, line)) {
+    std::istringstream ls(line);
+    std::string key;
+    nwg_plan_params p{};
+    if (ls >> key && params_from_stream(ls, &p)) g_cache[key] = p;
+  }
+}
+
+void store_file_cache(const std::string& key, const nwg_plan_params& p) {
+  const char* path = std::getenv("NWG_TUNE_CACHE");
+  if (!path) return;
+  std::ofstream out(path, std::ios::app);
+  out << key << ' ' << params_to_string(p) << '\n';
+}
+
+// in-degree histogram of the CSC, log2 buckets, as a cache key
+std::string graph_key(hipStream_t s, const nwg_graph* g) {
+  std::vector<uint32_t> rp((size_t)g->n + 1);
+  NWG_HIP(hipMemcpyAsync(rp.data(), g->csc_ptr.p, rp.size() * 4, hipMemcpyDeviceToHost, s));
+  NWG_HIP(hipStreamSynchronize(s));
+  uint64_t hist[33] = {};
+  for (uint32_t i = 0; i < g->n; ++i) {
+    const uint32_t d = rp[i + 1] - rp[i];
+    int b = 0;
+    while (b < 32 && (1u << b) <= d) ++b;
+    hist[b]++;
+  }
+  int dev = 0;
+  NWG_HIP(hipGetDevice(&dev));
+  hipDeviceProp_t prop{};
+  NWG_HIP(hipGetDeviceProperties(&prop, dev));
+  uint64_t h = 1469598103934665603ull;  // FNV-1a over the histogram
+  for (uint64_t v : hist) {
+    for (int q = 0; q < 8; ++q) {
+      h ^= (v >> (8 * q)) & 0xFF;
+      h *= 1099511628211ull;
+    }
+  }
+  std::ostringstream o;
+  o << prop.gcnArchName << ':' << prop.multiProcessorCount << ":n" << g->n << ":e" << g->nnz << ":h" << std::hex
+    << h;
+  std::string k = o.str();
+  for (auto& c : k)
+    if (c == ' ') c = '_';
+  return k;
+}
+
+}  // namespace
+
+double time_plan_sweeps(nwg_ctx* ctx, nwg_graph* g, uint64_t iters) {
+  nwg_pr_info info{};
+  info.flags = NWG_INFO_KERNEL_TIMING;
+  DevBuf<float> out(g->n);
+  pagerank_run(ctx, g, nullptr, 0.85f, -1.0f, iters, &info, out.p);
+  return info.sweep_ms_avg;
+}
+
+nwg_plan_params autotune_plan(nwg_ctx* ctx, nwg_graph* g) {
+  hipStream_t s = ctx->stream;
+  const std::string key = graph_key(s, g);
+  {
+    std::lock_guard<std::mutex> lk(g_cache_mu);
+    load_file_cache();
+    auto it = g_cache.find(key);
+    if (it != g_cache.end()) {
+      plan_build(s, g, it->second);
+      return it->second;
+    }
+  }
+  auto eval = [&](const nwg_plan_params& p) -> double {
+    try {
+      plan_build(s, g, p);
+    } catch (const Error&) {
+      return 1e30;  // invalid combination
+    }
+    double best = 1e30;
+    for (int r = 0; r < 2; ++r) best = std::min(best, time_plan_sweeps(ctx, g, 4));
+    return best;
+  };
+  nwg_plan_params best = default_plan_params();
+  double best_ms = eval(best);
+  auto try_field = [&](uint32_t nwg_plan_params::*field, std::initializer_list<uint32_t> values) {
+    for (uint32_t v : values) {
+      nwg_plan_params c = best;
+      c.*field = v;
+      if (c.*field == best.*field) continue;
+      const double ms = eval(c);
+      if (ms < best_ms) {
+        best_ms = ms;
+        best = c;
+      }
+    }
+  };
+  try_field(&nwg_plan_params::stream_hints, {0, 1});
+  try_field(&nwg_plan_params::short_max, {0, 2, 4, 8, 16, 32});
+  try_field(&nwg_plan_params::group_max, {16, 32, 64, 128, 256, 512, 1024});
+  try_field(&nwg_plan_params::hub_min, {2048, 4096, 8192, 16384, 32768, 65536, 262144});
+  try_field(&nwg_plan_params::hub_segment, {1024, 2048, 4096, 8192, 16384});
+  try_field(&nwg_plan_params::short_max, {0, 2, 4, 8, 16, 32});
+  plan_build(s, g, best);
+  {
+    std::lock_guard<std::mutex> lk(g_cache_mu);
+    g_cache[key] = best;
+    store_file_cache(key, best);
+  }
+  return best;
+}
+
+}  // namespace nwg

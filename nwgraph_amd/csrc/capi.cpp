@@ -7,62 +7,18 @@
 #include <cstring>
 #include <new>
 
+#include "capi_util.hpp"
 #include "nwg_internal.hpp"
-#include "philox.hpp"
 
+using nwg::check_graph;
+using nwg::check_mem_kind;
 using nwg::DevBuf;
+using nwg::DeviceGuard;
 using nwg::Error;
+using nwg::guarded;
+using nwg::stage_in;
 
 namespace {
-
-template <class F>
-int guarded(nwg_ctx* ctx, F&& f) {
-  try {
-    f();
-    if (ctx) ctx->last_error.clear();
-    return NWG_OK;
-  } catch (const Error& e) {
-    if (ctx) ctx->last_error = e.what();
-    return e.code;
-  } catch (const std::bad_alloc&) {
-    if (ctx) ctx->last_error = "host allocation failed";
-    return NWG_ENOMEM;
-  } catch (const std::exception& e) {
-    if (ctx) ctx->last_error = e.what();
-    return NWG_EHIP;
-  } catch (...) {
-    if (ctx) ctx->last_error = "unknown failure";
-    return NWG_EHIP;
-  }
-}
-
-struct DeviceGuard {
-  int prev = -1;
-  explicit DeviceGuard(int dev) {
-    if (hipGetDevice(&prev) != hipSuccess) prev = -1;
-    if (prev != dev) NWG_HIP(hipSetDevice(dev));
-  }
-  ~DeviceGuard() {
-    int cur = -1;
-    if (prev >= 0 && hipGetDevice(&cur) == hipSuccess && cur != prev) (void)hipSetDevice(prev);
-  }
-};
-
-void check_graph(nwg_ctx* ctx, const nwg_graph* g) {
-  NWG_REQUIRE(g != nullptr, "graph is null");
-  NWG_REQUIRE(g->device == ctx->device, "graph belongs to another device");
-}
-
-// copy `count` elements in (host or device source) to a device buffer
-template <class T>
-void stage_in(hipStream_t s, DevBuf<T>& dst, const T* src, size_t count, int mem_kind) {
-  dst.alloc(count ? count : 1);
-  if (count)
-    NWG_HIP(hipMemcpyAsync(dst.p, src, count * sizeof(T),
-                           mem_kind == NWG_MEM_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, s));
-}
-
-void check_mem_kind(int k) { NWG_REQUIRE(k == NWG_MEM_HOST || k == NWG_MEM_DEVICE, "mem_kind must be 0 or 1"); }
 
 // validate a host CSC: n+1 monotone pointers ending at nnz, entries < n
 void validate_host_csc(uint32_t n, uint64_t nnz, const uint32_t* ptr, const uint32_t* idx) {
@@ -96,7 +52,10 @@ int nwg_create(nwg_ctx** out, int device) {
     NWG_REQUIRE(device >= 0 && device < count, "no such HIP device");
     ctx->device = device;
     DeviceGuard dg(device);
-    NWG_HIP(hipStreamCreateWithFlags(&ctx->own_stream, hipStreamNonBlocking));
+    // a blocking stream: ordered after work a caller queued on the legacy default stream
+    NWG_HIP(hipStreamCreate(&ctx->own_stream));
+    // a private stream that only records HIP graphs (never executes work)
+    NWG_HIP(hipStreamCreateWithFlags(&ctx->capture_stream, hipStreamNonBlocking));
     ctx->stream = ctx->own_stream;
     NWG_HIP(hipEventCreate(&ctx->ev0));
     NWG_HIP(hipEventCreate(&ctx->ev1));
@@ -118,6 +77,7 @@ void nwg_destroy(nwg_ctx* ctx) {
   if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
   if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
   if (ctx->own_stream) (void)hipStreamDestroy(ctx->own_stream);
+  if (ctx->capture_stream) (void)hipStreamDestroy(ctx->capture_stream);
   if (prev >= 0) (void)hipSetDevice(prev);
   delete ctx;
 }

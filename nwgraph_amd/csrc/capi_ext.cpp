@@ -1,0 +1,184 @@
+// capi_ext.cpp -- extern "C" entry points of the batched, multi-graph, weighted and
+// autotuning paths (argument checks, staging, status codes).
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+#include <vector>
+
+#include "capi_util.hpp"
+#include "nwg_internal.hpp"
+
+using nwg::check_graph;
+using nwg::check_mem_kind;
+using nwg::DevBuf;
+using nwg::DeviceGuard;
+using nwg::guarded;
+using nwg::Staged;
+
+extern "C" {
+
+int nwg_graph_autotune(nwg_ctx* ctx, nwg_graph* g, nwg_plan_params* chosen) {
+  if (!ctx) return NWG_EINVAL;
+  return guarded(ctx, [&] {
+    check_graph(ctx, g);
+    DeviceGuard dg(ctx->device);
+    const nwg_plan_params p = nwg::autotune_plan(ctx, g);
+    if (chosen) *chosen = p;
+  });
+}
+
+int nwg_ppr_batched(nwg_ctx* ctx, nwg_graph* g, uint32_t k, const uint32_t* seed_ptr, const uint32_t* seeds,
+                    float* scores, float damping, float threshold, uint64_t max_iters, int mem_kind,
+                    uint64_t* iterations, nwg_pr_info* info) {
+  if (!ctx) return NWG_EINVAL;
+  return guarded(ctx, [&] {
+    check_graph(ctx, g);
+    check_mem_kind(mem_kind);
+    NWG_REQUIRE(k >= 1 && seed_ptr && seeds && scores, "k, seed_ptr, seeds and scores are required");
+    NWG_REQUIRE(seed_ptr[0] == 0, "seed_ptr[0] must be 0");
+    for (uint32_t j = 0; j < k; ++j) NWG_REQUIRE(seed_ptr[j] <= seed_ptr[j + 1], "seed_ptr must be non-decreasing");
+    DeviceGuard dg(ctx->device);
+    hipStream_t s = ctx->stream;
+    std::vector<uint32_t> sp(seed_ptr, seed_ptr + k + 1), sd(seeds, seeds + seed_ptr[k]);
+    // the k-tile: the plan's (set by nwg_ppr_autotune or nwg_graph_set_plan), else the
+    // widest tile that does not exceed k
+    const uint32_t req = g->plan.built ? g->plan.params.ppr_tile : 0;
+    const uint32_t tile = req ? req : (k >= 64 ? 64 : k >= 32 ? 32 : k >= 16 ? 16 : 8);
+    Staged<float> sc(s, scores, (size_t)g->n * k, mem_kind, false);
+    nwg::ppr_run(ctx, g, k, sp, sd, damping, threshold, max_iters, tile, sc.dev, iterations, info);
+    sc.finish(s);
+  });
+}
+
+int nwg_ppr_autotune(nwg_ctx* ctx, nwg_graph* g, uint32_t k, const uint32_t* seed_ptr, const uint32_t* seeds,
+                     uint32_t* tile_out) {
+  if (!ctx) return NWG_EINVAL;
+  return guarded(ctx, [&] {
+    check_graph(ctx, g);
+    NWG_REQUIRE(k >= 1 && seed_ptr && seeds, "k, seed_ptr and seeds are required");
+    DeviceGuard dg(ctx->device);
+    hipStream_t s = ctx->stream;
+    if (!g->plan.built) nwg::plan_build(s, g, nwg::default_plan_params());
+    std::vector<uint32_t> sp(seed_ptr, seed_ptr + k + 1), sd(seeds, seeds + seed_ptr[k]);
+    DevBuf<float> out((size_t)g->n * k);
+    uint32_t best = 0;
+    double best_ms = 1e30;
+    for (uint32_t t : {8u, 16u, 32u, 64u}) {
+      if (t > 8 && t / 2 >= k) break;  // wider tiles than k only idle lanes
+      double ms = 1e30;
+      for (int r = 0; r < 2; ++r) {
+        nwg_pr_info info{};
+        nwg::ppr_run(ctx, g, k, sp, sd, 0.85f, -1.0f, 3, t, out.p, nullptr, &info);
+        ms = std::min(ms, info.ms_per_iter);
+      }
+      if (ms < best_ms) {
+        best_ms = ms;
+        best = t;
+      }
+    }
+    g->plan.params.ppr_tile = best;
+    if (tile_out) *tile_out = best;
+  });
+}
+
+int nwg_multigraph_create(nwg_ctx* ctx, uint32_t ngraphs, nwg_graph* const* graphs, nwg_multigraph** out) {
+  if (!ctx || !out) return NWG_EINVAL;
+  *out = nullptr;
+  return guarded(ctx, [&] {
+    NWG_REQUIRE(graphs != nullptr, "graphs must not be null");
+    DeviceGuard dg(ctx->device);
+    *out = nwg::multigraph_create(ctx, ngraphs, graphs);
+  });
+}
+
+int nwg_multigraph_pagerank(nwg_ctx* ctx, nwg_multigraph* mg, float* scores, float damping, float threshold,
+                            uint64_t max_iters, int mem_kind, uint64_t* iterations, double* final_error,
+                            nwg_pr_info* info) {
+  if (!ctx) return NWG_EINVAL;
+  return guarded(ctx, [&] {
+    NWG_REQUIRE(mg != nullptr, "multigraph is null");
+    check_mem_kind(mem_kind);
+    NWG_REQUIRE(scores != nullptr, "scores must not be null");
+    DeviceGuard dg(ctx->device);
+    hipStream_t s = ctx->stream;
+    Staged<float> sc(s, scores, nwg::multigraph_vertices(mg), mem_kind, false);
+    nwg::multigraph_run(ctx, mg, damping, threshold, max_iters, sc.dev, iterations, final_error, info);
+    sc.finish(s);
+  });
+}
+
+int nwg_graph_set_weights(nwg_ctx* ctx, nwg_graph* g, const float* w_csc, int normalize, int mem_kind) {
+  if (!ctx) return NWG_EINVAL;
+  return guarded(ctx, [&] {
+    check_graph(ctx, g);
+    check_mem_kind(mem_kind);
+    NWG_REQUIRE(w_csc != nullptr || g->nnz == 0, "weights must not be null");
+    DeviceGuard dg(ctx->device);
+    hipStream_t s = ctx->stream;
+    Staged<float> w(s, const_cast<float*>(w_csc), g->nnz, mem_kind, true);
+    nwg::weights_set(s, g, w.dev, normalize != 0);
+  });
+}
+
+int nwg_graph_random_weights(nwg_ctx* ctx, nwg_graph* g, uint64_t seed, int normalize) {
+  if (!ctx) return NWG_EINVAL;
+  return guarded(ctx, [&] {
+    check_graph(ctx, g);
+    DeviceGuard dg(ctx->device);
+    hipStream_t s = ctx->stream;
+    DevBuf<float> w(g->nnz ? g->nnz : 1);
+    nwg::edge_weights_device(s, g->n, g->csc_ptr.p, g->csc_idx.p, seed, w.p);
+    nwg::weights_set(s, g, w.p, normalize != 0);
+  });
+}
+
+int nwg_graph_get_weights(nwg_ctx* ctx, const nwg_graph* g, float* w_csc) {
+  if (!ctx) return NWG_EINVAL;
+  return guarded(ctx, [&] {
+    check_graph(ctx, g);
+    NWG_REQUIRE(g->has_weights, "graph has no weights");
+    NWG_REQUIRE(w_csc != nullptr, "output must not be null");
+    DeviceGuard dg(ctx->device);
+    hipStream_t s = ctx->stream;
+    if (g->nnz) NWG_HIP(hipMemcpyAsync(w_csc, g->w_csc.p, g->nnz * 4, hipMemcpyDeviceToHost, s));
+    NWG_HIP(hipStreamSynchronize(s));
+  });
+}
+
+int nwg_wpr_forward(nwg_ctx* ctx, nwg_graph* g, const float* teleport, float* x, float damping, float threshold,
+                    uint64_t max_iters, int mem_kind, nwg_pr_info* info) {
+  if (!ctx) return NWG_EINVAL;
+  return guarded(ctx, [&] {
+    check_graph(ctx, g);
+    check_mem_kind(mem_kind);
+    NWG_REQUIRE(x != nullptr, "x must not be null");
+    DeviceGuard dg(ctx->device);
+    hipStream_t s = ctx->stream;
+    Staged<float> tv(s, const_cast<float*>(teleport), g->n, mem_kind, true);
+    Staged<float> xv(s, x, g->n, mem_kind, false);
+    nwg::wpr_forward(ctx, g, tv.dev, xv.dev, damping, threshold, max_iters, info);
+    xv.finish(s);
+  });
+}
+
+int nwg_wpr_backward(nwg_ctx* ctx, nwg_graph* g, const float* x, const float* grad_x, float* grad_w,
+                     float* grad_teleport, float damping, float threshold, uint64_t max_iters, int raw_weights,
+                     int mem_kind, nwg_pr_info* info) {
+  if (!ctx) return NWG_EINVAL;
+  return guarded(ctx, [&] {
+    check_graph(ctx, g);
+    check_mem_kind(mem_kind);
+    NWG_REQUIRE(x && grad_x, "x and grad_x are required");
+    DeviceGuard dg(ctx->device);
+    hipStream_t s = ctx->stream;
+    Staged<float> xv(s, const_cast<float*>(x), g->n, mem_kind, true);
+    Staged<float> gx(s, const_cast<float*>(grad_x), g->n, mem_kind, true);
+    Staged<float> gw(s, grad_w, g->nnz, mem_kind, false);
+    Staged<float> gv(s, grad_teleport, g->n, mem_kind, false);
+    nwg::wpr_backward(ctx, g, xv.dev, gx.dev, gw.dev, gv.dev, damping, threshold, max_iters, raw_weights != 0, info);
+    gw.finish(s);
+    gv.finish(s);
+  });
+}
+
+}  // extern "C"

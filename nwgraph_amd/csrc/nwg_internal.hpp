@@ -130,6 +130,7 @@ struct PrState {
   DevBuf<float> x, contrib0, contrib1, base;
   DevBuf<unsigned long long> acc;  // per-sweep L1 change, fixed point (see device_utils.hpp)
   DevBuf<float> hub_part;          // per hub segment partial sums
+  DevBuf<double> blk_err;          // per-workgroup L1 partials of one sweep
   Captured cap;
 };
 
@@ -139,6 +140,7 @@ struct nwg_ctx {
   int device = 0;
   hipStream_t own_stream = nullptr;
   hipStream_t stream = nullptr;
+  hipStream_t capture_stream = nullptr;  // used only to record HIP graphs
   std::string last_error;
   std::mutex mu;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -155,7 +157,11 @@ struct nwg_graph {
   nwg::DevBuf<uint32_t> csr_ptr, csr_idx, csr_to_csc;
   // optional weights (CSC order, normalised) and raw weights
   bool has_weights = false;
+  bool w_normalized = false;
   nwg::DevBuf<float> w_csc, w_raw, w_csr, w_rowsum;
+  // work items of the weighted sweeps over the CSC (forward) and the CSR (adjoint)
+  nwg::DevBuf<uint32_t> w_items_csc_r0, w_items_csc_r1, w_items_csr_r0, w_items_csr_r1;
+  uint32_t w_items_csc_n = 0, w_items_csr_n = 0;
   nwg::Plan plan;
   uint64_t plan_version = 1;
   std::unique_ptr<nwg::PrState> pr;
@@ -194,5 +200,30 @@ void plan_build(hipStream_t s, nwg_graph* g, const nwg_plan_params& pp);
 nwg_plan_params default_plan_params();
 void pagerank_run(nwg_ctx* ctx, nwg_graph* g, const float* teleport_dev, float damping, float threshold,
                   uint64_t max_iters, nwg_pr_info* info, float* scores_dev);
+
+// ---- batched personalised PageRank (ppr.hip) ----
+void ppr_run(nwg_ctx* ctx, nwg_graph* g, uint32_t k, const std::vector<uint32_t>& seed_ptr,
+             const std::vector<uint32_t>& seeds, float damping, float threshold, uint64_t max_iters, uint32_t tile,
+             float* scores_dev, uint64_t* iterations, nwg_pr_info* info);
+
+// ---- many graphs (multi.hip) ----
+nwg_multigraph* multigraph_create(nwg_ctx* ctx, uint32_t ngraphs, nwg_graph* const* graphs);
+void multigraph_run(nwg_ctx* ctx, nwg_multigraph* mg, float damping, float threshold, uint64_t max_iters,
+                    float* scores_dev, uint64_t* iterations, double* final_error, nwg_pr_info* info);
+uint64_t multigraph_vertices(const nwg_multigraph* mg);
+
+// ---- weighted PageRank and adjoint (wpr.hip) ----
+void build_items(hipStream_t s, const uint32_t* rp_dev, uint32_t n, DevBuf<uint32_t>& r0, DevBuf<uint32_t>& r1,
+                 uint32_t* count);
+void weights_set(hipStream_t s, nwg_graph* g, const float* w_dev, bool normalize);
+void wpr_forward(nwg_ctx* ctx, nwg_graph* g, const float* teleport_dev, float* x_dev, float damping, float threshold,
+                 uint64_t max_iters, nwg_pr_info* info);
+void wpr_backward(nwg_ctx* ctx, nwg_graph* g, const float* x_dev, const float* gx_dev, float* gw_dev, float* gv_dev,
+                  float damping, float threshold, uint64_t max_iters, bool raw, nwg_pr_info* info);
+
+// ---- autotuner (autotune.cpp) ----
+nwg_plan_params autotune_plan(nwg_ctx* ctx, nwg_graph* g);
+// time `iters` sweeps of the plain kernel under the graph's current plan (ms per sweep)
+double time_plan_sweeps(nwg_ctx* ctx, nwg_graph* g, uint64_t iters);
 
 }  // namespace nwg

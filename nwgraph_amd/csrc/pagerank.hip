@@ -13,8 +13,8 @@
 //   tiles      256 consecutive rows, one lane per row with in-degree <= short_max
 //   groups     rows listed per class, 4/8/16/32/64 lanes per row, shuffle reduction
 //   hub segs   hub_segment nonzeros of one hub row per workgroup -> partial sum
-// followed, when hubs exist, by pr_hub_finish_kernel, which adds each hub row's
-// segment partials in order and updates the row.
+// followed by pr_finish_kernel, which adds each hub row's segment partials in order,
+// updates the hub rows and folds the per-workgroup L1 partials.
 //
 // Convergence stays on the device: the L1 change of sweep k is accumulated in
 // fixed point in acc[k] (integer atomics: independent of block order); every
@@ -52,7 +52,7 @@ struct SweepParams {
   uint32_t cls_blk[kGroupClasses + 1];  // first workgroup of each group class (relative to n_tiles)
   uint32_t cls_off[kGroupClasses + 1];
   const uint32_t* group_rows;
-  uint32_t hub_blk;  // first hub-segment workgroup
+  uint32_t hub_blk;  // workgroups before the hub segments = entries of blk_err
   uint32_t n_hub_segs;
   uint32_t n_hub_rows;
   uint32_t hub_segment;
@@ -60,28 +60,70 @@ struct SweepParams {
   const uint32_t* seg_ptr;
   const uint32_t* seg_row;
   float* hub_part;
+  double* blk_err;  // L1 partial of every tile / group workgroup of the sweep
   unsigned long long* acc;
   uint32_t iter;
   double threshold;
 };
 
+// entries of blk_err reduced by one workgroup of the finishing kernel
+constexpr uint32_t kFinishChunk = 2048;
+
+// Streamed arrays (ids, row pointers, degrees, scores) may be read and written with
+// non-temporal hints so that they do not displace the gathered contributions from
+// L2 / MALL; the gathers themselves always use ordinary cached loads.
+template <bool NT, class T>
+__device__ __forceinline__ T ld(const T* p) {
+  if constexpr (NT) return __builtin_nontemporal_load(p);
+  else return *p;
+}
+template <bool NT, class T>
+__device__ __forceinline__ void st(T* p, T v) {
+  if constexpr (NT) __builtin_nontemporal_store(v, p);
+  else *p = v;
+}
+
 __device__ __forceinline__ bool converged_before(const SweepParams& P) {
   return P.iter > 0 && fixed_to_err(P.acc[P.iter - 1]) < P.threshold;
 }
 
+template <bool NT>
 __device__ __forceinline__ double update_row(const SweepParams& P, uint32_t i, float z) {
-  const float b = P.basev ? P.basev[i] : P.base;
-  const float xo = P.x[i];
+  const float b = P.basev ? ld<NT>(P.basev + i) : P.base;
+  const float xo = ld<NT>(P.x + i);
   const float xn = __builtin_fmaf(P.damp, z, b);
-  P.x[i] = xn;
-  P.cout[i] = xn / (float)P.deg[i];
+  st<NT>(P.x + i, xn);
+  st<NT>(P.cout + i, xn / (float)ld<NT>(P.deg + i));
   return (double)fabsf(xn - xo);
 }
 
-template <int G>
-__device__ __forceinline__ double group_rows_part(const SweepParams& P, uint32_t local_block, uint32_t cls) {
+// Tiles: 256 consecutive rows, one lane per row with in-degree <= short_max; the lane
+// adds its row's contributions in row order.
+template <bool NT>
+__global__ __launch_bounds__(kThreads) void pr_tile_kernel(SweepParams P) {
+  if (converged_before(P)) return;
+  __shared__ double red_d[kThreads / 64];
+  const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
+  double e = 0.0;
+  if (i < P.n) {
+    const uint32_t p0 = ld<NT>(P.rp + i), p1 = ld<NT>(P.rp + i + 1);
+    if (p1 - p0 <= P.short_max) {
+      float z = 0.f;
+      for (uint32_t p = p0; p < p1; ++p) z += P.cin[ld<NT>(P.col + p)];
+      e = update_row<NT>(P, i, z);
+    }
+  }
+  e = block_sum<kThreads>(e, red_d);
+  if (threadIdx.x == 0) P.blk_err[blockIdx.x] = e;
+}
+
+// Group classes: G lanes per listed row stride over the row, butterfly reduction.
+template <int G, bool NT>
+__global__ __launch_bounds__(kThreads) void pr_group_kernel(SweepParams P, uint32_t cls) {
+  if (converged_before(P)) return;
+  __shared__ double red_d[kThreads / 64];
   constexpr uint32_t kRowsPerBlock = kThreads / G;
-  const uint32_t slot = local_block * kRowsPerBlock + threadIdx.x / G;
+  const uint32_t slot = blockIdx.x * kRowsPerBlock + threadIdx.x / G;
   const uint32_t gl = threadIdx.x % G;
   const uint32_t cnt = P.cls_off[cls + 1] - P.cls_off[cls];
   const bool active = slot < cnt;
@@ -89,56 +131,37 @@ __device__ __forceinline__ double group_rows_part(const SweepParams& P, uint32_t
   uint32_t i = 0;
   if (active) {
     i = P.group_rows[P.cls_off[cls] + slot];
-    const uint32_t p1 = P.rp[i + 1];
-    for (uint32_t p = P.rp[i] + gl; p < p1; p += G) z += P.cin[P.col[p]];
+    const uint32_t p1 = ld<NT>(P.rp + i + 1);
+    for (uint32_t p = ld<NT>(P.rp + i) + gl; p < p1; p += G) z += P.cin[ld<NT>(P.col + p)];
   }
   z = group_sum<G>(z);
-  return (active && gl == 0) ? update_row(P, i, z) : 0.0;
-}
-
-__global__ __launch_bounds__(kThreads) void pr_sweep_kernel(SweepParams P) {
-  if (converged_before(P)) return;
-  __shared__ double red_d[kThreads / 64];
-  __shared__ float red_f[kThreads / 64];
-  const uint32_t b = blockIdx.x;
-  double e = 0.0;
-  if (b < P.n_tiles) {
-    // one lane per short row, rows in order inside the lane
-    const uint32_t i = b * kThreads + threadIdx.x;
-    if (i < P.n) {
-      const uint32_t p0 = P.rp[i], p1 = P.rp[i + 1];
-      if (p1 - p0 <= P.short_max) {
-        float z = 0.f;
-        for (uint32_t p = p0; p < p1; ++p) z += P.cin[P.col[p]];
-        e = update_row(P, i, z);
-      }
-    }
-  } else if (b < P.hub_blk) {
-    const uint32_t gb = b - P.n_tiles;
-    if (gb < P.cls_blk[1]) e = group_rows_part<4>(P, gb - P.cls_blk[0], 0);
-    else if (gb < P.cls_blk[2]) e = group_rows_part<8>(P, gb - P.cls_blk[1], 1);
-    else if (gb < P.cls_blk[3]) e = group_rows_part<16>(P, gb - P.cls_blk[2], 2);
-    else if (gb < P.cls_blk[4]) e = group_rows_part<32>(P, gb - P.cls_blk[3], 3);
-    else e = group_rows_part<64>(P, gb - P.cls_blk[4], 4);
-  } else {
-    // one segment of a hub row: partial sum only, finished by pr_hub_finish_kernel
-    const uint32_t s = b - P.hub_blk;
-    const uint32_t h = P.seg_row[s];
-    const uint32_t i = P.hub_rows[h];
-    const uint32_t p0 = P.rp[i] + (s - P.seg_ptr[h]) * P.hub_segment;
-    const uint32_t pe = P.rp[i + 1];
-    const uint32_t p1 = (pe - p0 > P.hub_segment) ? p0 + P.hub_segment : pe;
-    float z = 0.f;
-    for (uint32_t p = p0 + threadIdx.x; p < p1; p += kThreads) z += P.cin[P.col[p]];
-    z = block_sum<kThreads>(z, red_f);
-    if (threadIdx.x == 0) P.hub_part[s] = z;
-    return;
-  }
+  double e = (active && gl == 0) ? update_row<NT>(P, i, z) : 0.0;
   e = block_sum<kThreads>(e, red_d);
-  if (threadIdx.x == 0 && e > 0.0) atomicAdd(P.acc + P.iter, err_to_fixed(e));
+  if (threadIdx.x == 0) P.blk_err[P.n_tiles + P.cls_blk[cls] + blockIdx.x] = e;
 }
 
-__global__ __launch_bounds__(kThreads) void pr_hub_finish_kernel(SweepParams P) {
+// Hub segments: hub_segment nonzeros of one hub row per workgroup -> partial sum.
+template <bool NT>
+__global__ __launch_bounds__(kThreads) void pr_hubseg_kernel(SweepParams P) {
+  if (converged_before(P)) return;
+  __shared__ float red_f[kThreads / 64];
+  const uint32_t s = blockIdx.x;
+  const uint32_t h = P.seg_row[s];
+  const uint32_t i = P.hub_rows[h];
+  const uint32_t p0 = P.rp[i] + (s - P.seg_ptr[h]) * P.hub_segment;
+  const uint32_t pe = P.rp[i + 1];
+  const uint32_t p1 = (pe - p0 > P.hub_segment) ? p0 + P.hub_segment : pe;
+  float z = 0.f;
+  for (uint32_t p = p0 + threadIdx.x; p < p1; p += kThreads) z += P.cin[ld<NT>(P.col + p)];
+  z = block_sum<kThreads>(z, red_f);
+  if (threadIdx.x == 0) P.hub_part[s] = z;
+}
+
+// Last launch of a sweep: finishes hub rows (segment partials added in segment order)
+// and folds the workgroup L1 partials of the sweep into acc[iter].  Only this kernel's
+// few workgroups touch acc, so the atomics never queue up on one address.
+template <bool NT>
+__global__ __launch_bounds__(kThreads) void pr_finish_kernel(SweepParams P) {
   if (converged_before(P)) return;
   __shared__ double red_d[kThreads / 64];
   const uint32_t h = blockIdx.x * kThreads + threadIdx.x;
@@ -147,8 +170,10 @@ __global__ __launch_bounds__(kThreads) void pr_hub_finish_kernel(SweepParams P) 
     float z = 0.f;
     const uint32_t s1 = P.seg_ptr[h + 1];
     for (uint32_t s = P.seg_ptr[h]; s < s1; ++s) z += P.hub_part[s];
-    e = update_row(P, P.hub_rows[h], z);
+    e = update_row<NT>(P, P.hub_rows[h], z);
   }
+  const uint32_t c0 = blockIdx.x * kFinishChunk;
+  for (uint32_t q = c0 + threadIdx.x; q < c0 + kFinishChunk && q < P.hub_blk; q += kThreads) e += P.blk_err[q];
   e = block_sum<kThreads>(e, red_d);
   if (threadIdx.x == 0 && e > 0.0) atomicAdd(P.acc + P.iter, err_to_fixed(e));
 }
@@ -256,6 +281,9 @@ static void validate_params(const nwg_plan_params& p) {
   NWG_REQUIRE(p.group_max < p.hub_min, "plan: group_max must be below hub_min");
   NWG_REQUIRE(p.hub_segment >= 256 && p.hub_segment % 256 == 0, "plan: hub_segment must be a multiple of 256");
   NWG_REQUIRE(p.lds_hub_prefix == 0, "plan: lds_hub_prefix is not supported by this sweep");
+  NWG_REQUIRE(p.stream_hints <= 1, "plan: stream_hints must be 0 or 1");
+  NWG_REQUIRE(p.ppr_tile == 0 || p.ppr_tile == 8 || p.ppr_tile == 16 || p.ppr_tile == 32 || p.ppr_tile == 64,
+              "plan: ppr_tile must be 0, 8, 16, 32 or 64");
 }
 
 void plan_build(hipStream_t s, nwg_graph* g, const nwg_plan_params& pp) {
@@ -301,6 +329,20 @@ void plan_build(hipStream_t s, nwg_graph* g, const nwg_plan_params& pp) {
   g->plan_version++;
 }
 
+static constexpr uint32_t kLanes[kGroupClasses] = {4, 8, 16, 32, 64};
+
+// workgroups of a sweep that precede the hub segments (tiles + group classes)
+static uint64_t sweep_workgroups_before_hubs(const Plan& pl) {
+  uint64_t b = pl.n_tiles;
+  for (int c = 0; c < kGroupClasses; ++c) b += ceil_div(pl.cls_count[c], kThreads / kLanes[c]);
+  return b;
+}
+
+static uint32_t finish_blocks(const SweepParams& P) {
+  const uint64_t a = ceil_div(P.n_hub_rows, kThreads), b = ceil_div(P.hub_blk, kFinishChunk);
+  return (uint32_t)std::max<uint64_t>(std::max(a, b), 1);
+}
+
 static SweepParams make_params(nwg_graph* g, PrState& st, bool personalised, float damping, float threshold) {
   const Plan& pl = g->plan;
   SweepParams P{};
@@ -314,10 +356,9 @@ static SweepParams make_params(nwg_graph* g, PrState& st, bool personalised, flo
   P.n = g->n;
   P.short_max = pl.params.short_max;
   P.n_tiles = pl.n_tiles;
-  static constexpr uint32_t lanes[kGroupClasses] = {4, 8, 16, 32, 64};
   P.cls_blk[0] = 0;
   for (int c = 0; c < kGroupClasses; ++c) {
-    const uint32_t rows_per_block = kThreads / lanes[c];
+    const uint32_t rows_per_block = kThreads / kLanes[c];
     P.cls_blk[c + 1] = P.cls_blk[c] + (uint32_t)ceil_div(pl.cls_count[c], rows_per_block);
     P.cls_off[c] = pl.cls_off[c];
   }
@@ -331,6 +372,7 @@ static SweepParams make_params(nwg_graph* g, PrState& st, bool personalised, flo
   P.seg_ptr = pl.hub_seg_ptr.p;
   P.seg_row = pl.hub_seg_row.p;
   P.hub_part = st.hub_part.p;
+  P.blk_err = st.blk_err.p;
   P.acc = st.acc.p;
   P.threshold = (double)threshold;  // page_rank.hpp:144 compares double error with (double)threshold
   return P;
@@ -338,6 +380,26 @@ static SweepParams make_params(nwg_graph* g, PrState& st, bool personalised, flo
 
 // Enqueue init + max_iters sweeps on stream s.  With ev (2*iters events) every sweep
 // kernel is bracketed by an event pair.
+// one sweep: tiles, each non-empty group class, hub segments, finish
+template <bool NT>
+static void enqueue_sweep(hipStream_t s, const SweepParams& P) {
+  if (P.n_tiles) pr_tile_kernel<NT><<<P.n_tiles, kThreads, 0, s>>>(P);
+  for (uint32_t c = 0; c < (uint32_t)kGroupClasses; ++c) {
+    const uint32_t nb = P.cls_blk[c + 1] - P.cls_blk[c];
+    if (!nb) continue;
+    switch (c) {
+      case 0: pr_group_kernel<4, NT><<<nb, kThreads, 0, s>>>(P, c); break;
+      case 1: pr_group_kernel<8, NT><<<nb, kThreads, 0, s>>>(P, c); break;
+      case 2: pr_group_kernel<16, NT><<<nb, kThreads, 0, s>>>(P, c); break;
+      case 3: pr_group_kernel<32, NT><<<nb, kThreads, 0, s>>>(P, c); break;
+      default: pr_group_kernel<64, NT><<<nb, kThreads, 0, s>>>(P, c); break;
+    }
+  }
+  if (P.n_hub_segs) pr_hubseg_kernel<NT><<<P.n_hub_segs, kThreads, 0, s>>>(P);
+  pr_finish_kernel<NT><<<finish_blocks(P), kThreads, 0, s>>>(P);
+  NWG_LAUNCH_CHECK();
+}
+
 static void enqueue_solve(hipStream_t s, nwg_graph* g, PrState& st, SweepParams P, uint64_t iters,
                           hipEvent_t* ev) {
   const uint32_t n = g->n;
@@ -345,19 +407,15 @@ static void enqueue_solve(hipStream_t s, nwg_graph* g, PrState& st, SweepParams 
   pr_init_kernel<<<blocks_for(std::max<uint64_t>(n, iters)), kThreads, 0, s>>>(n, g->outdeg.p, init, st.x.p,
                                                                                st.contrib0.p, st.acc.p, iters);
   NWG_LAUNCH_CHECK();
-  const uint32_t grid = P.hub_blk + P.n_hub_segs;
+  const bool nt = g->plan.params.stream_hints != 0;
   for (uint64_t k = 0; k < iters; ++k) {
     P.iter = (uint32_t)k;
     P.cin = (k & 1) ? st.contrib1.p : st.contrib0.p;
     P.cout = (k & 1) ? st.contrib0.p : st.contrib1.p;
     if (ev) NWG_HIP(hipEventRecord(ev[2 * k], s));
-    pr_sweep_kernel<<<grid, kThreads, 0, s>>>(P);
-    NWG_LAUNCH_CHECK();
+    if (nt) enqueue_sweep<true>(s, P);
+    else enqueue_sweep<false>(s, P);
     if (ev) NWG_HIP(hipEventRecord(ev[2 * k + 1], s));
-    if (P.n_hub_rows) {
-      pr_hub_finish_kernel<<<blocks_for(P.n_hub_rows), kThreads, 0, s>>>(P);
-      NWG_LAUNCH_CHECK();
-    }
   }
 }
 
@@ -374,6 +432,7 @@ void pagerank_run(nwg_ctx* ctx, nwg_graph* g, const float* teleport_dev, float d
   st.contrib1.ensure(n);
   st.acc.ensure(std::max<uint64_t>(max_iters, 1));
   st.hub_part.ensure(std::max<uint32_t>(g->plan.n_hub_segs, 1));
+  st.blk_err.ensure(std::max<uint64_t>(sweep_workgroups_before_hubs(g->plan), 1));
   const bool personalised = teleport_dev != nullptr;
   if (personalised) {
     st.base.ensure(n);
@@ -394,22 +453,24 @@ void pagerank_run(nwg_ctx* ctx, nwg_graph* g, const float* teleport_dev, float d
     key.ptrs[4] = st.hub_part.p;
     key.ptrs[5] = personalised ? st.base.p : nullptr;
     key.ptrs[6] = g->csc_ptr.p;
+    key.ptrs[7] = st.blk_err.p;
     key.plan_version = g->plan_version;
     key.iters = max_iters;
     key.damping = damping;
     key.threshold = threshold;
     if (!st.cap.valid || !(st.cap.key == key)) {
       st.cap.reset();
-      NWG_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+      hipStream_t cs = ctx->capture_stream;
+      NWG_HIP(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
       try {
-        enqueue_solve(s, g, st, P, max_iters, nullptr);
+        enqueue_solve(cs, g, st, P, max_iters, nullptr);
       } catch (...) {
         hipGraph_t tmp = nullptr;
-        (void)hipStreamEndCapture(s, &tmp);
+        (void)hipStreamEndCapture(cs, &tmp);
         if (tmp) (void)hipGraphDestroy(tmp);
         throw;
       }
-      NWG_HIP(hipStreamEndCapture(s, &st.cap.graph));
+      NWG_HIP(hipStreamEndCapture(cs, &st.cap.graph));
       NWG_HIP(hipGraphInstantiate(&st.cap.exec, st.cap.graph, nullptr, nullptr, 0));
       st.cap.key = key;
       st.cap.valid = true;

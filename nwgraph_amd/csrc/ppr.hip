@@ -1,0 +1,360 @@
+// ppr.hip -- k personalised PageRank vectors as one SpMM per sweep.
+//
+// Each vector j is the reference's PageRank loop (page_rank_v10 Jacobi schedule,
+// include/nwgraph/experimental/algorithms/page_rank.hpp:377-432) with the uniform
+// base (1-d)/N replaced by (1-d) v_j, v_j uniform over a small seed set, and its own
+// stopping rule (first sweep with L1 change < threshold).  Scores and contributions
+// are row-major N x k, so one in-edge u of row i gathers the contiguous k-float
+// segment C[u][0..k): a T-lane slice of a wavefront (T = k-tile, 8/16/32/64) loads
+// it in one coalesced access while the column-id stream is shared by all k vectors.
+//
+// Work split per sweep (one launch + one finishing launch):
+//   row workgroups   4 * (64/T) consecutive rows x one k-tile; rows with in-degree
+//                    >= hub_min are skipped here
+//   hub segments     hub_segment in-edges of one hub row x one k-tile -> partial vectors
+//   finish           hub rows (partials added in segment order) and per-vector L1
+//                    partials folded into acc[iter][j] (fixed point, order independent)
+// A vector whose previous sweep converged is skipped (its lanes idle); a workgroup
+// whose k-tile is entirely converged exits at once.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <vector>
+
+#include "device_utils.hpp"
+#include "nwg_internal.hpp"
+
+namespace nwg {
+
+namespace {
+
+constexpr int kThreads = 256;
+constexpr uint32_t kSeedFlag = 0x80000000u;
+
+struct PprParams {
+  const uint32_t* rp;
+  const uint32_t* col;
+  const uint32_t* degflag;   // out-degree | kSeedFlag when the row is a seed of some vector
+  const uint32_t* seed_row;  // sorted (row, vector) seed entries
+  const uint32_t* seed_vec;
+  const float* seed_base;    // (1-d) v_j[row] for the entry
+  uint32_t n_seed;
+  const float* cin;
+  float* cout;
+  float* x;
+  uint32_t n, k;
+  float damp;
+  uint32_t hub_min, hub_segment;
+  uint32_t row_blocks;       // workgroups of rows per k-tile
+  uint32_t n_hub_rows, n_hub_segs;
+  const uint32_t* hub_rows;
+  const uint32_t* seg_ptr;
+  const uint32_t* seg_row;
+  float* hub_part;           // n_hub_segs x k
+  double* blk_err;           // (row_blocks * tiles) x T
+  unsigned long long* acc;   // iters x k
+  uint32_t iter;
+  double threshold;
+};
+
+__device__ __forceinline__ bool vec_done(const PprParams& P, uint32_t j) {
+  return P.iter > 0 && fixed_to_err(P.acc[(uint64_t)(P.iter - 1) * P.k + j]) < P.threshold;
+}
+
+__device__ __forceinline__ float seed_base(const PprParams& P, uint32_t i, uint32_t j) {
+  // lower_bound of i in seed_row, then scan the row's entries for vector j
+  uint32_t lo = 0, hi = P.n_seed;
+  while (lo < hi) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (P.seed_row[mid] < i) lo = mid + 1;
+    else hi = mid;
+  }
+  float b = 0.f;
+  for (uint32_t q = lo; q < P.n_seed && P.seed_row[q] == i; ++q)
+    if (P.seed_vec[q] == j) b = P.seed_base[q];
+  return b;
+}
+
+__device__ __forceinline__ double ppr_update(const PprParams& P, uint32_t i, uint32_t j, float z) {
+  const uint32_t df = P.degflag[i];
+  const float b = (df & kSeedFlag) ? seed_base(P, i, j) : 0.f;
+  const uint64_t o = (uint64_t)i * P.k + j;
+  const float xo = P.x[o];
+  const float xn = __builtin_fmaf(P.damp, z, b);
+  P.x[o] = xn;
+  P.cout[o] = xn / (float)(df & ~kSeedFlag);
+  return (double)fabsf(xn - xo);
+}
+
+// per-vector sum over the 64/T row slots of a wave, then over the 4 waves; result for
+// vector lane vl (< T) in threads [0, T)
+template <int T>
+__device__ __forceinline__ double slot_reduce(double e, double* lds /* 4*T */) {
+#pragma unroll
+  for (int o = T; o < 64; o <<= 1) e += __shfl_xor(e, o, 64);
+  const uint32_t w = threadIdx.x / 64, l = threadIdx.x % 64;
+  if (l < (uint32_t)T) lds[w * T + l] = e;
+  __syncthreads();
+  double r = 0.0;
+  if (threadIdx.x < (uint32_t)T)
+    for (int q = 0; q < 4; ++q) r += lds[q * T + threadIdx.x];
+  __syncthreads();
+  return r;
+}
+
+template <int T>
+__global__ __launch_bounds__(kThreads) void ppr_sweep_kernel(PprParams P) {
+  __shared__ double red[4 * T];
+  __shared__ float hub_red[kThreads];
+  __shared__ int any_live;
+  const uint32_t tile = blockIdx.y;
+  const uint32_t vl = threadIdx.x % T;
+  const uint32_t j = tile * T + vl;
+  const bool live = j < P.k && !vec_done(P, j);
+  if (threadIdx.x == 0) any_live = 0;
+  __syncthreads();
+  if (live) any_live = 1;
+  __syncthreads();
+  if (!any_live) return;
+  const uint32_t b = blockIdx.x;
+  if (b < P.row_blocks) {
+    constexpr uint32_t kSlots = kThreads / T;
+    const uint32_t i = b * kSlots + threadIdx.x / T;
+    double e = 0.0;
+    if (i < P.n && live) {
+      const uint32_t p0 = P.rp[i], p1 = P.rp[i + 1];
+      if (p1 - p0 < P.hub_min) {
+        float z = 0.f;
+        for (uint32_t p = p0; p < p1; ++p) z += P.cin[(uint64_t)P.col[p] * P.k + j];
+        e = ppr_update(P, i, j, z);
+      }
+    }
+    e = slot_reduce<T>(e, red);
+    if (threadIdx.x < (uint32_t)T) P.blk_err[((uint64_t)tile * P.row_blocks + b) * T + threadIdx.x] = e;
+  } else {
+    // hub segment s of one k-tile: kThreads/T edge groups stride over the segment
+    const uint32_t s = b - P.row_blocks;
+    const uint32_t h = P.seg_row[s];
+    const uint32_t i = P.hub_rows[h];
+    const uint32_t p0 = P.rp[i] + (s - P.seg_ptr[h]) * P.hub_segment;
+    const uint32_t pe = P.rp[i + 1];
+    const uint32_t p1 = (pe - p0 > P.hub_segment) ? p0 + P.hub_segment : pe;
+    constexpr uint32_t kGroups = kThreads / T;
+    float z = 0.f;
+    if (live)
+      for (uint32_t p = p0 + threadIdx.x / T; p < p1; p += kGroups) z += P.cin[(uint64_t)P.col[p] * P.k + j];
+    hub_red[threadIdx.x] = z;
+    __syncthreads();
+    if (threadIdx.x < (uint32_t)T) {
+      float t = 0.f;
+      for (uint32_t g = 0; g < kGroups; ++g) t += hub_red[g * T + threadIdx.x];
+      if (j < P.k) P.hub_part[(uint64_t)s * P.k + j] = t;
+    }
+  }
+}
+
+// finishing launch: hub rows + fold of the per-workgroup, per-vector L1 partials.
+// grid.x covers max(hub rows * k / 256, partial chunks); each thread owns one (item, vector).
+template <int T>
+__global__ __launch_bounds__(kThreads) void ppr_finish_kernel(PprParams P, uint32_t tiles) {
+  __shared__ double red[kThreads];
+  // part 1: hub rows, one thread per (hub row, vector)
+  const uint64_t t = (uint64_t)blockIdx.x * kThreads + threadIdx.x;
+  const uint64_t hub_items = (uint64_t)P.n_hub_rows * P.k;
+  const uint32_t j = (uint32_t)(t % P.k);
+  double e = 0.0;
+  if (t < hub_items && !vec_done(P, j)) {
+    const uint32_t h = (uint32_t)(t / P.k);
+    float z = 0.f;
+    for (uint32_t s = P.seg_ptr[h]; s < P.seg_ptr[h + 1]; ++s) z += P.hub_part[(uint64_t)s * P.k + j];
+    e = ppr_update(P, P.hub_rows[h], j, z);
+  }
+  if (t < hub_items && e > 0.0) atomicAdd(P.acc + (uint64_t)P.iter * P.k + j, err_to_fixed(e));
+  // part 2: fold blk_err rows; this workgroup folds 32 workgroup rows of every tile
+  const uint64_t rows_total = (uint64_t)P.row_blocks;
+  const uint64_t r0 = (uint64_t)blockIdx.x * 32;
+  for (uint32_t tl = 0; tl < tiles; ++tl) {
+    double acc = 0.0;
+    const uint32_t vl = threadIdx.x % T;
+    const uint32_t slot = threadIdx.x / T;  // kThreads/T slots stride over the 32 rows
+    for (uint64_t r = r0 + slot; r < r0 + 32 && r < rows_total; r += kThreads / T)
+      acc += P.blk_err[((uint64_t)tl * P.row_blocks + r) * T + vl];
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    if (threadIdx.x < (uint32_t)T) {
+      double sum = 0.0;
+      for (uint32_t q = 0; q < kThreads / T; ++q) sum += red[q * T + threadIdx.x];
+      const uint32_t jj = tl * T + threadIdx.x;
+      if (jj < P.k && sum > 0.0 && !vec_done(P, jj))
+        atomicAdd(P.acc + (uint64_t)P.iter * P.k + jj, err_to_fixed(sum));
+    }
+    __syncthreads();
+  }
+}
+
+__global__ __launch_bounds__(kThreads) void ppr_init_kernel(uint32_t n, uint32_t k, const uint32_t* __restrict__ deg,
+                                                            float init, float* __restrict__ x,
+                                                            float* __restrict__ c0) {
+  const uint64_t t = (uint64_t)blockIdx.x * kThreads + threadIdx.x;
+  if (t < (uint64_t)n * k) {
+    const uint32_t i = (uint32_t)(t / k);
+    x[t] = init;
+    c0[t] = init / (float)deg[i];
+  }
+}
+
+__global__ __launch_bounds__(kThreads) void ppr_zero_acc_kernel(unsigned long long* __restrict__ acc, uint64_t m) {
+  const uint64_t t = (uint64_t)blockIdx.x * kThreads + threadIdx.x;
+  if (t < m) acc[t] = 0ull;
+}
+
+__global__ __launch_bounds__(kThreads) void ppr_degflag_kernel(uint32_t n, const uint32_t* __restrict__ deg,
+                                                               const uint32_t* __restrict__ seed_row, uint32_t ns,
+                                                               uint32_t* __restrict__ out) {
+  const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
+  if (i < n) out[i] = deg[i];
+  if (i < ns) atomicOr(&out[seed_row[i]], kSeedFlag);
+}
+
+inline uint32_t grid1(uint64_t m) { return (uint32_t)ceil_div(m ? m : 1, kThreads); }
+
+}  // namespace
+
+// Host driver; tile = k-tile width (8, 16, 32 or 64).
+void ppr_run(nwg_ctx* ctx, nwg_graph* g, uint32_t k, const std::vector<uint32_t>& seed_ptr,
+             const std::vector<uint32_t>& seeds, float damping, float threshold, uint64_t max_iters, uint32_t tile,
+             float* scores_dev, uint64_t* iterations, nwg_pr_info* info) {
+  hipStream_t s = ctx->stream;
+  const uint32_t n = g->n;
+  NWG_REQUIRE(k >= 1, "k must be >= 1");
+  NWG_REQUIRE(tile == 8 || tile == 16 || tile == 32 || tile == 64, "k-tile must be 8, 16, 32 or 64");
+  NWG_REQUIRE(max_iters < (1ull << 31), "max_iters too large");
+  if (!g->plan.built) plan_build(s, g, default_plan_params());
+  const Plan& pl = g->plan;
+  // seed entries sorted by (row, vector)
+  std::vector<std::pair<uint32_t, uint32_t>> ent;
+  std::vector<float> ebase;
+  for (uint32_t j = 0; j < k; ++j) {
+    const uint32_t c = seed_ptr[j + 1] - seed_ptr[j];
+    NWG_REQUIRE(c >= 1, "every teleport vector needs at least one seed");
+    for (uint32_t q = seed_ptr[j]; q < seed_ptr[j + 1]; ++q) {
+      NWG_REQUIRE(seeds[q] < n, "seed id >= n");
+      ent.emplace_back(seeds[q], j);
+    }
+  }
+  std::sort(ent.begin(), ent.end());
+  for (size_t q = 1; q < ent.size(); ++q) NWG_REQUIRE(ent[q] != ent[q - 1], "duplicate seed in a teleport vector");
+  std::vector<uint32_t> erow(ent.size()), evec(ent.size());
+  ebase.resize(ent.size());
+  for (size_t q = 0; q < ent.size(); ++q) {
+    erow[q] = ent[q].first;
+    evec[q] = ent[q].second;
+    const uint32_t c = seed_ptr[ent[q].second + 1] - seed_ptr[ent[q].second];
+    const float v = 1.0f / (float)c;                                  // teleport value v_j[row]
+    ebase[q] = (float)((1.0 - (double)damping) * (double)v);          // same rounding as the plain path
+  }
+  const uint32_t ns = (uint32_t)ent.size();
+  DevBuf<uint32_t> d_row(ns), d_vec(ns), degflag(n);
+  DevBuf<float> d_base(ns);
+  NWG_HIP(hipMemcpyAsync(d_row.p, erow.data(), ns * 4ull, hipMemcpyHostToDevice, s));
+  NWG_HIP(hipMemcpyAsync(d_vec.p, evec.data(), ns * 4ull, hipMemcpyHostToDevice, s));
+  NWG_HIP(hipMemcpyAsync(d_base.p, ebase.data(), ns * 4ull, hipMemcpyHostToDevice, s));
+  ppr_degflag_kernel<<<grid1(std::max(n, ns)), kThreads, 0, s>>>(n, g->outdeg.p, d_row.p, ns, degflag.p);
+  NWG_LAUNCH_CHECK();
+
+  const uint32_t tiles = (uint32_t)ceil_div(k, tile);
+  const uint32_t row_blocks = (uint32_t)ceil_div(n, kThreads / tile);
+  const uint64_t nk = (uint64_t)n * k;
+  DevBuf<float> c0(nk), c1(nk);
+  DevBuf<float> xbuf;
+  float* x = scores_dev;
+  DevBuf<float> hub_part((uint64_t)std::max<uint32_t>(pl.n_hub_segs, 1) * k);
+  DevBuf<double> blk_err((uint64_t)row_blocks * tiles * tile);
+  DevBuf<unsigned long long> acc(std::max<uint64_t>(max_iters, 1) * k);
+
+  PprParams P{};
+  P.rp = g->csc_ptr.p;
+  P.col = g->csc_idx.p;
+  P.degflag = degflag.p;
+  P.seed_row = d_row.p;
+  P.seed_vec = d_vec.p;
+  P.seed_base = d_base.p;
+  P.n_seed = ns;
+  P.x = x;
+  P.n = n;
+  P.k = k;
+  P.damp = damping;
+  P.hub_min = pl.params.hub_min;
+  P.hub_segment = pl.params.hub_segment;
+  P.row_blocks = row_blocks;
+  P.n_hub_rows = pl.n_hub_rows;
+  P.n_hub_segs = pl.n_hub_segs;
+  P.hub_rows = pl.hub_rows.p;
+  P.seg_ptr = pl.hub_seg_ptr.p;
+  P.seg_row = pl.hub_seg_row.p;
+  P.hub_part = hub_part.p;
+  P.blk_err = blk_err.p;
+  P.acc = acc.p;
+  P.threshold = (double)threshold;
+
+  const float init = (float)(1.0 / n);
+  NWG_HIP(hipEventRecord(ctx->ev0, s));
+  ppr_init_kernel<<<grid1(nk), kThreads, 0, s>>>(n, k, g->outdeg.p, init, x, c0.p);
+  NWG_LAUNCH_CHECK();
+  ppr_zero_acc_kernel<<<grid1(max_iters * k), kThreads, 0, s>>>(acc.p, max_iters * k);
+  NWG_LAUNCH_CHECK();
+  const dim3 grid(row_blocks + pl.n_hub_segs, tiles);
+  const uint64_t fin_a = ceil_div((uint64_t)pl.n_hub_rows * k, kThreads), fin_b = ceil_div(row_blocks, 32);
+  const uint32_t fin = (uint32_t)std::max<uint64_t>(std::max(fin_a, fin_b), 1);
+  for (uint64_t it = 0; it < max_iters; ++it) {
+    P.iter = (uint32_t)it;
+    P.cin = (it & 1) ? c1.p : c0.p;
+    P.cout = (it & 1) ? c0.p : c1.p;
+    switch (tile) {
+      case 8:
+        ppr_sweep_kernel<8><<<grid, kThreads, 0, s>>>(P);
+        ppr_finish_kernel<8><<<fin, kThreads, 0, s>>>(P, tiles);
+        break;
+      case 16:
+        ppr_sweep_kernel<16><<<grid, kThreads, 0, s>>>(P);
+        ppr_finish_kernel<16><<<fin, kThreads, 0, s>>>(P, tiles);
+        break;
+      case 32:
+        ppr_sweep_kernel<32><<<grid, kThreads, 0, s>>>(P);
+        ppr_finish_kernel<32><<<fin, kThreads, 0, s>>>(P, tiles);
+        break;
+      default:
+        ppr_sweep_kernel<64><<<grid, kThreads, 0, s>>>(P);
+        ppr_finish_kernel<64><<<fin, kThreads, 0, s>>>(P, tiles);
+        break;
+    }
+    NWG_LAUNCH_CHECK();
+  }
+  NWG_HIP(hipEventRecord(ctx->ev1, s));
+  std::vector<unsigned long long> h_acc(std::max<uint64_t>(max_iters, 1) * k);
+  NWG_HIP(hipMemcpyAsync(h_acc.data(), acc.p, max_iters * k * 8ull, hipMemcpyDeviceToHost, s));
+  NWG_HIP(hipStreamSynchronize(s));
+  float ms = 0.f;
+  NWG_HIP(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+  uint64_t max_it = 0;
+  for (uint32_t j = 0; j < k; ++j) {
+    uint64_t itj = max_iters;
+    for (uint64_t t = 0; t < max_iters; ++t)
+      if (fixed_to_err(h_acc[t * k + j]) < (double)threshold) {
+        itj = t + 1;
+        break;
+      }
+    if (iterations) iterations[j] = itj;
+    max_it = std::max(max_it, itj);
+  }
+  if (info) {
+    info->iterations = max_it;
+    info->final_error = 0.0;
+    for (uint32_t j = 0; j < k && max_it; ++j) info->final_error += fixed_to_err(h_acc[(max_it - 1) * k + j]);
+    info->ms_total = ms;
+    info->ms_per_iter = max_it ? ms / (double)max_it : 0.0;
+    info->sweep_ms_avg = 0.0;
+  }
+}
+
+}  // namespace nwg

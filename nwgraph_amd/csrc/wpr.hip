@@ -1,0 +1,343 @@
+// wpr.hip -- weighted PageRank forward, implicit adjoint and edge gradient.
+//
+// Forward (same fixed-point form as the reference loop, page_rank.hpp:121-147, with a
+// per-edge transition weight instead of 1/outdeg and a teleport vector v):
+//   x_i <- fma(d, z_i, (1-d) v_i),   z_i = sum_{p in CSC row i} w[p] * x[col[p]]
+// Jacobi, x double-buffered, stop at the first sweep with L1 change < threshold.
+// Adjoint of L(x) with g = dL/dx at the fixed point x = (1-d) v + d A x, A[i][u] = w(u->i):
+//   lambda = g + d A^T lambda   -- solved by the same sweep in pull form over the CSR
+// (row u gathers its out-edges u -> i: lambda_u <- fma(d, sum w(u->i) lambda_i, g_u)), so
+// no unrolled forward iterate is stored and no atomics are needed.  Then
+//   dL/dw(u->i) = d * lambda_i * x_u,   dL/dv = (1-d) lambda,
+// and, for raw weights normalised per source (w = w~ / s_u, s_u = sum_k w~(u->k)),
+//   dL/dw~(u->k) = (dL/dw(u->k) - sum_j dL/dw(u->j) w(u->j)) / s_u.
+//
+// Sweep schedule: the rows are cut into work items of up to 256 consecutive rows with at
+// most kItemNnz nonzeros (values staged in LDS, one lane per row) or one longer row
+// (whole-workgroup reduction); a persistent grid walks the items round-robin and adds
+// its L1 partial to acc[k] once per sweep.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <vector>
+
+#include "device_utils.hpp"
+#include "nwg_internal.hpp"
+
+namespace nwg {
+
+namespace {
+
+constexpr int kThreads = 256;
+constexpr uint32_t kItemNnz = 4096;
+constexpr uint32_t kItemRows = 256;
+
+struct WParams {
+  const uint32_t* rp;
+  const uint32_t* col;
+  const float* w;
+  const uint32_t* item_r0;
+  const uint32_t* item_r1;
+  uint32_t n_items;
+  const float* base;  // per row: (1-d) v_i forward, g_u adjoint
+  const float* xin;
+  float* xout;
+  float damp;
+  unsigned long long* acc;
+  uint32_t iter;
+  double threshold;
+};
+
+__global__ __launch_bounds__(kThreads) void wsweep_kernel(WParams P) {
+  if (P.iter > 0 && fixed_to_err(P.acc[P.iter - 1]) < P.threshold) return;
+  __shared__ float vals[kItemNnz];
+  __shared__ float red_f[kThreads / 64];
+  __shared__ double red_d[kThreads / 64];
+  double e = 0.0;
+  for (uint32_t it = blockIdx.x; it < P.n_items; it += gridDim.x) {
+    const uint32_t r0 = P.item_r0[it], r1 = P.item_r1[it];
+    const uint32_t p0 = P.rp[r0], p1 = P.rp[r1];
+    if (p1 - p0 > kItemNnz) {
+      float z = 0.f;
+      for (uint32_t p = p0 + threadIdx.x; p < p1; p += kThreads) z = __builtin_fmaf(P.w[p], P.xin[P.col[p]], z);
+      z = block_sum<kThreads>(z, red_f);
+      if (threadIdx.x == 0) {
+        const float xn = __builtin_fmaf(P.damp, z, P.base[r0]);
+        e += (double)fabsf(xn - P.xin[r0]);
+        P.xout[r0] = xn;
+      }
+    } else {
+      for (uint32_t q = threadIdx.x; q < p1 - p0; q += kThreads) vals[q] = P.w[p0 + q] * P.xin[P.col[p0 + q]];
+      __syncthreads();
+      const uint32_t i = r0 + threadIdx.x;
+      if (i < r1) {
+        const uint32_t a = P.rp[i] - p0, b = P.rp[i + 1] - p0;
+        float z = 0.f;
+        for (uint32_t q = a; q < b; ++q) z += vals[q];
+        const float xn = __builtin_fmaf(P.damp, z, P.base[i]);
+        e += (double)fabsf(xn - P.xin[i]);
+        P.xout[i] = xn;
+      }
+      __syncthreads();
+    }
+  }
+  e = block_sum<kThreads>(e, red_d);
+  if (threadIdx.x == 0 && e > 0.0) atomicAdd(P.acc + P.iter, err_to_fixed(e));
+}
+
+__global__ __launch_bounds__(kThreads) void wfill_kernel(uint32_t n, float v, float* __restrict__ x,
+                                                         unsigned long long* __restrict__ acc, uint64_t nacc) {
+  const uint64_t t = (uint64_t)blockIdx.x * kThreads + threadIdx.x;
+  if (t < n) x[t] = v;
+  if (t < nacc) acc[t] = 0ull;
+}
+
+__global__ __launch_bounds__(kThreads) void wcopy_kernel(uint32_t n, const float* __restrict__ src,
+                                                         float* __restrict__ dst, unsigned long long* __restrict__ acc,
+                                                         uint64_t nacc) {
+  const uint64_t t = (uint64_t)blockIdx.x * kThreads + threadIdx.x;
+  if (t < n) dst[t] = src[t];
+  if (t < nacc) acc[t] = 0ull;
+}
+
+// base_i = (float)((1-d) * v_i) (v null: uniform 1/n)
+__global__ __launch_bounds__(kThreads) void wbase_kernel(uint32_t n, const float* __restrict__ v, double omd,
+                                                         float* __restrict__ base) {
+  const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
+  if (i < n) base[i] = v ? (float)(omd * (double)v[i]) : (float)(omd / (double)n);
+}
+
+// per-source sums of raw weights in CSR order (double), then normalised CSC weights
+__global__ __launch_bounds__(kThreads) void rowsum_kernel(uint32_t n, const uint32_t* __restrict__ csr_ptr,
+                                                          const uint32_t* __restrict__ csr_to_csc,
+                                                          const float* __restrict__ w_raw, float* __restrict__ s) {
+  const uint32_t u = blockIdx.x * kThreads + threadIdx.x;
+  if (u >= n) return;
+  double acc = 0.0;
+  for (uint32_t q = csr_ptr[u]; q < csr_ptr[u + 1]; ++q) acc += (double)w_raw[csr_to_csc[q]];
+  s[u] = (float)acc;
+}
+
+__global__ __launch_bounds__(kThreads) void normalise_kernel(uint64_t nnz, const uint32_t* __restrict__ col,
+                                                             const float* __restrict__ w_raw,
+                                                             const float* __restrict__ s, float* __restrict__ w) {
+  const uint64_t p = (uint64_t)blockIdx.x * kThreads + threadIdx.x;
+  if (p < nnz) w[p] = w_raw[p] / s[col[p]];
+}
+
+__global__ __launch_bounds__(kThreads) void gather_kernel(uint64_t nnz, const uint32_t* __restrict__ perm,
+                                                          const float* __restrict__ src, float* __restrict__ dst) {
+  const uint64_t q = (uint64_t)blockIdx.x * kThreads + threadIdx.x;
+  if (q < nnz) dst[q] = src[perm[q]];
+}
+
+// grad_w[p] = d * lambda[row] * x[col[p]] over the CSC, one thread per row
+__global__ __launch_bounds__(kThreads) void edge_grad_kernel(uint32_t n, const uint32_t* __restrict__ rp,
+                                                             const uint32_t* __restrict__ col, float d,
+                                                             const float* __restrict__ lambda,
+                                                             const float* __restrict__ x, float* __restrict__ gw) {
+  const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
+  if (i >= n) return;
+  const float dl = d * lambda[i];
+  for (uint32_t p = rp[i]; p < rp[i + 1]; ++p) gw[p] = dl * x[col[p]];
+}
+
+// t_u = sum_j g(u->j) w(u->j) over the CSR (double); gw~[p] = (g[p] - t_u) / s_u
+__global__ __launch_bounds__(kThreads) void chain_sum_kernel(uint32_t n, const uint32_t* __restrict__ csr_ptr,
+                                                             const uint32_t* __restrict__ csr_to_csc,
+                                                             const float* __restrict__ gw,
+                                                             const float* __restrict__ w, float* __restrict__ t) {
+  const uint32_t u = blockIdx.x * kThreads + threadIdx.x;
+  if (u >= n) return;
+  double acc = 0.0;
+  for (uint32_t q = csr_ptr[u]; q < csr_ptr[u + 1]; ++q) {
+    const uint32_t p = csr_to_csc[q];
+    acc += (double)gw[p] * (double)w[p];
+  }
+  t[u] = (float)acc;
+}
+
+__global__ __launch_bounds__(kThreads) void chain_apply_kernel(uint64_t nnz, const uint32_t* __restrict__ col,
+                                                               const float* __restrict__ t,
+                                                               const float* __restrict__ s, float* __restrict__ gw) {
+  const uint64_t p = (uint64_t)blockIdx.x * kThreads + threadIdx.x;
+  if (p < nnz) {
+    const uint32_t u = col[p];
+    gw[p] = (gw[p] - t[u]) / s[u];
+  }
+}
+
+__global__ __launch_bounds__(kThreads) void scale_kernel(uint32_t n, float a, const float* __restrict__ src,
+                                                         float* __restrict__ dst) {
+  const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
+  if (i < n) dst[i] = a * src[i];
+}
+
+inline uint32_t grid1(uint64_t m) { return (uint32_t)ceil_div(m ? m : 1, kThreads); }
+
+}  // namespace
+
+// Work items over a compressed structure (host-side, from its row pointers).
+void build_items(hipStream_t s, const uint32_t* rp_dev, uint32_t n, DevBuf<uint32_t>& r0, DevBuf<uint32_t>& r1,
+                 uint32_t* count) {
+  std::vector<uint32_t> rp((size_t)n + 1);
+  NWG_HIP(hipMemcpyAsync(rp.data(), rp_dev, ((size_t)n + 1) * 4, hipMemcpyDeviceToHost, s));
+  NWG_HIP(hipStreamSynchronize(s));
+  std::vector<uint32_t> a, b;
+  uint64_t r = 0;
+  while (r < n) {
+    uint64_t e = r + 1;
+    if (rp[e] - rp[r] <= kItemNnz)
+      while (e < n && e - r < kItemRows && rp[e + 1] - rp[r] <= kItemNnz) ++e;
+    a.push_back((uint32_t)r);
+    b.push_back((uint32_t)e);
+    r = e;
+  }
+  r0.alloc(std::max<size_t>(a.size(), 1));
+  r1.alloc(std::max<size_t>(b.size(), 1));
+  if (!a.empty()) {
+    NWG_HIP(hipMemcpyAsync(r0.p, a.data(), a.size() * 4, hipMemcpyHostToDevice, s));
+    NWG_HIP(hipMemcpyAsync(r1.p, b.data(), b.size() * 4, hipMemcpyHostToDevice, s));
+  }
+  NWG_HIP(hipStreamSynchronize(s));
+  *count = (uint32_t)a.size();
+}
+
+void weights_set(hipStream_t s, nwg_graph* g, const float* w_dev, bool normalize) {
+  const uint64_t nnz = g->nnz;
+  build_csr_device(s, g);
+  g->w_raw.alloc(nnz ? nnz : 1);
+  g->w_csc.alloc(nnz ? nnz : 1);
+  g->w_csr.alloc(nnz ? nnz : 1);
+  g->w_rowsum.alloc(g->n);
+  if (nnz) NWG_HIP(hipMemcpyAsync(g->w_raw.p, w_dev, nnz * 4, hipMemcpyDeviceToDevice, s));
+  if (normalize) {
+    rowsum_kernel<<<grid1(g->n), kThreads, 0, s>>>(g->n, g->csr_ptr.p, g->csr_to_csc.p, g->w_raw.p, g->w_rowsum.p);
+    NWG_LAUNCH_CHECK();
+    normalise_kernel<<<grid1(nnz), kThreads, 0, s>>>(nnz, g->csc_idx.p, g->w_raw.p, g->w_rowsum.p, g->w_csc.p);
+    NWG_LAUNCH_CHECK();
+  } else {
+    std::vector<float> ones(g->n, 1.0f);
+    NWG_HIP(hipMemcpyAsync(g->w_rowsum.p, ones.data(), (size_t)g->n * 4, hipMemcpyHostToDevice, s));
+    if (nnz) NWG_HIP(hipMemcpyAsync(g->w_csc.p, w_dev, nnz * 4, hipMemcpyDeviceToDevice, s));
+  }
+  gather_kernel<<<grid1(nnz), kThreads, 0, s>>>(nnz, g->csr_to_csc.p, g->w_csc.p, g->w_csr.p);
+  NWG_LAUNCH_CHECK();
+  NWG_HIP(hipStreamSynchronize(s));
+  g->has_weights = true;
+  g->w_normalized = normalize;
+}
+
+// Runs the fixed point y <- fma(d, sum w y, base) from y0 over (rp, col, w); returns the
+// sweeps executed and leaves the result in `out`.
+static uint64_t fixed_point(nwg_ctx* ctx, const uint32_t* rp, const uint32_t* col, const float* w,
+                            const DevBuf<uint32_t>& r0, const DevBuf<uint32_t>& r1, uint32_t n_items, uint32_t n,
+                            const float* base, const float* y0, bool y0_uniform, float damping, float threshold,
+                            uint64_t max_iters, float* out, nwg_pr_info* info) {
+  hipStream_t s = ctx->stream;
+  DevBuf<float> y[2] = {DevBuf<float>(n), DevBuf<float>(n)};
+  DevBuf<unsigned long long> acc(std::max<uint64_t>(max_iters, 1));
+  int dev = 0, cus = 0;
+  NWG_HIP(hipGetDevice(&dev));
+  NWG_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+  const uint32_t grid = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(n_items, (uint64_t)cus * 8));
+  NWG_HIP(hipEventRecord(ctx->ev0, s));
+  if (y0_uniform)
+    wfill_kernel<<<grid1(std::max<uint64_t>(n, max_iters)), kThreads, 0, s>>>(n, (float)(1.0 / n), y[0].p, acc.p,
+                                                                            max_iters);
+  else
+    wcopy_kernel<<<grid1(std::max<uint64_t>(n, max_iters)), kThreads, 0, s>>>(n, y0, y[0].p, acc.p, max_iters);
+  NWG_LAUNCH_CHECK();
+  WParams P{};
+  P.rp = rp;
+  P.col = col;
+  P.w = w;
+  P.item_r0 = r0.p;
+  P.item_r1 = r1.p;
+  P.n_items = n_items;
+  P.base = base;
+  P.damp = damping;
+  P.acc = acc.p;
+  P.threshold = (double)threshold;
+  for (uint64_t k = 0; k < max_iters; ++k) {
+    P.iter = (uint32_t)k;
+    P.xin = y[k & 1].p;
+    P.xout = y[(k + 1) & 1].p;
+    wsweep_kernel<<<grid, kThreads, 0, s>>>(P);
+    NWG_LAUNCH_CHECK();
+  }
+  NWG_HIP(hipEventRecord(ctx->ev1, s));
+  std::vector<unsigned long long> h(std::max<uint64_t>(max_iters, 1));
+  NWG_HIP(hipMemcpyAsync(h.data(), acc.p, max_iters * 8, hipMemcpyDeviceToHost, s));
+  NWG_HIP(hipStreamSynchronize(s));
+  uint64_t iters = max_iters;
+  double last = 0.0;
+  for (uint64_t k = 0; k < max_iters; ++k) {
+    last = fixed_to_err(h[k]);
+    if (last < (double)threshold) {
+      iters = k + 1;
+      break;
+    }
+  }
+  NWG_HIP(hipMemcpyAsync(out, y[iters & 1].p, (size_t)n * 4, hipMemcpyDeviceToDevice, s));
+  NWG_HIP(hipStreamSynchronize(s));
+  if (info) {
+    float ms = 0.f;
+    NWG_HIP(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+    info->iterations = iters;
+    info->final_error = max_iters ? last : 0.0;
+    info->ms_total = ms;
+    info->ms_per_iter = iters ? ms / (double)iters : 0.0;
+    info->sweep_ms_avg = 0.0;
+    if (info->error_trace)
+      for (uint64_t k = 0; k < iters && k < info->error_trace_len; ++k) info->error_trace[k] = fixed_to_err(h[k]);
+  }
+  return iters;
+}
+
+void wpr_forward(nwg_ctx* ctx, nwg_graph* g, const float* teleport_dev, float* x_dev, float damping, float threshold,
+                 uint64_t max_iters, nwg_pr_info* info) {
+  hipStream_t s = ctx->stream;
+  NWG_REQUIRE(g->has_weights, "graph has no weights (nwg_graph_set_weights)");
+  NWG_REQUIRE(max_iters < (1ull << 31), "max_iters too large");
+  if (!g->w_items_csc_n && g->n) build_items(s, g->csc_ptr.p, g->n, g->w_items_csc_r0, g->w_items_csc_r1, &g->w_items_csc_n);
+  DevBuf<float> base(g->n);
+  wbase_kernel<<<grid1(g->n), kThreads, 0, s>>>(g->n, teleport_dev, 1.0 - (double)damping, base.p);
+  NWG_LAUNCH_CHECK();
+  fixed_point(ctx, g->csc_ptr.p, g->csc_idx.p, g->w_csc.p, g->w_items_csc_r0, g->w_items_csc_r1, g->w_items_csc_n,
+              g->n, base.p, nullptr, true, damping, threshold, max_iters, x_dev, info);
+}
+
+void wpr_backward(nwg_ctx* ctx, nwg_graph* g, const float* x_dev, const float* gx_dev, float* gw_dev, float* gv_dev,
+                  float damping, float threshold, uint64_t max_iters, bool raw, nwg_pr_info* info) {
+  hipStream_t s = ctx->stream;
+  NWG_REQUIRE(g->has_weights, "graph has no weights (nwg_graph_set_weights)");
+  NWG_REQUIRE(max_iters < (1ull << 31), "max_iters too large");
+  NWG_REQUIRE(!raw || g->w_normalized, "raw-weight gradients need weights set with normalize=1");
+  if (!g->w_items_csr_n && g->n) build_items(s, g->csr_ptr.p, g->n, g->w_items_csr_r0, g->w_items_csr_r1, &g->w_items_csr_n);
+  const uint32_t n = g->n;
+  const uint64_t nnz = g->nnz;
+  DevBuf<float> lambda(n);
+  // adjoint fixed point: lambda = g + d A^T lambda, started from g
+  fixed_point(ctx, g->csr_ptr.p, g->csr_idx.p, g->w_csr.p, g->w_items_csr_r0, g->w_items_csr_r1, g->w_items_csr_n, n,
+              gx_dev, gx_dev, false, damping, threshold, max_iters, lambda.p, info);
+  if (gw_dev && nnz) {
+    edge_grad_kernel<<<grid1(n), kThreads, 0, s>>>(n, g->csc_ptr.p, g->csc_idx.p, damping, lambda.p, x_dev, gw_dev);
+    NWG_LAUNCH_CHECK();
+    if (raw) {
+      DevBuf<float> t(n);
+      chain_sum_kernel<<<grid1(n), kThreads, 0, s>>>(n, g->csr_ptr.p, g->csr_to_csc.p, gw_dev, g->w_csc.p, t.p);
+      NWG_LAUNCH_CHECK();
+      chain_apply_kernel<<<grid1(nnz), kThreads, 0, s>>>(nnz, g->csc_idx.p, t.p, g->w_rowsum.p, gw_dev);
+      NWG_LAUNCH_CHECK();
+      NWG_HIP(hipStreamSynchronize(s));
+    }
+  }
+  if (gv_dev) {
+    scale_kernel<<<grid1(n), kThreads, 0, s>>>(n, 1.0f - damping, lambda.p, gv_dev);
+    NWG_LAUNCH_CHECK();
+  }
+  NWG_HIP(hipStreamSynchronize(s));
+}
+
+}  // namespace nwg

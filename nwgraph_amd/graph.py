@@ -96,6 +96,9 @@ class Context:
         self.check(self.lib.nwg_generate_grid(self.handle, width, keep, long_range, seed, ctypes.byref(h)))
         return self._wrap(h)
 
+    def multigraph(self, graphs):
+        return MultiGraph(self, graphs)
+
     def grid_edges(self, width, keep=8500, long_range=50, seed=1):
         slots = int(self.lib.nwg_grid_slots(width))
         src = np.empty(slots, np.uint32)
@@ -207,6 +210,139 @@ class DeviceGraph:
                                                        float(damping), float(threshold), int(max_iters),
                                                        NWG_MEM_DEVICE, ctypes.byref(info)))
         return self._info_dict(info, trace)
+
+    # ---- batched personalised PageRank -----------------------------------------
+    @staticmethod
+    def _seed_csr(seeds):
+        seeds = [np.asarray(s, np.uint32).ravel() for s in seeds]
+        ptr = np.zeros(len(seeds) + 1, np.uint32)
+        ptr[1:] = np.cumsum([s.size for s in seeds])
+        flat = np.concatenate(seeds).astype(np.uint32) if seeds else np.zeros(0, np.uint32)
+        return len(seeds), ptr, np.ascontiguousarray(flat)
+
+    def ppr(self, seeds, damping=0.85, threshold=1e-4, max_iters=20):
+        """k personalised PageRank vectors; seeds[j] = seed ids of teleport vector j (uniform over them).
+        Returns (scores float32[n, k], iterations uint64[k], info dict)."""
+        k, ptr, flat = self._seed_csr(seeds)
+        scores = np.empty((self.n, k), np.float32)
+        its = np.zeros(k, np.uint64)
+        info, trace = self._info(max_iters)
+        self.ctx.check(self.ctx.lib.nwg_ppr_batched(self.ctx.handle, self.handle, k, _ptr(ptr), _ptr(flat),
+                                                    _ptr(scores), float(damping), float(threshold), int(max_iters),
+                                                    NWG_MEM_HOST, _ptr(its), ctypes.byref(info)))
+        return scores, its, self._info_dict(info, trace)
+
+    def ppr_device(self, seeds, scores_ptr, damping=0.85, threshold=1e-4, max_iters=20):
+        k, ptr, flat = self._seed_csr(seeds)
+        its = np.zeros(k, np.uint64)
+        info, trace = self._info(max_iters)
+        self.ctx.check(self.ctx.lib.nwg_ppr_batched(self.ctx.handle, self.handle, k, _ptr(ptr), _ptr(flat),
+                                                    ctypes.c_void_p(scores_ptr), float(damping), float(threshold),
+                                                    int(max_iters), NWG_MEM_DEVICE, _ptr(its), ctypes.byref(info)))
+        return its, self._info_dict(info, trace)
+
+    def ppr_autotune(self, seeds):
+        k, ptr, flat = self._seed_csr(seeds)
+        t = ctypes.c_uint32()
+        self.ctx.check(self.ctx.lib.nwg_ppr_autotune(self.ctx.handle, self.handle, k, _ptr(ptr), _ptr(flat),
+                                                     ctypes.byref(t)))
+        return t.value
+
+    # ---- weighted PageRank and adjoint ----------------------------------------
+    def set_weights(self, w_csc, normalize=True):
+        w = np.ascontiguousarray(w_csc, np.float32)
+        assert w.size == self.nnz
+        self.ctx.check(self.ctx.lib.nwg_graph_set_weights(self.ctx.handle, self.handle, _ptr(w), 1 if normalize else 0,
+                                                          NWG_MEM_HOST))
+
+    def set_weights_device(self, w_ptr, normalize=True):
+        self.ctx.check(self.ctx.lib.nwg_graph_set_weights(self.ctx.handle, self.handle, ctypes.c_void_p(w_ptr),
+                                                          1 if normalize else 0, NWG_MEM_DEVICE))
+
+    def random_weights(self, seed=1, normalize=True):
+        self.ctx.check(self.ctx.lib.nwg_graph_random_weights(self.ctx.handle, self.handle, int(seed),
+                                                             1 if normalize else 0))
+
+    def get_weights(self):
+        w = np.empty(self.nnz, np.float32)
+        self.ctx.check(self.ctx.lib.nwg_graph_get_weights(self.ctx.handle, self.handle, _ptr(w)))
+        return w
+
+    def wpr_forward(self, teleport=None, damping=0.85, threshold=1e-4, max_iters=100):
+        x = np.empty(self.n, np.float32)
+        tele = None if teleport is None else np.ascontiguousarray(teleport, np.float32)
+        info, trace = self._info(max_iters)
+        self.ctx.check(self.ctx.lib.nwg_wpr_forward(self.ctx.handle, self.handle, _ptr(tele), _ptr(x), float(damping),
+                                                    float(threshold), int(max_iters), NWG_MEM_HOST,
+                                                    ctypes.byref(info)))
+        return x, self._info_dict(info, trace)
+
+    def wpr_backward(self, x, grad_x, damping=0.85, threshold=1e-4, max_iters=100, raw_weights=True):
+        xa = np.ascontiguousarray(x, np.float32)
+        ga = np.ascontiguousarray(grad_x, np.float32)
+        gw = np.empty(self.nnz, np.float32)
+        gv = np.empty(self.n, np.float32)
+        info, trace = self._info(max_iters)
+        self.ctx.check(self.ctx.lib.nwg_wpr_backward(self.ctx.handle, self.handle, _ptr(xa), _ptr(ga), _ptr(gw),
+                                                     _ptr(gv), float(damping), float(threshold), int(max_iters),
+                                                     1 if raw_weights else 0, NWG_MEM_HOST, ctypes.byref(info)))
+        return gw, gv, self._info_dict(info, trace)
+
+    def wpr_forward_device(self, teleport_ptr, x_ptr, damping=0.85, threshold=1e-4, max_iters=100):
+        info, trace = self._info(max_iters)
+        self.ctx.check(self.ctx.lib.nwg_wpr_forward(self.ctx.handle, self.handle,
+                                                    ctypes.c_void_p(teleport_ptr) if teleport_ptr else None,
+                                                    ctypes.c_void_p(x_ptr), float(damping), float(threshold),
+                                                    int(max_iters), NWG_MEM_DEVICE, ctypes.byref(info)))
+        return self._info_dict(info, trace)
+
+    def wpr_backward_device(self, x_ptr, gx_ptr, gw_ptr, gv_ptr, damping=0.85, threshold=1e-4, max_iters=100,
+                            raw_weights=True):
+        info, trace = self._info(max_iters)
+        self.ctx.check(self.ctx.lib.nwg_wpr_backward(self.ctx.handle, self.handle, ctypes.c_void_p(x_ptr),
+                                                     ctypes.c_void_p(gx_ptr),
+                                                     ctypes.c_void_p(gw_ptr) if gw_ptr else None,
+                                                     ctypes.c_void_p(gv_ptr) if gv_ptr else None, float(damping),
+                                                     float(threshold), int(max_iters), 1 if raw_weights else 0,
+                                                     NWG_MEM_DEVICE, ctypes.byref(info)))
+        return self._info_dict(info, trace)
+
+
+class MultiGraph:
+    """Block-diagonal union of device graphs solved by one persistent launch per sweep."""
+
+    def __init__(self, ctx, graphs):
+        self.ctx = ctx
+        self.graphs = list(graphs)
+        arr = (ctypes.c_void_p * len(self.graphs))(*[g.handle.value for g in self.graphs])
+        h = ctypes.c_void_p()
+        ctx.check(ctx.lib.nwg_multigraph_create(ctx.handle, len(self.graphs), arr, ctypes.byref(h)))
+        self.handle = h
+        ng, nv, ne = ctypes.c_uint32(), ctypes.c_uint64(), ctypes.c_uint64()
+        ctx.check(ctx.lib.nwg_multigraph_info(h, ctypes.byref(ng), ctypes.byref(nv), ctypes.byref(ne)))
+        self.ngraphs, self.n, self.nnz = ng.value, nv.value, ne.value
+        self.offsets = np.concatenate([[0], np.cumsum([g.n for g in self.graphs])]).astype(np.int64)
+
+    def pagerank(self, damping=0.85, threshold=1e-4, max_iters=20):
+        scores = np.empty(self.n, np.float32)
+        its = np.zeros(self.ngraphs, np.uint64)
+        errs = np.zeros(self.ngraphs, np.float64)
+        info, trace = DeviceGraph._info(max_iters)
+        self.ctx.check(self.ctx.lib.nwg_multigraph_pagerank(self.ctx.handle, self.handle, _ptr(scores),
+                                                            float(damping), float(threshold), int(max_iters),
+                                                            NWG_MEM_HOST, _ptr(its), _ptr(errs), ctypes.byref(info)))
+        return scores, its, errs, DeviceGraph._info_dict(info, trace)
+
+    def close(self):
+        if getattr(self, "handle", None):
+            self.ctx.lib.nwg_multigraph_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 _default_ctx = None

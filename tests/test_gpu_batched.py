@@ -1,0 +1,120 @@
+"""GPU parity of the batched personalised PageRank (SpMM) and of the many-graphs
+persistent solve, through the C ABI, against the oracle's per-vector / per-graph runs."""
+import numpy as np
+import pytest
+
+import pyoracle as O
+
+pytestmark = pytest.mark.gpu
+
+D = 0.85
+
+
+def teleport(n, ids):
+    v = np.zeros(n, np.float32)
+    v[ids] = np.float32(1.0 / len(ids))
+    return v
+
+
+@pytest.mark.parametrize("k", [1, 5, 16, 37, 64])
+def test_ppr_matches_oracle_per_vector(ctx, k):
+    g = ctx.rmat(12, 16, seed=8)
+    n, ind, col, deg = O.rmat_csc(12, 16, seed=8)
+    seeds = O.ppr_sources(n, k, 8, seed=k)
+    X, its, info = g.ppr(seeds, D, 1e-4, 30)
+    assert X.shape == (n, k)
+    for j in range(k):
+        ref, it, _, _ = O.pr_v10(n, ind, col, deg, D, 1e-4, 30, teleport=teleport(n, seeds[j]))
+        assert its[j] == it, (j, its[j], it)
+        assert np.max(np.abs(X[:, j].astype(np.float64) - ref)) <= 1e-6
+
+
+@pytest.mark.parametrize("tile", [8, 16, 32, 64])
+def test_ppr_tiles_agree(ctx, tile):
+    g = ctx.rmat(11, 16, seed=3)
+    plan = g.get_plan()
+    plan["ppr_tile"] = tile
+    g.set_plan(**plan)
+    n, ind, col, deg = O.rmat_csc(11, 16, seed=3)
+    seeds = O.ppr_sources(n, 24, 8, seed=1)
+    X, its, _ = g.ppr(seeds, D, 1e-5, 40)
+    for j in (0, 7, 23):
+        ref, it, _, _ = O.pr_v10(n, ind, col, deg, D, 1e-5, 40, teleport=teleport(n, seeds[j]))
+        assert its[j] == it and np.max(np.abs(X[:, j] - ref)) <= 1e-6
+
+
+def test_ppr_with_hub_rows_and_single_solve(ctx):
+    # a small hub threshold routes many rows through the segment path
+    g = ctx.rmat(13, 16, seed=5)
+    g.set_plan(short_max=8, group_max=128, hub_min=256, hub_segment=256)
+    n, ind, col, deg = O.rmat_csc(13, 16, seed=5)
+    seeds = O.ppr_sources(n, 8, 8, seed=2)
+    X, its, _ = g.ppr(seeds, D, 1e-4, 20)
+    for j in range(8):
+        v = teleport(n, seeds[j])
+        single, info = g.pagerank(D, 1e-4, 20, teleport=v)
+        assert info["iterations"] == its[j]
+        assert np.max(np.abs(X[:, j] - single)) <= 1e-6
+
+
+def test_ppr_autotune_picks_a_tile(ctx):
+    g = ctx.rmat(12, 16, seed=2)
+    seeds = O.ppr_sources(g.n, 64, 8, seed=3)
+    t = g.ppr_autotune(seeds)
+    assert t in (8, 16, 32, 64) and g.get_plan()["ppr_tile"] == t
+    X, its, _ = g.ppr(seeds, D, 1e-4, 20)
+    n, ind, col, deg = O.rmat_csc(12, 16, seed=2)
+    ref, it, _, _ = O.pr_v10(n, ind, col, deg, D, 1e-4, 20, teleport=teleport(n, seeds[11]))
+    assert its[11] == it and np.max(np.abs(X[:, 11] - ref)) <= 1e-6
+
+
+def test_ppr_rejects_bad_seeds(ctx):
+    from nwgraph_amd import NwgError
+
+    g = ctx.rmat(8, 4, seed=1)
+    with pytest.raises(NwgError):
+        g.ppr([np.array([1, 1], np.uint32)])  # duplicate seed
+    with pytest.raises(NwgError):
+        g.ppr([np.array([g.n + 3], np.uint32)])  # out of range
+
+
+def test_multigraph_matches_per_graph_oracle(ctx):
+    rng = np.random.default_rng(11)
+    scales = rng.integers(4, 13, 24)
+    graphs, refs = [], []
+    for q, sc in enumerate(scales):
+        graphs.append(ctx.rmat(int(sc), 16, seed=100 + q))
+        refs.append(O.rmat_csc(int(sc), 16, seed=100 + q))
+    mg = ctx.multigraph(graphs)
+    assert mg.ngraphs == len(graphs) and mg.n == sum(g.n for g in graphs)
+    X, its, errs, info = mg.pagerank(D, 1e-4, 50)
+    converged = 0
+    for q, (n, ind, col, deg) in enumerate(refs):
+        ref, it, tr, _ = O.pr_v10(n, ind, col, deg, D, 1e-4, 50)
+        xs = X[mg.offsets[q]:mg.offsets[q + 1]]
+        assert its[q] == it, (q, its[q], it)
+        assert np.max(np.abs(xs.astype(np.float64) - ref)) <= 1e-6
+        assert abs(errs[q] - tr[-1]) <= 1e-3 * tr[-1] + 1e-12
+        converged += it < 50
+    assert converged > 0  # per-graph stopping is exercised
+
+
+def test_multigraph_with_hub_items_and_repeat(ctx):
+    # a star whose centre has 5999 in-edges (more than one LDS work item holds) plus a
+    # ring, next to an RMAT graph and a grid
+    n0 = 6000
+    s = np.concatenate([np.arange(1, n0), np.arange(n0)]).astype(np.uint32)
+    d = np.concatenate([np.zeros(n0 - 1), (np.arange(n0) + 1) % n0]).astype(np.uint32)
+    star = ctx.graph_from_edges(n0, s, d)
+    cs, cd = O.clean_edges(s, d)
+    ind0, col0 = O.csc(n0, cs, cd)
+    graphs = [star, ctx.rmat(9, 8, seed=2), ctx.grid(64, seed=3)]
+    refs = [(n0, ind0, col0, O.out_degrees(n0, ind0, col0)), O.rmat_csc(9, 8, seed=2), O.grid_csc(64, seed=3)]
+    assert int(np.diff(ind0.astype(np.int64)).max()) > 4096
+    mg = ctx.multigraph(graphs)
+    X1, its1, _, _ = mg.pagerank(D, 1e-4, 100)
+    X2, its2, _, _ = mg.pagerank(D, 1e-4, 100)
+    assert np.array_equal(X1, X2) and np.array_equal(its1, its2)
+    for q, (n, ind, col, deg) in enumerate(refs):
+        ref, it, _, _ = O.pr_v10(n, ind, col, deg, D, 1e-4, 100)
+        assert its1[q] == it and np.max(np.abs(X1[mg.offsets[q]:mg.offsets[q + 1]] - ref)) <= 1e-6

@@ -194,7 +194,7 @@ def test_edge_cases(ctx):
     n, ind, col, deg = O.rmat_csc(10, 8, seed=1)
     x3, info3 = g2.pagerank(D, 10.0, 20)
     ref, it, _, _ = O.pr_v10(n, ind, col, deg, D, 10.0, 20)
-    assert info3["iterations"] == it == 1 and np.max(np.abs(x3 - ref)) < 1e-9
+    assert info3["iterations"] == it == 1 and np.max(np.abs(x3 - ref) / ref) < 1e-6
     # dangling vertices lose mass: sum of scores stays below 1 (no redistribution)
     assert x3.sum() < 1.0
 
