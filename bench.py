@@ -1,15 +1,19 @@
 #!/usr/bin/env python3
 """Benchmark of the PageRank hot path on one MI355X.
 
-Workload: plain PageRank on a directed RMAT graph, scale 25 (33,554,432 vertices),
-edge factor 16, A/B/C/D = .57/.19/.19/.05, self-loops and duplicates removed, int32
-CSC, float32 scores, damping 0.85, L1 tolerance 1e-4, 20 sweeps.  The graph is
-generated and built on the device (not timed).  One step = one page_rank solve
-(init + up to 20 sweeps, one captured HIP graph) with the CSC already resident in HBM.
+Default workload (the headline): plain PageRank on a directed RMAT graph, scale 25
+(33,554,432 vertices), edge factor 16, A/B/C/D = .57/.19/.19/.05, self-loops and
+duplicates removed, int32 CSC, float32 scores, damping 0.85, L1 tolerance 1e-4, 20
+sweeps.  The graph is generated and built on the device (not timed).  One step = one
+page_rank solve (init + up to 20 sweeps, one captured HIP graph) with the CSC resident in
+HBM.  Output: one JSON line with GTEPS, the roofline of the sweep and the CPU baseline
+(the oracle's restatement of the reference's v11 loop on this host's cores, over a
+bounded sample of sweeps on the same graph).
 
-Output: one JSON line (rank 0) with GTEPS, the roofline of the dominant kernel
-(the sweep) and the CPU baseline (the oracle's restatement of the reference's v11
-loop on this host's cores, over a bounded sample of sweeps on the same graph).
+Other workloads (--workload): grid (4096^2 lattice, max 100 sweeps), ppr (64 personalised
+vectors on RMAT-22, batched vs 64 sequential solves), multi (256 RMAT graphs of scale
+10..18 in one persistent launch per sweep), wpr (weighted PageRank forward + implicit
+adjoint on RMAT-20).
 """
 import argparse
 import json
@@ -36,15 +40,19 @@ def bytes_per_iter(n, nnz):
     return 4 * nnz + 4 * (n + 1) + 20 * n
 
 
-def cpu_baseline(ptr, idx, deg, sweeps):
-    sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    import pyoracle as O
-
+def host_threads():
     try:
         ncpu = len(os.sched_getaffinity(0))
     except AttributeError:
         ncpu = os.cpu_count() or 1
-    threads = max(1, min(ncpu, 16))
+    return max(1, min(ncpu, 16))
+
+
+def cpu_baseline(ptr, idx, deg, sweeps):
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle as O
+
+    threads = host_threads()
     n = ptr.size - 1
     _, it, _, sec = O.pr_v11_parallel(n, ptr, idx, deg, 0.85, 0.0, sweeps, threads)
     t = float(np.median(sec))
@@ -54,7 +62,7 @@ def cpu_baseline(ptr, idx, deg, sweeps):
         "cores": threads,
         "kind": "port",
         "sample": f"{it} sweeps of the v11 in-place loop (page_rank.hpp:124-139 restated in C, OpenMP static "
-                  f"chunks, -O3 -march=x86-64-v3) on the same RMAT CSC copied to the host; median sweep "
+                  f"chunks, -O3 -march=x86-64-v3) on the same CSC copied to the host; median sweep "
                   f"{t * 1e3:.1f} ms",
     }
 
@@ -85,76 +93,11 @@ def validate(g, dev_buf, ptr, idx, deg, d=0.85):
             "ok": bool(worst < 1e-5 and abs(lhs - rhs) / rhs < 1e-5)}
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
-    ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--scale", type=int, default=25)
-    ap.add_argument("--edge-factor", type=int, default=16)
-    ap.add_argument("--iters", type=int, default=20)
-    ap.add_argument("--tol", type=float, default=1e-4)
-    ap.add_argument("--seed", type=int, default=1)
-    ap.add_argument("--cpu-sweeps", type=int, default=3)
-    ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--no-validate", action="store_true")
-    ap.add_argument("--autotune", action="store_true")
-    ap.add_argument("--plan", default="", help="k=v,... overrides of the sweep plan")
-    args = ap.parse_args()
-
-    world = int(os.environ.get("WORLD_SIZE", "1"))
-    if args.gpus != 1 or world != 1:
-        print(json.dumps({"error": "this build runs on exactly one GPU; multi-GPU sharding is designed "
-                                   "(see the sharding section of the design notes) but not implemented",
-                          "n_gpus": args.gpus}))
-        sys.exit(2)
-
-    import torch
-
-    import nwgraph_amd
-
-    ctx = nwgraph_amd.Context(0)
-    t0 = time.perf_counter()
-    g = ctx.rmat(args.scale, args.edge_factor, seed=args.seed)
-    build_s = time.perf_counter() - t0
-    n, nnz = g.n, g.nnz
-    log(f"RMAT-{args.scale} ef{args.edge_factor}: n={n} nnz={nnz} generated+built on device in {build_s:.2f} s")
-    if args.plan:
-        g.set_plan(**{k: int(v) for k, v in (kv.split("=") for kv in args.plan.split(","))})
-    if args.autotune:
-        t0 = time.perf_counter()
-        chosen = g.autotune()
-        log(f"autotune chose {chosen} in {time.perf_counter() - t0:.1f} s")
-    plan = g.get_plan()
-
-    buf = torch.empty(n, dtype=torch.float32, device="cuda")
-    torch.cuda.synchronize()
-    for _ in range(args.warmup):
-        g.pagerank_device(buf.data_ptr(), None, 0.85, args.tol, args.iters)
-    log(f"warmup {args.warmup} done")
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    infos = [g.pagerank_device(buf.data_ptr(), None, 0.85, args.tol, args.iters) for _ in range(args.steps)]
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    ms_per_step = wall * 1e3 / args.steps
-    iters = infos[-1]["iterations"]
-    ev_ms = float(np.mean([i["ms_total"] for i in infos]))
-    log(f"timed {args.steps} steps: {ms_per_step:.3f} ms/step wall, {ev_ms:.3f} ms/step events, {iters} sweeps")
-
-    # live per-kernel timing of the sweep (HIP events on the library's stream)
-    kinfo = [g.pagerank_device(buf.data_ptr(), None, 0.85, args.tol, args.iters, kernel_timing=True)
-             for _ in range(3)]
-    sweep_ms = float(np.median([k["sweep_ms_avg"] for k in kinfo]))
-    b_iter = bytes_per_iter(n, nnz)
-    achieved = b_iter / (sweep_ms * 1e-3) / 1e9
-    log(f"sweep kernel {sweep_ms:.4f} ms -> {achieved:.1f} GB/s algorithmic ({achieved / HBM_PEAK_GBS:.3f} of peak)")
-
-    value = nnz * iters / (ms_per_step * 1e-3) / 1e9
-    out = {
+def base_line(args, value, unit, ms_per_step, dtype, config):
+    return {
         "metric": METRIC,
-        "value": round(value, 3),
-        "unit": "GTEPS",
+        "value": value,
+        "unit": unit,
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
@@ -162,31 +105,88 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32",
+        "dtype": dtype,
         "data": "synthetic",
-        "config": {
-            "workload": f"plain PageRank, directed RMAT scale {args.scale}, edge factor {args.edge_factor}, "
-                        f"A/B/C/D .57/.19/.19/.05, cleaned, int32 CSC, d=0.85, L1 tol {args.tol}, "
-                        f"max {args.iters} sweeps",
-            "n": n,
-            "nnz": nnz,
-            "sweeps_per_step": iters,
-            "ms_per_sweep_events": round(ev_ms / max(iters, 1), 5),
-            "build_s": round(build_s, 3),
-            "plan": plan,
-            "parallelism": "single GPU",
-        },
-        "roofline": {
-            "bound": "hbm",
-            "achieved": round(achieved, 2),
-            "peak": HBM_PEAK_GBS,
-            "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": None,
-            "kernel": "pr_sweep_kernel",
-            "kernel_ms": round(sweep_ms, 5),
-            "algorithmic_bytes": b_iter,
-        },
+        "config": config,
+    }
+
+
+def apply_plan(g, args):
+    if args.plan:
+        plan = g.get_plan()
+        plan.update({k: int(v) for k, v in (kv.split("=") for kv in args.plan.split(","))})
+        g.set_plan(**plan)
+    if args.autotune:
+        t0 = time.perf_counter()
+        chosen = g.autotune()
+        log(f"autotune chose {chosen} in {time.perf_counter() - t0:.1f} s")
+    return g.get_plan()
+
+
+# ---------------------------------------------------------------------------- plain / grid
+def run_plain(args, ctx, grid=False):
+    import torch
+
+    t0 = time.perf_counter()
+    if grid:
+        g = ctx.grid(args.width, 8500, 50, seed=args.seed)
+        iters = args.iters if args.iters_given else 100
+        workload = (f"plain PageRank, {args.width}x{args.width} 4-neighbour grid, each undirected edge kept with "
+                    f"p=0.85 and stored both ways, +0.5% random long-range edges, cleaned, d=0.85, L1 tol "
+                    f"{args.tol}, max {iters} sweeps")
+    else:
+        g = ctx.rmat(args.scale, args.edge_factor, seed=args.seed)
+        iters = args.iters
+        workload = (f"plain PageRank, directed RMAT scale {args.scale}, edge factor {args.edge_factor}, "
+                    f"A/B/C/D .57/.19/.19/.05, cleaned, int32 CSC, d=0.85, L1 tol {args.tol}, max {iters} sweeps")
+    build_s = time.perf_counter() - t0
+    n, nnz = g.n, g.nnz
+    log(f"n={n} nnz={nnz} generated+built on device in {build_s:.2f} s")
+    plan = apply_plan(g, args)
+
+    buf = torch.empty(n, dtype=torch.float32, device="cuda")
+    torch.cuda.synchronize()
+    for _ in range(args.warmup):
+        g.pagerank_device(buf.data_ptr(), None, 0.85, args.tol, iters)
+    log(f"warmup {args.warmup} done")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    infos = [g.pagerank_device(buf.data_ptr(), None, 0.85, args.tol, iters) for _ in range(args.steps)]
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ms_per_step = wall * 1e3 / args.steps
+    sweeps = infos[-1]["iterations"]
+    ev_ms = float(np.mean([i["ms_total"] for i in infos]))
+    log(f"timed {args.steps} steps: {ms_per_step:.3f} ms/step wall, {ev_ms:.3f} ms/step events, {sweeps} sweeps")
+
+    # live per-sweep kernel timing (HIP events on the library's stream around each sweep's launches)
+    kinfo = [g.pagerank_device(buf.data_ptr(), None, 0.85, args.tol, iters, kernel_timing=True) for _ in range(3)]
+    sweep_ms = float(np.median([k["sweep_ms_avg"] for k in kinfo]))
+    b_iter = bytes_per_iter(n, nnz)
+    achieved = b_iter / (sweep_ms * 1e-3) / 1e9
+    log(f"sweep {sweep_ms:.4f} ms -> {achieved:.1f} GB/s algorithmic ({achieved / HBM_PEAK_GBS:.3f} of peak)")
+
+    value = nnz * sweeps / (ms_per_step * 1e-3) / 1e9
+    out = base_line(args, round(value, 3), "GTEPS", ms_per_step, "f32", {
+        "workload": workload,
+        "n": n,
+        "nnz": nnz,
+        "sweeps_per_step": sweeps,
+        "ms_per_sweep_events": round(ev_ms / max(sweeps, 1), 5),
+        "build_s": round(build_s, 3),
+        "plan": plan,
+        "parallelism": "single GPU",
+    })
+    out["roofline"] = {
+        "bound": "hbm",
+        "achieved": round(achieved, 2),
+        "peak": HBM_PEAK_GBS,
+        "unit": "GB/s",
+        "frac": round(achieved / HBM_PEAK_GBS, 4),
+        "traffic": args.traffic,
+        "kernel": "one sweep (pr_tile_kernel + pr_group_kernel<4..64> + pr_hubseg_kernel + pr_finish_kernel)",
+        "kernel_ms": round(sweep_ms, 5),
+        "algorithmic_bytes": b_iter,
     }
     if not args.no_cpu or not args.no_validate:
         ptr, idx, deg = g.csc()
@@ -197,8 +197,181 @@ def main():
         t0 = time.perf_counter()
         out["cpu_baseline"] = cpu_baseline(ptr, idx, deg, args.cpu_sweeps)
         log(f"cpu baseline {out['cpu_baseline']['value']} GTEPS in {time.perf_counter() - t0:.1f} s")
-    print(json.dumps(out), flush=True)
     g.close()
+    return out
+
+
+# ---------------------------------------------------------------------------- batched PPR
+def run_ppr(args, ctx):
+    import torch
+
+    import nwgraph_amd
+
+    g = ctx.rmat(args.ppr_scale, args.edge_factor, seed=args.seed)
+    n, nnz, k = g.n, g.nnz, args.k
+    plan = apply_plan(g, args)
+    seeds = nwgraph_amd.ppr_seeds(n, k, 8, args.seed)
+    tile = g.ppr_autotune(seeds)
+    log(f"RMAT-{args.ppr_scale}: n={n} nnz={nnz}; k={k}, autotuned k-tile {tile}")
+    X = torch.empty((n, k), dtype=torch.float32, device="cuda")
+    torch.cuda.synchronize()
+    for _ in range(args.warmup):
+        g.ppr_device(seeds, X.data_ptr(), 0.85, args.tol, args.iters)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        its, info = g.ppr_device(seeds, X.data_ptr(), 0.85, args.tol, args.iters)
+    torch.cuda.synchronize()
+    ms_batched = (time.perf_counter() - t0) * 1e3 / args.steps
+    # 64 sequential personalised solves with the plain kernel
+    x1 = torch.empty(n, dtype=torch.float32, device="cuda")
+    teles = []
+    for j in range(k):
+        v = torch.zeros(n, dtype=torch.float32, device="cuda")
+        v[torch.as_tensor(seeds[j].astype(np.int64), device="cuda")] = 1.0 / seeds.shape[1]
+        teles.append(v)
+    torch.cuda.synchronize()
+    for j in range(min(k, 2)):
+        g.pagerank_device(x1.data_ptr(), teles[j].data_ptr(), 0.85, args.tol, args.iters)
+    t0 = time.perf_counter()
+    seq_its = []
+    for j in range(k):
+        seq_its.append(g.pagerank_device(x1.data_ptr(), teles[j].data_ptr(), 0.85, args.tol, args.iters)["iterations"])
+    torch.cuda.synchronize()
+    ms_seq = (time.perf_counter() - t0) * 1e3
+    vec_sweeps = int(np.sum(its))
+    value = nnz * vec_sweeps / (ms_batched * 1e-3) / 1e9
+    out = base_line(args, round(value, 3), "GTEPS (vector-sweeps x edges / s)", ms_batched, "f32", {
+        "workload": f"batched personalised PageRank, k={k} teleports uniform over 8 seeded sources each, RMAT scale "
+                    f"{args.ppr_scale} ef {args.edge_factor}, d=0.85, tol {args.tol}, max {args.iters} sweeps",
+        "n": n, "nnz": nnz, "k": k, "k_tile": tile, "plan": plan,
+        "vector_sweeps": vec_sweeps, "sequential_vector_sweeps": int(np.sum(seq_its)),
+        "ms_batched": round(ms_batched, 3), "ms_64_sequential": round(ms_seq, 3),
+        "speedup_vs_sequential": round(ms_seq / ms_batched, 3),
+    })
+    g.close()
+    return out
+
+
+# ---------------------------------------------------------------------------- many graphs
+def run_multi(args, ctx):
+    rng = np.random.default_rng(args.seed)
+    scales = rng.integers(10, 19, args.ngraphs)
+    graphs = [ctx.rmat(int(s), args.edge_factor, seed=1000 + q) for q, s in enumerate(scales)]
+    mg = ctx.multigraph(graphs)
+    log(f"{mg.ngraphs} graphs, scales 10..18: n={mg.n} nnz={mg.nnz}")
+    for _ in range(args.warmup):
+        mg.pagerank(0.85, args.tol, args.iters)
+    t0 = time.perf_counter()
+    ev = []
+    for _ in range(args.steps):
+        _, its, _, info = mg.pagerank(0.85, args.tol, args.iters)
+        ev.append(info["ms_total"])
+    ms = (time.perf_counter() - t0) * 1e3 / args.steps
+    edge_sweeps = int(sum(int(g.nnz) * int(i) for g, i in zip(graphs, its)))
+    value = edge_sweeps / (np.mean(ev) * 1e-3) / 1e9
+    out = base_line(args, round(value, 3), "GTEPS (sum over graphs, device time)", ms, "f32", {
+        "workload": f"{mg.ngraphs} independent RMAT graphs, scales U{{10..18}} (seed {args.seed}), ef "
+                    f"{args.edge_factor}, one persistent launch per sweep in one HIP graph, per-graph tol "
+                    f"{args.tol}, max {args.iters}",
+        "n": mg.n, "nnz": mg.nnz, "ms_device": round(float(np.mean(ev)), 3),
+        "iterations_min_max": [int(its.min()), int(its.max())],
+    })
+    mg.close()
+    for g in graphs:
+        g.close()
+    return out
+
+
+# ---------------------------------------------------------------------------- weighted
+def run_wpr(args, ctx):
+    import torch
+
+    g = ctx.rmat(args.wpr_scale, args.edge_factor, seed=args.seed)
+    n, nnz = g.n, g.nnz
+    g.random_weights(seed=args.seed, normalize=True)
+    v = torch.full((n,), 1.0 / n, dtype=torch.float32, device="cuda")
+    x = torch.empty(n, dtype=torch.float32, device="cuda")
+    c = torch.randn(n, generator=torch.Generator().manual_seed(args.seed)).float().cuda()
+    gw = torch.empty(nnz, dtype=torch.float32, device="cuda")
+    gv = torch.empty(n, dtype=torch.float32, device="cuda")
+    torch.cuda.synchronize()
+    for _ in range(args.warmup):
+        g.wpr_forward_device(v.data_ptr(), x.data_ptr(), 0.85, args.tol, args.iters)
+        g.wpr_backward_device(x.data_ptr(), c.data_ptr(), gw.data_ptr(), gv.data_ptr(), 0.85, args.tol, args.iters)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        fi = g.wpr_forward_device(v.data_ptr(), x.data_ptr(), 0.85, args.tol, args.iters)
+    torch.cuda.synchronize()
+    ms_f = (time.perf_counter() - t0) * 1e3 / args.steps
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        bi = g.wpr_backward_device(x.data_ptr(), c.data_ptr(), gw.data_ptr(), gv.data_ptr(), 0.85, args.tol,
+                                   args.iters)
+    torch.cuda.synchronize()
+    ms_b = (time.perf_counter() - t0) * 1e3 / args.steps
+    value = nnz * fi["iterations"] / (ms_f * 1e-3) / 1e9
+    out = base_line(args, round(value, 3), "GTEPS (forward)", ms_f + ms_b, "f32", {
+        "workload": f"weighted PageRank + implicit adjoint, RMAT scale {args.wpr_scale} ef {args.edge_factor}, "
+                    f"w ~ U(0.1,1) row-normalised, L = sum c_i x_i, c ~ N(0,1), d=0.85, tol {args.tol}, "
+                    f"max {args.iters}",
+        "n": n, "nnz": nnz, "forward_ms": round(ms_f, 3), "backward_ms": round(ms_b, 3),
+        "backward_over_forward": round(ms_b / ms_f, 3),
+        "forward_sweeps": fi["iterations"], "adjoint_sweeps": bi["iterations"],
+    })
+    g.close()
+    return out
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", default="plain", choices=["plain", "grid", "ppr", "multi", "wpr"])
+    ap.add_argument("--scale", type=int, default=25)
+    ap.add_argument("--edge-factor", type=int, default=16)
+    ap.add_argument("--width", type=int, default=4096)
+    ap.add_argument("--ppr-scale", type=int, default=22)
+    ap.add_argument("--k", type=int, default=64)
+    ap.add_argument("--ngraphs", type=int, default=256)
+    ap.add_argument("--wpr-scale", type=int, default=20)
+    ap.add_argument("--iters", type=int, default=None)
+    ap.add_argument("--tol", type=float, default=1e-4)
+    ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--cpu-sweeps", type=int, default=3)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-validate", action="store_true")
+    ap.add_argument("--autotune", action="store_true")
+    ap.add_argument("--plan", default="", help="k=v,... overrides of the sweep plan")
+    ap.add_argument("--traffic", type=float, default=None, help="measured HBM bytes per sweep (rocprofv3 PMC)")
+    args = ap.parse_args()
+    args.iters_given = args.iters is not None
+    if args.iters is None:
+        args.iters = 20
+
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print(json.dumps({"error": "this build runs on exactly one GPU; multi-GPU sharding is designed "
+                                   "(see the sharding section of the design notes) but not implemented",
+                          "n_gpus": args.gpus}))
+        sys.exit(2)
+
+    import torch  # noqa: F401  (device buffers; loaded before the library: one HIP runtime)
+
+    import nwgraph_amd
+
+    ctx = nwgraph_amd.Context(0)
+    if args.workload == "plain":
+        out = run_plain(args, ctx)
+    elif args.workload == "grid":
+        out = run_plain(args, ctx, grid=True)
+    elif args.workload == "ppr":
+        out = run_ppr(args, ctx)
+    elif args.workload == "multi":
+        out = run_multi(args, ctx)
+    else:
+        out = run_wpr(args, ctx)
+    print(json.dumps(out), flush=True)
     ctx.close()
 
 

@@ -68,7 +68,7 @@ typedef struct nwg_plan_params {
   uint32_t lds_hub_prefix; /* contributions of vertices [0, lds_hub_prefix) staged in LDS (0 = off) */
   uint32_t stream_hints; /* 1: non-temporal loads/stores for the streamed arrays (ids, pointers, scores) */
   uint32_t ppr_tile;     /* k-tile of the batched solve: 8, 16, 32 or 64 vectors per lane group (0 = widest <= k) */
-  uint32_t reserved;
+  uint32_t relabel;      /* 1: sweep a copy of the CSC renumbered by out-degree (results returned in original ids) */
 } nwg_plan_params;
 
 /* ---------------------------------------------------------------- context */
@@ -141,6 +141,10 @@ int nwg_generate_grid_edges(nwg_ctx* ctx, uint32_t width, uint32_t keep_per10000
                             int mem_kind);
 /* number of raw slots nwg_generate_grid_edges writes */
 uint64_t nwg_grid_slots(uint32_t width);
+/* Teleport seed sets for personalised PageRank: k vectors of s distinct ids in [0, n),
+ * vector j draws mulhi(Philox({j, t, 2, 'PPRS'}, seed)[0], n) for t = 0, 1, ... keeping
+ * the first s distinct ids.  ids: [host] k*s, row-major. */
+int nwg_generate_ppr_seeds(uint32_t n, uint32_t k, uint32_t s, uint64_t seed, uint32_t* ids);
 
 /* ------------------------------------------------------ plain PageRank solve */
 /* PageRank on a resident graph.  scores: n floats ([host] or [device] per mem_kind).

@@ -1,0 +1,180 @@
+// nwgraph_hip/page_rank.hpp -- C++ drop-in for nw::graph::page_rank on MI355X.
+//
+// Replaces (pnnl/NWGraph) include/nwgraph/algorithms/page_rank.hpp:89-148:
+//
+//   template <adjacency_list_graph Graph, typename Real>
+//   void page_rank(const Graph& graph, const std::vector<vertex_id_type>& degrees,
+//                  std::vector<Real>& page_rank, Real damping_factor, Real threshold,
+//                  size_t max_iters, size_t num_threads);
+//
+// with the same parameters, the same in-place output, the same per-sweep trace on
+// stdout (pagerank::trace, page_rank.hpp:42-52) and the same init timers
+// (life_timer, util/timer.hpp:120-133).  The work runs in libnwgraph_hip.so through the
+// C ABI of <nwgraph_hip.h>: the CSC is uploaded, the whole sweep loop runs on the GPU as
+// one captured HIP graph (Jacobi schedule of page_rank_v10), and the scores are copied
+// back.  Errors become exceptions: std::invalid_argument, std::bad_alloc or
+// nw::graph::hip::error.
+//
+// Accepted graphs: anything laid out like adjacency<1> -- `indices_` (N+1 row offsets)
+// and `std::get<0>(to_be_indexed_)` (source id of every in-edge), both 32-bit -- such as
+// the reference's adjacency<1> and nw::graph::hip::csc_graph below.
+//
+// Including this header after the reference's <nwgraph/adjacency.hpp> adds a
+// non-template overload nw::graph::page_rank(const adjacency<1>&, ...) for float scores,
+// which overload resolution prefers to the reference's template, so the reference's
+// drivers (bench/gapbs/pr.cpp:137-139) bind to the GPU path unchanged.
+#pragma once
+
+#include <nwgraph_hip.h>
+
+#include <chrono>
+#include <cstdint>
+#include <iomanip>
+#include <iostream>
+#include <new>
+#include <stdexcept>
+#include <string>
+#include <tuple>
+#include <type_traits>
+#include <vector>
+
+namespace nw {
+namespace graph {
+namespace hip {
+
+class error : public std::runtime_error {
+public:
+  error(int code, const std::string& msg) : std::runtime_error(msg), code_(code) {}
+  int code() const noexcept { return code_; }
+
+private:
+  int code_;
+};
+
+inline void check(nwg_ctx* ctx, int rc, const char* what) {
+  if (rc == NWG_OK) return;
+  std::string msg = std::string(what) + ": " + (ctx ? nwg_last_error(ctx) : "no context");
+  if (rc == NWG_ENOMEM) throw std::bad_alloc();
+  if (rc == NWG_EINVAL) throw std::invalid_argument(msg);
+  throw error(rc, msg);
+}
+
+// One device context (stream + reusable device buffers).  Not shareable between
+// threads; thread_default() gives each host thread its own on device 0.
+class context {
+public:
+  explicit context(int device = 0) {
+    nwg_ctx* c = nullptr;
+    check(nullptr, nwg_create(&c, device), "nwg_create");
+    ctx_ = c;
+  }
+  ~context() { nwg_destroy(ctx_); }
+  context(const context&) = delete;
+  context& operator=(const context&) = delete;
+  nwg_ctx* get() const noexcept { return ctx_; }
+  static context& thread_default() {
+    thread_local context c(0);
+    return c;
+  }
+
+private:
+  nwg_ctx* ctx_ = nullptr;
+};
+
+// The CSC container with adjacency<1>'s layout (compressed.hpp:64-102).
+struct csc_graph {
+  using vertex_id_type = uint32_t;
+  std::vector<uint32_t> indices_;
+  std::tuple<std::vector<uint32_t>> to_be_indexed_;
+  size_t size() const { return indices_.empty() ? 0 : indices_.size() - 1; }
+  size_t num_edges() const { return std::get<0>(to_be_indexed_).size(); }
+};
+
+// Print the reference's per-sweep trace (page_rank.hpp:42-52) and init timers?
+inline bool& trace_enabled() {
+  static bool on = true;
+  return on;
+}
+
+namespace detail {
+inline void trace_row(const std::string& a, const std::string& b, const std::string& c, const std::string& d) {
+  std::cout << std::setw(5) << std::left << a << std::setw(20) << std::left << b << std::setw(20) << std::left << c
+            << std::setw(20) << std::left << d << "\n";
+}
+inline void trace_iter(uint64_t iter, double error, double seconds) {
+  auto p = std::cout.precision(10);
+  auto f = std::cout.setf(std::ios::fixed, std::ios::floatfield);
+  std::cout << std::setw(5) << std::left << iter << std::setw(20) << std::left << error << std::setw(20) << std::left
+            << seconds << std::setw(20) << std::left << 0 << "\n";
+  std::cout.precision(p);
+  std::cout.setf(f, std::ios::floatfield);
+}
+}  // namespace detail
+
+// Core entry on raw CSC arrays (host memory).  Returns the sweeps executed.
+inline uint64_t page_rank_csc(uint32_t n, uint64_t nnz, const uint32_t* indices, const uint32_t* sources,
+                              const uint32_t* degrees, float* scores, float damping_factor, float threshold,
+                              size_t max_iters, nwg_ctx* ctx = nullptr) {
+  if (!ctx) ctx = context::thread_default().get();
+  std::vector<double> trace(max_iters ? max_iters : 1);
+  nwg_pr_info info{};
+  info.error_trace = trace.data();
+  info.error_trace_len = trace.size();
+  const auto t0 = std::chrono::steady_clock::now();
+  check(ctx, nwg_pagerank_f32(ctx, n, nnz, indices, sources, degrees, scores, damping_factor, threshold, max_iters,
+                              &info),
+        "nwg_pagerank_f32");
+  if (trace_enabled()) {
+    const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    std::cout << "# [ init page rank ]: " << ms - info.ms_total << " ms" << std::endl;
+    detail::trace_row("iter", "error", "time", "outgoing");
+    const double per = info.iterations ? info.ms_per_iter * 1e-3 : 0.0;
+    for (uint64_t k = 0; k < info.iterations; ++k) detail::trace_iter(k, trace[k], per);
+  }
+  return info.iterations;
+}
+
+template <class Graph>
+concept csc_layout = requires(const Graph& g) {
+  { g.indices_.data() };
+  { std::get<0>(g.to_be_indexed_).data() };
+};
+
+// Same signature as the reference template (page_rank.hpp:89-91).  Real must be float
+// (the device path computes in FP32); ids and offsets must be 32-bit.
+template <csc_layout Graph, typename Real>
+void page_rank(const Graph& graph, const std::vector<typename Graph::vertex_id_type>& degrees,
+               std::vector<Real>& page_rank, Real damping_factor, Real threshold, size_t max_iters,
+               size_t /*num_threads: the reference ignores it too*/) {
+  static_assert(std::is_same_v<Real, float>, "the MI355X path computes float scores");
+  static_assert(sizeof(typename Graph::vertex_id_type) == 4, "vertex ids must be 32-bit");
+  static_assert(sizeof(graph.indices_[0]) == 4, "row offsets must be 32-bit");
+  const auto& src = std::get<0>(graph.to_be_indexed_);
+  const size_t n = graph.indices_.size() - 1;
+  if (page_rank.size() != n) throw std::invalid_argument("page_rank must be pre-sized to graph.size()");
+  if (!degrees.empty() && degrees.size() != n) throw std::invalid_argument("degrees must have graph.size() entries");
+  page_rank_csc((uint32_t)n, src.size(), reinterpret_cast<const uint32_t*>(graph.indices_.data()),
+                reinterpret_cast<const uint32_t*>(src.data()),
+                degrees.empty() ? nullptr : reinterpret_cast<const uint32_t*>(degrees.data()), page_rank.data(),
+                damping_factor, threshold, max_iters);
+}
+
+}  // namespace hip
+
+#ifdef NW_GRAPH_ADJACENCY_HPP
+// Exact-match overload for the reference's CSC with float scores: preferred over the
+// reference's template by overload resolution.
+inline void page_rank(const adjacency<1>& graph, const std::vector<adjacency<1>::vertex_id_type>& degrees,
+                      std::vector<float>& page_rank, float damping_factor, float threshold, size_t max_iters,
+                      size_t num_threads) {
+  hip::page_rank(graph, degrees, page_rank, damping_factor, threshold, max_iters, num_threads);
+}
+#endif
+
+#ifndef NW_GRAPH_PAGE_RANK_HPP
+// Without the reference's algorithm header, nw::graph::page_rank is the GPU path.
+using hip::page_rank;
+#endif
+
+}  // namespace graph
+}  // namespace nw

@@ -18,6 +18,7 @@ from .graph import (  # noqa: F401
     build_degrees,
     default_context,
     page_rank,
+    ppr_seeds,
     read_mm,
     read_nw,
     write_nw,

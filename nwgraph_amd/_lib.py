@@ -53,10 +53,12 @@ class PlanParams(ctypes.Structure):
         ("hub_segment", c_u32),
         ("lds_hub_prefix", c_u32),
         ("stream_hints", c_u32),
-        ("reserved", c_u32 * 2),
+        ("ppr_tile", c_u32),
+        ("relabel", c_u32),
     ]
 
-    FIELDS = ("short_max", "group_max", "hub_min", "hub_segment", "lds_hub_prefix", "stream_hints")
+    FIELDS = ("short_max", "group_max", "hub_min", "hub_segment", "lds_hub_prefix", "stream_hints", "ppr_tile",
+              "relabel")
 
     def as_dict(self):
         return {k: int(getattr(self, k)) for k in self.FIELDS}
@@ -87,6 +89,7 @@ _SIGNATURES = {
     "nwg_generate_grid": (c_int, [c_void_p, c_u32, c_u32, c_u32, c_u64, ctypes.POINTER(c_void_p)]),
     "nwg_generate_grid_edges": (c_int, [c_void_p, c_u32, c_u32, c_u32, c_u64, c_void_p, c_void_p, c_int]),
     "nwg_grid_slots": (c_u64, [c_u32]),
+    "nwg_generate_ppr_seeds": (c_int, [c_u32, c_u32, c_u32, c_u64, c_void_p]),
     "nwg_graph_pagerank": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_float, c_float, c_u64, c_int,
                                    ctypes.POINTER(PrInfo)]),
     "nwg_graph_set_plan": (c_int, [c_void_p, c_void_p, ctypes.POINTER(PlanParams)]),
@@ -123,6 +126,13 @@ def load():
         return _lib
     if not os.path.exists(LIB_PATH):
         raise ImportError(f"{LIB_PATH} is missing: build it with `python -c 'import __graft_entry__ as g; g.build()'`")
+    # One HIP runtime per process: torch bundles its own libamdhip64.  When torch is
+    # installed, load it first so that the library and torch share torch's runtime
+    # (loading the system runtime first leaves torch unable to initialise its device).
+    try:
+        import torch  # noqa: F401
+    except Exception:
+        pass
     lib = ctypes.CDLL(LIB_PATH)
     for name, (res, args) in _SIGNATURES.items():
         fn = getattr(lib, name)

@@ -117,6 +117,18 @@ uint32_t read_back_u32(hipStream_t s, const uint32_t* d) {
 
 }  // namespace
 
+int key_bits_for(uint32_t n) { return bits_for(n); }
+
+void sorted_keys_to_csx(hipStream_t s, const uint64_t* keys, uint64_t m, uint32_t n, int b, uint32_t* ptr,
+                        uint32_t* ent) {
+  if (m) {
+    csx_kernel<<<grid_for(m), 256, 0, s>>>(keys, m, n, b, ptr, ent);
+  } else {
+    fill_u32_kernel<<<grid_for((uint64_t)n + 1), 256, 0, s>>>(ptr, (uint64_t)n + 1, 0u);
+  }
+  NWG_LAUNCH_CHECK();
+}
+
 void out_degrees_device(hipStream_t s, uint32_t n, uint64_t nnz, const uint32_t* col, uint32_t* deg) {
   fill_u32_kernel<<<grid_for(n), 256, 0, s>>>(deg, n, 0u);
   NWG_LAUNCH_CHECK();

@@ -9,6 +9,7 @@
 
 #include "capi_util.hpp"
 #include "nwg_internal.hpp"
+#include "philox.hpp"
 
 using nwg::check_graph;
 using nwg::check_mem_kind;
@@ -236,6 +237,23 @@ int nwg_generate_rmat_edges(nwg_ctx* ctx, uint32_t scale, uint32_t a, uint32_t b
     }
     NWG_HIP(hipStreamSynchronize(s));
   });
+}
+
+int nwg_generate_ppr_seeds(uint32_t n, uint32_t k, uint32_t s, uint64_t seed, uint32_t* ids) {
+  if (!ids || n == 0 || s > n) return NWG_EINVAL;
+  for (uint32_t j = 0; j < k; ++j) {
+    uint32_t* row = ids + (uint64_t)j * s;
+    uint32_t got = 0;
+    for (uint64_t t = 0; got < s; ++t) {
+      const nwg::U32x4 r = nwg::philox4x32_10(j, (uint32_t)t, 2u, nwg::kTagPprs, (uint32_t)seed,
+                                              (uint32_t)(seed >> 32));
+      const uint32_t id = (uint32_t)(((uint64_t)r.v[0] * n) >> 32);
+      bool dup = false;
+      for (uint32_t q = 0; q < got; ++q) dup |= (row[q] == id);
+      if (!dup) row[got++] = id;
+    }
+  }
+  return NWG_OK;
 }
 
 uint64_t nwg_grid_slots(uint32_t width) {

@@ -125,6 +125,20 @@ struct Captured {
   ~Captured() { reset(); }
 };
 
+// Degree-ordered copy of a CSC (relabel.hip): new id r is original vertex order[r].
+struct Relabeled {
+  DevBuf<uint32_t> order, rank;  // new -> original, original -> new
+  DevBuf<uint32_t> csc_ptr, csc_idx, outdeg;
+};
+
+// The CSC arrays a sweep runs on (original or relabelled).
+struct ActiveCsc {
+  const uint32_t* ptr;
+  const uint32_t* idx;
+  const uint32_t* deg;
+  const uint32_t* order;  // null when not relabelled
+};
+
 // Cached state of PageRank solves on one graph (buffers + captured HIP graph).
 struct PrState {
   DevBuf<float> x, contrib0, contrib1, base;
@@ -164,6 +178,11 @@ struct nwg_graph {
   uint32_t w_items_csc_n = 0, w_items_csr_n = 0;
   nwg::Plan plan;
   uint64_t plan_version = 1;
+  std::unique_ptr<nwg::Relabeled> rl;  // built when a plan asks for relabel=1
+  nwg::ActiveCsc active(bool relabel) const {
+    if (relabel && rl) return {rl->csc_ptr.p, rl->csc_idx.p, rl->outdeg.p, rl->order.p};
+    return {csc_ptr.p, csc_idx.p, outdeg.p, nullptr};
+  }
   std::unique_ptr<nwg::PrState> pr;
   std::unique_ptr<nwg::PrState> wpr;
 };
@@ -176,6 +195,10 @@ namespace nwg {
 void build_csc_device(hipStream_t s, uint32_t n, uint64_t ne, uint32_t* src, uint32_t* dst,
                       bool clean, nwg_graph* g);
 void build_csr_device(hipStream_t s, nwg_graph* g);
+int key_bits_for(uint32_t n);  // bits of a vertex id field in packed sort keys
+// row pointers + entries from keys (row << b | entry) sorted ascending
+void sorted_keys_to_csx(hipStream_t s, const uint64_t* keys, uint64_t m, uint32_t n, int b, uint32_t* ptr,
+                        uint32_t* ent);
 void out_degrees_device(hipStream_t s, uint32_t n, uint64_t nnz, const uint32_t* col, uint32_t* deg);
 
 // ---- primitives (scan.hip / sort.hip) ----
@@ -197,6 +220,7 @@ void edge_weights_device(hipStream_t s, uint32_t n, const uint32_t* csc_ptr, con
 // ---- PageRank (pagerank.hip) ----
 // (re)build the sweep plan of g with the given parameters; g is unchanged on failure
 void plan_build(hipStream_t s, nwg_graph* g, const nwg_plan_params& pp);
+void relabel_build(hipStream_t s, nwg_graph* g);
 nwg_plan_params default_plan_params();
 void pagerank_run(nwg_ctx* ctx, nwg_graph* g, const float* teleport_dev, float damping, float threshold,
                   uint64_t max_iters, nwg_pr_info* info, float* scores_dev);

@@ -190,11 +190,20 @@ __global__ __launch_bounds__(kThreads) void pr_init_kernel(uint32_t n, const uin
   if (i < iters) acc[i] = 0ull;
 }
 
-// personalised base (1-d) * v_i rounded once from double
-__global__ __launch_bounds__(kThreads) void pr_base_kernel(uint32_t n, const float* __restrict__ v, double omd,
+// personalised base (1-d) * v rounded once from double; with a relabelled CSC row i
+// is original vertex order[i]
+__global__ __launch_bounds__(kThreads) void pr_base_kernel(uint32_t n, const float* __restrict__ v,
+                                                           const uint32_t* __restrict__ order, double omd,
                                                            float* __restrict__ base) {
   const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
-  if (i < n) base[i] = (float)(omd * (double)v[i]);
+  if (i < n) base[i] = (float)(omd * (double)v[order ? order[i] : i]);
+}
+
+// scores in original ids from the relabelled solve: out[order[i]] = x[i]
+__global__ __launch_bounds__(kThreads) void pr_unpermute_kernel(uint32_t n, const uint32_t* __restrict__ order,
+                                                                const float* __restrict__ x, float* __restrict__ out) {
+  const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
+  if (i < n) out[order[i]] = x[i];
 }
 
 // ---------------------------------------------------------------------------
@@ -250,10 +259,9 @@ uint32_t d2h_u32(hipStream_t s, const uint32_t* p) {
 }
 
 // rows of class `cls` in ascending order -> out (allocated); returns count
-uint32_t select_class(hipStream_t s, const nwg_graph* g, const nwg_plan_params& pp, uint32_t cls,
+uint32_t select_class(hipStream_t s, uint32_t n, const uint32_t* rp, const nwg_plan_params& pp, uint32_t cls,
                       DevBuf<uint32_t>& flag, DevBuf<uint32_t>& pos, uint32_t* out /*may be null to count*/) {
-  const uint32_t n = g->n;
-  class_flag_kernel<<<blocks_for(n), kThreads, 0, s>>>(n, g->csc_ptr.p, pp, cls, flag.p);
+  class_flag_kernel<<<blocks_for(n), kThreads, 0, s>>>(n, rp, pp, cls, flag.p);
   NWG_LAUNCH_CHECK();
   exclusive_scan_u32(s, flag.p, pos.p, n);
   const uint32_t cnt = d2h_u32(s, pos.p + n);
@@ -284,18 +292,21 @@ static void validate_params(const nwg_plan_params& p) {
   NWG_REQUIRE(p.stream_hints <= 1, "plan: stream_hints must be 0 or 1");
   NWG_REQUIRE(p.ppr_tile == 0 || p.ppr_tile == 8 || p.ppr_tile == 16 || p.ppr_tile == 32 || p.ppr_tile == 64,
               "plan: ppr_tile must be 0, 8, 16, 32 or 64");
+  NWG_REQUIRE(p.relabel <= 1, "plan: relabel must be 0 or 1");
 }
 
 void plan_build(hipStream_t s, nwg_graph* g, const nwg_plan_params& pp) {
   validate_params(pp);
   const uint32_t n = g->n;
+  if (pp.relabel) relabel_build(s, g);
+  const uint32_t* rp = g->active(pp.relabel != 0).ptr;
   Plan pl;
   pl.params = pp;
   DevBuf<uint32_t> flag(n), pos((uint64_t)n + 1);
   uint32_t counts[kGroupClasses];
   uint32_t total = 0;
   for (int c = 0; c < kGroupClasses; ++c) {
-    counts[c] = select_class(s, g, pp, 1 + c, flag, pos, nullptr);
+    counts[c] = select_class(s, n, rp, pp, 1 + c, flag, pos, nullptr);
     total += counts[c];
   }
   pl.group_rows.alloc(total ? total : 1);
@@ -303,16 +314,16 @@ void plan_build(hipStream_t s, nwg_graph* g, const nwg_plan_params& pp) {
   for (int c = 0; c < kGroupClasses; ++c) {
     pl.cls_count[c] = counts[c];
     pl.cls_off[c + 1] = pl.cls_off[c] + counts[c];
-    if (counts[c]) select_class(s, g, pp, 1 + c, flag, pos, pl.group_rows.p + pl.cls_off[c]);
+    if (counts[c]) select_class(s, n, rp, pp, 1 + c, flag, pos, pl.group_rows.p + pl.cls_off[c]);
   }
-  const uint32_t nh = select_class(s, g, pp, 6, flag, pos, nullptr);
+  const uint32_t nh = select_class(s, n, rp, pp, 6, flag, pos, nullptr);
   pl.n_hub_rows = nh;
   pl.hub_rows.alloc(nh ? nh : 1);
   pl.hub_seg_ptr.alloc((uint64_t)nh + 1);
   pl.n_hub_segs = 0;
   if (nh) {
-    select_class(s, g, pp, 6, flag, pos, pl.hub_rows.p);
-    hub_nseg_kernel<<<blocks_for(nh), kThreads, 0, s>>>(nh, pl.hub_rows.p, g->csc_ptr.p, pp.hub_segment, flag.p);
+    select_class(s, n, rp, pp, 6, flag, pos, pl.hub_rows.p);
+    hub_nseg_kernel<<<blocks_for(nh), kThreads, 0, s>>>(nh, pl.hub_rows.p, rp, pp.hub_segment, flag.p);
     NWG_LAUNCH_CHECK();
     exclusive_scan_u32(s, flag.p, pl.hub_seg_ptr.p, nh);
     pl.n_hub_segs = d2h_u32(s, pl.hub_seg_ptr.p + nh);
@@ -346,9 +357,10 @@ static uint32_t finish_blocks(const SweepParams& P) {
 static SweepParams make_params(nwg_graph* g, PrState& st, bool personalised, float damping, float threshold) {
   const Plan& pl = g->plan;
   SweepParams P{};
-  P.rp = g->csc_ptr.p;
-  P.col = g->csc_idx.p;
-  P.deg = g->outdeg.p;
+  const ActiveCsc a = g->active(pl.params.relabel != 0);
+  P.rp = a.ptr;
+  P.col = a.idx;
+  P.deg = a.deg;
   P.x = st.x.p;
   P.basev = personalised ? st.base.p : nullptr;
   P.base = (float)((1.0 - (double)damping) / g->n);  // page_rank.hpp:94
@@ -404,7 +416,7 @@ static void enqueue_solve(hipStream_t s, nwg_graph* g, PrState& st, SweepParams 
                           hipEvent_t* ev) {
   const uint32_t n = g->n;
   const float init = (float)(1.0 / n);  // page_rank.hpp:93
-  pr_init_kernel<<<blocks_for(std::max<uint64_t>(n, iters)), kThreads, 0, s>>>(n, g->outdeg.p, init, st.x.p,
+  pr_init_kernel<<<blocks_for(std::max<uint64_t>(n, iters)), kThreads, 0, s>>>(n, P.deg, init, st.x.p,
                                                                                st.contrib0.p, st.acc.p, iters);
   NWG_LAUNCH_CHECK();
   const bool nt = g->plan.params.stream_hints != 0;
@@ -436,7 +448,8 @@ void pagerank_run(nwg_ctx* ctx, nwg_graph* g, const float* teleport_dev, float d
   const bool personalised = teleport_dev != nullptr;
   if (personalised) {
     st.base.ensure(n);
-    pr_base_kernel<<<blocks_for(n), kThreads, 0, s>>>(n, teleport_dev, 1.0 - (double)damping, st.base.p);
+    pr_base_kernel<<<blocks_for(n), kThreads, 0, s>>>(n, teleport_dev, g->active(g->plan.params.relabel != 0).order,
+                                                      1.0 - (double)damping, st.base.p);
     NWG_LAUNCH_CHECK();
   }
   const SweepParams P = make_params(g, st, personalised, damping, threshold);
@@ -452,7 +465,7 @@ void pagerank_run(nwg_ctx* ctx, nwg_graph* g, const float* teleport_dev, float d
     key.ptrs[3] = st.acc.p;
     key.ptrs[4] = st.hub_part.p;
     key.ptrs[5] = personalised ? st.base.p : nullptr;
-    key.ptrs[6] = g->csc_ptr.p;
+    key.ptrs[6] = P.rp;
     key.ptrs[7] = st.blk_err.p;
     key.plan_version = g->plan_version;
     key.iters = max_iters;
@@ -492,7 +505,15 @@ void pagerank_run(nwg_ctx* ctx, nwg_graph* g, const float* teleport_dev, float d
   }
   std::vector<unsigned long long> acc(std::max<uint64_t>(max_iters, 1));
   NWG_HIP(hipMemcpyAsync(acc.data(), st.acc.p, max_iters * sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
-  if (scores_dev) NWG_HIP(hipMemcpyAsync(scores_dev, st.x.p, (size_t)n * sizeof(float), hipMemcpyDeviceToDevice, s));
+  if (scores_dev) {
+    const uint32_t* order = g->active(g->plan.params.relabel != 0).order;
+    if (order) {
+      pr_unpermute_kernel<<<blocks_for(n), kThreads, 0, s>>>(n, order, st.x.p, scores_dev);
+      NWG_LAUNCH_CHECK();
+    } else {
+      NWG_HIP(hipMemcpyAsync(scores_dev, st.x.p, (size_t)n * sizeof(float), hipMemcpyDeviceToDevice, s));
+    }
+  }
   NWG_HIP(hipStreamSynchronize(s));
   NWG_HIP(hipEventElapsedTime(&ms_total, ctx->ev0, ctx->ev1));
   uint64_t iters = max_iters;

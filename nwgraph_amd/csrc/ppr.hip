@@ -208,12 +208,39 @@ __global__ __launch_bounds__(kThreads) void ppr_zero_acc_kernel(unsigned long lo
   if (t < m) acc[t] = 0ull;
 }
 
+// out[i] = deg[i] | kSeedFlag when row i appears in the sorted seed rows (binary search,
+// so no two threads ever write the same entry)
 __global__ __launch_bounds__(kThreads) void ppr_degflag_kernel(uint32_t n, const uint32_t* __restrict__ deg,
                                                                const uint32_t* __restrict__ seed_row, uint32_t ns,
                                                                uint32_t* __restrict__ out) {
   const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
-  if (i < n) out[i] = deg[i];
-  if (i < ns) atomicOr(&out[seed_row[i]], kSeedFlag);
+  if (i >= n) return;
+  uint32_t lo = 0, hi = ns;
+  while (lo < hi) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (seed_row[mid] < i) lo = mid + 1;
+    else hi = mid;
+  }
+  out[i] = deg[i] | ((lo < ns && seed_row[lo] == i) ? kSeedFlag : 0u);
+}
+
+__global__ __launch_bounds__(kThreads) void gather_u32_kernel(uint32_t m, const uint32_t* __restrict__ idx,
+                                                              const uint32_t* __restrict__ table,
+                                                              uint32_t* __restrict__ out) {
+  const uint32_t q = blockIdx.x * kThreads + threadIdx.x;
+  if (q < m) out[q] = table[idx[q]];
+}
+
+// rows of the relabelled solve back to original ids: out[order[r]][:] = x[r][:]
+__global__ __launch_bounds__(kThreads) void unpermute_rows_kernel(uint32_t n, uint32_t k,
+                                                                  const uint32_t* __restrict__ order,
+                                                                  const float* __restrict__ x,
+                                                                  float* __restrict__ out) {
+  const uint64_t t = (uint64_t)blockIdx.x * kThreads + threadIdx.x;
+  if (t < (uint64_t)n * k) {
+    const uint32_t r = (uint32_t)(t / k), j = (uint32_t)(t % k);
+    out[(uint64_t)order[r] * k + j] = x[t];
+  }
 }
 
 inline uint32_t grid1(uint64_t m) { return (uint32_t)ceil_div(m ? m : 1, kThreads); }
@@ -231,7 +258,8 @@ void ppr_run(nwg_ctx* ctx, nwg_graph* g, uint32_t k, const std::vector<uint32_t>
   NWG_REQUIRE(max_iters < (1ull << 31), "max_iters too large");
   if (!g->plan.built) plan_build(s, g, default_plan_params());
   const Plan& pl = g->plan;
-  // seed entries sorted by (row, vector)
+  const ActiveCsc a = g->active(pl.params.relabel != 0);
+  // seed entries sorted by (row, vector); rows in the active numbering
   std::vector<std::pair<uint32_t, uint32_t>> ent;
   std::vector<float> ebase;
   for (uint32_t j = 0; j < k; ++j) {
@@ -244,6 +272,19 @@ void ppr_run(nwg_ctx* ctx, nwg_graph* g, uint32_t k, const std::vector<uint32_t>
   }
   std::sort(ent.begin(), ent.end());
   for (size_t q = 1; q < ent.size(); ++q) NWG_REQUIRE(ent[q] != ent[q - 1], "duplicate seed in a teleport vector");
+  if (a.order) {  // map seed ids through rank (original -> relabelled) and re-sort
+    const uint32_t m = (uint32_t)ent.size();
+    std::vector<uint32_t> ids(m);
+    for (uint32_t q = 0; q < m; ++q) ids[q] = ent[q].first;
+    DevBuf<uint32_t> d_ids(m), d_new(m);
+    NWG_HIP(hipMemcpyAsync(d_ids.p, ids.data(), m * 4ull, hipMemcpyHostToDevice, s));
+    gather_u32_kernel<<<grid1(m), kThreads, 0, s>>>(m, d_ids.p, g->rl->rank.p, d_new.p);
+    NWG_LAUNCH_CHECK();
+    NWG_HIP(hipMemcpyAsync(ids.data(), d_new.p, m * 4ull, hipMemcpyDeviceToHost, s));
+    NWG_HIP(hipStreamSynchronize(s));
+    for (uint32_t q = 0; q < m; ++q) ent[q].first = ids[q];
+    std::sort(ent.begin(), ent.end());
+  }
   std::vector<uint32_t> erow(ent.size()), evec(ent.size());
   ebase.resize(ent.size());
   for (size_t q = 0; q < ent.size(); ++q) {
@@ -259,7 +300,7 @@ void ppr_run(nwg_ctx* ctx, nwg_graph* g, uint32_t k, const std::vector<uint32_t>
   NWG_HIP(hipMemcpyAsync(d_row.p, erow.data(), ns * 4ull, hipMemcpyHostToDevice, s));
   NWG_HIP(hipMemcpyAsync(d_vec.p, evec.data(), ns * 4ull, hipMemcpyHostToDevice, s));
   NWG_HIP(hipMemcpyAsync(d_base.p, ebase.data(), ns * 4ull, hipMemcpyHostToDevice, s));
-  ppr_degflag_kernel<<<grid1(std::max(n, ns)), kThreads, 0, s>>>(n, g->outdeg.p, d_row.p, ns, degflag.p);
+  ppr_degflag_kernel<<<grid1(n), kThreads, 0, s>>>(n, a.deg, d_row.p, ns, degflag.p);
   NWG_LAUNCH_CHECK();
 
   const uint32_t tiles = (uint32_t)ceil_div(k, tile);
@@ -267,14 +308,15 @@ void ppr_run(nwg_ctx* ctx, nwg_graph* g, uint32_t k, const std::vector<uint32_t>
   const uint64_t nk = (uint64_t)n * k;
   DevBuf<float> c0(nk), c1(nk);
   DevBuf<float> xbuf;
-  float* x = scores_dev;
+  if (a.order) xbuf.alloc(nk);
+  float* x = a.order ? xbuf.p : scores_dev;
   DevBuf<float> hub_part((uint64_t)std::max<uint32_t>(pl.n_hub_segs, 1) * k);
   DevBuf<double> blk_err((uint64_t)row_blocks * tiles * tile);
   DevBuf<unsigned long long> acc(std::max<uint64_t>(max_iters, 1) * k);
 
   PprParams P{};
-  P.rp = g->csc_ptr.p;
-  P.col = g->csc_idx.p;
+  P.rp = a.ptr;
+  P.col = a.idx;
   P.degflag = degflag.p;
   P.seed_row = d_row.p;
   P.seed_vec = d_vec.p;
@@ -299,7 +341,7 @@ void ppr_run(nwg_ctx* ctx, nwg_graph* g, uint32_t k, const std::vector<uint32_t>
 
   const float init = (float)(1.0 / n);
   NWG_HIP(hipEventRecord(ctx->ev0, s));
-  ppr_init_kernel<<<grid1(nk), kThreads, 0, s>>>(n, k, g->outdeg.p, init, x, c0.p);
+  ppr_init_kernel<<<grid1(nk), kThreads, 0, s>>>(n, k, a.deg, init, x, c0.p);
   NWG_LAUNCH_CHECK();
   ppr_zero_acc_kernel<<<grid1(max_iters * k), kThreads, 0, s>>>(acc.p, max_iters * k);
   NWG_LAUNCH_CHECK();
@@ -331,6 +373,10 @@ void ppr_run(nwg_ctx* ctx, nwg_graph* g, uint32_t k, const std::vector<uint32_t>
     NWG_LAUNCH_CHECK();
   }
   NWG_HIP(hipEventRecord(ctx->ev1, s));
+  if (a.order) {
+    unpermute_rows_kernel<<<grid1(nk), kThreads, 0, s>>>(n, k, a.order, x, scores_dev);
+    NWG_LAUNCH_CHECK();
+  }
   std::vector<unsigned long long> h_acc(std::max<uint64_t>(max_iters, 1) * k);
   NWG_HIP(hipMemcpyAsync(h_acc.data(), acc.p, max_iters * k * 8ull, hipMemcpyDeviceToHost, s));
   NWG_HIP(hipStreamSynchronize(s));

@@ -1,0 +1,252 @@
+// pr.cpp -- PageRank benchmark driver over libnwgraph_hip, the counterpart of the
+// reference's bench/gapbs/pr.cpp (usage pr.cpp:16-34, main pr.cpp:54-177).
+//
+//   pr [--version ID...] -f FILE... [-i NUM] [-t NUM] [-n NUM] [-dvV] [--log FILE] [--log-header] [THREADS]...
+//
+// FILE is a Matrix Market file, an NW binary edge list (load_graph,
+// bench/gapbs/common.hpp:82-101), or a synthetic spec generated on the device:
+//   rmat:SCALE[:EDGE_FACTOR[:SEED]]   RMAT, A/B/C/D = .57/.19/.19/.05, cleaned
+//   grid:WIDTH[:SEED]                 4-neighbour lattice (p=0.85) + 0.5% long-range edges
+// Versions 10 and 11 both run the GPU sweep (Jacobi schedule of page_rank_v10; v11's
+// in-place update order is a CPU schedule artefact).  Defaults follow the reference:
+// version 11, -i 20 (0 means 1), -t 1e-4, -n 1 (0 means 1).  THREADS are accepted and
+// reported like the reference's (which only uses them for TBB); the GPU ignores them.
+#include <nwgraph_hip.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <fstream>
+#include <iomanip>
+#include <iostream>
+#include <map>
+#include <numeric>
+#include <string>
+#include <tuple>
+#include <vector>
+
+#include <nwgraph_hip/page_rank.hpp>
+
+namespace {
+
+struct Args {
+  std::vector<long> versions;
+  std::vector<std::string> files;
+  long max_iters = 20;
+  float tolerance = 1e-4f;
+  long trials = 1;
+  bool debug = false, verify = false, verbose = false, log_header = false;
+  std::string log_file;
+  std::vector<long> threads;
+};
+
+[[noreturn]] void usage(int rc) {
+  std::cout << "pr: MI355X page rank benchmark driver.\n"
+               "  Usage:\n"
+               "      pr (-h | --help)\n"
+               "      pr [--version ID...] -f FILE... [-i NUM] [-t NUM] [-n NUM] [-dvV] [--log FILE] [--log-header] "
+               "[THREADS]...\n\n"
+               "  Options:\n"
+               "      -h, --help          show this screen\n"
+               "      --version ID        algorithm version to run [default: 11]\n"
+               "      -f FILE             input file path (.mtx, NW binary, rmat:SCALE[:EF[:SEED]], grid:W[:SEED])\n"
+               "      -i NUM              maximum iteration [default: 20]\n"
+               "      -t NUM              tolerance [default: 1e-4]\n"
+               "      -n NUM              number of trials [default: 1]\n"
+               "      --log FILE          log times to a file\n"
+               "      --log-header        add a header to the log file\n"
+               "      -d, --debug         run in debug mode\n"
+               "      -v, --verify        verify results\n"
+               "      -V, --verbose       run in verbose mode\n";
+  std::exit(rc);
+}
+
+Args parse(int argc, char** argv) {
+  Args a;
+  for (int i = 1; i < argc; ++i) {
+    std::string s = argv[i];
+    auto need = [&](const char* what) -> std::string {
+      if (i + 1 >= argc) {
+        std::cerr << "missing value for " << what << "\n";
+        usage(1);
+      }
+      return argv[++i];
+    };
+    if (s == "-h" || s == "--help") usage(0);
+    else if (s == "--version") a.versions.push_back(std::stol(need("--version")));
+    else if (s == "-f") a.files.push_back(need("-f"));
+    else if (s == "-i") a.max_iters = std::stol(need("-i"));
+    else if (s == "-t") a.tolerance = std::stof(need("-t"));
+    else if (s == "-n") a.trials = std::stol(need("-n"));
+    else if (s == "--log") a.log_file = need("--log");
+    else if (s == "--log-header") a.log_header = true;
+    else if (s.size() > 1 && s[0] == '-' && s[1] != '-' && !std::isdigit((unsigned char)s[1])) {
+      for (size_t k = 1; k < s.size(); ++k) {
+        if (s[k] == 'd') a.debug = true;
+        else if (s[k] == 'v') a.verify = true;
+        else if (s[k] == 'V') a.verbose = true;
+        else usage(1);
+      }
+    } else if (s == "--debug") a.debug = true;
+    else if (s == "--verify") a.verify = true;
+    else if (s == "--verbose") a.verbose = true;
+    else a.threads.push_back(std::stol(s));
+  }
+  if (a.files.empty()) usage(1);
+  if (a.versions.empty()) a.versions.push_back(11);
+  if (a.max_iters == 0) a.max_iters = 1;  // `?: 1` in the reference (pr.cpp:63)
+  if (a.trials == 0) a.trials = 1;        // pr.cpp:62
+  if (a.threads.empty()) a.threads.push_back(1);
+  return a;
+}
+
+void check(nwg_ctx* ctx, int rc, const char* what) { nw::graph::hip::check(ctx, rc, what); }
+
+struct LifeTimer {  // util/timer.hpp:120-133
+  std::string msg;
+  std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
+  explicit LifeTimer(std::string m) : msg(std::move(m)) {}
+  ~LifeTimer() {
+    const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    std::cout << "# [ " + msg + " ]: " << ms << " ms" << std::endl;
+  }
+};
+
+nwg_graph* load(nwg_ctx* ctx, const std::string& file, bool verbose) {
+  nwg_graph* g = nullptr;
+  auto field = [&](size_t k, long dflt) {
+    size_t pos = 0;
+    for (size_t q = 0; q < k; ++q) {
+      pos = file.find(':', pos);
+      if (pos == std::string::npos) return dflt;
+      ++pos;
+    }
+    return std::stol(file.substr(pos));
+  };
+  if (file.rfind("rmat:", 0) == 0) {
+    LifeTimer t("generate rmat + build adjacency");
+    check(ctx, nwg_generate_rmat(ctx, (uint32_t)field(1, 20), (uint32_t)field(2, 16), 5700, 1900, 1900,
+                                 (uint64_t)field(3, 1), &g),
+          "nwg_generate_rmat");
+    return g;
+  }
+  if (file.rfind("grid:", 0) == 0) {
+    LifeTimer t("generate grid + build adjacency");
+    check(ctx, nwg_generate_grid(ctx, (uint32_t)field(1, 1024), 8500, 50, (uint64_t)field(2, 1), &g),
+          "nwg_generate_grid");
+    return g;
+  }
+  std::ifstream in(file);
+  std::string type;
+  in >> type;
+  uint32_t n = 0;
+  uint64_t ne = 0;
+  uint32_t *src = nullptr, *dst = nullptr;
+  if (type == "NW") {
+    LifeTimer t("deserialize");
+    check(ctx, nwg_read_nw(file.c_str(), &n, &ne, &src, &dst), "nwg_read_nw");
+  } else if (type == "%%MatrixMarket") {
+    std::cout << "Reading matrix market input " << file << " (slow)\n";
+    LifeTimer t("read mm");
+    check(ctx, nwg_read_mm(file.c_str(), &n, &ne, &src, &dst), "nwg_read_mm");
+  } else {
+    std::cerr << "Did not recognize graph input file " << file << "\n";
+    std::exit(1);
+  }
+  if (verbose) std::cout << "% edge list: " << n << " vertices, " << ne << " edges\n";
+  {
+    LifeTimer t("build adjacency");  // adjacency<1> keeps self-loops and duplicates
+    const int rc = nwg_graph_build_from_edges(ctx, n, ne, src, dst, 0, NWG_MEM_HOST, &g);
+    nwg_free(src);
+    nwg_free(dst);
+    check(ctx, rc, "nwg_graph_build_from_edges");
+  }
+  return g;
+}
+
+}  // namespace
+
+int main(int argc, char** argv) {
+  Args args = parse(argc, argv);
+  nw::graph::hip::context context(0);
+  nwg_ctx* ctx = context.get();
+  // Times (common.hpp:176-255): (file, version, threads) -> samples
+  std::map<std::tuple<std::string, long, long>, std::vector<double>> times;
+
+  for (const auto& file : args.files) {
+    nwg_graph* g = load(ctx, file, args.verbose);
+    uint32_t n = 0;
+    uint64_t nnz = 0;
+    nwg_graph_info(g, &n, &nnz);
+    if (args.verbose) std::cout << "% adjacency<1>: " << n << " vertices, " << nnz << " edges\n";
+    std::cout << "# [ degrees ]: built on the device with the adjacency" << std::endl;
+    if (args.debug) {
+      std::vector<uint32_t> ptr(n + 1);
+      check(ctx, nwg_graph_get_csc(ctx, g, ptr.data(), nullptr, nullptr), "nwg_graph_get_csc");
+      for (uint32_t i = 0; i <= std::min<uint32_t>(n, 32); ++i) std::cout << ptr[i] << " ";
+      std::cout << "\n";
+    }
+    std::vector<float> rankings(n);
+    std::vector<double> trace(args.max_iters);
+    for (long thread : args.threads) {
+      for (long id : args.versions) {
+        for (long trial = 0; trial < args.trials; ++trial) {
+          if (id != 10 && id != 11) {
+            std::cerr << "Unknown version id " << id << std::endl;
+            continue;
+          }
+          nwg_pr_info info{};
+          info.error_trace = trace.data();
+          info.error_trace_len = trace.size();
+          const auto t0 = std::chrono::steady_clock::now();
+          check(ctx,
+                nwg_graph_pagerank(ctx, g, rankings.data(), nullptr, 0.85f, args.tolerance, (uint64_t)args.max_iters,
+                                   NWG_MEM_HOST, &info),
+                "nwg_graph_pagerank");
+          const double sec = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+          nw::graph::hip::detail::trace_row("iter", "error", "time", "outgoing");
+          for (uint64_t k = 0; k < info.iterations; ++k)
+            nw::graph::hip::detail::trace_iter(k, trace[k], info.ms_per_iter * 1e-3);
+          times[std::make_tuple(file, id, thread)].push_back(sec);
+        }
+        if (args.verify) {  // print_n_ranks (pr.cpp:46-52): top 10 by score
+          std::cout << "Verifying\n";
+          std::vector<size_t> perm(n);
+          std::iota(perm.begin(), perm.end(), 0);
+          std::stable_sort(perm.begin(), perm.end(), [&](size_t a, size_t b) { return rankings[a] > rankings[b]; });
+          for (size_t i = 0; i < 10 && i < n; ++i)
+            std::cout << std::to_string(perm[i]) + ": " << std::to_string(rankings[perm[i]]) << std::endl;
+        }
+      }
+    }
+    nwg_graph_destroy(g);
+  }
+
+  // Times::print (common.hpp:210-236)
+  size_t w = 4;
+  for (auto& [cfg, s] : times) w = std::max(w, std::get<0>(cfg).size());
+  std::cout << std::setw(w + 2) << std::left << "File" << std::setw(10) << std::left << "Version" << std::setw(10)
+            << std::left << "Threads" << std::setw(20) << std::left << "Min" << std::setw(20) << std::left << "Avg"
+            << std::setw(20) << std::left << "Max" << "\n";
+  for (auto& [cfg, s] : times) {
+    const auto [mn, mx] = std::minmax_element(s.begin(), s.end());
+    const double avg = std::accumulate(s.begin(), s.end(), 0.0) / s.size();
+    std::cout << std::setw(w + 2) << std::left << std::get<0>(cfg) << std::setw(10) << std::left << std::get<1>(cfg)
+              << std::setw(10) << std::left << std::get<2>(cfg) << std::setw(20) << std::left << std::setprecision(6)
+              << std::fixed << *mn << std::setw(20) << std::left << std::setprecision(6) << std::fixed << avg
+              << std::setw(20) << std::left << std::setprecision(6) << std::fixed << *mx << "\n";
+  }
+  if (!args.log_file.empty()) {
+    std::ofstream log(args.log_file, std::ios::app);
+    if (args.log_header) log << "Benchmark Version Threads File Min Avg Max Tolerance Device\n";
+    for (auto& [cfg, s] : times) {
+      const auto [mn, mx] = std::minmax_element(s.begin(), s.end());
+      const double avg = std::accumulate(s.begin(), s.end(), 0.0) / s.size();
+      log << "pr " << std::get<1>(cfg) << " " << std::get<2>(cfg) << " " << std::get<0>(cfg) << " " << *mn << " "
+          << avg << " " << *mx << " " << args.tolerance << " gfx950\n";
+    }
+  }
+  return 0;
+}

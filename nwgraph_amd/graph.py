@@ -446,6 +446,16 @@ def page_rank(graph, degrees, page_rank, damping_factor=0.85, threshold=1e-4, ma
         info.update(DeviceGraph._info_dict(pi, trace))
 
 
+def ppr_seeds(n, k, s, seed=1):
+    """k teleport seed sets of s distinct ids in [0, n) (nwg_generate_ppr_seeds); shape (k, s)."""
+    lib = _lib.load()
+    ids = np.empty(k * s, np.uint32)
+    rc = lib.nwg_generate_ppr_seeds(int(n), int(k), int(s), int(seed), _ptr(ids))
+    if rc != 0:
+        raise NwgError(rc, "nwg_generate_ppr_seeds: invalid arguments")
+    return ids.reshape(k, s)
+
+
 # ---------------------------------------------------------------------------
 # input formats
 # ---------------------------------------------------------------------------

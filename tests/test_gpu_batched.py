@@ -116,5 +116,12 @@ def test_multigraph_with_hub_items_and_repeat(ctx):
     X2, its2, _, _ = mg.pagerank(D, 1e-4, 100)
     assert np.array_equal(X1, X2) and np.array_equal(its1, its2)
     for q, (n, ind, col, deg) in enumerate(refs):
-        ref, it, _, _ = O.pr_v10(n, ind, col, deg, D, 1e-4, 100)
-        assert its1[q] == it and np.max(np.abs(X1[mg.offsets[q]:mg.offsets[q + 1]] - ref)) <= 1e-6
+        # the stopping sweep follows the FP64 iteration: the oracle's sequential FP32 sum over
+        # the 5999-term hub row carries enough rounding noise (propagated round the ring) to
+        # keep its L1 change above 1e-4 two sweeps longer than FP64 and the tree-summed GPU
+        x64, it64 = O.fp64_jacobi(n, ind, col, deg, tol=1e-4, max_iters=100)
+        assert its1[q] == it64
+        assert np.max(np.abs(X1[mg.offsets[q]:mg.offsets[q + 1]] - x64)) <= 1e-6
+        # and at a fixed sweep count the FP32 oracle agrees within the 1e-6 gate
+        ref, _, _, _ = O.pr_v10(n, ind, col, deg, D, 0.0, int(its1[q]))
+        assert np.max(np.abs(X1[mg.offsets[q]:mg.offsets[q + 1]] - ref)) <= 1e-6

@@ -134,6 +134,8 @@ def test_pagerank_rmat_matches_v10(ctx, scale):
         dict(short_max=32, group_max=32, hub_min=33, hub_segment=512),
         dict(short_max=4, group_max=1024, hub_min=100000, hub_segment=4096),  # no hubs
         dict(short_max=100000, group_max=100000, hub_min=100001, hub_segment=4096),  # all short
+        dict(short_max=8, group_max=128, hub_min=2048, hub_segment=2048, relabel=1),  # degree-ordered copy
+        dict(short_max=2, group_max=64, hub_min=512, hub_segment=512, relabel=1, stream_hints=1),
     ],
 )
 def test_pagerank_plans_agree(ctx, plan):
@@ -167,13 +169,41 @@ def test_pagerank_karate_golden(ctx):
     assert info["iterations"] == it and np.max(np.abs(scores - ref)) < 1e-7
 
 
-def test_pagerank_personalised_teleport(ctx):
+@pytest.mark.parametrize("relabel", [0, 1])
+def test_pagerank_personalised_teleport(ctx, relabel):
     g = ctx.rmat(13, 16, seed=4)
+    if relabel:
+        p = g.get_plan()
+        p["relabel"] = 1
+        g.set_plan(**p)
     n, ind, col, deg = O.rmat_csc(13, 16, seed=4)
     ids = O.ppr_sources(n, 1, 8, seed=4)[0]
     v = np.zeros(n, np.float32)
     v[ids] = 1.0 / 8
     check_against_oracle(g, n, ind, col, deg, teleport=v)
+
+
+def test_pr_driver_karate(tmp_path):
+    """The pr driver (bench/gapbs/pr.cpp counterpart) on karate.mtx: trace, --verify top 10
+    and the Times table."""
+    import subprocess
+
+    exe = os.path.join(os.path.dirname(GOLDEN), "..", "nwgraph_amd", "bin", "pr")
+    out = subprocess.run([exe, "-f", os.path.join(GOLDEN, "karate.mtx"), "-i", "100", "-t", "1e-7", "-n", "2",
+                          "--verify", "--version", "11", "--version", "10", "4"], capture_output=True, text=True,
+                         timeout=120)
+    assert out.returncode == 0, out.stderr
+    text = out.stdout
+    assert "iter" in text and "Verifying" in text and "File" in text and "Version" in text
+    # top-10 lines "id: score" match the oracle's ordering (adjacency<1> keeps duplicates: none here)
+    kg = golden("karate_pagerank.json")
+    ind, col = O.csc(34, np.array(kg["src"], np.uint32), np.array(kg["dst"], np.uint32))
+    ref, _, _, _ = O.pr_v10(34, ind, col, O.out_degrees(34, ind, col), 0.85, 1e-7, 100)
+    top = [int(l.split(":")[0]) for l in text.splitlines() if ": " in l and l.split(":")[0].isdigit()][:10]
+    assert top == list(np.argsort(-ref, kind="stable")[:10])
+    rows = [l for l in text.splitlines() if "karate.mtx" in l and not l.startswith("Reading")]
+    assert len(rows) == 2 and {int(r.split()[1]) for r in rows} == {10, 11}
+    assert all(int(r.split()[2]) == 4 for r in rows)
 
 
 def test_edge_cases(ctx):
