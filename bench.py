@@ -322,12 +322,42 @@ def run_wpr(args, ctx):
     return out
 
 
+# ---------------------------------------------------------------------------- calibration
+def run_copy(args, ctx):
+    """Streaming device copy of a known size: the measured HBM ceiling, and the reference
+    transfer for scaling the FETCH_SIZE / WRITE_SIZE counters of the sweep."""
+    import ctypes
+
+    import torch
+
+    nbytes = args.copy_mib << 20
+    a = torch.empty(nbytes // 4, dtype=torch.float32, device="cuda").fill_(1.0)
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    ms = ctypes.c_double()
+    for _ in range(args.warmup):
+        ctx.check(ctx.lib.nwg_stream_copy(ctx.handle, ctypes.c_void_p(b.data_ptr()), ctypes.c_void_p(a.data_ptr()),
+                                          nbytes, ctypes.byref(ms)))
+    times = []
+    for _ in range(args.steps):
+        ctx.check(ctx.lib.nwg_stream_copy(ctx.handle, ctypes.c_void_p(b.data_ptr()), ctypes.c_void_p(a.data_ptr()),
+                                          nbytes, ctypes.byref(ms)))
+        times.append(ms.value)
+    t = float(np.median(times))
+    gbs = 2 * nbytes / (t * 1e-3) / 1e9
+    return base_line(args, round(gbs, 1), "GB/s (read + write)", t, "u8", {
+        "workload": f"device stream copy of {args.copy_mib} MiB (16-byte vector loads/stores)",
+        "bytes_read": nbytes, "bytes_written": nbytes, "kernel_ms": round(t, 4),
+        "frac_of_8TBs": round(gbs / HBM_PEAK_GBS, 4),
+    })
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="plain", choices=["plain", "grid", "ppr", "multi", "wpr"])
+    ap.add_argument("--workload", default="plain", choices=["plain", "grid", "ppr", "multi", "wpr", "copy"])
     ap.add_argument("--scale", type=int, default=25)
     ap.add_argument("--edge-factor", type=int, default=16)
     ap.add_argument("--width", type=int, default=4096)
@@ -335,13 +365,15 @@ def main():
     ap.add_argument("--k", type=int, default=64)
     ap.add_argument("--ngraphs", type=int, default=256)
     ap.add_argument("--wpr-scale", type=int, default=20)
+    ap.add_argument("--copy-mib", type=int, default=4096)
     ap.add_argument("--iters", type=int, default=None)
     ap.add_argument("--tol", type=float, default=1e-4)
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--cpu-sweeps", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-validate", action="store_true")
-    ap.add_argument("--autotune", action="store_true")
+    ap.add_argument("--no-autotune", dest="autotune", action="store_false",
+                    help="keep the default sweep plan instead of autotuning it before the timed region")
     ap.add_argument("--plan", default="", help="k=v,... overrides of the sweep plan")
     ap.add_argument("--traffic", type=float, default=None, help="measured HBM bytes per sweep (rocprofv3 PMC)")
     args = ap.parse_args()
@@ -369,6 +401,8 @@ def main():
         out = run_ppr(args, ctx)
     elif args.workload == "multi":
         out = run_multi(args, ctx)
+    elif args.workload == "copy":
+        out = run_copy(args, ctx)
     else:
         out = run_wpr(args, ctx)
     print(json.dumps(out), flush=True)

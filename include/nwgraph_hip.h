@@ -213,6 +213,12 @@ int nwg_wpr_backward(nwg_ctx* ctx, nwg_graph* g, const float* x, const float* gr
                      float* grad_w, float* grad_teleport, float damping, float threshold,
                      uint64_t max_iters, int raw_weights, int mem_kind, nwg_pr_info* info);
 
+/* ------------------------------------------------------------ calibration */
+/* Plain device copy of `bytes` (multiple of 16) from src to dst ([device] pointers) with a
+ * streaming kernel; returns the kernel time in *ms.  Used to calibrate HBM bandwidth and
+ * the hardware byte counters against a transfer of known size. */
+int nwg_stream_copy(nwg_ctx* ctx, void* dst, const void* src, uint64_t bytes, double* ms);
+
 /* ------------------------------------------------------------------ input */
 /* Matrix Market reader (read_mm<directed>, io/mmio.hpp:171-217 with mm_fill :86-103):
  * 1-based -> 0-based, symmetric files mirror off-diagonal entries, n = max(header

@@ -110,7 +110,8 @@ _SIGNATURES = {
                                 ctypes.POINTER(PrInfo)]),
     "nwg_wpr_backward": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_float, c_float, c_u64,
                                  c_int, c_int, ctypes.POINTER(PrInfo)]),
-    "nwg_read_mm": (c_int, [ctypes.c_char_p, P_u32, P_u64, ctypes.POINTER(P_u32), ctypes.POINTER(P_u32)]),
+    "nwg_stream_copy": (c_int, [c_void_p, c_void_p, c_void_p, c_u64, P_double]),
+    "nwg_read_mm":(c_int, [ctypes.c_char_p, P_u32, P_u64, ctypes.POINTER(P_u32), ctypes.POINTER(P_u32)]),
     "nwg_read_nw": (c_int, [ctypes.c_char_p, P_u32, P_u64, ctypes.POINTER(P_u32), ctypes.POINTER(P_u32)]),
     "nwg_write_nw": (c_int, [ctypes.c_char_p, c_u32, c_u64, c_void_p, c_void_p]),
     "nwg_free": (None, [c_void_p]),

@@ -142,6 +142,7 @@ nwg_plan_params autotune_plan(nwg_ctx* ctx, nwg_graph* g) {
       }
     }
   };
+  try_field(&nwg_plan_params::relabel, {0, 1});
   try_field(&nwg_plan_params::stream_hints, {0, 1});
   try_field(&nwg_plan_params::short_max, {0, 2, 4, 8, 16, 32});
   try_field(&nwg_plan_params::group_max, {16, 32, 64, 128, 256, 512, 1024});

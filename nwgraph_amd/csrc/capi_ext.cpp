@@ -17,6 +17,16 @@ using nwg::Staged;
 
 extern "C" {
 
+int nwg_stream_copy(nwg_ctx* ctx, void* dst, const void* src, uint64_t bytes, double* ms) {
+  if (!ctx) return NWG_EINVAL;
+  return guarded(ctx, [&] {
+    NWG_REQUIRE(dst && src, "copy buffers must not be null");
+    DeviceGuard dg(ctx->device);
+    const float t = nwg::stream_copy_device(ctx->stream, ctx->ev0, ctx->ev1, dst, src, bytes);
+    if (ms) *ms = t;
+  });
+}
+
 int nwg_graph_autotune(nwg_ctx* ctx, nwg_graph* g, nwg_plan_params* chosen) {
   if (!ctx) return NWG_EINVAL;
   return guarded(ctx, [&] {

@@ -8,6 +8,8 @@
 // Philox({e.lo, e.hi, l/4, 'RMAT'}, seed) and picks quadrant A=(0,0) if r < tA,
 // B=(0,1) if r < tAB, C=(1,0) if r < tABC, else D=(1,1), as (source bit, target bit).
 // No noise and no vertex permutation.
+#include <algorithm>
+
 #include "device_utils.hpp"
 #include "nwg_internal.hpp"
 #include "philox.hpp"
@@ -122,4 +124,35 @@ void edge_weights_device(hipStream_t s, uint32_t n, const uint32_t* csc_ptr, con
   NWG_LAUNCH_CHECK();
 }
 
+}  // namespace nwg
+
+// ---------------------------------------------------------------------------
+// calibration copy: 16-byte vector loads and stores, grid-stride over the buffer
+// ---------------------------------------------------------------------------
+namespace nwg {
+namespace {
+__global__ __launch_bounds__(256) void stream_copy_kernel(const uint4* __restrict__ src, uint4* __restrict__ dst,
+                                                          uint64_t n16) {
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n16; i += stride) dst[i] = src[i];
+}
+}  // namespace
+
+float stream_copy_device(hipStream_t s, hipEvent_t e0, hipEvent_t e1, void* dst, const void* src, uint64_t bytes) {
+  NWG_REQUIRE(bytes % 16 == 0, "copy size must be a multiple of 16 bytes");
+  NWG_REQUIRE(((uintptr_t)dst | (uintptr_t)src) % 16 == 0, "copy buffers must be 16-byte aligned");
+  const uint64_t n16 = bytes / 16;
+  int dev = 0, cus = 0;
+  NWG_HIP(hipGetDevice(&dev));
+  NWG_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+  const uint32_t grid = (uint32_t)std::min<uint64_t>(ceil_div(n16 ? n16 : 1, 256), (uint64_t)cus * 16);
+  NWG_HIP(hipEventRecord(e0, s));
+  stream_copy_kernel<<<grid, 256, 0, s>>>(static_cast<const uint4*>(src), static_cast<uint4*>(dst), n16);
+  NWG_LAUNCH_CHECK();
+  NWG_HIP(hipEventRecord(e1, s));
+  NWG_HIP(hipEventSynchronize(e1));
+  float ms = 0.f;
+  NWG_HIP(hipEventElapsedTime(&ms, e0, e1));
+  return ms;
+}
 }  // namespace nwg

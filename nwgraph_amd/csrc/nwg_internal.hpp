@@ -125,6 +125,13 @@ struct Captured {
   ~Captured() { reset(); }
 };
 
+// Cached buffers of the weighted forward / adjoint (wpr.hip).
+struct WState {
+  DevBuf<float> y0, y1, base, lambda, t;
+  DevBuf<unsigned long long> acc;
+  DevBuf<uint32_t> row_of;  // CSC position -> row
+};
+
 // Degree-ordered copy of a CSC (relabel.hip): new id r is original vertex order[r].
 struct Relabeled {
   DevBuf<uint32_t> order, rank;  // new -> original, original -> new
@@ -184,7 +191,7 @@ struct nwg_graph {
     return {csc_ptr.p, csc_idx.p, outdeg.p, nullptr};
   }
   std::unique_ptr<nwg::PrState> pr;
-  std::unique_ptr<nwg::PrState> wpr;
+  std::unique_ptr<nwg::WState> wst;
 };
 
 namespace nwg {
@@ -216,11 +223,17 @@ void grid_edges_device(hipStream_t s, uint32_t width, uint32_t keep, uint32_t ln
                        uint32_t* src, uint32_t* dst);
 void edge_weights_device(hipStream_t s, uint32_t n, const uint32_t* csc_ptr, const uint32_t* csc_idx,
                          uint64_t seed, float* w);
+// streaming copy for bandwidth / counter calibration; returns the kernel time in ms
+float stream_copy_device(hipStream_t s, hipEvent_t e0, hipEvent_t e1, void* dst, const void* src, uint64_t bytes);
 
 // ---- PageRank (pagerank.hip) ----
 // (re)build the sweep plan of g with the given parameters; g is unchanged on failure
 void plan_build(hipStream_t s, nwg_graph* g, const nwg_plan_params& pp);
 void relabel_build(hipStream_t s, nwg_graph* g);
+// rows with in-degree >= min_deg (ascending) and their split into `seg`-nonzero segments
+void build_segments(hipStream_t s, const uint32_t* rp, uint32_t n, uint32_t min_deg, uint32_t seg,
+                    DevBuf<uint32_t>& rows, DevBuf<uint32_t>& seg_ptr, DevBuf<uint32_t>& seg_row, uint32_t* nrows,
+                    uint32_t* nsegs);
 nwg_plan_params default_plan_params();
 void pagerank_run(nwg_ctx* ctx, nwg_graph* g, const float* teleport_dev, float damping, float threshold,
                   uint64_t max_iters, nwg_pr_info* info, float* scores_dev);

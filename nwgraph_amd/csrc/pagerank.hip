@@ -274,6 +274,38 @@ uint32_t select_class(hipStream_t s, uint32_t n, const uint32_t* rp, const nwg_p
 
 }  // namespace
 
+void build_segments(hipStream_t s, const uint32_t* rp, uint32_t n, uint32_t min_deg, uint32_t seg,
+                    DevBuf<uint32_t>& rows, DevBuf<uint32_t>& seg_ptr, DevBuf<uint32_t>& seg_row, uint32_t* nrows,
+                    uint32_t* nsegs) {
+  // rows with in-degree >= min_deg form the hub class of a plan whose other cutoffs are below it
+  nwg_plan_params pp{};
+  pp.short_max = 0;
+  pp.group_max = 0;
+  pp.hub_min = min_deg;
+  DevBuf<uint32_t> flag(n ? n : 1), pos((uint64_t)n + 1);
+  const uint32_t nh = select_class(s, n, rp, pp, 6, flag, pos, nullptr);
+  rows.alloc(nh ? nh : 1);
+  seg_ptr.alloc((uint64_t)nh + 1);
+  uint32_t ns = 0;
+  if (nh) {
+    select_class(s, n, rp, pp, 6, flag, pos, rows.p);
+    hub_nseg_kernel<<<blocks_for(nh), kThreads, 0, s>>>(nh, rows.p, rp, seg, flag.p);
+    NWG_LAUNCH_CHECK();
+    exclusive_scan_u32(s, flag.p, seg_ptr.p, nh);
+    ns = d2h_u32(s, seg_ptr.p + nh);
+    seg_row.alloc(ns);
+    hub_segrow_kernel<<<blocks_for(nh), kThreads, 0, s>>>(nh, seg_ptr.p, seg_row.p);
+    NWG_LAUNCH_CHECK();
+  } else {
+    const uint32_t zero = 0;
+    NWG_HIP(hipMemcpyAsync(seg_ptr.p, &zero, 4, hipMemcpyHostToDevice, s));
+    seg_row.alloc(1);
+  }
+  NWG_HIP(hipStreamSynchronize(s));
+  *nrows = nh;
+  *nsegs = ns;
+}
+
 nwg_plan_params default_plan_params() {
   nwg_plan_params p{};
   p.short_max = 8;

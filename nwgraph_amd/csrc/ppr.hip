@@ -9,9 +9,11 @@
 // it in one coalesced access while the column-id stream is shared by all k vectors.
 //
 // Work split per sweep (one launch + one finishing launch):
-//   row workgroups   4 * (64/T) consecutive rows x one k-tile; rows with in-degree
-//                    >= hub_min are skipped here
-//   hub segments     hub_segment in-edges of one hub row x one k-tile -> partial vectors
+//   row workgroups   4 * (64/T) consecutive rows x one k-tile; a T-lane group walks the
+//                    in-edges of its row one by one, so only light rows (in-degree <= 64)
+//                    are done here
+//   heavy segments   (256/T)*32 in-edges of one heavier row x one k-tile, 256/T edge
+//                    groups striding over them -> partial vectors
 //   finish           hub rows (partials added in segment order) and per-vector L1
 //                    partials folded into acc[iter][j] (fixed point, order independent)
 // A vector whose previous sweep converged is skipped (its lanes idle); a workgroup
@@ -44,7 +46,7 @@ struct PprParams {
   float* x;
   uint32_t n, k;
   float damp;
-  uint32_t hub_min, hub_segment;
+  uint32_t light_max, hub_segment;  // rows with in-degree <= light_max take the row path
   uint32_t row_blocks;       // workgroups of rows per k-tile
   uint32_t n_hub_rows, n_hub_segs;
   const uint32_t* hub_rows;
@@ -123,7 +125,7 @@ __global__ __launch_bounds__(kThreads) void ppr_sweep_kernel(PprParams P) {
     double e = 0.0;
     if (i < P.n && live) {
       const uint32_t p0 = P.rp[i], p1 = P.rp[i + 1];
-      if (p1 - p0 < P.hub_min) {
+      if (p1 - p0 <= P.light_max) {
         float z = 0.f;
         for (uint32_t p = p0; p < p1; ++p) z += P.cin[(uint64_t)P.col[p] * P.k + j];
         e = ppr_update(P, i, j, z);
@@ -259,6 +261,13 @@ void ppr_run(nwg_ctx* ctx, nwg_graph* g, uint32_t k, const std::vector<uint32_t>
   if (!g->plan.built) plan_build(s, g, default_plan_params());
   const Plan& pl = g->plan;
   const ActiveCsc a = g->active(pl.params.relabel != 0);
+  // Heavy rows (in-degree > light_max) are split into segments of kSegIters edges per
+  // lane group so that no lane group walks more than ~32 dependent gathers of a row.
+  const uint32_t light_max = 64;
+  const uint32_t seg_edges = (kThreads / tile) * 32;
+  DevBuf<uint32_t> heavy_rows, heavy_seg_ptr, heavy_seg_row;
+  uint32_t n_heavy = 0, n_segs = 0;
+  build_segments(s, a.ptr, n, light_max + 1, seg_edges, heavy_rows, heavy_seg_ptr, heavy_seg_row, &n_heavy, &n_segs);
   // seed entries sorted by (row, vector); rows in the active numbering
   std::vector<std::pair<uint32_t, uint32_t>> ent;
   std::vector<float> ebase;
@@ -310,7 +319,7 @@ void ppr_run(nwg_ctx* ctx, nwg_graph* g, uint32_t k, const std::vector<uint32_t>
   DevBuf<float> xbuf;
   if (a.order) xbuf.alloc(nk);
   float* x = a.order ? xbuf.p : scores_dev;
-  DevBuf<float> hub_part((uint64_t)std::max<uint32_t>(pl.n_hub_segs, 1) * k);
+  DevBuf<float> hub_part((uint64_t)std::max<uint32_t>(n_segs, 1) * k);
   DevBuf<double> blk_err((uint64_t)row_blocks * tiles * tile);
   DevBuf<unsigned long long> acc(std::max<uint64_t>(max_iters, 1) * k);
 
@@ -326,14 +335,14 @@ void ppr_run(nwg_ctx* ctx, nwg_graph* g, uint32_t k, const std::vector<uint32_t>
   P.n = n;
   P.k = k;
   P.damp = damping;
-  P.hub_min = pl.params.hub_min;
-  P.hub_segment = pl.params.hub_segment;
+  P.light_max = light_max;
+  P.hub_segment = seg_edges;
   P.row_blocks = row_blocks;
-  P.n_hub_rows = pl.n_hub_rows;
-  P.n_hub_segs = pl.n_hub_segs;
-  P.hub_rows = pl.hub_rows.p;
-  P.seg_ptr = pl.hub_seg_ptr.p;
-  P.seg_row = pl.hub_seg_row.p;
+  P.n_hub_rows = n_heavy;
+  P.n_hub_segs = n_segs;
+  P.hub_rows = heavy_rows.p;
+  P.seg_ptr = heavy_seg_ptr.p;
+  P.seg_row = heavy_seg_row.p;
   P.hub_part = hub_part.p;
   P.blk_err = blk_err.p;
   P.acc = acc.p;
@@ -345,8 +354,8 @@ void ppr_run(nwg_ctx* ctx, nwg_graph* g, uint32_t k, const std::vector<uint32_t>
   NWG_LAUNCH_CHECK();
   ppr_zero_acc_kernel<<<grid1(max_iters * k), kThreads, 0, s>>>(acc.p, max_iters * k);
   NWG_LAUNCH_CHECK();
-  const dim3 grid(row_blocks + pl.n_hub_segs, tiles);
-  const uint64_t fin_a = ceil_div((uint64_t)pl.n_hub_rows * k, kThreads), fin_b = ceil_div(row_blocks, 32);
+  const dim3 grid(row_blocks + n_segs, tiles);
+  const uint64_t fin_a = ceil_div((uint64_t)n_heavy * k, kThreads), fin_b = ceil_div(row_blocks, 32);
   const uint32_t fin = (uint32_t)std::max<uint64_t>(std::max(fin_a, fin_b), 1);
   for (uint64_t it = 0; it < max_iters; ++it) {
     P.iter = (uint32_t)it;

@@ -107,15 +107,58 @@ __global__ __launch_bounds__(kThreads) void wbase_kernel(uint32_t n, const float
   if (i < n) base[i] = v ? (float)(omd * (double)v[i]) : (float)(omd / (double)n);
 }
 
-// per-source sums of raw weights in CSR order (double), then normalised CSC weights
-__global__ __launch_bounds__(kThreads) void rowsum_kernel(uint32_t n, const uint32_t* __restrict__ csr_ptr,
-                                                          const uint32_t* __restrict__ csr_to_csc,
-                                                          const float* __restrict__ w_raw, float* __restrict__ s) {
-  const uint32_t u = blockIdx.x * kThreads + threadIdx.x;
-  if (u >= n) return;
-  double acc = 0.0;
-  for (uint32_t q = csr_ptr[u]; q < csr_ptr[u + 1]; ++q) acc += (double)w_raw[csr_to_csc[q]];
-  s[u] = (float)acc;
+// Per-row sums over the CSR of v[q] = a[perm[q]] * (b ? b[perm[q]] : 1), accumulated in
+// double in CSR order (deterministic), on the same LDS work items as the sweep.
+__global__ __launch_bounds__(kThreads) void csr_rowsum_kernel(const uint32_t* __restrict__ rp,
+                                                              const uint32_t* __restrict__ perm,
+                                                              const float* __restrict__ a,
+                                                              const float* __restrict__ b,
+                                                              const uint32_t* __restrict__ item_r0,
+                                                              const uint32_t* __restrict__ item_r1, uint32_t n_items,
+                                                              float* __restrict__ out) {
+  __shared__ double vals[kItemNnz];
+  __shared__ double red_d[kThreads / 64];
+  for (uint32_t it = blockIdx.x; it < n_items; it += gridDim.x) {
+    const uint32_t r0 = item_r0[it], r1 = item_r1[it];
+    const uint32_t p0 = rp[r0], p1 = rp[r1];
+    if (p1 - p0 > kItemNnz) {
+      // a single long row: strided partial sums, fixed-tree reduction
+      double acc = 0.0;
+      for (uint32_t q = p0 + threadIdx.x; q < p1; q += kThreads) {
+        const uint32_t e = perm[q];
+        acc += (double)a[e] * (b ? (double)b[e] : 1.0);
+      }
+      acc = block_sum<kThreads>(acc, red_d);
+      if (threadIdx.x == 0) out[r0] = (float)acc;
+    } else {
+      for (uint32_t q = threadIdx.x; q < p1 - p0; q += kThreads) {
+        const uint32_t e = perm[p0 + q];
+        vals[q] = (double)a[e] * (b ? (double)b[e] : 1.0);
+      }
+      __syncthreads();
+      const uint32_t u = r0 + threadIdx.x;
+      if (u < r1) {
+        double acc = 0.0;
+        for (uint32_t q = rp[u] - p0; q < rp[u + 1] - p0; ++q) acc += vals[q];
+        out[u] = (float)acc;
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// row of every CSC position (upper bound in the row pointers)
+__global__ __launch_bounds__(kThreads) void row_of_kernel(uint32_t n, uint64_t nnz, const uint32_t* __restrict__ rp,
+                                                          uint32_t* __restrict__ row) {
+  const uint64_t p = (uint64_t)blockIdx.x * kThreads + threadIdx.x;
+  if (p >= nnz) return;
+  uint32_t lo = 0, hi = n;  // first r with rp[r + 1] > p
+  while (lo < hi) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (rp[mid + 1] <= p) lo = mid + 1;
+    else hi = mid;
+  }
+  row[p] = lo;
 }
 
 __global__ __launch_bounds__(kThreads) void normalise_kernel(uint64_t nnz, const uint32_t* __restrict__ col,
@@ -131,30 +174,13 @@ __global__ __launch_bounds__(kThreads) void gather_kernel(uint64_t nnz, const ui
   if (q < nnz) dst[q] = src[perm[q]];
 }
 
-// grad_w[p] = d * lambda[row] * x[col[p]] over the CSC, one thread per row
-__global__ __launch_bounds__(kThreads) void edge_grad_kernel(uint32_t n, const uint32_t* __restrict__ rp,
+// grad_w[p] = d * lambda[row(p)] * x[col[p]], one thread per CSC position
+__global__ __launch_bounds__(kThreads) void edge_grad_kernel(uint64_t nnz, const uint32_t* __restrict__ row,
                                                              const uint32_t* __restrict__ col, float d,
                                                              const float* __restrict__ lambda,
                                                              const float* __restrict__ x, float* __restrict__ gw) {
-  const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
-  if (i >= n) return;
-  const float dl = d * lambda[i];
-  for (uint32_t p = rp[i]; p < rp[i + 1]; ++p) gw[p] = dl * x[col[p]];
-}
-
-// t_u = sum_j g(u->j) w(u->j) over the CSR (double); gw~[p] = (g[p] - t_u) / s_u
-__global__ __launch_bounds__(kThreads) void chain_sum_kernel(uint32_t n, const uint32_t* __restrict__ csr_ptr,
-                                                             const uint32_t* __restrict__ csr_to_csc,
-                                                             const float* __restrict__ gw,
-                                                             const float* __restrict__ w, float* __restrict__ t) {
-  const uint32_t u = blockIdx.x * kThreads + threadIdx.x;
-  if (u >= n) return;
-  double acc = 0.0;
-  for (uint32_t q = csr_ptr[u]; q < csr_ptr[u + 1]; ++q) {
-    const uint32_t p = csr_to_csc[q];
-    acc += (double)gw[p] * (double)w[p];
-  }
-  t[u] = (float)acc;
+  const uint64_t p = (uint64_t)blockIdx.x * kThreads + threadIdx.x;
+  if (p < nnz) gw[p] = (d * lambda[row[p]]) * x[col[p]];
 }
 
 __global__ __launch_bounds__(kThreads) void chain_apply_kernel(uint64_t nnz, const uint32_t* __restrict__ col,
@@ -203,17 +229,33 @@ void build_items(hipStream_t s, const uint32_t* rp_dev, uint32_t n, DevBuf<uint3
   *count = (uint32_t)a.size();
 }
 
+static uint32_t persistent_grid(uint32_t n_items) {
+  int dev = 0, cus = 0;
+  NWG_HIP(hipGetDevice(&dev));
+  NWG_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+  return (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(n_items, (uint64_t)cus * 8));
+}
+
+static void ensure_csr_items(hipStream_t s, nwg_graph* g) {
+  build_csr_device(s, g);
+  if (!g->w_items_csr_n && g->n) build_items(s, g->csr_ptr.p, g->n, g->w_items_csr_r0, g->w_items_csr_r1, &g->w_items_csr_n);
+}
+
 void weights_set(hipStream_t s, nwg_graph* g, const float* w_dev, bool normalize) {
   const uint64_t nnz = g->nnz;
-  build_csr_device(s, g);
-  g->w_raw.alloc(nnz ? nnz : 1);
-  g->w_csc.alloc(nnz ? nnz : 1);
-  g->w_csr.alloc(nnz ? nnz : 1);
-  g->w_rowsum.alloc(g->n);
+  ensure_csr_items(s, g);
+  g->w_raw.ensure(nnz ? nnz : 1);
+  g->w_csc.ensure(nnz ? nnz : 1);
+  g->w_csr.ensure(nnz ? nnz : 1);
+  g->w_rowsum.ensure(g->n);
   if (nnz) NWG_HIP(hipMemcpyAsync(g->w_raw.p, w_dev, nnz * 4, hipMemcpyDeviceToDevice, s));
   if (normalize) {
-    rowsum_kernel<<<grid1(g->n), kThreads, 0, s>>>(g->n, g->csr_ptr.p, g->csr_to_csc.p, g->w_raw.p, g->w_rowsum.p);
-    NWG_LAUNCH_CHECK();
+    if (g->w_items_csr_n) {
+      csr_rowsum_kernel<<<persistent_grid(g->w_items_csr_n), kThreads, 0, s>>>(
+          g->csr_ptr.p, g->csr_to_csc.p, g->w_raw.p, nullptr, g->w_items_csr_r0.p, g->w_items_csr_r1.p,
+          g->w_items_csr_n, g->w_rowsum.p);
+      NWG_LAUNCH_CHECK();
+    }
     normalise_kernel<<<grid1(nnz), kThreads, 0, s>>>(nnz, g->csc_idx.p, g->w_raw.p, g->w_rowsum.p, g->w_csc.p);
     NWG_LAUNCH_CHECK();
   } else {
@@ -228,25 +270,24 @@ void weights_set(hipStream_t s, nwg_graph* g, const float* w_dev, bool normalize
   g->w_normalized = normalize;
 }
 
-// Runs the fixed point y <- fma(d, sum w y, base) from y0 over (rp, col, w); returns the
-// sweeps executed and leaves the result in `out`.
-static uint64_t fixed_point(nwg_ctx* ctx, const uint32_t* rp, const uint32_t* col, const float* w,
+// Runs the fixed point y <- fma(d, sum w y, base) from y0 over (rp, col, w) in the cached
+// buffers of `st`; returns the sweeps executed and leaves the result in `out`.
+static uint64_t fixed_point(nwg_ctx* ctx, WState& st, const uint32_t* rp, const uint32_t* col, const float* w,
                             const DevBuf<uint32_t>& r0, const DevBuf<uint32_t>& r1, uint32_t n_items, uint32_t n,
                             const float* base, const float* y0, bool y0_uniform, float damping, float threshold,
                             uint64_t max_iters, float* out, nwg_pr_info* info) {
   hipStream_t s = ctx->stream;
-  DevBuf<float> y[2] = {DevBuf<float>(n), DevBuf<float>(n)};
-  DevBuf<unsigned long long> acc(std::max<uint64_t>(max_iters, 1));
-  int dev = 0, cus = 0;
-  NWG_HIP(hipGetDevice(&dev));
-  NWG_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-  const uint32_t grid = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(n_items, (uint64_t)cus * 8));
+  st.y0.ensure(n);
+  st.y1.ensure(n);
+  st.acc.ensure(std::max<uint64_t>(max_iters, 1));
+  float* y[2] = {st.y0.p, st.y1.p};
+  const uint32_t grid = persistent_grid(n_items);
   NWG_HIP(hipEventRecord(ctx->ev0, s));
   if (y0_uniform)
-    wfill_kernel<<<grid1(std::max<uint64_t>(n, max_iters)), kThreads, 0, s>>>(n, (float)(1.0 / n), y[0].p, acc.p,
+    wfill_kernel<<<grid1(std::max<uint64_t>(n, max_iters)), kThreads, 0, s>>>(n, (float)(1.0 / n), y[0], st.acc.p,
                                                                             max_iters);
   else
-    wcopy_kernel<<<grid1(std::max<uint64_t>(n, max_iters)), kThreads, 0, s>>>(n, y0, y[0].p, acc.p, max_iters);
+    wcopy_kernel<<<grid1(std::max<uint64_t>(n, max_iters)), kThreads, 0, s>>>(n, y0, y[0], st.acc.p, max_iters);
   NWG_LAUNCH_CHECK();
   WParams P{};
   P.rp = rp;
@@ -257,18 +298,18 @@ static uint64_t fixed_point(nwg_ctx* ctx, const uint32_t* rp, const uint32_t* co
   P.n_items = n_items;
   P.base = base;
   P.damp = damping;
-  P.acc = acc.p;
+  P.acc = st.acc.p;
   P.threshold = (double)threshold;
   for (uint64_t k = 0; k < max_iters; ++k) {
     P.iter = (uint32_t)k;
-    P.xin = y[k & 1].p;
-    P.xout = y[(k + 1) & 1].p;
+    P.xin = y[k & 1];
+    P.xout = y[(k + 1) & 1];
     wsweep_kernel<<<grid, kThreads, 0, s>>>(P);
     NWG_LAUNCH_CHECK();
   }
   NWG_HIP(hipEventRecord(ctx->ev1, s));
   std::vector<unsigned long long> h(std::max<uint64_t>(max_iters, 1));
-  NWG_HIP(hipMemcpyAsync(h.data(), acc.p, max_iters * 8, hipMemcpyDeviceToHost, s));
+  NWG_HIP(hipMemcpyAsync(h.data(), st.acc.p, max_iters * 8, hipMemcpyDeviceToHost, s));
   NWG_HIP(hipStreamSynchronize(s));
   uint64_t iters = max_iters;
   double last = 0.0;
@@ -279,7 +320,7 @@ static uint64_t fixed_point(nwg_ctx* ctx, const uint32_t* rp, const uint32_t* co
       break;
     }
   }
-  NWG_HIP(hipMemcpyAsync(out, y[iters & 1].p, (size_t)n * 4, hipMemcpyDeviceToDevice, s));
+  NWG_HIP(hipMemcpyAsync(out, y[iters & 1], (size_t)n * 4, hipMemcpyDeviceToDevice, s));
   NWG_HIP(hipStreamSynchronize(s));
   if (info) {
     float ms = 0.f;
@@ -295,17 +336,24 @@ static uint64_t fixed_point(nwg_ctx* ctx, const uint32_t* rp, const uint32_t* co
   return iters;
 }
 
+static WState& wstate(nwg_graph* g) {
+  if (!g->wst) g->wst.reset(new WState());
+  return *g->wst;
+}
+
 void wpr_forward(nwg_ctx* ctx, nwg_graph* g, const float* teleport_dev, float* x_dev, float damping, float threshold,
                  uint64_t max_iters, nwg_pr_info* info) {
   hipStream_t s = ctx->stream;
   NWG_REQUIRE(g->has_weights, "graph has no weights (nwg_graph_set_weights)");
   NWG_REQUIRE(max_iters < (1ull << 31), "max_iters too large");
-  if (!g->w_items_csc_n && g->n) build_items(s, g->csc_ptr.p, g->n, g->w_items_csc_r0, g->w_items_csc_r1, &g->w_items_csc_n);
-  DevBuf<float> base(g->n);
-  wbase_kernel<<<grid1(g->n), kThreads, 0, s>>>(g->n, teleport_dev, 1.0 - (double)damping, base.p);
+  if (!g->w_items_csc_n && g->n)
+    build_items(s, g->csc_ptr.p, g->n, g->w_items_csc_r0, g->w_items_csc_r1, &g->w_items_csc_n);
+  WState& st = wstate(g);
+  st.base.ensure(g->n);
+  wbase_kernel<<<grid1(g->n), kThreads, 0, s>>>(g->n, teleport_dev, 1.0 - (double)damping, st.base.p);
   NWG_LAUNCH_CHECK();
-  fixed_point(ctx, g->csc_ptr.p, g->csc_idx.p, g->w_csc.p, g->w_items_csc_r0, g->w_items_csc_r1, g->w_items_csc_n,
-              g->n, base.p, nullptr, true, damping, threshold, max_iters, x_dev, info);
+  fixed_point(ctx, st, g->csc_ptr.p, g->csc_idx.p, g->w_csc.p, g->w_items_csc_r0, g->w_items_csc_r1,
+              g->w_items_csc_n, g->n, st.base.p, nullptr, true, damping, threshold, max_iters, x_dev, info);
 }
 
 void wpr_backward(nwg_ctx* ctx, nwg_graph* g, const float* x_dev, const float* gx_dev, float* gw_dev, float* gv_dev,
@@ -314,27 +362,37 @@ void wpr_backward(nwg_ctx* ctx, nwg_graph* g, const float* x_dev, const float* g
   NWG_REQUIRE(g->has_weights, "graph has no weights (nwg_graph_set_weights)");
   NWG_REQUIRE(max_iters < (1ull << 31), "max_iters too large");
   NWG_REQUIRE(!raw || g->w_normalized, "raw-weight gradients need weights set with normalize=1");
-  if (!g->w_items_csr_n && g->n) build_items(s, g->csr_ptr.p, g->n, g->w_items_csr_r0, g->w_items_csr_r1, &g->w_items_csr_n);
+  ensure_csr_items(s, g);
   const uint32_t n = g->n;
   const uint64_t nnz = g->nnz;
-  DevBuf<float> lambda(n);
+  WState& st = wstate(g);
+  st.lambda.ensure(n);
   // adjoint fixed point: lambda = g + d A^T lambda, started from g
-  fixed_point(ctx, g->csr_ptr.p, g->csr_idx.p, g->w_csr.p, g->w_items_csr_r0, g->w_items_csr_r1, g->w_items_csr_n, n,
-              gx_dev, gx_dev, false, damping, threshold, max_iters, lambda.p, info);
+  fixed_point(ctx, st, g->csr_ptr.p, g->csr_idx.p, g->w_csr.p, g->w_items_csr_r0, g->w_items_csr_r1,
+              g->w_items_csr_n, n, gx_dev, gx_dev, false, damping, threshold, max_iters, st.lambda.p, info);
   if (gw_dev && nnz) {
-    edge_grad_kernel<<<grid1(n), kThreads, 0, s>>>(n, g->csc_ptr.p, g->csc_idx.p, damping, lambda.p, x_dev, gw_dev);
+    if (!st.row_of.n) {
+      st.row_of.alloc(nnz);
+      row_of_kernel<<<grid1(nnz), kThreads, 0, s>>>(n, nnz, g->csc_ptr.p, st.row_of.p);
+      NWG_LAUNCH_CHECK();
+    }
+    edge_grad_kernel<<<grid1(nnz), kThreads, 0, s>>>(nnz, st.row_of.p, g->csc_idx.p, damping, st.lambda.p, x_dev,
+                                                     gw_dev);
     NWG_LAUNCH_CHECK();
     if (raw) {
-      DevBuf<float> t(n);
-      chain_sum_kernel<<<grid1(n), kThreads, 0, s>>>(n, g->csr_ptr.p, g->csr_to_csc.p, gw_dev, g->w_csc.p, t.p);
+      st.t.ensure(n);
+      if (g->w_items_csr_n) {
+        csr_rowsum_kernel<<<persistent_grid(g->w_items_csr_n), kThreads, 0, s>>>(
+            g->csr_ptr.p, g->csr_to_csc.p, gw_dev, g->w_csc.p, g->w_items_csr_r0.p, g->w_items_csr_r1.p,
+            g->w_items_csr_n, st.t.p);
+        NWG_LAUNCH_CHECK();
+      }
+      chain_apply_kernel<<<grid1(nnz), kThreads, 0, s>>>(nnz, g->csc_idx.p, st.t.p, g->w_rowsum.p, gw_dev);
       NWG_LAUNCH_CHECK();
-      chain_apply_kernel<<<grid1(nnz), kThreads, 0, s>>>(nnz, g->csc_idx.p, t.p, g->w_rowsum.p, gw_dev);
-      NWG_LAUNCH_CHECK();
-      NWG_HIP(hipStreamSynchronize(s));
     }
   }
   if (gv_dev) {
-    scale_kernel<<<grid1(n), kThreads, 0, s>>>(n, 1.0f - damping, lambda.p, gv_dev);
+    scale_kernel<<<grid1(n), kThreads, 0, s>>>(n, 1.0f - damping, st.lambda.p, gv_dev);
     NWG_LAUNCH_CHECK();
   }
   NWG_HIP(hipStreamSynchronize(s));
