@@ -184,7 +184,8 @@ def run_plain(args, ctx, grid=False):
         "unit": "GB/s",
         "frac": round(achieved / HBM_PEAK_GBS, 4),
         "traffic": args.traffic,
-        "kernel": "one sweep (pr_tile_kernel + pr_group_kernel<4..64> + pr_hubseg_kernel + pr_finish_kernel)",
+        "kernel": ("one sweep (pr_tile_kernel + pr_group_kernel<4..64> + pr_sgroup_kernel<4..64> + pr_hubseg_kernel"
+                   " + pr_finish_kernel, those the plan uses)"),
         "kernel_ms": round(sweep_ms, 5),
         "algorithmic_bytes": b_iter,
     }
@@ -233,9 +234,11 @@ def run_ppr(args, ctx):
     for j in range(min(k, 2)):
         g.pagerank_device(x1.data_ptr(), teles[j].data_ptr(), 0.85, args.tol, args.iters)
     t0 = time.perf_counter()
-    seq_its = []
+    seq_its, seq_dev = [], 0.0
     for j in range(k):
-        seq_its.append(g.pagerank_device(x1.data_ptr(), teles[j].data_ptr(), 0.85, args.tol, args.iters)["iterations"])
+        si = g.pagerank_device(x1.data_ptr(), teles[j].data_ptr(), 0.85, args.tol, args.iters)
+        seq_its.append(si["iterations"])
+        seq_dev += si["ms_total"]
     torch.cuda.synchronize()
     ms_seq = (time.perf_counter() - t0) * 1e3
     vec_sweeps = int(np.sum(its))
@@ -246,6 +249,7 @@ def run_ppr(args, ctx):
         "n": n, "nnz": nnz, "k": k, "k_tile": tile, "plan": plan,
         "vector_sweeps": vec_sweeps, "sequential_vector_sweeps": int(np.sum(seq_its)),
         "ms_batched": round(ms_batched, 3), "ms_64_sequential": round(ms_seq, 3),
+        "device_ms_batched": round(info["ms_total"], 3), "device_ms_64_sequential": round(seq_dev, 3),
         "speedup_vs_sequential": round(ms_seq / ms_batched, 3),
     })
     g.close()

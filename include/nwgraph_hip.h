@@ -65,8 +65,10 @@ typedef struct nwg_plan_params {
   uint32_t group_max;    /* rows with in-degree <= group_max: a 4/8/16/32-lane group per row */
   uint32_t hub_min;      /* rows with in-degree >= hub_min: split over several workgroups */
   uint32_t hub_segment;  /* nonzeros per workgroup segment of a hub row */
-  uint32_t lds_hub_prefix; /* contributions of vertices [0, lds_hub_prefix) staged in LDS (0 = off) */
-  uint32_t stream_hints; /* 1: non-temporal loads/stores for the streamed arrays (ids, pointers, scores) */
+  uint32_t slice_min;    /* rows with slice_min <= in-degree < hub_min split their in-edges by source
+                            slice ((id / 32) mod 8) so that each XCD gathers from one slice (0 = off;
+                            else above short_max); costs a second copy of the column ids */
+  uint32_t stream_hints; /* non-temporal hints: 0 none, 1 the column-id stream, 2 every streamed array */
   uint32_t ppr_tile;     /* k-tile of the batched solve: 8, 16, 32 or 64 vectors per lane group (0 = widest <= k) */
   uint32_t relabel;      /* 1: sweep a copy of the CSC renumbered by out-degree (results returned in original ids) */
 } nwg_plan_params;

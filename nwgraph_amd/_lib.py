@@ -51,13 +51,13 @@ class PlanParams(ctypes.Structure):
         ("group_max", c_u32),
         ("hub_min", c_u32),
         ("hub_segment", c_u32),
-        ("lds_hub_prefix", c_u32),
+        ("slice_min", c_u32),
         ("stream_hints", c_u32),
         ("ppr_tile", c_u32),
         ("relabel", c_u32),
     ]
 
-    FIELDS = ("short_max", "group_max", "hub_min", "hub_segment", "lds_hub_prefix", "stream_hints", "ppr_tile",
+    FIELDS = ("short_max", "group_max", "hub_min", "hub_segment", "slice_min", "stream_hints", "ppr_tile",
               "relabel")
 
     def as_dict(self):

@@ -131,10 +131,19 @@ void edge_weights_device(hipStream_t s, uint32_t n, const uint32_t* csc_ptr, con
 // ---------------------------------------------------------------------------
 namespace nwg {
 namespace {
+// four independent 16-byte loads in flight per lane, then four stores
 __global__ __launch_bounds__(256) void stream_copy_kernel(const uint4* __restrict__ src, uint4* __restrict__ dst,
                                                           uint64_t n16) {
   const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n16; i += stride) dst[i] = src[i];
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  for (; i + 3 * stride < n16; i += 4 * stride) {
+    const uint4 a = src[i], b = src[i + stride], c = src[i + 2 * stride], d = src[i + 3 * stride];
+    dst[i] = a;
+    dst[i + stride] = b;
+    dst[i + 2 * stride] = c;
+    dst[i + 3 * stride] = d;
+  }
+  for (; i < n16; i += stride) dst[i] = src[i];
 }
 }  // namespace
 
@@ -145,7 +154,7 @@ float stream_copy_device(hipStream_t s, hipEvent_t e0, hipEvent_t e1, void* dst,
   int dev = 0, cus = 0;
   NWG_HIP(hipGetDevice(&dev));
   NWG_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-  const uint32_t grid = (uint32_t)std::min<uint64_t>(ceil_div(n16 ? n16 : 1, 256), (uint64_t)cus * 16);
+  const uint32_t grid = (uint32_t)std::min<uint64_t>(ceil_div(n16 ? n16 : 1, 256), (uint64_t)cus * 8);
   NWG_HIP(hipEventRecord(e0, s));
   stream_copy_kernel<<<grid, 256, 0, s>>>(static_cast<const uint4*>(src), static_cast<uint4*>(dst), n16);
   NWG_LAUNCH_CHECK();

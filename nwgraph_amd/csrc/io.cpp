@@ -16,8 +16,8 @@ struct Edges {
   std::vector<uint32_t> s, d;
 };
 
-int export_edges(Edges& e, uint32_t* ne_n, uint64_t* ne, uint32_t** src, uint32_t** dst, uint64_t n) {
-  (void)ne_n;
+// hand the edges to the caller in malloc'ed arrays (freed with nwg_free)
+int export_edges(const Edges& e, uint64_t* ne, uint32_t** src, uint32_t** dst) {
   const size_t m = e.s.size();
   uint32_t* a = static_cast<uint32_t*>(std::malloc((m ? m : 1) * sizeof(uint32_t)));
   uint32_t* b = static_cast<uint32_t*>(std::malloc((m ? m : 1) * sizeof(uint32_t)));
@@ -33,7 +33,6 @@ int export_edges(Edges& e, uint32_t* ne_n, uint64_t* ne, uint32_t** src, uint32_
   *ne = m;
   *src = a;
   *dst = b;
-  (void)n;
   return NWG_OK;
 }
 
@@ -83,7 +82,7 @@ int nwg_read_mm(const char* path, uint32_t* n, uint64_t* ne, uint32_t** src, uin
   }
   if (N > 0xFFFFFFFFull) return NWG_EFORMAT;
   *n = (uint32_t)N;
-  return export_edges(e, nullptr, ne, src, dst, N);
+  return export_edges(e, ne, src, dst);
 }
 
 // NW binary edge list: char magic[27] "NW Graph index_edge_list", int directedness,
@@ -120,7 +119,7 @@ int nwg_read_nw(const char* path, uint32_t* n, uint64_t* ne, uint32_t** src, uin
     if (!in) return NWG_EFORMAT;
   }
   *n = (uint32_t)N;
-  return export_edges(e, nullptr, ne, src, dst, N);
+  return export_edges(e, ne, src, dst);
 }
 
 int nwg_write_nw(const char* path, uint32_t n, uint64_t ne, const uint32_t* src, const uint32_t* dst) {

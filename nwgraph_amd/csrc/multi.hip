@@ -141,7 +141,6 @@ struct nwg_multigraph {
   nwg::DevBuf<unsigned long long> acc;
   uint32_t n_items = 0;
   nwg::Captured cap;
-  float cap_damping = -1.f;
 };
 
 namespace nwg {

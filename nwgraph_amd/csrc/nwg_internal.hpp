@@ -81,6 +81,11 @@ struct DevBuf {
 //   d >= hub_min               hub: hub_segment nonzeros per workgroup, partials combined
 //                              in a second kernel in segment order
 constexpr int kGroupClasses = 5;  // 4, 8, 16, 32, 64 lanes
+constexpr uint32_t kSlices = 8;   // source slices of the sliced gather: one per XCD of gfx950
+// A source's slice is its 128-byte line of contributions (32 floats) mod kSlices, so that
+// every L2 line an XCD fetches for its slice holds only that slice's sources.
+constexpr int kSliceShift = 5;
+__host__ __device__ inline uint32_t slice_of(uint32_t u) { return (u >> kSliceShift) & (kSlices - 1); }
 struct Plan {
   nwg_plan_params params{};
   bool built = false;
@@ -92,17 +97,28 @@ struct Plan {
   DevBuf<uint32_t> hub_rows;     // ascending row ids
   DevBuf<uint32_t> hub_seg_ptr;  // n_hub_rows + 1: first segment of each hub row
   DevBuf<uint32_t> hub_seg_row;  // segment -> index into hub_rows
+  // Source-sliced rows (slice_min <= in-degree < hub_min): their in-edges are regrouped
+  // by source slice (slice_of) inside the row, in `scol` (a full copy of the column
+  // ids, other rows unchanged); sseg[k*(kSlices+1) + x] is where slice x of sliced row k
+  // starts.
+  uint32_t n_srows = 0;
+  uint32_t scls_off[kGroupClasses + 1] = {};  // sliced rows per lane class (by in-degree / kSlices)
+  DevBuf<uint32_t> srows;  // sliced row ids, ascending
+  DevBuf<uint32_t> sseg;   // n_srows x (kSlices + 1) segment starts
+  DevBuf<uint32_t> slist;  // indices into srows grouped by lane class
+  DevBuf<uint32_t> scol;
 };
 
 // Identity of a captured sweep loop: re-capture when any of it changes.
 struct CaptureKey {
-  const void* ptrs[8] = {};
+  static constexpr int kPtrs = 10;
+  const void* ptrs[kPtrs] = {};
   uint64_t plan_version = 0;
   uint64_t iters = 0;
   float damping = 0.f, threshold = 0.f;
   uint32_t extra = 0;
   bool operator==(const CaptureKey& o) const {
-    for (int i = 0; i < 8; ++i)
+    for (int i = 0; i < kPtrs; ++i)
       if (ptrs[i] != o.ptrs[i]) return false;
     return plan_version == o.plan_version && iters == o.iters && damping == o.damping &&
            threshold == o.threshold && extra == o.extra;
@@ -152,7 +168,21 @@ struct PrState {
   DevBuf<unsigned long long> acc;  // per-sweep L1 change, fixed point (see device_utils.hpp)
   DevBuf<float> hub_part;          // per hub segment partial sums
   DevBuf<double> blk_err;          // per-workgroup L1 partials of one sweep
+  DevBuf<float> spart;             // kSlices x n_srows partial sums of the sliced rows
   Captured cap;
+};
+
+// Reusable state of the batched personalised solve: the heavy-row segments of the active
+// CSC (rebuilt when the plan or the k-tile changes) and grow-only buffers.
+struct PprState {
+  uint64_t seg_version = 0;  // plan_version the segments were built for (0 = none)
+  uint32_t seg_tile = 0;
+  uint32_t n_heavy = 0, n_segs = 0;
+  DevBuf<uint32_t> heavy_rows, heavy_seg_ptr, heavy_seg_row;
+  DevBuf<uint32_t> degflag, seed_row, seed_vec, ids, ids_new;
+  DevBuf<float> seed_base, c0, c1, x, hub_part;
+  DevBuf<double> blk_err;
+  DevBuf<unsigned long long> acc;
 };
 
 }  // namespace nwg
@@ -192,6 +222,7 @@ struct nwg_graph {
   }
   std::unique_ptr<nwg::PrState> pr;
   std::unique_ptr<nwg::WState> wst;
+  std::unique_ptr<nwg::PprState> ppr;
 };
 
 namespace nwg {

@@ -8,13 +8,16 @@
 //   err += |x_i' - x_i|  (float difference, double sum) (page_rank.hpp:134)
 //   contrib'_i = x_i' / (float)outdeg_i                 (page_rank.hpp:116,135)
 // contrib is double-buffered (read contrib[k&1], write contrib[(k+1)&1]); x is
-// updated in place (each row owns its entry).  One sweep is one launch of
-// pr_sweep_kernel, whose workgroups are of four kinds (degree-binned schedule):
-//   tiles      256 consecutive rows, one lane per row with in-degree <= short_max
-//   groups     rows listed per class, 4/8/16/32/64 lanes per row, shuffle reduction
-//   hub segs   hub_segment nonzeros of one hub row per workgroup -> partial sum
-// followed by pr_finish_kernel, which adds each hub row's segment partials in order,
-// updates the hub rows and folds the per-workgroup L1 partials.
+// updated in place (each row owns its entry).  One sweep is a short sequence of
+// launches, one per row class (degree-binned schedule):
+//   pr_tile_kernel    256 consecutive rows, one lane per row with in-degree <= short_max
+//   pr_group_kernel   rows listed per class, 4/8/16/32/64 lanes per row, butterfly reduction
+//   pr_sgroup_kernel  source-sliced rows (slice_min <= d < hub_min): workgroup b gathers
+//                     only sources u with (u / 32) % 8 == b % 8 -> one partial per (row, slice)
+//   pr_hubseg_kernel  hub_segment nonzeros of one hub row per workgroup -> partial sum
+//   pr_finish_kernel  adds each hub row's segment partials and each sliced row's slice
+//                     partials in order, updates those rows and folds the per-workgroup
+//                     L1 partials
 //
 // Convergence stays on the device: the L1 change of sweep k is accumulated in
 // fixed point in acc[k] (integer atomics: independent of block order); every
@@ -61,6 +64,13 @@ struct SweepParams {
   const uint32_t* seg_row;
   float* hub_part;
   double* blk_err;  // L1 partial of every tile / group workgroup of the sweep
+  // source-sliced rows
+  const uint32_t* srows;
+  const uint32_t* sseg;
+  const uint32_t* slist;
+  uint32_t scls_off[kGroupClasses + 1];
+  uint32_t n_srows;
+  float* spart;  // kSlices x n_srows
   unsigned long long* acc;
   uint32_t iter;
   double threshold;
@@ -68,26 +78,55 @@ struct SweepParams {
 
 // entries of blk_err reduced by one workgroup of the finishing kernel
 constexpr uint32_t kFinishChunk = 2048;
+// gathers in flight per lane: group, hub and sliced rows / one-lane tile rows
+constexpr uint32_t kUnroll = 4;
+constexpr uint32_t kTileUnroll = 4;
 
 // Streamed arrays (ids, row pointers, degrees, scores) may be read and written with
 // non-temporal hints so that they do not displace the gathered contributions from
 // L2 / MALL; the gathers themselves always use ordinary cached loads.
-template <bool NT, class T>
+// NT is the hint level: 0 none, 1 the column-id stream only, 2 every streamed array
+// (loads and stores).
+template <int NT, class T>
 __device__ __forceinline__ T ld(const T* p) {
-  if constexpr (NT) return __builtin_nontemporal_load(p);
+  if constexpr (NT >= 2) return __builtin_nontemporal_load(p);
   else return *p;
 }
-template <bool NT, class T>
+template <int NT, class T>
+__device__ __forceinline__ T ld_col(const T* p) {
+  if constexpr (NT >= 1) return __builtin_nontemporal_load(p);
+  else return *p;
+}
+template <int NT, class T>
 __device__ __forceinline__ void st(T* p, T v) {
-  if constexpr (NT) __builtin_nontemporal_store(v, p);
+  if constexpr (NT >= 2) __builtin_nontemporal_store(v, p);
   else *p = v;
+}
+
+// z + sum of cin[col[q]] over q = p, p + S, p + 2S, ... < p1, added in that order (one
+// sequential sum), with kU gathers issued before they are added: a lane keeps kU
+// misses in flight instead of one.
+template <int NT, uint32_t S, uint32_t kU>
+__device__ __forceinline__ float gather_sum(const SweepParams& P, uint32_t p0, uint32_t p1, float z) {
+  for (uint64_t p = p0; p < p1; p += S * kU) {  // 64-bit: no wrap near 2^32 nonzeros
+    float v[kU];
+#pragma unroll
+    for (uint32_t r = 0; r < kU; ++r) {
+      const uint64_t q = p + r * S;
+      v[r] = q < p1 ? P.cin[ld_col<NT>(P.col + q)] : 0.f;
+    }
+#pragma unroll
+    for (uint32_t r = 0; r < kU; ++r)
+      if (p + r * S < p1) z += v[r];
+  }
+  return z;
 }
 
 __device__ __forceinline__ bool converged_before(const SweepParams& P) {
   return P.iter > 0 && fixed_to_err(P.acc[P.iter - 1]) < P.threshold;
 }
 
-template <bool NT>
+template <int NT>
 __device__ __forceinline__ double update_row(const SweepParams& P, uint32_t i, float z) {
   const float b = P.basev ? ld<NT>(P.basev + i) : P.base;
   const float xo = ld<NT>(P.x + i);
@@ -99,7 +138,7 @@ __device__ __forceinline__ double update_row(const SweepParams& P, uint32_t i, f
 
 // Tiles: 256 consecutive rows, one lane per row with in-degree <= short_max; the lane
 // adds its row's contributions in row order.
-template <bool NT>
+template <int NT>
 __global__ __launch_bounds__(kThreads) void pr_tile_kernel(SweepParams P) {
   if (converged_before(P)) return;
   __shared__ double red_d[kThreads / 64];
@@ -108,9 +147,7 @@ __global__ __launch_bounds__(kThreads) void pr_tile_kernel(SweepParams P) {
   if (i < P.n) {
     const uint32_t p0 = ld<NT>(P.rp + i), p1 = ld<NT>(P.rp + i + 1);
     if (p1 - p0 <= P.short_max) {
-      float z = 0.f;
-      for (uint32_t p = p0; p < p1; ++p) z += P.cin[ld<NT>(P.col + p)];
-      e = update_row<NT>(P, i, z);
+      e = update_row<NT>(P, i, gather_sum<NT, 1, kTileUnroll>(P, p0, p1, 0.f));
     }
   }
   e = block_sum<kThreads>(e, red_d);
@@ -118,7 +155,7 @@ __global__ __launch_bounds__(kThreads) void pr_tile_kernel(SweepParams P) {
 }
 
 // Group classes: G lanes per listed row stride over the row, butterfly reduction.
-template <int G, bool NT>
+template <int G, int NT>
 __global__ __launch_bounds__(kThreads) void pr_group_kernel(SweepParams P, uint32_t cls) {
   if (converged_before(P)) return;
   __shared__ double red_d[kThreads / 64];
@@ -131,8 +168,7 @@ __global__ __launch_bounds__(kThreads) void pr_group_kernel(SweepParams P, uint3
   uint32_t i = 0;
   if (active) {
     i = P.group_rows[P.cls_off[cls] + slot];
-    const uint32_t p1 = ld<NT>(P.rp + i + 1);
-    for (uint32_t p = ld<NT>(P.rp + i) + gl; p < p1; p += G) z += P.cin[ld<NT>(P.col + p)];
+    z = gather_sum<NT, G, kUnroll>(P, ld<NT>(P.rp + i) + gl, ld<NT>(P.rp + i + 1), z);
   }
   z = group_sum<G>(z);
   double e = (active && gl == 0) ? update_row<NT>(P, i, z) : 0.0;
@@ -141,7 +177,7 @@ __global__ __launch_bounds__(kThreads) void pr_group_kernel(SweepParams P, uint3
 }
 
 // Hub segments: hub_segment nonzeros of one hub row per workgroup -> partial sum.
-template <bool NT>
+template <int NT>
 __global__ __launch_bounds__(kThreads) void pr_hubseg_kernel(SweepParams P) {
   if (converged_before(P)) return;
   __shared__ float red_f[kThreads / 64];
@@ -151,16 +187,39 @@ __global__ __launch_bounds__(kThreads) void pr_hubseg_kernel(SweepParams P) {
   const uint32_t p0 = P.rp[i] + (s - P.seg_ptr[h]) * P.hub_segment;
   const uint32_t pe = P.rp[i + 1];
   const uint32_t p1 = (pe - p0 > P.hub_segment) ? p0 + P.hub_segment : pe;
-  float z = 0.f;
-  for (uint32_t p = p0 + threadIdx.x; p < p1; p += kThreads) z += P.cin[ld<NT>(P.col + p)];
+  float z = gather_sum<NT, kThreads, kUnroll>(P, p0 + threadIdx.x, p1, 0.f);
   z = block_sum<kThreads>(z, red_f);
   if (threadIdx.x == 0) P.hub_part[s] = z;
 }
 
+// Source-sliced rows of lane class `cls`: workgroup b gathers only from source slice
+// b % kSlices.  Workgroups are dispatched to the XCDs round-robin, so every slice stays
+// on one XCD and that XCD's L2 caches the contributions of its own slice only: the eight
+// L2s hold eight different parts of the hot set instead of eight copies of one.
+template <int G, int NT>
+__global__ __launch_bounds__(kThreads) void pr_sgroup_kernel(SweepParams P, uint32_t cls) {
+  if (converged_before(P)) return;
+  constexpr uint32_t kRowsPerBlock = kThreads / G;
+  const uint32_t x = blockIdx.x % kSlices, lb = blockIdx.x / kSlices;
+  const uint32_t slot = lb * kRowsPerBlock + threadIdx.x / G;
+  const uint32_t gl = threadIdx.x % G;
+  const bool active = slot < P.scls_off[cls + 1] - P.scls_off[cls];
+  float z = 0.f;
+  uint32_t k = 0;
+  if (active) {
+    k = P.slist[P.scls_off[cls] + slot];
+    const uint32_t* sg = P.sseg + (uint64_t)k * (kSlices + 1) + x;
+    z = gather_sum<NT, G, kUnroll>(P, sg[0] + gl, sg[1], z);
+  }
+  z = group_sum<G>(z);
+  if (active && gl == 0) P.spart[(uint64_t)x * P.n_srows + k] = z;
+}
+
 // Last launch of a sweep: finishes hub rows (segment partials added in segment order)
-// and folds the workgroup L1 partials of the sweep into acc[iter].  Only this kernel's
-// few workgroups touch acc, so the atomics never queue up on one address.
-template <bool NT>
+// and source-sliced rows (slice partials added in slice order), and folds the workgroup
+// L1 partials of the sweep into acc[iter].  Only this kernel's few workgroups touch acc,
+// so the atomics never queue up on one address.
+template <int NT>
 __global__ __launch_bounds__(kThreads) void pr_finish_kernel(SweepParams P) {
   if (converged_before(P)) return;
   __shared__ double red_d[kThreads / 64];
@@ -171,6 +230,12 @@ __global__ __launch_bounds__(kThreads) void pr_finish_kernel(SweepParams P) {
     const uint32_t s1 = P.seg_ptr[h + 1];
     for (uint32_t s = P.seg_ptr[h]; s < s1; ++s) z += P.hub_part[s];
     e = update_row<NT>(P, P.hub_rows[h], z);
+  } else if (h < P.n_hub_rows + P.n_srows) {
+    const uint32_t k = h - P.n_hub_rows;
+    float z = 0.f;
+#pragma unroll
+    for (uint32_t x = 0; x < kSlices; ++x) z += P.spart[(uint64_t)x * P.n_srows + k];
+    e = update_row<NT>(P, P.srows[k], z);
   }
   const uint32_t c0 = blockIdx.x * kFinishChunk;
   for (uint32_t q = c0 + threadIdx.x; q < c0 + kFinishChunk && q < P.hub_blk; q += kThreads) e += P.blk_err[q];
@@ -209,12 +274,14 @@ __global__ __launch_bounds__(kThreads) void pr_unpermute_kernel(uint32_t n, cons
 // ---------------------------------------------------------------------------
 // plan construction
 // ---------------------------------------------------------------------------
-// class id of a row: 0 tile, 1..5 group classes (4..64 lanes), 6 hub
-__device__ __forceinline__ uint32_t row_class(uint32_t d, const nwg_plan_params& pp) {
+// class id of a row: 0 tile, 1..5 group classes (4..64 lanes), 6 hub, 7 source-sliced
+// (slice_min <= d < hub_min; slice_min > short_max is a plan invariant)
+__host__ __device__ __forceinline__ uint32_t row_class(uint32_t d, const nwg_plan_params& pp) {
   if (d <= pp.short_max) return 0;
+  if (d >= pp.hub_min) return 6;
+  if (pp.slice_min && d >= pp.slice_min) return 7;
   if (d <= pp.group_max) return 1u + (d > 16) + (d > 32) + (d > 64);
-  if (d < pp.hub_min) return 5;
-  return 6;
+  return 5;
 }
 
 __global__ __launch_bounds__(kThreads) void class_flag_kernel(uint32_t n, const uint32_t* __restrict__ rp,
@@ -247,6 +314,65 @@ __global__ __launch_bounds__(kThreads) void hub_segrow_kernel(uint32_t nh, const
   const uint32_t h = blockIdx.x * kThreads + threadIdx.x;
   if (h < nh)
     for (uint32_t s = seg_ptr[h]; s < seg_ptr[h + 1]; ++s) seg_row[s] = h;
+}
+
+// lane class of a sliced row by its mean slice length (4, 8, 16, 32 or 64 lanes)
+__global__ __launch_bounds__(kThreads) void slabel_kernel(uint32_t ns, const uint32_t* __restrict__ srows,
+                                                          const uint32_t* __restrict__ rp,
+                                                          uint32_t* __restrict__ label) {
+  const uint32_t k = blockIdx.x * kThreads + threadIdx.x;
+  if (k >= ns) return;
+  const uint32_t i = srows[k];
+  const uint32_t len = (rp[i + 1] - rp[i] + kSlices - 1) / kSlices;
+  label[k] = (len > 16) + (len > 32) + (len > 64) + (len > 128);
+}
+
+__global__ __launch_bounds__(kThreads) void label_flag_kernel(uint32_t m, const uint32_t* __restrict__ label,
+                                                              uint32_t c, uint32_t* __restrict__ flag) {
+  const uint32_t k = blockIdx.x * kThreads + threadIdx.x;
+  if (k < m) flag[k] = label[k] == c ? 1u : 0u;
+}
+
+// Sort key of CSC position p: its row, then (sliced rows only) the slice of its source.
+// A stable sort of these keys carrying the column id keeps every row's sources
+// ascending inside each slice and leaves the other rows as they are.
+__global__ __launch_bounds__(kThreads) void slice_key_kernel(uint32_t n, uint64_t nnz, const uint32_t* __restrict__ rp,
+                                                             const uint32_t* __restrict__ col, nwg_plan_params pp,
+                                                             uint64_t* __restrict__ keys, uint32_t* __restrict__ vals) {
+  const uint64_t p = (uint64_t)blockIdx.x * kThreads + threadIdx.x;
+  if (p >= nnz) return;
+  uint32_t lo = 0, hi = n;  // row of p: first r with rp[r + 1] > p
+  while (lo < hi) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (rp[mid + 1] <= p) lo = mid + 1;
+    else hi = mid;
+  }
+  const uint32_t u = col[p];
+  const uint32_t x = row_class(rp[lo + 1] - rp[lo], pp) == 7 ? slice_of(u) : 0u;
+  keys[p] = ((uint64_t)lo << 3) | x;
+  vals[p] = u;
+}
+
+// sseg[k][x]: first position of slice x in sliced row k (binary search on the slice field)
+__global__ __launch_bounds__(kThreads) void sseg_kernel(uint32_t ns, const uint32_t* __restrict__ srows,
+                                                        const uint32_t* __restrict__ rp,
+                                                        const uint32_t* __restrict__ scol,
+                                                        uint32_t* __restrict__ sseg) {
+  const uint64_t t = (uint64_t)blockIdx.x * kThreads + threadIdx.x;
+  if (t >= (uint64_t)ns * (kSlices + 1)) return;
+  const uint32_t k = (uint32_t)(t / (kSlices + 1)), x = (uint32_t)(t % (kSlices + 1));
+  const uint32_t i = srows[k];
+  uint32_t lo = rp[i], hi = rp[i + 1];
+  if (x == kSlices) {
+    lo = hi;
+  } else {
+    while (lo < hi) {
+      const uint32_t mid = (lo + hi) >> 1;
+      if (slice_of(scol[mid]) < x) lo = mid + 1;
+      else hi = mid;
+    }
+  }
+  sseg[t] = lo;
 }
 
 inline uint32_t blocks_for(uint64_t n) { return (uint32_t)ceil_div(n ? n : 1, kThreads); }
@@ -312,7 +438,7 @@ nwg_plan_params default_plan_params() {
   p.group_max = 128;
   p.hub_min = 16384;
   p.hub_segment = 4096;
-  p.lds_hub_prefix = 0;
+  p.slice_min = 0;
   return p;
 }
 
@@ -320,11 +446,12 @@ static void validate_params(const nwg_plan_params& p) {
   NWG_REQUIRE(p.short_max <= p.group_max, "plan: short_max must not exceed group_max");
   NWG_REQUIRE(p.group_max < p.hub_min, "plan: group_max must be below hub_min");
   NWG_REQUIRE(p.hub_segment >= 256 && p.hub_segment % 256 == 0, "plan: hub_segment must be a multiple of 256");
-  NWG_REQUIRE(p.lds_hub_prefix == 0, "plan: lds_hub_prefix is not supported by this sweep");
-  NWG_REQUIRE(p.stream_hints <= 1, "plan: stream_hints must be 0 or 1");
+  NWG_REQUIRE(p.stream_hints <= 2, "plan: stream_hints must be 0, 1 or 2");
   NWG_REQUIRE(p.ppr_tile == 0 || p.ppr_tile == 8 || p.ppr_tile == 16 || p.ppr_tile == 32 || p.ppr_tile == 64,
               "plan: ppr_tile must be 0, 8, 16, 32 or 64");
   NWG_REQUIRE(p.relabel <= 1, "plan: relabel must be 0 or 1");
+  NWG_REQUIRE(p.slice_min == 0 || (p.slice_min >= 2 && p.slice_min > p.short_max),
+              "plan: slice_min must be 0 or above both 1 and short_max");
 }
 
 void plan_build(hipStream_t s, nwg_graph* g, const nwg_plan_params& pp) {
@@ -365,6 +492,43 @@ void plan_build(hipStream_t s, nwg_graph* g, const nwg_plan_params& pp) {
   } else {
     pl.hub_seg_row.alloc(1);
   }
+  const uint32_t ns = pp.slice_min ? select_class(s, n, rp, pp, 7, flag, pos, nullptr) : 0;
+  pl.n_srows = ns;
+  pl.srows.alloc(ns ? ns : 1);
+  pl.sseg.alloc((uint64_t)(ns ? ns : 1) * (kSlices + 1));
+  pl.slist.alloc(ns ? ns : 1);
+  if (ns) {
+    select_class(s, n, rp, pp, 7, flag, pos, pl.srows.p);
+    const ActiveCsc a = g->active(pp.relabel != 0);
+    const uint64_t nnz = g->nnz;
+    pl.scol.alloc(nnz);
+    {
+      DevBuf<uint64_t> keys(nnz), keys_tmp(nnz);
+      DevBuf<uint32_t> vals_tmp(nnz);
+      slice_key_kernel<<<blocks_for(nnz), kThreads, 0, s>>>(n, nnz, rp, a.idx, pp, keys.p, pl.scol.p);
+      NWG_LAUNCH_CHECK();
+      radix_sort_pairs(s, keys.p, keys_tmp.p, pl.scol.p, vals_tmp.p, nnz, key_bits_for(n) + 3);
+    }
+    sseg_kernel<<<blocks_for((uint64_t)ns * (kSlices + 1)), kThreads, 0, s>>>(ns, pl.srows.p, rp, pl.scol.p,
+                                                                              pl.sseg.p);
+    NWG_LAUNCH_CHECK();
+    DevBuf<uint32_t> label(ns);
+    slabel_kernel<<<blocks_for(ns), kThreads, 0, s>>>(ns, pl.srows.p, rp, label.p);
+    NWG_LAUNCH_CHECK();
+    for (uint32_t c = 0; c < (uint32_t)kGroupClasses; ++c) {
+      label_flag_kernel<<<blocks_for(ns), kThreads, 0, s>>>(ns, label.p, c, flag.p);
+      NWG_LAUNCH_CHECK();
+      exclusive_scan_u32(s, flag.p, pos.p, ns);
+      const uint32_t cnt = d2h_u32(s, pos.p + ns);
+      pl.scls_off[c + 1] = pl.scls_off[c] + cnt;
+      if (cnt) {
+        class_scatter_kernel<<<blocks_for(ns), kThreads, 0, s>>>(ns, flag.p, pos.p, pl.slist.p + pl.scls_off[c]);
+        NWG_LAUNCH_CHECK();
+      }
+    }
+  } else {
+    pl.scol.release();
+  }
   pl.n_tiles = (uint32_t)ceil_div(n, kThreads);
   NWG_HIP(hipStreamSynchronize(s));
   pl.built = true;
@@ -382,7 +546,7 @@ static uint64_t sweep_workgroups_before_hubs(const Plan& pl) {
 }
 
 static uint32_t finish_blocks(const SweepParams& P) {
-  const uint64_t a = ceil_div(P.n_hub_rows, kThreads), b = ceil_div(P.hub_blk, kFinishChunk);
+  const uint64_t a = ceil_div((uint64_t)P.n_hub_rows + P.n_srows, kThreads), b = ceil_div(P.hub_blk, kFinishChunk);
   return (uint32_t)std::max<uint64_t>(std::max(a, b), 1);
 }
 
@@ -391,7 +555,7 @@ static SweepParams make_params(nwg_graph* g, PrState& st, bool personalised, flo
   SweepParams P{};
   const ActiveCsc a = g->active(pl.params.relabel != 0);
   P.rp = a.ptr;
-  P.col = a.idx;
+  P.col = pl.n_srows ? pl.scol.p : a.idx;  // scol differs from the CSC only inside sliced rows
   P.deg = a.deg;
   P.x = st.x.p;
   P.basev = personalised ? st.base.p : nullptr;
@@ -417,6 +581,12 @@ static SweepParams make_params(nwg_graph* g, PrState& st, bool personalised, flo
   P.seg_row = pl.hub_seg_row.p;
   P.hub_part = st.hub_part.p;
   P.blk_err = st.blk_err.p;
+  P.srows = pl.srows.p;
+  P.sseg = pl.sseg.p;
+  P.slist = pl.slist.p;
+  for (int c = 0; c <= kGroupClasses; ++c) P.scls_off[c] = pl.scls_off[c];
+  P.n_srows = pl.n_srows;
+  P.spart = st.spart.p;
   P.acc = st.acc.p;
   P.threshold = (double)threshold;  // page_rank.hpp:144 compares double error with (double)threshold
   return P;
@@ -425,7 +595,7 @@ static SweepParams make_params(nwg_graph* g, PrState& st, bool personalised, flo
 // Enqueue init + max_iters sweeps on stream s.  With ev (2*iters events) every sweep
 // kernel is bracketed by an event pair.
 // one sweep: tiles, each non-empty group class, hub segments, finish
-template <bool NT>
+template <int NT>
 static void enqueue_sweep(hipStream_t s, const SweepParams& P) {
   if (P.n_tiles) pr_tile_kernel<NT><<<P.n_tiles, kThreads, 0, s>>>(P);
   for (uint32_t c = 0; c < (uint32_t)kGroupClasses; ++c) {
@@ -437,6 +607,18 @@ static void enqueue_sweep(hipStream_t s, const SweepParams& P) {
       case 2: pr_group_kernel<16, NT><<<nb, kThreads, 0, s>>>(P, c); break;
       case 3: pr_group_kernel<32, NT><<<nb, kThreads, 0, s>>>(P, c); break;
       default: pr_group_kernel<64, NT><<<nb, kThreads, 0, s>>>(P, c); break;
+    }
+  }
+  for (uint32_t c = 0; c < (uint32_t)kGroupClasses; ++c) {
+    const uint32_t cnt = P.scls_off[c + 1] - P.scls_off[c];
+    if (!cnt) continue;
+    const uint32_t nb = kSlices * (uint32_t)ceil_div(cnt, kThreads / kLanes[c]);
+    switch (c) {
+      case 0: pr_sgroup_kernel<4, NT><<<nb, kThreads, 0, s>>>(P, c); break;
+      case 1: pr_sgroup_kernel<8, NT><<<nb, kThreads, 0, s>>>(P, c); break;
+      case 2: pr_sgroup_kernel<16, NT><<<nb, kThreads, 0, s>>>(P, c); break;
+      case 3: pr_sgroup_kernel<32, NT><<<nb, kThreads, 0, s>>>(P, c); break;
+      default: pr_sgroup_kernel<64, NT><<<nb, kThreads, 0, s>>>(P, c); break;
     }
   }
   if (P.n_hub_segs) pr_hubseg_kernel<NT><<<P.n_hub_segs, kThreads, 0, s>>>(P);
@@ -451,14 +633,15 @@ static void enqueue_solve(hipStream_t s, nwg_graph* g, PrState& st, SweepParams 
   pr_init_kernel<<<blocks_for(std::max<uint64_t>(n, iters)), kThreads, 0, s>>>(n, P.deg, init, st.x.p,
                                                                                st.contrib0.p, st.acc.p, iters);
   NWG_LAUNCH_CHECK();
-  const bool nt = g->plan.params.stream_hints != 0;
+  const uint32_t nt = g->plan.params.stream_hints;
   for (uint64_t k = 0; k < iters; ++k) {
     P.iter = (uint32_t)k;
     P.cin = (k & 1) ? st.contrib1.p : st.contrib0.p;
     P.cout = (k & 1) ? st.contrib0.p : st.contrib1.p;
     if (ev) NWG_HIP(hipEventRecord(ev[2 * k], s));
-    if (nt) enqueue_sweep<true>(s, P);
-    else enqueue_sweep<false>(s, P);
+    if (nt == 2) enqueue_sweep<2>(s, P);
+    else if (nt == 1) enqueue_sweep<1>(s, P);
+    else enqueue_sweep<0>(s, P);
     if (ev) NWG_HIP(hipEventRecord(ev[2 * k + 1], s));
   }
 }
@@ -477,6 +660,7 @@ void pagerank_run(nwg_ctx* ctx, nwg_graph* g, const float* teleport_dev, float d
   st.acc.ensure(std::max<uint64_t>(max_iters, 1));
   st.hub_part.ensure(std::max<uint32_t>(g->plan.n_hub_segs, 1));
   st.blk_err.ensure(std::max<uint64_t>(sweep_workgroups_before_hubs(g->plan), 1));
+  st.spart.ensure(std::max<uint64_t>((uint64_t)kSlices * g->plan.n_srows, 1));
   const bool personalised = teleport_dev != nullptr;
   if (personalised) {
     st.base.ensure(n);
@@ -499,6 +683,7 @@ void pagerank_run(nwg_ctx* ctx, nwg_graph* g, const float* teleport_dev, float d
     key.ptrs[5] = personalised ? st.base.p : nullptr;
     key.ptrs[6] = P.rp;
     key.ptrs[7] = st.blk_err.p;
+    key.ptrs[8] = st.spart.p;
     key.plan_version = g->plan_version;
     key.iters = max_iters;
     key.damping = damping;

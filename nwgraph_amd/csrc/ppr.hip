@@ -32,6 +32,8 @@ namespace {
 
 constexpr int kThreads = 256;
 constexpr uint32_t kSeedFlag = 0x80000000u;
+// gathers a lane issues before it adds them (in edge order): memory-level parallelism
+constexpr uint32_t kUnroll = 8;
 
 struct PprParams {
   const uint32_t* rp;
@@ -123,13 +125,31 @@ __global__ __launch_bounds__(kThreads) void ppr_sweep_kernel(PprParams P) {
     constexpr uint32_t kSlots = kThreads / T;
     const uint32_t i = b * kSlots + threadIdx.x / T;
     double e = 0.0;
-    if (i < P.n && live) {
-      const uint32_t p0 = P.rp[i], p1 = P.rp[i + 1];
-      if (p1 - p0 <= P.light_max) {
-        float z = 0.f;
-        for (uint32_t p = p0; p < p1; ++p) z += P.cin[(uint64_t)P.col[p] * P.k + j];
-        e = ppr_update(P, i, j, z);
+    uint32_t p0 = 0, p1 = 0;
+    if (i < P.n) {
+      p0 = P.rp[i];
+      p1 = P.rp[i + 1];
+    }
+    // The row test is uniform over the T-lane group, so every lane of the group takes
+    // part in the shuffles below (converged vectors only skip their gathers).
+    if (i < P.n && p1 - p0 <= P.light_max) {
+      float z = 0.f;
+      for (uint32_t p = p0; p < p1; p += T) {
+        const uint32_t m = min((uint32_t)T, p1 - p);
+        const uint32_t c = vl < m ? P.col[p + vl] : 0u;  // up to T ids in one coalesced load
+        for (uint32_t q = 0; q < m; q += kUnroll) {    // q + r < T: q is a multiple of 8 below m <= T
+          float v[kUnroll];
+#pragma unroll
+          for (uint32_t r = 0; r < kUnroll; ++r) {
+            const uint32_t u = (uint32_t)__shfl((int)c, (int)(q + r), T);
+            v[r] = (live && q + r < m) ? P.cin[(uint64_t)u * P.k + j] : 0.f;
+          }
+#pragma unroll
+          for (uint32_t r = 0; r < kUnroll; ++r)
+            if (q + r < m) z += v[r];  // in edge order, as one sequential sum
+        }
       }
+      if (live) e = ppr_update(P, i, j, z);
     }
     e = slot_reduce<T>(e, red);
     if (threadIdx.x < (uint32_t)T) P.blk_err[((uint64_t)tile * P.row_blocks + b) * T + threadIdx.x] = e;
@@ -143,8 +163,20 @@ __global__ __launch_bounds__(kThreads) void ppr_sweep_kernel(PprParams P) {
     const uint32_t p1 = (pe - p0 > P.hub_segment) ? p0 + P.hub_segment : pe;
     constexpr uint32_t kGroups = kThreads / T;
     float z = 0.f;
-    if (live)
-      for (uint32_t p = p0 + threadIdx.x / T; p < p1; p += kGroups) z += P.cin[(uint64_t)P.col[p] * P.k + j];
+    if (live) {
+      // edge group g takes p0 + g, p0 + g + kGroups, ...: kUnroll of them in flight at once
+      for (uint64_t p = p0 + threadIdx.x / T; p < p1; p += kGroups * kUnroll) {  // 64-bit: no wrap
+        float v[kUnroll];
+#pragma unroll
+        for (uint32_t r = 0; r < kUnroll; ++r) {
+          const uint64_t q = p + r * kGroups;
+          v[r] = q < p1 ? P.cin[(uint64_t)P.col[q] * P.k + j] : 0.f;
+        }
+#pragma unroll
+        for (uint32_t r = 0; r < kUnroll; ++r)
+          if (p + r * kGroups < p1) z += v[r];
+      }
+    }
     hub_red[threadIdx.x] = z;
     __syncthreads();
     if (threadIdx.x < (uint32_t)T) {
@@ -261,13 +293,21 @@ void ppr_run(nwg_ctx* ctx, nwg_graph* g, uint32_t k, const std::vector<uint32_t>
   if (!g->plan.built) plan_build(s, g, default_plan_params());
   const Plan& pl = g->plan;
   const ActiveCsc a = g->active(pl.params.relabel != 0);
-  // Heavy rows (in-degree > light_max) are split into segments of kSegIters edges per
-  // lane group so that no lane group walks more than ~32 dependent gathers of a row.
+  if (!g->ppr) g->ppr.reset(new PprState());
+  PprState& st = *g->ppr;
+  // Heavy rows (in-degree > light_max) are split into segments of seg_edges edges so
+  // that no lane group walks more than 32 edges of a row; rebuilt only when the plan
+  // (hence the active CSC) or the k-tile changes.
   const uint32_t light_max = 64;
   const uint32_t seg_edges = (kThreads / tile) * 32;
-  DevBuf<uint32_t> heavy_rows, heavy_seg_ptr, heavy_seg_row;
-  uint32_t n_heavy = 0, n_segs = 0;
-  build_segments(s, a.ptr, n, light_max + 1, seg_edges, heavy_rows, heavy_seg_ptr, heavy_seg_row, &n_heavy, &n_segs);
+  if (st.seg_version != g->plan_version || st.seg_tile != tile) {
+    st.seg_version = 0;
+    build_segments(s, a.ptr, n, light_max + 1, seg_edges, st.heavy_rows, st.heavy_seg_ptr, st.heavy_seg_row,
+                   &st.n_heavy, &st.n_segs);
+    st.seg_version = g->plan_version;
+    st.seg_tile = tile;
+  }
+  const uint32_t n_heavy = st.n_heavy, n_segs = st.n_segs;
   // seed entries sorted by (row, vector); rows in the active numbering
   std::vector<std::pair<uint32_t, uint32_t>> ent;
   std::vector<float> ebase;
@@ -281,55 +321,59 @@ void ppr_run(nwg_ctx* ctx, nwg_graph* g, uint32_t k, const std::vector<uint32_t>
   }
   std::sort(ent.begin(), ent.end());
   for (size_t q = 1; q < ent.size(); ++q) NWG_REQUIRE(ent[q] != ent[q - 1], "duplicate seed in a teleport vector");
+  const uint32_t ns = (uint32_t)ent.size();
   if (a.order) {  // map seed ids through rank (original -> relabelled) and re-sort
-    const uint32_t m = (uint32_t)ent.size();
-    std::vector<uint32_t> ids(m);
-    for (uint32_t q = 0; q < m; ++q) ids[q] = ent[q].first;
-    DevBuf<uint32_t> d_ids(m), d_new(m);
-    NWG_HIP(hipMemcpyAsync(d_ids.p, ids.data(), m * 4ull, hipMemcpyHostToDevice, s));
-    gather_u32_kernel<<<grid1(m), kThreads, 0, s>>>(m, d_ids.p, g->rl->rank.p, d_new.p);
+    std::vector<uint32_t> ids(ns);
+    for (uint32_t q = 0; q < ns; ++q) ids[q] = ent[q].first;
+    st.ids.ensure(ns);
+    st.ids_new.ensure(ns);
+    NWG_HIP(hipMemcpyAsync(st.ids.p, ids.data(), ns * 4ull, hipMemcpyHostToDevice, s));
+    gather_u32_kernel<<<grid1(ns), kThreads, 0, s>>>(ns, st.ids.p, g->rl->rank.p, st.ids_new.p);
     NWG_LAUNCH_CHECK();
-    NWG_HIP(hipMemcpyAsync(ids.data(), d_new.p, m * 4ull, hipMemcpyDeviceToHost, s));
+    NWG_HIP(hipMemcpyAsync(ids.data(), st.ids_new.p, ns * 4ull, hipMemcpyDeviceToHost, s));
     NWG_HIP(hipStreamSynchronize(s));
-    for (uint32_t q = 0; q < m; ++q) ent[q].first = ids[q];
+    for (uint32_t q = 0; q < ns; ++q) ent[q].first = ids[q];
     std::sort(ent.begin(), ent.end());
   }
-  std::vector<uint32_t> erow(ent.size()), evec(ent.size());
-  ebase.resize(ent.size());
-  for (size_t q = 0; q < ent.size(); ++q) {
+  std::vector<uint32_t> erow(ns), evec(ns);
+  ebase.resize(ns);
+  for (size_t q = 0; q < ns; ++q) {
     erow[q] = ent[q].first;
     evec[q] = ent[q].second;
     const uint32_t c = seed_ptr[ent[q].second + 1] - seed_ptr[ent[q].second];
     const float v = 1.0f / (float)c;                                  // teleport value v_j[row]
     ebase[q] = (float)((1.0 - (double)damping) * (double)v);          // same rounding as the plain path
   }
-  const uint32_t ns = (uint32_t)ent.size();
-  DevBuf<uint32_t> d_row(ns), d_vec(ns), degflag(n);
-  DevBuf<float> d_base(ns);
-  NWG_HIP(hipMemcpyAsync(d_row.p, erow.data(), ns * 4ull, hipMemcpyHostToDevice, s));
-  NWG_HIP(hipMemcpyAsync(d_vec.p, evec.data(), ns * 4ull, hipMemcpyHostToDevice, s));
-  NWG_HIP(hipMemcpyAsync(d_base.p, ebase.data(), ns * 4ull, hipMemcpyHostToDevice, s));
-  ppr_degflag_kernel<<<grid1(n), kThreads, 0, s>>>(n, a.deg, d_row.p, ns, degflag.p);
+  st.seed_row.ensure(ns);
+  st.seed_vec.ensure(ns);
+  st.seed_base.ensure(ns);
+  st.degflag.ensure(n);
+  NWG_HIP(hipMemcpyAsync(st.seed_row.p, erow.data(), ns * 4ull, hipMemcpyHostToDevice, s));
+  NWG_HIP(hipMemcpyAsync(st.seed_vec.p, evec.data(), ns * 4ull, hipMemcpyHostToDevice, s));
+  NWG_HIP(hipMemcpyAsync(st.seed_base.p, ebase.data(), ns * 4ull, hipMemcpyHostToDevice, s));
+  ppr_degflag_kernel<<<grid1(n), kThreads, 0, s>>>(n, a.deg, st.seed_row.p, ns, st.degflag.p);
   NWG_LAUNCH_CHECK();
 
   const uint32_t tiles = (uint32_t)ceil_div(k, tile);
   const uint32_t row_blocks = (uint32_t)ceil_div(n, kThreads / tile);
   const uint64_t nk = (uint64_t)n * k;
-  DevBuf<float> c0(nk), c1(nk);
-  DevBuf<float> xbuf;
-  if (a.order) xbuf.alloc(nk);
-  float* x = a.order ? xbuf.p : scores_dev;
-  DevBuf<float> hub_part((uint64_t)std::max<uint32_t>(n_segs, 1) * k);
-  DevBuf<double> blk_err((uint64_t)row_blocks * tiles * tile);
-  DevBuf<unsigned long long> acc(std::max<uint64_t>(max_iters, 1) * k);
+  st.c0.ensure(nk);
+  st.c1.ensure(nk);
+  if (a.order) st.x.ensure(nk);
+  float* x = a.order ? st.x.p : scores_dev;
+  st.hub_part.ensure((uint64_t)std::max<uint32_t>(n_segs, 1) * k);
+  st.blk_err.ensure((uint64_t)row_blocks * tiles * tile);
+  st.acc.ensure(std::max<uint64_t>(max_iters, 1) * k);
+  float* const c0p = st.c0.p;
+  float* const c1p = st.c1.p;
 
   PprParams P{};
   P.rp = a.ptr;
   P.col = a.idx;
-  P.degflag = degflag.p;
-  P.seed_row = d_row.p;
-  P.seed_vec = d_vec.p;
-  P.seed_base = d_base.p;
+  P.degflag = st.degflag.p;
+  P.seed_row = st.seed_row.p;
+  P.seed_vec = st.seed_vec.p;
+  P.seed_base = st.seed_base.p;
   P.n_seed = ns;
   P.x = x;
   P.n = n;
@@ -340,27 +384,27 @@ void ppr_run(nwg_ctx* ctx, nwg_graph* g, uint32_t k, const std::vector<uint32_t>
   P.row_blocks = row_blocks;
   P.n_hub_rows = n_heavy;
   P.n_hub_segs = n_segs;
-  P.hub_rows = heavy_rows.p;
-  P.seg_ptr = heavy_seg_ptr.p;
-  P.seg_row = heavy_seg_row.p;
-  P.hub_part = hub_part.p;
-  P.blk_err = blk_err.p;
-  P.acc = acc.p;
+  P.hub_rows = st.heavy_rows.p;
+  P.seg_ptr = st.heavy_seg_ptr.p;
+  P.seg_row = st.heavy_seg_row.p;
+  P.hub_part = st.hub_part.p;
+  P.blk_err = st.blk_err.p;
+  P.acc = st.acc.p;
   P.threshold = (double)threshold;
 
   const float init = (float)(1.0 / n);
   NWG_HIP(hipEventRecord(ctx->ev0, s));
-  ppr_init_kernel<<<grid1(nk), kThreads, 0, s>>>(n, k, a.deg, init, x, c0.p);
+  ppr_init_kernel<<<grid1(nk), kThreads, 0, s>>>(n, k, a.deg, init, x, c0p);
   NWG_LAUNCH_CHECK();
-  ppr_zero_acc_kernel<<<grid1(max_iters * k), kThreads, 0, s>>>(acc.p, max_iters * k);
+  ppr_zero_acc_kernel<<<grid1(max_iters * k), kThreads, 0, s>>>(st.acc.p, max_iters * k);
   NWG_LAUNCH_CHECK();
   const dim3 grid(row_blocks + n_segs, tiles);
   const uint64_t fin_a = ceil_div((uint64_t)n_heavy * k, kThreads), fin_b = ceil_div(row_blocks, 32);
   const uint32_t fin = (uint32_t)std::max<uint64_t>(std::max(fin_a, fin_b), 1);
   for (uint64_t it = 0; it < max_iters; ++it) {
     P.iter = (uint32_t)it;
-    P.cin = (it & 1) ? c1.p : c0.p;
-    P.cout = (it & 1) ? c0.p : c1.p;
+    P.cin = (it & 1) ? c1p : c0p;
+    P.cout = (it & 1) ? c0p : c1p;
     switch (tile) {
       case 8:
         ppr_sweep_kernel<8><<<grid, kThreads, 0, s>>>(P);
@@ -387,7 +431,7 @@ void ppr_run(nwg_ctx* ctx, nwg_graph* g, uint32_t k, const std::vector<uint32_t>
     NWG_LAUNCH_CHECK();
   }
   std::vector<unsigned long long> h_acc(std::max<uint64_t>(max_iters, 1) * k);
-  NWG_HIP(hipMemcpyAsync(h_acc.data(), acc.p, max_iters * k * 8ull, hipMemcpyDeviceToHost, s));
+  NWG_HIP(hipMemcpyAsync(h_acc.data(), st.acc.p, max_iters * k * 8ull, hipMemcpyDeviceToHost, s));
   NWG_HIP(hipStreamSynchronize(s));
   float ms = 0.f;
   NWG_HIP(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));

@@ -42,15 +42,21 @@ __global__ __launch_bounds__(kThreads) void order_kernel(uint32_t n, const uint6
   deg_new[r] = deg[v];
 }
 
-// relabelled edge keys (rank[row] << b) | rank[src], one thread per original row
-__global__ __launch_bounds__(kThreads) void edge_key_kernel(uint32_t n, const uint32_t* __restrict__ ptr,
+// relabelled edge keys (rank[row] << b) | rank[src], one thread per CSC position (the
+// row found by binary search, so hub rows do not serialise on one lane)
+__global__ __launch_bounds__(kThreads) void edge_key_kernel(uint32_t n, uint64_t nnz, const uint32_t* __restrict__ ptr,
                                                             const uint32_t* __restrict__ col,
                                                             const uint32_t* __restrict__ rank, int b,
                                                             uint64_t* __restrict__ keys) {
-  const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
-  if (i >= n) return;
-  const uint64_t ri = (uint64_t)rank[i] << b;
-  for (uint32_t p = ptr[i]; p < ptr[i + 1]; ++p) keys[p] = ri | rank[col[p]];
+  const uint64_t p = (uint64_t)blockIdx.x * kThreads + threadIdx.x;
+  if (p >= nnz) return;
+  uint32_t lo = 0, hi = n;  // first row r with ptr[r + 1] > p
+  while (lo < hi) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (ptr[mid + 1] <= p) lo = mid + 1;
+    else hi = mid;
+  }
+  keys[p] = ((uint64_t)rank[lo] << b) | rank[col[p]];
 }
 
 }  // namespace
@@ -78,7 +84,7 @@ void relabel_build(hipStream_t s, nwg_graph* g) {
   {
     DevBuf<uint64_t> keys(nnz ? nnz : 1), tmp(nnz ? nnz : 1);
     if (nnz) {
-      edge_key_kernel<<<grid1(n), kThreads, 0, s>>>(n, g->csc_ptr.p, g->csc_idx.p, rl->rank.p, b, keys.p);
+      edge_key_kernel<<<grid1(nnz), kThreads, 0, s>>>(n, nnz, g->csc_ptr.p, g->csc_idx.p, rl->rank.p, b, keys.p);
       NWG_LAUNCH_CHECK();
       radix_sort_pairs(s, keys.p, tmp.p, nullptr, nullptr, nnz, 2 * b);
     }

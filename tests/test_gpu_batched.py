@@ -68,6 +68,30 @@ def test_ppr_autotune_picks_a_tile(ctx):
     assert its[11] == it and np.max(np.abs(X[:, 11] - ref)) <= 1e-6
 
 
+def test_ppr_reuses_state_across_calls_and_plans(ctx):
+    """Back-to-back solves on one graph reuse its cached segments and buffers: a larger k,
+    then a smaller k, then a relabelled plan (segments rebuilt) and a new tile all agree
+    with the oracle."""
+    g = ctx.rmat(12, 16, seed=6)
+    n, ind, col, deg = O.rmat_csc(12, 16, seed=6)
+
+    def check(seeds, js):
+        X, its, _ = g.ppr(seeds, D, 1e-4, 25)
+        for j in js:
+            ref, it, _, _ = O.pr_v10(n, ind, col, deg, D, 1e-4, 25, teleport=teleport(n, seeds[j]))
+            assert its[j] == it and np.max(np.abs(X[:, j] - ref)) <= 1e-6
+
+    s64 = O.ppr_sources(n, 64, 8, seed=11)
+    check(s64, (0, 33, 63))
+    s5 = O.ppr_sources(n, 5, 3, seed=12)
+    check(s5, range(5))
+    plan = g.get_plan()
+    plan.update(relabel=1, ppr_tile=16, hub_min=512, hub_segment=512)
+    g.set_plan(**plan)
+    check(s64, (1, 40, 62))
+    check(s5, range(5))
+
+
 def test_ppr_rejects_bad_seeds(ctx):
     from nwgraph_amd import NwgError
 

@@ -136,6 +136,12 @@ def test_pagerank_rmat_matches_v10(ctx, scale):
         dict(short_max=100000, group_max=100000, hub_min=100001, hub_segment=4096),  # all short
         dict(short_max=8, group_max=128, hub_min=2048, hub_segment=2048, relabel=1),  # degree-ordered copy
         dict(short_max=2, group_max=64, hub_min=512, hub_segment=512, relabel=1, stream_hints=1),
+        # source-sliced rows of every lane class (mean slice length 2..256), hubs above 2047
+        dict(short_max=4, group_max=128, hub_min=2048, hub_segment=2048, slice_min=9),
+        # sliced rows on the degree-ordered copy, no hubs, every stream hinted
+        dict(short_max=8, group_max=64, hub_min=100000, hub_segment=4096, slice_min=65, relabel=1, stream_hints=2),
+        # nearly every row sliced: most slices empty or a single edge
+        dict(short_max=0, group_max=16, hub_min=1 << 20, hub_segment=256, slice_min=2),
     ],
 )
 def test_pagerank_plans_agree(ctx, plan):
@@ -267,3 +273,19 @@ def test_full_size_properties_rmat22(ctx):
     assert abs(lhs - rhs) < 1e-5 * rhs
     # L1 trace equals the host L1 between the two iterates
     assert abs(i20["error_trace"][-1] - np.abs(x20.astype(np.float64) - x19).sum()) < 1e-9
+
+
+def test_full_size_sliced_plan_rmat22(ctx):
+    """At RMAT-22 the degree-ordered, source-sliced plan gives the default plan's scores
+    (only the FP32 summation order of the sliced rows differs) and leaves the CSC as built."""
+    g = ctx.rmat(22, 16, seed=1)
+    ptr0, idx0, _ = g.csc()
+    x_def, i_def = g.pagerank(D, 1e-4, 20)
+    p = g.get_plan()
+    p.update(short_max=4, group_max=128, hub_min=65536, hub_segment=8192, slice_min=65, relabel=1)
+    g.set_plan(**p)
+    x_sl, i_sl = g.pagerank(D, 1e-4, 20)
+    assert i_sl["iterations"] == i_def["iterations"]
+    assert np.max(np.abs(x_sl - x_def) / np.maximum(x_def, 1e-30)) < 1e-5
+    ptr1, idx1, _ = g.csc()
+    assert np.array_equal(ptr0, ptr1) and np.array_equal(idx0, idx1)
