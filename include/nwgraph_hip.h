@@ -71,6 +71,8 @@ typedef struct nwg_plan_params {
   uint32_t stream_hints; /* non-temporal hints: 0 none, 1 the column-id stream, 2 every streamed array */
   uint32_t ppr_tile;     /* k-tile of the batched solve: 8, 16, 32 or 64 vectors per lane group (0 = widest <= k) */
   uint32_t relabel;      /* 1: sweep a copy of the CSC renumbered by out-degree (results returned in original ids) */
+  uint32_t binned;       /* 1: binned sweep (propagation blocking over a tiled copy of the CSC, 8 bytes per
+                            entry; row sums exact in fixed point) instead of the degree-binned pull sweep */
 } nwg_plan_params;
 
 /* ---------------------------------------------------------------- context */

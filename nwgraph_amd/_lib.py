@@ -55,10 +55,11 @@ class PlanParams(ctypes.Structure):
         ("stream_hints", c_u32),
         ("ppr_tile", c_u32),
         ("relabel", c_u32),
+        ("binned", c_u32),
     ]
 
     FIELDS = ("short_max", "group_max", "hub_min", "hub_segment", "slice_min", "stream_hints", "ppr_tile",
-              "relabel")
+              "relabel", "binned")
 
     def as_dict(self):
         return {k: int(getattr(self, k)) for k in self.FIELDS}

@@ -30,14 +30,14 @@ bool g_file_loaded = false;
 std::string params_to_string(const nwg_plan_params& p) {
   std::ostringstream o;
   o << p.short_max << ' ' << p.group_max << ' ' << p.hub_min << ' ' << p.hub_segment << ' ' << p.slice_min
-    << ' ' << p.stream_hints << ' ' << p.ppr_tile << ' ' << p.relabel;
+    << ' ' << p.stream_hints << ' ' << p.ppr_tile << ' ' << p.relabel << ' ' << p.binned;
   return o.str();
 }
 
 bool params_from_stream(std::istream& in, nwg_plan_params* p) {
   nwg_plan_params q{};
   if (!(in >> q.short_max >> q.group_max >> q.hub_min >> q.hub_segment >> q.slice_min >> q.stream_hints >>
-        q.ppr_tile >> q.relabel))
+        q.ppr_tile >> q.relabel >> q.binned))
     return false;
   *p = q;
   return true;
@@ -144,13 +144,19 @@ nwg_plan_params autotune_plan(nwg_ctx* ctx, nwg_graph* g) {
     }
   };
   try_field(&nwg_plan_params::relabel, {0, 1});
-  try_field(&nwg_plan_params::stream_hints, {0, 1, 2});
-  try_field(&nwg_plan_params::short_max, {0, 2, 4, 8, 16, 32});
-  try_field(&nwg_plan_params::group_max, {16, 32, 64, 128, 256, 512, 1024});
-  try_field(&nwg_plan_params::hub_min, {2048, 4096, 8192, 16384, 32768, 65536, 262144});
-  try_field(&nwg_plan_params::hub_segment, {1024, 2048, 4096, 8192, 16384});
-  try_field(&nwg_plan_params::slice_min, {0, 33, 65, 129, 257, 1025});
-  try_field(&nwg_plan_params::short_max, {0, 2, 4, 8, 16, 32});
+  try_field(&nwg_plan_params::binned, {0, 1});
+  if (best.binned) {
+    // the binned sweep ignores the row-class cutoffs; only the numbering matters
+    try_field(&nwg_plan_params::relabel, {0, 1});
+  } else {
+    try_field(&nwg_plan_params::stream_hints, {0, 1, 2});
+    try_field(&nwg_plan_params::short_max, {0, 2, 4, 8, 16, 32});
+    try_field(&nwg_plan_params::group_max, {16, 32, 64, 128, 256, 512, 1024});
+    try_field(&nwg_plan_params::hub_min, {2048, 4096, 8192, 16384, 32768, 65536, 262144});
+    try_field(&nwg_plan_params::hub_segment, {1024, 2048, 4096, 8192, 16384});
+    try_field(&nwg_plan_params::slice_min, {0, 33, 65, 129, 257, 1025});
+    try_field(&nwg_plan_params::short_max, {0, 2, 4, 8, 16, 32});
+  }
   plan_build(s, g, best);
   {
     std::lock_guard<std::mutex> lk(g_cache_mu);

@@ -86,6 +86,27 @@ constexpr uint32_t kSlices = 8;   // source slices of the sliced gather: one per
 // every L2 line an XCD fetches for its slice holds only that slice's sources.
 constexpr int kSliceShift = 5;
 __host__ __device__ inline uint32_t slice_of(uint32_t u) { return (u >> kSliceShift) & (kSlices - 1); }
+// Binned sweep layout (propagation blocking) of the active CSC.  Rows form bins of
+// 2^kBinBits, sources form chunks of 2^kChunkBits; every CSC entry (row v <- source u)
+// belongs to tile (bin v >> kBinBits, chunk u >> kChunkBits).  Entries are stored tile
+// by tile, bin-major, each tile in CSC order; per entry only the low bits of its row and
+// source are kept.  The chunk pass writes contrib[u] into every entry of a chunk (from
+// the chunk staged in LDS), the bin pass streams each bin and adds its entries into LDS
+// row accumulators.
+constexpr int kBinBits = 14;    // rows per bin: 16384 int64 accumulators = 128 KiB of LDS
+constexpr int kChunkBits = 15;  // sources per chunk: 32768 floats = 128 KiB of LDS
+constexpr uint32_t kBinMaxSegs = 2048;  // tiles of one chunk-pass work item
+struct BinnedLayout {
+  bool built = false;
+  uint32_t n_bins = 0, n_chunks = 0, n_items2 = 0, n_items1 = 0, n_segs = 0;
+  uint64_t nnz = 0, padded = 0;  // entries, and the length of the per-entry arrays
+  DevBuf<uint16_t> brow, bsrc;   // row & (2^kBinBits - 1), source & (2^kChunkBits - 1)
+  DevBuf<uint4> items2;          // bin pass: {bin, first entry, end entry, items of the bin}
+  DevBuf<uint4> items1;          // chunk pass: {chunk, first tile, end tile, entries}
+  DevBuf<uint32_t> seg_start;    // non-empty tiles of each chunk: first entry
+  DevBuf<uint32_t> seg_flat;     // and offset inside their chunk-pass item
+};
+
 struct Plan {
   nwg_plan_params params{};
   bool built = false;
@@ -107,11 +128,36 @@ struct Plan {
   DevBuf<uint32_t> sseg;   // n_srows x (kSlices + 1) segment starts
   DevBuf<uint32_t> slist;  // indices into srows grouped by lane class
   DevBuf<uint32_t> scol;
+  BinnedLayout bl;  // built when params.binned
+};
+
+// Device parameters of one binned sweep.
+struct BinParams {
+  const uint16_t* brow;
+  const uint16_t* bsrc;
+  float* vals;
+  const uint4* items2;
+  const uint4* items1;
+  const uint32_t* seg_start;
+  const uint32_t* seg_flat;
+  unsigned long long* gacc;  // n_bins << kBinBits row sums of split bins (kept zero between sweeps)
+  uint32_t* bin_done;        // finished items per bin (kept zero between sweeps)
+  const uint32_t* deg;
+  float* x;
+  float* contrib;
+  const float* basev;
+  float base;
+  float damp;
+  uint32_t n;
+  uint32_t n_items2, n_items1;
+  unsigned long long* acc;
+  uint32_t iter;
+  double threshold;
 };
 
 // Identity of a captured sweep loop: re-capture when any of it changes.
 struct CaptureKey {
-  static constexpr int kPtrs = 10;
+  static constexpr int kPtrs = 12;
   const void* ptrs[kPtrs] = {};
   uint64_t plan_version = 0;
   uint64_t iters = 0;
@@ -169,6 +215,9 @@ struct PrState {
   DevBuf<float> hub_part;          // per hub segment partial sums
   DevBuf<double> blk_err;          // per-workgroup L1 partials of one sweep
   DevBuf<float> spart;             // kSlices x n_srows partial sums of the sliced rows
+  DevBuf<float> bvals;             // binned sweep: one value per entry
+  DevBuf<unsigned long long> gacc; // binned sweep: row sums of split bins
+  DevBuf<uint32_t> bin_done;       // binned sweep: item counters per bin
   Captured cap;
 };
 
@@ -266,6 +315,9 @@ void build_segments(hipStream_t s, const uint32_t* rp, uint32_t n, uint32_t min_
                     DevBuf<uint32_t>& rows, DevBuf<uint32_t>& seg_ptr, DevBuf<uint32_t>& seg_row, uint32_t* nrows,
                     uint32_t* nsegs);
 nwg_plan_params default_plan_params();
+// binned sweep (binned.hip)
+void binned_build(hipStream_t s, const ActiveCsc& a, uint32_t n, uint64_t nnz, BinnedLayout& L);
+void binned_enqueue_sweep(hipStream_t s, const BinParams& P);
 void pagerank_run(nwg_ctx* ctx, nwg_graph* g, const float* teleport_dev, float damping, float threshold,
                   uint64_t max_iters, nwg_pr_info* info, float* scores_dev);
 

@@ -450,6 +450,7 @@ static void validate_params(const nwg_plan_params& p) {
   NWG_REQUIRE(p.ppr_tile == 0 || p.ppr_tile == 8 || p.ppr_tile == 16 || p.ppr_tile == 32 || p.ppr_tile == 64,
               "plan: ppr_tile must be 0, 8, 16, 32 or 64");
   NWG_REQUIRE(p.relabel <= 1, "plan: relabel must be 0 or 1");
+  NWG_REQUIRE(p.binned <= 1, "plan: binned must be 0 or 1");
   NWG_REQUIRE(p.slice_min == 0 || (p.slice_min >= 2 && p.slice_min > p.short_max),
               "plan: slice_min must be 0 or above both 1 and short_max");
 }
@@ -530,6 +531,7 @@ void plan_build(hipStream_t s, nwg_graph* g, const nwg_plan_params& pp) {
     pl.scol.release();
   }
   pl.n_tiles = (uint32_t)ceil_div(n, kThreads);
+  if (pp.binned) binned_build(s, g->active(pp.relabel != 0), n, g->nnz, pl.bl);
   NWG_HIP(hipStreamSynchronize(s));
   pl.built = true;
   g->plan = std::move(pl);
@@ -634,14 +636,47 @@ static void enqueue_solve(hipStream_t s, nwg_graph* g, PrState& st, SweepParams 
                                                                                st.contrib0.p, st.acc.p, iters);
   NWG_LAUNCH_CHECK();
   const uint32_t nt = g->plan.params.stream_hints;
+  const BinnedLayout& L = g->plan.bl;
+  BinParams B{};
+  if (g->plan.params.binned) {
+    // contrib is one buffer here: the chunk pass of sweep k reads it before the bin pass
+    // of sweep k rewrites it
+    B.brow = L.brow.p;
+    B.bsrc = L.bsrc.p;
+    B.vals = st.bvals.p;
+    B.items2 = L.items2.p;
+    B.items1 = L.items1.p;
+    B.seg_start = L.seg_start.p;
+    B.seg_flat = L.seg_flat.p;
+    B.gacc = st.gacc.p;
+    B.bin_done = st.bin_done.p;
+    B.deg = P.deg;
+    B.x = P.x;
+    B.contrib = st.contrib0.p;
+    B.basev = P.basev;
+    B.base = P.base;
+    B.damp = P.damp;
+    B.n = n;
+    B.n_items2 = L.n_items2;
+    B.n_items1 = L.n_items1;
+    B.acc = P.acc;
+    B.threshold = P.threshold;
+  }
   for (uint64_t k = 0; k < iters; ++k) {
     P.iter = (uint32_t)k;
     P.cin = (k & 1) ? st.contrib1.p : st.contrib0.p;
     P.cout = (k & 1) ? st.contrib0.p : st.contrib1.p;
     if (ev) NWG_HIP(hipEventRecord(ev[2 * k], s));
-    if (nt == 2) enqueue_sweep<2>(s, P);
-    else if (nt == 1) enqueue_sweep<1>(s, P);
-    else enqueue_sweep<0>(s, P);
+    if (g->plan.params.binned) {
+      B.iter = (uint32_t)k;
+      binned_enqueue_sweep(s, B);
+    } else if (nt == 2) {
+      enqueue_sweep<2>(s, P);
+    } else if (nt == 1) {
+      enqueue_sweep<1>(s, P);
+    } else {
+      enqueue_sweep<0>(s, P);
+    }
     if (ev) NWG_HIP(hipEventRecord(ev[2 * k + 1], s));
   }
 }
@@ -661,6 +696,15 @@ void pagerank_run(nwg_ctx* ctx, nwg_graph* g, const float* teleport_dev, float d
   st.hub_part.ensure(std::max<uint32_t>(g->plan.n_hub_segs, 1));
   st.blk_err.ensure(std::max<uint64_t>(sweep_workgroups_before_hubs(g->plan), 1));
   st.spart.ensure(std::max<uint64_t>((uint64_t)kSlices * g->plan.n_srows, 1));
+  if (g->plan.params.binned) {
+    const BinnedLayout& L = g->plan.bl;
+    st.bvals.ensure(L.padded);
+    st.gacc.ensure((uint64_t)L.n_bins << kBinBits);
+    st.bin_done.ensure(L.n_bins);
+    // the bin pass keeps these zero between sweeps; clear them here, outside any capture
+    NWG_HIP(hipMemsetAsync(st.gacc.p, 0, st.gacc.n * sizeof(unsigned long long), s));
+    NWG_HIP(hipMemsetAsync(st.bin_done.p, 0, st.bin_done.n * sizeof(uint32_t), s));
+  }
   const bool personalised = teleport_dev != nullptr;
   if (personalised) {
     st.base.ensure(n);
@@ -684,6 +728,9 @@ void pagerank_run(nwg_ctx* ctx, nwg_graph* g, const float* teleport_dev, float d
     key.ptrs[6] = P.rp;
     key.ptrs[7] = st.blk_err.p;
     key.ptrs[8] = st.spart.p;
+    key.ptrs[9] = st.bvals.p;
+    key.ptrs[10] = st.gacc.p;
+    key.ptrs[11] = st.bin_done.p;
     key.plan_version = g->plan_version;
     key.iters = max_iters;
     key.damping = damping;

@@ -142,6 +142,9 @@ def test_pagerank_rmat_matches_v10(ctx, scale):
         dict(short_max=8, group_max=64, hub_min=100000, hub_segment=4096, slice_min=65, relabel=1, stream_hints=2),
         # nearly every row sliced: most slices empty or a single edge
         dict(short_max=0, group_max=16, hub_min=1 << 20, hub_segment=256, slice_min=2),
+        # binned sweep: one bin and one chunk at this size
+        dict(binned=1),
+        dict(binned=1, relabel=1),
     ],
 )
 def test_pagerank_plans_agree(ctx, plan):
@@ -289,3 +292,39 @@ def test_full_size_sliced_plan_rmat22(ctx):
     assert np.max(np.abs(x_sl - x_def) / np.maximum(x_def, 1e-30)) < 1e-5
     ptr1, idx1, _ = g.csc()
     assert np.array_equal(ptr0, ptr1) and np.array_equal(idx0, idx1)
+    # and so does the binned sweep (bins split over many items at this size)
+    p.update(binned=1)
+    g.set_plan(**p)
+    x_b, i_b = g.pagerank(D, 1e-4, 20)
+    assert i_b["iterations"] == i_def["iterations"]
+    assert np.max(np.abs(x_b - x_def) / np.maximum(x_def, 1e-30)) < 1e-5
+
+
+@pytest.mark.parametrize("relabel", [0, 1])
+def test_binned_sweep_many_bins_matches_v10(ctx, relabel):
+    """Binned sweep at RMAT-18: 16 bins of 2^14 rows, 8 chunks of 2^15 sources, the hot
+    bins split over several items (combined through the global accumulators)."""
+    g = ctx.rmat(18, 16, seed=3)
+    p = g.get_plan()
+    p.update(binned=1, relabel=relabel)
+    g.set_plan(**p)
+    n, ind, col, deg = O.rmat_csc(18, 16, seed=3)
+    x, info, ref = check_against_oracle(g, n, ind, col, deg)
+    assert np.max(np.abs(x - ref) / np.maximum(ref, 1e-30)) < 1e-5  # FP32 summation order only
+    # deterministic and independent of the schedule: a second call is bit-identical
+    x2, info2 = g.pagerank(D, 1e-4, 20)
+    assert np.array_equal(x, x2) and info2["iterations"] == info["iterations"]
+
+
+def test_binned_sweep_grid_and_personalised(ctx):
+    g = ctx.grid(256, seed=5)
+    p = g.get_plan()
+    p.update(binned=1)
+    g.set_plan(**p)
+    n, ind, col, deg = O.grid_csc(256, seed=5)
+    x, info, ref = check_against_oracle(g, n, ind, col, deg, max_iters=100)
+    assert info["iterations"] < 100
+    ids = O.ppr_sources(n, 1, 8, seed=9)[0]
+    v = np.zeros(n, np.float32)
+    v[ids] = 1.0 / 8
+    check_against_oracle(g, n, ind, col, deg, teleport=v)
