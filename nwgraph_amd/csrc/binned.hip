@@ -8,16 +8,17 @@
 //   chunk pass  work item = one source chunk (2^15 sources) and a run of its tiles:
 //               stage contrib[chunk] in LDS, then vals[e] = contrib[src(e)] for every
 //               entry e of those tiles (2-byte source offsets in, 4-byte values out)
-//   bin pass    work item = a contiguous entry range of one bin (2^14 rows): add
-//               vals[e] into LDS row accumulators (4 + 2 bytes in per entry); a bin
-//               split over several items adds its partial rows into gacc with global
-//               atomics and the last item of the bin finishes it.  Finishing a row is
-//               the reference's update (page_rank.hpp:133-135): x = fma(d, z, base),
+//   bin pass    work item = a contiguous entry range of one bin (2^13 rows): add
+//               vals[e] into LDS row accumulators (4 + 2 bytes in per entry); a bin that
+//               is one item finishes its rows, the items of a split bin store partial
+//               rows that the combine pass adds and finishes.  Finishing a row is the
+//               reference's update (page_rank.hpp:133-135): x = fma(d, z, base),
 //               |x - x_old| into the sweep's L1 change, contrib = x / deg.
 //
-// Row sums are exact fixed-point sums (2^-60 units, integer adds), so they do not depend
-// on how the entries are split or ordered: results are deterministic, and z is the float
-// rounding of the (nearly) exact sum of the float contributions.  The L1 change is
+// Row sums are exact sums of the contributions rounded to units of 2^-52 (integer-valued
+// doubles, see quantize), so they do not depend on how the entries are split or ordered:
+// results are deterministic, and z is within a few 2^-52 of the exact sum of the float
+// contributions.  The L1 change is
 // accumulated in fixed point as in the pull sweep (device_utils.hpp), and every
 // workgroup first checks the previous sweep's change against the threshold
 // (page_rank.hpp:144-146), so a captured loop stops working after convergence.
@@ -38,13 +39,16 @@ constexpr uint32_t kBinRows = 1u << kBinBits;
 constexpr uint32_t kChunkSrc = 1u << kChunkBits;
 constexpr uint32_t kItem2Entries = 1u << 18;  // entries per bin-pass item
 constexpr uint32_t kItem1Entries = 1u << 19;  // entries per chunk-pass item
-constexpr float kSumScale = 0x1p60f;          // fixed point of row sums: z < 8 fits in 63 bits
-constexpr double kSumUnit = 0x1p-60;
+constexpr uint32_t kPadSrc = 0xFFFFu;           // source offset of a pad entry (value 0)
+constexpr uint32_t kNoSlot = 0xFFFFFFFFu;     // bin-pass item of a bin that is one item
 
-__device__ __forceinline__ unsigned long long sum_to_fixed(float c) {
-  return (unsigned long long)__builtin_rintf(c * kSumScale);  // c >= 0; exact for c >= 2^-37
-}
-__device__ __forceinline__ float fixed_to_sum(unsigned long long v) { return (float)((double)v * kSumUnit); }
+// Row sums are kept as integer-valued doubles in units of 2^-52: each contribution is
+// rounded to the nearest unit (ties to even), and as a row sum z <= 1 stays below 2^53
+// units, every add is exact: the sum does not depend on the order or the split of the
+// entries.  (z <= sum of all x <= 1: x starts summing to 1 and a sweep maps a vector of
+// sum <= 1 to one of sum <= 1.)
+__device__ __forceinline__ double quantize(float c) { return __builtin_rint((double)c * 0x1p52); }
+__device__ __forceinline__ float dequantize(double v) { return (float)(v * 0x1p-52); }
 
 __device__ __forceinline__ bool converged_before(const BinParams& P) {
   return P.iter > 0 && fixed_to_err(P.acc[P.iter - 1]) < P.threshold;
@@ -71,13 +75,15 @@ __device__ __forceinline__ double block_sum_d(double v, double* red /* kBT/64 */
   return t;  // valid in thread 0
 }
 
-// chunk pass: vals[e] = contrib[src(e)] over the tiles of one item
+// chunk pass: vals[e] = contrib[src(e)] over the tiles of one item.  Tiles are padded to
+// multiples of 8 entries (pads have source kPadSrc and get 0), so a lane moves 8 entries
+// with one 16-byte id load and two 16-byte value stores.
 __global__ __launch_bounds__(kBT) void pr_chunk_kernel(BinParams P) {
   if (converged_before(P)) return;
   __shared__ float cst[kChunkSrc];
   __shared__ uint32_t sstart[kBinMaxSegs];
   __shared__ uint32_t sflat[kBinMaxSegs + 1];
-  const uint4 it = P.items1[blockIdx.x];  // {chunk, first tile, end tile, entries}
+  const uint4 it = P.items1[blockIdx.x];  // {chunk, first tile, end tile, groups of 8 entries}
   const uint32_t ns = it.z - it.y, m = it.w;
   for (uint32_t k = threadIdx.x; k < ns; k += kBT) {
     sstart[k] = P.seg_start[it.y + k];
@@ -87,94 +93,158 @@ __global__ __launch_bounds__(kBT) void pr_chunk_kernel(BinParams P) {
   const uint32_t u0 = it.x << kChunkBits;
   for (uint32_t q = threadIdx.x; q < kChunkSrc; q += kBT) cst[q] = (u0 + q < P.n) ? P.contrib[u0 + q] : 0.f;
   __syncthreads();
-  // each wave takes blocks of 512 flat entries; lane l handles entries k0 + l + 64 r
-  const uint32_t w = threadIdx.x >> 6, l = threadIdx.x & 63;
-  for (uint32_t k0 = w * 512; k0 < m; k0 += (kBT / 64) * 512) {
-    const uint32_t k = k0 + l;
-    if (k >= m) break;
-    uint32_t lo = 0, hi = ns - 1;  // last tile j with sflat[j] <= k
-    while (lo < hi) {
-      const uint32_t mid = (lo + hi + 1) >> 1;
-      if (sflat[mid] <= k) lo = mid;
-      else hi = mid - 1;
-    }
-    uint32_t j = lo;
-    uint32_t p[8];
-    uint16_t u[8];
+  for (uint32_t g0 = 0; g0 < m; g0 += 2 * kBT) {
+    uint32_t p[2];
+    uint4 uu[2];
 #pragma unroll
-    for (uint32_t r = 0; r < 8; ++r) {
-      const uint32_t kk = k + 64 * r;
-      if (kk < m) {
-        while (sflat[j + 1] <= kk) ++j;
-        p[r] = sstart[j] + (kk - sflat[j]);
-        u[r] = P.bsrc[p[r]];
+    for (uint32_t h = 0; h < 2; ++h) {
+      const uint32_t g = g0 + h * kBT + threadIdx.x;
+      if (g < m) {
+        uint32_t lo = 0, hi = ns - 1;  // last tile j with sflat[j] <= g
+        while (lo < hi) {
+          const uint32_t mid = (lo + hi + 1) >> 1;
+          if (sflat[mid] <= g) lo = mid;
+          else hi = mid - 1;
+        }
+        p[h] = sstart[lo] + 8u * (g - sflat[lo]);
+        uu[h] = *reinterpret_cast<const uint4*>(P.bsrc + p[h]);
       }
     }
 #pragma unroll
-    for (uint32_t r = 0; r < 8; ++r)
-      if (k + 64 * r < m) P.vals[p[r]] = cst[u[r]];
+    for (uint32_t h = 0; h < 2; ++h) {
+      if (g0 + h * kBT + threadIdx.x >= m) continue;
+      const uint32_t w[4] = {uu[h].x, uu[h].y, uu[h].z, uu[h].w};
+      float o[8];
+#pragma unroll
+      for (uint32_t t = 0; t < 8; ++t) {
+        const uint32_t u = (w[t >> 1] >> (16 * (t & 1))) & 0xFFFFu;
+        o[t] = u == kPadSrc ? 0.f : cst[u];
+      }
+      *reinterpret_cast<float4*>(P.vals + p[h]) = make_float4(o[0], o[1], o[2], o[3]);
+      *reinterpret_cast<float4*>(P.vals + p[h] + 4) = make_float4(o[4], o[5], o[6], o[7]);
+    }
   }
 }
 
-// bin pass: row sums of one entry range of one bin in LDS, then finish or combine
+constexpr uint32_t kNoRow = 0xFFFFFFFFu;
+
+// Adds each lane's run (row cur, quantized sum run; cur == kNoRow: none) into lacc.  When
+// every run in the wave belongs to one row (a hub row filling the wave), the runs are summed
+// across the wave first and one lane issues the atomic, instead of 64 atomics on one address.
+// The values are integers below 2^53, so every add is exact and the order does not matter.
+// All lanes of the wave must call it together.
+__device__ __forceinline__ void wave_flush(double* lacc, uint32_t cur, double run, uint32_t lane) {
+  const bool has = cur != kNoRow;
+  const unsigned long long m = __ballot(has);
+  if (!m) return;
+  const int first = __builtin_ctzll(m);
+  const uint32_t row0 = (uint32_t)__shfl((int)cur, first);
+  if (__ballot(has && cur != row0) == 0ull) {
+    double t = has ? run : 0.0;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) t += __shfl_xor(t, o);
+    if ((int)lane == first) atomicAdd(&lacc[row0], t);
+  } else if (has) {
+    atomicAdd(&lacc[cur], run);
+  }
+}
+
+// bin pass: row sums of one entry range of one bin in LDS; a bin that is one item finishes
+// its rows here, an item of a split bin stores its partial rows in its slot of P.part
 __global__ __launch_bounds__(kBT) void pr_bin_kernel(BinParams P) {
   if (converged_before(P)) return;
-  __shared__ unsigned long long lacc[kBinRows];
+  __shared__ double lacc[kBinRows];
   __shared__ double red[kBT / 64];
-  __shared__ uint32_t is_last;
-  const uint4 it = P.items2[blockIdx.x];  // {bin, first entry, end entry, items of the bin}
+  const uint4 it = P.items2[blockIdx.x];  // {bin, first entry, end entry, partial slot or kNoSlot}
   const uint32_t b = it.x;
-  for (uint32_t q = threadIdx.x; q < kBinRows; q += kBT) lacc[q] = 0ull;
+  for (uint32_t q = threadIdx.x; q < kBinRows; q += kBT) lacc[q] = 0.0;
   __syncthreads();
-  // 8 consecutive entries per lane from an 8-aligned start (the arrays are padded), runs of
-  // one row added in registers before one LDS atomic
-  for (uint64_t q = (uint64_t)(it.y & ~7u) + 8u * threadIdx.x; q < it.z; q += 8u * kBT) {
-    const uint4 rw = *reinterpret_cast<const uint4*>(P.brow + q);
-    const float4 va = *reinterpret_cast<const float4*>(P.vals + q);
-    const float4 vb = *reinterpret_cast<const float4*>(P.vals + q + 4);
-    const uint32_t rows[8] = {rw.x & 0xFFFFu, rw.x >> 16, rw.y & 0xFFFFu, rw.y >> 16,
-                              rw.z & 0xFFFFu, rw.z >> 16, rw.w & 0xFFFFu, rw.w >> 16};
-    const float vs[8] = {va.x, va.y, va.z, va.w, vb.x, vb.y, vb.z, vb.w};
-    uint32_t cur = 0xFFFFFFFFu;
-    unsigned long long run = 0ull;
+  // A wave takes 512 consecutive entries (8 per lane, from an 8-aligned start: the arrays
+  // are padded), two such blocks in flight; the loop bounds are uniform over the wave so
+  // every lane takes part in the votes of wave_flush.  Runs of one row are added in
+  // registers; a lane's last run goes through wave_flush.
+  const uint32_t w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  for (uint64_t wq = (uint64_t)(it.y & ~7u) + 512u * w; wq < it.z; wq += 2u * 512u * (kBT / 64)) {
+    uint4 rw[2];
+    float4 va[2], vb[2];
 #pragma unroll
-    for (uint32_t r = 0; r < 8; ++r) {
-      const uint64_t e = q + r;
-      if (e < it.y || e >= it.z) continue;
-      const unsigned long long f = sum_to_fixed(vs[r]);
-      if (rows[r] == cur) {
-        run += f;
-      } else {
-        if (cur != 0xFFFFFFFFu) atomicAdd(&lacc[cur], run);
-        cur = rows[r];
-        run = f;
-      }
+    for (uint32_t h = 0; h < 2; ++h) {
+      // unconditional loads (clamped into the item's first group when past its end, then
+      // masked below) so that both groups' loads are issued before any is waited on
+      const uint64_t q = wq + h * 512u * (kBT / 64) + 8u * lane;
+      const uint64_t qc = q < it.z ? q : (uint64_t)(it.y & ~7u);
+      rw[h] = *reinterpret_cast<const uint4*>(P.brow + qc);
+      va[h] = *reinterpret_cast<const float4*>(P.vals + qc);
+      vb[h] = *reinterpret_cast<const float4*>(P.vals + qc + 4);
     }
-    if (cur != 0xFFFFFFFFu) atomicAdd(&lacc[cur], run);
+#pragma unroll
+    for (uint32_t h = 0; h < 2; ++h) {
+      const uint64_t q = wq + h * 512u * (kBT / 64) + 8u * lane;
+      uint32_t cur = kNoRow;
+      double run = 0.0;
+      if (q < it.z) {
+        const uint32_t rows[8] = {rw[h].x & 0xFFFFu, rw[h].x >> 16, rw[h].y & 0xFFFFu, rw[h].y >> 16,
+                                  rw[h].z & 0xFFFFu, rw[h].z >> 16, rw[h].w & 0xFFFFu, rw[h].w >> 16};
+        const float vs[8] = {va[h].x, va[h].y, va[h].z, va[h].w, vb[h].x, vb[h].y, vb[h].z, vb[h].w};
+#pragma unroll
+        for (uint32_t r = 0; r < 8; ++r) {
+          const uint64_t e = q + r;
+          if (e < it.y || e >= it.z) continue;
+          const double f = quantize(vs[r]);
+          if (rows[r] == cur) {
+            run += f;
+          } else {
+            if (cur != kNoRow) atomicAdd(&lacc[cur], run);
+            cur = rows[r];
+            run = f;
+          }
+        }
+      }
+      wave_flush(lacc, cur, run, lane);
+    }
   }
   __syncthreads();
   const uint32_t r0 = b << kBinBits;
-  double e = 0.0;
-  if (it.w == 1) {  // the bin is one item: finish its rows from LDS
-    for (uint32_t v = threadIdx.x; v < kBinRows && r0 + v < P.n; v += kBT) e += finish_row(P, r0 + v, fixed_to_sum(lacc[v]));
-  } else {
-    for (uint32_t v = threadIdx.x; v < kBinRows; v += kBT)
-      if (lacc[v]) atomicAdd(P.gacc + r0 + v, lacc[v]);
-    __threadfence();
-    __syncthreads();
-    if (threadIdx.x == 0) is_last = (atomicAdd(P.bin_done + b, 1u) == it.w - 1) ? 1u : 0u;
-    __syncthreads();
-    if (!is_last) return;
-    __threadfence();
-    for (uint32_t v = threadIdx.x; v < kBinRows && r0 + v < P.n; v += kBT) {
-      const unsigned long long g = __hip_atomic_load(P.gacc + r0 + v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      P.gacc[r0 + v] = 0ull;  // zero again for the next sweep
-      e += finish_row(P, r0 + v, fixed_to_sum(g));
-    }
-    if (threadIdx.x == 0) P.bin_done[b] = 0u;
+  if (it.w != kNoSlot) {
+    double* dst = P.part + (uint64_t)it.w * kBinRows;
+    for (uint32_t v = threadIdx.x; v < kBinRows; v += kBT) dst[v] = lacc[v];
+    return;
   }
+  double e = 0.0;
+  for (uint32_t v = threadIdx.x; v < kBinRows && r0 + v < P.n; v += kBT) e += finish_row(P, r0 + v, dequantize(lacc[v]));
   e = block_sum_d(e, red);
   if (threadIdx.x == 0 && e > 0.0) atomicAdd(P.acc + P.iter, err_to_fixed(e));
+}
+
+// rows of the bins split over several bin-pass items: add the items' partial rows (exact,
+// so in any order) and finish the rows; grid (kBinRows / 256, split bins)
+__global__ __launch_bounds__(256) void pr_bin_combine_kernel(BinParams P) {
+  if (converged_before(P)) return;
+  __shared__ double red[4];
+  const uint4 it = P.items3[blockIdx.y];  // {bin, first slot, slots}
+  const uint32_t v = blockIdx.x * 256 + threadIdx.x;
+  const uint32_t i = (it.x << kBinBits) + v;
+  double e = 0.0;
+  if (i < P.n) {
+    const double* src = P.part + (uint64_t)it.y * kBinRows + v;
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+    uint32_t k = 0;
+    for (; k + 4 <= it.z; k += 4) {
+      s0 += src[(uint64_t)k * kBinRows];
+      s1 += src[(uint64_t)(k + 1) * kBinRows];
+      s2 += src[(uint64_t)(k + 2) * kBinRows];
+      s3 += src[(uint64_t)(k + 3) * kBinRows];
+    }
+    for (; k < it.z; ++k) s0 += src[(uint64_t)k * kBinRows];
+    e = finish_row(P, i, dequantize((s0 + s1) + (s2 + s3)));
+  }
+  e = wave_sum(e);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = e;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const double t = (red[0] + red[1]) + (red[2] + red[3]);
+    if (t > 0.0) atomicAdd(P.acc + P.iter, err_to_fixed(t));
+  }
 }
 
 // ---- layout construction ----
@@ -195,12 +265,33 @@ __global__ __launch_bounds__(256) void bin_key_kernel(uint32_t n, uint64_t nnz, 
   low[p] = ((lo & (kBinRows - 1)) << 16) | (u & (kChunkSrc - 1));
 }
 
-__global__ __launch_bounds__(256) void bin_split_kernel(uint64_t nnz, const uint32_t* __restrict__ low,
+// sorted entry q of tile t goes to padded position ptp[t] + (q - tp[t])
+__global__ __launch_bounds__(256) void bin_place_kernel(uint64_t nnz, const uint64_t* __restrict__ keys,
+                                                        const uint32_t* __restrict__ low,
+                                                        const uint32_t* __restrict__ tp,
+                                                        const uint32_t* __restrict__ ptp,
                                                         uint16_t* __restrict__ brow, uint16_t* __restrict__ bsrc) {
-  const uint64_t p = (uint64_t)blockIdx.x * 256 + threadIdx.x;
-  if (p >= nnz) return;
-  brow[p] = (uint16_t)(low[p] >> 16);
-  bsrc[p] = (uint16_t)(low[p] & 0xFFFFu);
+  const uint64_t q = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+  if (q >= nnz) return;
+  const uint64_t t = keys[q];
+  const uint64_t np = ptp[t] + (q - tp[t]);
+  brow[np] = (uint16_t)(low[q] >> 16);
+  bsrc[np] = (uint16_t)(low[q] & 0xFFFFu);
+}
+
+// pads of tile t: row of the tile's last entry (so they join its run), source kPadSrc
+__global__ __launch_bounds__(256) void bin_pad_kernel(uint64_t n_tiles, const uint32_t* __restrict__ tp,
+                                                      const uint32_t* __restrict__ ptp, uint16_t* __restrict__ brow,
+                                                      uint16_t* __restrict__ bsrc) {
+  const uint64_t t = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+  if (t >= n_tiles) return;
+  const uint32_t cnt = tp[t + 1] - tp[t];
+  if (!cnt) return;
+  const uint32_t last = ptp[t] + cnt - 1;
+  for (uint32_t r = cnt; r < ptp[t + 1] - ptp[t]; ++r) {
+    brow[ptp[t] + r] = brow[last];
+    bsrc[ptp[t] + r] = (uint16_t)kPadSrc;
+  }
 }
 
 // tile_ptr[t] = first sorted position with key >= t, for t in [0, n_tiles]
@@ -227,13 +318,9 @@ void binned_build(hipStream_t s, const ActiveCsc& a, uint32_t n, uint64_t nnz, B
   L.n_bins = (uint32_t)ceil_div(n, kBinRows);
   L.n_chunks = (uint32_t)ceil_div(n, kChunkSrc);
   L.nnz = nnz;
-  L.padded = ceil_div(nnz, 8) * 8 + 8;
   const uint64_t n_tiles = (uint64_t)L.n_bins * L.n_chunks;
-  L.brow.alloc(L.padded);
-  L.bsrc.alloc(L.padded);
-  NWG_HIP(hipMemsetAsync(L.brow.p, 0, L.padded * 2, s));
-  NWG_HIP(hipMemsetAsync(L.bsrc.p, 0, L.padded * 2, s));
-  std::vector<uint32_t> tp(n_tiles + 1, 0);
+  std::vector<uint32_t> tp(n_tiles + 1, 0);  // becomes the padded tile pointers
+  uint64_t total = 0;
   if (nnz) {
     DevBuf<uint64_t> keys(nnz), keys_tmp(nnz);
     DevBuf<uint32_t> low(nnz), low_tmp(nnz);
@@ -243,52 +330,84 @@ void binned_build(hipStream_t s, const ActiveCsc& a, uint32_t n, uint64_t nnz, B
     while (kb < 64 && (1ull << kb) < n_tiles) ++kb;
     // stable: entries of one tile keep their CSC order (row, then source)
     radix_sort_pairs(s, keys.p, keys_tmp.p, low.p, low_tmp.p, nnz, kb);
-    bin_split_kernel<<<grid256(nnz), 256, 0, s>>>(nnz, low.p, L.brow.p, L.bsrc.p);
-    NWG_LAUNCH_CHECK();
-    DevBuf<uint32_t> d_tp(n_tiles + 1);
+    DevBuf<uint32_t> d_tp(n_tiles + 1), d_ptp(n_tiles + 1);
     tile_ptr_kernel<<<grid256(n_tiles + 1), 256, 0, s>>>(nnz, keys.p, n_tiles, d_tp.p);
     NWG_LAUNCH_CHECK();
     NWG_HIP(hipMemcpyAsync(tp.data(), d_tp.p, (n_tiles + 1) * 4, hipMemcpyDeviceToHost, s));
     NWG_HIP(hipStreamSynchronize(s));
+    std::vector<uint32_t> ptp(n_tiles + 1, 0);
+    for (uint64_t t = 0; t < n_tiles; ++t) {
+      total += (uint64_t)(tp[t + 1] - tp[t] + 7) / 8 * 8;
+      NWG_REQUIRE(total < (1ull << 32) - 16, "binned layout needs fewer than 2^32 - 16 padded entries");
+      ptp[t + 1] = (uint32_t)total;
+    }
+    L.padded = total + 8;
+    L.brow.alloc(L.padded);
+    L.bsrc.alloc(L.padded);
+    NWG_HIP(hipMemsetAsync(L.brow.p, 0, L.padded * 2, s));
+    NWG_HIP(hipMemsetAsync(L.bsrc.p, 0xFF, L.padded * 2, s));
+    NWG_HIP(hipMemcpyAsync(d_ptp.p, ptp.data(), (n_tiles + 1) * 4, hipMemcpyHostToDevice, s));
+    bin_place_kernel<<<grid256(nnz), 256, 0, s>>>(nnz, keys.p, low.p, d_tp.p, d_ptp.p, L.brow.p, L.bsrc.p);
+    NWG_LAUNCH_CHECK();
+    bin_pad_kernel<<<grid256(n_tiles), 256, 0, s>>>(n_tiles, d_tp.p, d_ptp.p, L.brow.p, L.bsrc.p);
+    NWG_LAUNCH_CHECK();
+    NWG_HIP(hipStreamSynchronize(s));
+    tp.swap(ptp);
+  } else {
+    L.padded = 8;
+    L.brow.alloc(L.padded);
+    L.bsrc.alloc(L.padded);
+    NWG_HIP(hipMemsetAsync(L.brow.p, 0, L.padded * 2, s));
+    NWG_HIP(hipMemsetAsync(L.bsrc.p, 0xFF, L.padded * 2, s));
   }
   const uint32_t C = L.n_chunks;
-  // bin pass items: every bin, cut into ranges of at most kItem2Entries entries
-  std::vector<uint4> it2;
+  // bin pass items: every bin, cut into ranges of at most kItem2Entries entries; the items
+  // of a split bin get consecutive partial slots, and the bin a combine item
+  std::vector<uint4> it2, it3;
+  uint32_t slots = 0;
   for (uint32_t b = 0; b < L.n_bins; ++b) {
     const uint32_t p0 = tp[(uint64_t)b * C], p1 = tp[(uint64_t)(b + 1) * C];
     const uint32_t cnt = (uint32_t)std::max<uint64_t>(ceil_div(p1 - p0, kItem2Entries), 1);
+    if (cnt > 1) it3.push_back(make_uint4(b, slots, cnt, 0));
     for (uint32_t q = 0; q < cnt; ++q) {
       const uint32_t a0 = p0 + (uint32_t)((uint64_t)(p1 - p0) * q / cnt);
       const uint32_t a1 = p0 + (uint32_t)((uint64_t)(p1 - p0) * (q + 1) / cnt);
-      it2.push_back(make_uint4(b, a0, a1, cnt));
+      it2.push_back(make_uint4(b, a0, a1, cnt > 1 ? slots++ : kNoSlot));
     }
   }
-  // chunk pass items: the non-empty tiles of each chunk, cut at kItem1Entries entries or
-  // kBinMaxSegs tiles (a tile longer than kItem1Entries is an item of its own)
+  // chunk pass items: the non-empty tiles of each chunk (long tiles cut into pieces), cut
+  // at kItem1Entries entries or kBinMaxSegs pieces; offsets and sizes in groups of 8
+  // entries (every padded tile, and kItem1Entries, is a whole number of groups)
   std::vector<uint4> it1;
   std::vector<uint32_t> sst, sfl;
   for (uint32_t c = 0; c < C; ++c) {
     uint32_t first = (uint32_t)sst.size(), flat = 0;
     for (uint32_t b = 0; b < L.n_bins; ++b) {
       const uint32_t t0 = tp[(uint64_t)b * C + c], t1 = tp[(uint64_t)b * C + c + 1];
-      if (t1 == t0) continue;
-      const uint32_t len = t1 - t0;
-      if ((uint32_t)sst.size() > first &&
-          (flat + len > kItem1Entries || (uint32_t)sst.size() - first >= kBinMaxSegs)) {
-        it1.push_back(make_uint4(c, first, (uint32_t)sst.size(), flat));
-        first = (uint32_t)sst.size();
-        flat = 0;
+      // a long tile is cut into pieces of at most kItem1Entries entries
+      for (uint32_t q0 = t0; q0 < t1; q0 += kItem1Entries) {
+        const uint32_t len = (std::min<uint32_t>(t1 - q0, kItem1Entries)) / 8;
+        if ((uint32_t)sst.size() > first &&
+            (flat + len > kItem1Entries / 8 || (uint32_t)sst.size() - first >= kBinMaxSegs)) {
+          it1.push_back(make_uint4(c, first, (uint32_t)sst.size(), flat));
+          first = (uint32_t)sst.size();
+          flat = 0;
+        }
+        sst.push_back(q0);
+        sfl.push_back(flat);
+        flat += len;
       }
-      sst.push_back(t0);
-      sfl.push_back(flat);
-      flat += len;
     }
     if ((uint32_t)sst.size() > first) it1.push_back(make_uint4(c, first, (uint32_t)sst.size(), flat));
   }
   L.n_items2 = (uint32_t)it2.size();
+  L.n_items3 = (uint32_t)it3.size();
+  L.n_slots = slots;
   L.n_items1 = (uint32_t)it1.size();
   L.n_segs = (uint32_t)sst.size();
   L.items2.alloc(std::max<size_t>(it2.size(), 1));
+  L.items3.alloc(std::max<size_t>(it3.size(), 1));
+  if (!it3.empty()) NWG_HIP(hipMemcpyAsync(L.items3.p, it3.data(), it3.size() * sizeof(uint4), hipMemcpyHostToDevice, s));
   L.items1.alloc(std::max<size_t>(it1.size(), 1));
   L.seg_start.alloc(std::max<size_t>(sst.size(), 1));
   L.seg_flat.alloc(std::max<size_t>(sfl.size(), 1));
@@ -305,6 +424,7 @@ void binned_build(hipStream_t s, const ActiveCsc& a, uint32_t n, uint64_t nnz, B
 void binned_enqueue_sweep(hipStream_t s, const BinParams& P) {
   if (P.n_items1) pr_chunk_kernel<<<P.n_items1, kBT, 0, s>>>(P);
   pr_bin_kernel<<<P.n_items2, kBT, 0, s>>>(P);
+  if (P.n_items3) pr_bin_combine_kernel<<<dim3(kBinRows / 256, P.n_items3), 256, 0, s>>>(P);
   NWG_LAUNCH_CHECK();
 }
 

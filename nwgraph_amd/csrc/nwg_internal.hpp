@@ -93,15 +93,16 @@ __host__ __device__ inline uint32_t slice_of(uint32_t u) { return (u >> kSliceSh
 // source are kept.  The chunk pass writes contrib[u] into every entry of a chunk (from
 // the chunk staged in LDS), the bin pass streams each bin and adds its entries into LDS
 // row accumulators.
-constexpr int kBinBits = 14;    // rows per bin: 16384 int64 accumulators = 128 KiB of LDS
+constexpr int kBinBits = 13;    // rows per bin: 8192 int64 accumulators = 64 KiB of LDS (2 workgroups per CU)
 constexpr int kChunkBits = 15;  // sources per chunk: 32768 floats = 128 KiB of LDS
 constexpr uint32_t kBinMaxSegs = 2048;  // tiles of one chunk-pass work item
 struct BinnedLayout {
   bool built = false;
-  uint32_t n_bins = 0, n_chunks = 0, n_items2 = 0, n_items1 = 0, n_segs = 0;
+  uint32_t n_bins = 0, n_chunks = 0, n_items2 = 0, n_items1 = 0, n_items3 = 0, n_segs = 0, n_slots = 0;
   uint64_t nnz = 0, padded = 0;  // entries, and the length of the per-entry arrays
   DevBuf<uint16_t> brow, bsrc;   // row & (2^kBinBits - 1), source & (2^kChunkBits - 1)
-  DevBuf<uint4> items2;          // bin pass: {bin, first entry, end entry, items of the bin}
+  DevBuf<uint4> items2;          // bin pass: {bin, first entry, end entry, partial slot (or none)}
+  DevBuf<uint4> items3;          // combine pass: {split bin, first slot, slots}
   DevBuf<uint4> items1;          // chunk pass: {chunk, first tile, end tile, entries}
   DevBuf<uint32_t> seg_start;    // non-empty tiles of each chunk: first entry
   DevBuf<uint32_t> seg_flat;     // and offset inside their chunk-pass item
@@ -138,10 +139,10 @@ struct BinParams {
   float* vals;
   const uint4* items2;
   const uint4* items1;
+  const uint4* items3;
   const uint32_t* seg_start;
   const uint32_t* seg_flat;
-  unsigned long long* gacc;  // n_bins << kBinBits row sums of split bins (kept zero between sweeps)
-  uint32_t* bin_done;        // finished items per bin (kept zero between sweeps)
+  double* part;              // n_slots << kBinBits partial row sums of the split bins' items
   const uint32_t* deg;
   float* x;
   float* contrib;
@@ -149,7 +150,7 @@ struct BinParams {
   float base;
   float damp;
   uint32_t n;
-  uint32_t n_items2, n_items1;
+  uint32_t n_items2, n_items1, n_items3;
   unsigned long long* acc;
   uint32_t iter;
   double threshold;
@@ -216,8 +217,7 @@ struct PrState {
   DevBuf<double> blk_err;          // per-workgroup L1 partials of one sweep
   DevBuf<float> spart;             // kSlices x n_srows partial sums of the sliced rows
   DevBuf<float> bvals;             // binned sweep: one value per entry
-  DevBuf<unsigned long long> gacc; // binned sweep: row sums of split bins
-  DevBuf<uint32_t> bin_done;       // binned sweep: item counters per bin
+  DevBuf<double> bpart;            // binned sweep: partial rows of split bins
   Captured cap;
 };
 
@@ -228,6 +228,7 @@ struct PprState {
   uint32_t seg_tile = 0;
   uint32_t n_heavy = 0, n_segs = 0;
   DevBuf<uint32_t> heavy_rows, heavy_seg_ptr, heavy_seg_row;
+  DevBuf<uint32_t> lrp, lcol;  // light rows' in-edges, compacted (heavy rows empty)
   DevBuf<uint32_t> degflag, seed_row, seed_vec, ids, ids_new;
   DevBuf<float> seed_base, c0, c1, x, hub_part;
   DevBuf<double> blk_err;

@@ -648,8 +648,9 @@ static void enqueue_solve(hipStream_t s, nwg_graph* g, PrState& st, SweepParams 
     B.items1 = L.items1.p;
     B.seg_start = L.seg_start.p;
     B.seg_flat = L.seg_flat.p;
-    B.gacc = st.gacc.p;
-    B.bin_done = st.bin_done.p;
+    B.part = st.bpart.p;
+    B.items3 = L.items3.p;
+    B.n_items3 = L.n_items3;
     B.deg = P.deg;
     B.x = P.x;
     B.contrib = st.contrib0.p;
@@ -699,11 +700,7 @@ void pagerank_run(nwg_ctx* ctx, nwg_graph* g, const float* teleport_dev, float d
   if (g->plan.params.binned) {
     const BinnedLayout& L = g->plan.bl;
     st.bvals.ensure(L.padded);
-    st.gacc.ensure((uint64_t)L.n_bins << kBinBits);
-    st.bin_done.ensure(L.n_bins);
-    // the bin pass keeps these zero between sweeps; clear them here, outside any capture
-    NWG_HIP(hipMemsetAsync(st.gacc.p, 0, st.gacc.n * sizeof(unsigned long long), s));
-    NWG_HIP(hipMemsetAsync(st.bin_done.p, 0, st.bin_done.n * sizeof(uint32_t), s));
+    st.bpart.ensure(std::max<uint64_t>((uint64_t)L.n_slots << kBinBits, 1));
   }
   const bool personalised = teleport_dev != nullptr;
   if (personalised) {
@@ -729,8 +726,7 @@ void pagerank_run(nwg_ctx* ctx, nwg_graph* g, const float* teleport_dev, float d
     key.ptrs[7] = st.blk_err.p;
     key.ptrs[8] = st.spart.p;
     key.ptrs[9] = st.bvals.p;
-    key.ptrs[10] = st.gacc.p;
-    key.ptrs[11] = st.bin_done.p;
+    key.ptrs[10] = st.bpart.p;
     key.plan_version = g->plan_version;
     key.iters = max_iters;
     key.damping = damping;

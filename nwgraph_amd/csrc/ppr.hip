@@ -9,11 +9,12 @@
 // it in one coalesced access while the column-id stream is shared by all k vectors.
 //
 // Work split per sweep (one launch + one finishing launch):
-//   row workgroups   4 * (64/T) consecutive rows x one k-tile; a T-lane group walks the
-//                    in-edges of its row one by one, so only light rows (in-degree <= 64)
-//                    are done here
-//   heavy segments   (256/T)*32 in-edges of one heavier row x one k-tile, 256/T edge
-//                    groups striding over them -> partial vectors
+//   row workgroups   (256/T) lane groups x 8 (T = 8: 4) consecutive rows x one k-tile; a
+//                    group walks the in-edges of its rows as one flat list over a compacted
+//                    copy of the light rows (in-degree <= 64), T ids per coalesced load
+//   heavy segments   (256/T)*32 in-edges of one heavier row x one k-tile; lane group g
+//                    takes 32 contiguous edges with all 32 gathers in flight -> partial
+//                    vectors
 //   finish           hub rows (partials added in segment order) and per-vector L1
 //                    partials folded into acc[iter][j] (fixed point, order independent)
 // A vector whose previous sweep converged is skipped (its lanes idle); a workgroup
@@ -34,10 +35,19 @@ constexpr int kThreads = 256;
 constexpr uint32_t kSeedFlag = 0x80000000u;
 // gathers a lane issues before it adds them (in edge order): memory-level parallelism
 constexpr uint32_t kUnroll = 8;
+// edges of a heavy-row segment per lane group (all their gathers are in flight at once)
+constexpr uint32_t kSegPerGroup = 32;
+// consecutive light rows per lane group (lanes 0..kR of the group hold their row pointers)
+template <int T>
+struct PprRows {
+  static constexpr uint32_t value = T >= 16 ? 8u : 4u;
+};
 
 struct PprParams {
   const uint32_t* rp;
   const uint32_t* col;
+  const uint32_t* lrp;   // light CSC: rows with in-degree <= light_max, heavy rows empty
+  const uint32_t* lcol;
   const uint32_t* degflag;   // out-degree | kSeedFlag when the row is a seed of some vector
   const uint32_t* seed_row;  // sorted (row, vector) seed entries
   const uint32_t* seed_vec;
@@ -79,15 +89,19 @@ __device__ __forceinline__ float seed_base(const PprParams& P, uint32_t i, uint3
   return b;
 }
 
-__device__ __forceinline__ double ppr_update(const PprParams& P, uint32_t i, uint32_t j, float z) {
-  const uint32_t df = P.degflag[i];
+// update of row i, vector j from its sum z, its degree/seed word df and x_old xo
+__device__ __forceinline__ double ppr_finish_row(const PprParams& P, uint32_t i, uint32_t j, float z, uint32_t df,
+                                                 float xo) {
   const float b = (df & kSeedFlag) ? seed_base(P, i, j) : 0.f;
   const uint64_t o = (uint64_t)i * P.k + j;
-  const float xo = P.x[o];
   const float xn = __builtin_fmaf(P.damp, z, b);
   P.x[o] = xn;
   P.cout[o] = xn / (float)(df & ~kSeedFlag);
   return (double)fabsf(xn - xo);
+}
+
+__device__ __forceinline__ double ppr_update(const PprParams& P, uint32_t i, uint32_t j, float z) {
+  return ppr_finish_row(P, i, j, z, P.degflag[i], P.x[(uint64_t)i * P.k + j]);
 }
 
 // per-vector sum over the 64/T row slots of a wave, then over the 4 waves; result for
@@ -110,6 +124,7 @@ template <int T>
 __global__ __launch_bounds__(kThreads) void ppr_sweep_kernel(PprParams P) {
   __shared__ double red[4 * T];
   __shared__ float hub_red[kThreads];
+  __shared__ float zbuf[kThreads * PprRows<T>::value];
   __shared__ int any_live;
   const uint32_t tile = blockIdx.y;
   const uint32_t vl = threadIdx.x % T;
@@ -122,64 +137,105 @@ __global__ __launch_bounds__(kThreads) void ppr_sweep_kernel(PprParams P) {
   if (!any_live) return;
   const uint32_t b = blockIdx.x;
   if (b < P.row_blocks) {
+    // Light rows: each T-lane group takes kR consecutive rows and walks their in-edges as
+    // one flat list over the compacted light CSC (heavy rows are empty there), T column
+    // ids per coalesced load and kUnroll gathers in flight; each row's sum (in edge order)
+    // goes to LDS, and the kR row updates are issued together at the end.  Every test
+    // that steers the loops is uniform over the group, so all its lanes take part in the
+    // shuffles (converged vectors only skip their gathers and updates).
+    constexpr uint32_t kR = PprRows<T>::value;
     constexpr uint32_t kSlots = kThreads / T;
-    const uint32_t i = b * kSlots + threadIdx.x / T;
+    const uint32_t slot = threadIdx.x / T;
+    const uint32_t g0 = (b * kSlots + slot) * kR;
     double e = 0.0;
-    uint32_t p0 = 0, p1 = 0;
-    if (i < P.n) {
-      p0 = P.rp[i];
-      p1 = P.rp[i + 1];
-    }
-    // The row test is uniform over the T-lane group, so every lane of the group takes
-    // part in the shuffles below (converged vectors only skip their gathers).
-    if (i < P.n && p1 - p0 <= P.light_max) {
+    if (g0 < P.n) {
+      const uint32_t nr = min(kR, P.n - g0);
+      const uint32_t q = min(vl, nr);  // lanes 0..nr hold the row pointers of rows g0..g0+nr
+      const uint32_t my_lrp = P.lrp[g0 + q];
+      const uint32_t my_rp = P.rp[g0 + q];
+      const uint32_t pend = (uint32_t)__shfl((int)my_lrp, (int)nr, T);
+      uint32_t r = 0;
+      uint32_t rend = (uint32_t)__shfl((int)my_lrp, 1, T);
       float z = 0.f;
-      for (uint32_t p = p0; p < p1; p += T) {
-        const uint32_t m = min((uint32_t)T, p1 - p);
-        const uint32_t c = vl < m ? P.col[p + vl] : 0u;  // up to T ids in one coalesced load
-        for (uint32_t q = 0; q < m; q += kUnroll) {    // q + r < T: q is a multiple of 8 below m <= T
+      for (uint32_t p = (uint32_t)__shfl((int)my_lrp, 0, T); p < pend; p += T) {
+        const uint32_t m = min((uint32_t)T, pend - p);
+        const uint32_t c = vl < m ? P.lcol[p + vl] : 0u;
+        for (uint32_t q0 = 0; q0 < m; q0 += kUnroll) {  // q0 + t < T: q0 is a multiple of 8 below m <= T
           float v[kUnroll];
 #pragma unroll
-          for (uint32_t r = 0; r < kUnroll; ++r) {
-            const uint32_t u = (uint32_t)__shfl((int)c, (int)(q + r), T);
-            v[r] = (live && q + r < m) ? P.cin[(uint64_t)u * P.k + j] : 0.f;
+          for (uint32_t t = 0; t < kUnroll; ++t) {
+            const uint32_t u = (uint32_t)__shfl((int)c, (int)(q0 + t), T);
+            v[t] = (live && q0 + t < m) ? P.cin[(uint64_t)u * P.k + j] : 0.f;
           }
 #pragma unroll
-          for (uint32_t r = 0; r < kUnroll; ++r)
-            if (q + r < m) z += v[r];  // in edge order, as one sequential sum
+          for (uint32_t t = 0; t < kUnroll; ++t) {
+            if (q0 + t >= m) break;
+            while (p + q0 + t >= rend) {  // row r ends before this edge (r + 1 < nr here)
+              zbuf[(slot * kR + r) * T + vl] = z;
+              z = 0.f;
+              ++r;
+              rend = (uint32_t)__shfl((int)my_lrp, (int)(r + 1), T);
+            }
+            z += v[t];
+          }
         }
       }
-      if (live) e = ppr_update(P, i, j, z);
+      for (; r < nr; ++r) {
+        zbuf[(slot * kR + r) * T + vl] = z;
+        z = 0.f;
+      }
+      float xo[kR], zz[kR];
+      uint32_t df[kR];
+      bool light[kR];
+#pragma unroll
+      for (uint32_t t = 0; t < kR; ++t) {
+        const uint32_t d = (uint32_t)__shfl((int)my_rp, (int)(t + 1), T) - (uint32_t)__shfl((int)my_rp, (int)t, T);
+        light[t] = t < nr && live && d <= P.light_max;  // heavy rows are finished by the segment path
+        if (light[t]) {
+          const uint32_t i = g0 + t;
+          df[t] = P.degflag[i];
+          xo[t] = P.x[(uint64_t)i * P.k + j];
+          zz[t] = zbuf[(slot * kR + t) * T + vl];
+        }
+      }
+#pragma unroll
+      for (uint32_t t = 0; t < kR; ++t)
+        if (light[t]) e += ppr_finish_row(P, g0 + t, j, zz[t], df[t], xo[t]);
     }
     e = slot_reduce<T>(e, red);
     if (threadIdx.x < (uint32_t)T) P.blk_err[((uint64_t)tile * P.row_blocks + b) * T + threadIdx.x] = e;
   } else {
-    // hub segment s of one k-tile: kThreads/T edge groups stride over the segment
+    // heavy segment s of one k-tile: edge group g takes the contiguous kSegPerGroup edges
+    // [p0 + g*kSegPerGroup, ...), loads their ids in one pass and issues all the gathers
+    // before adding them in edge order; the group partials are added in group order
     const uint32_t s = b - P.row_blocks;
     const uint32_t h = P.seg_row[s];
     const uint32_t i = P.hub_rows[h];
-    const uint32_t p0 = P.rp[i] + (s - P.seg_ptr[h]) * P.hub_segment;
+    const uint32_t ps = P.rp[i] + (s - P.seg_ptr[h]) * P.hub_segment;
     const uint32_t pe = P.rp[i + 1];
-    const uint32_t p1 = (pe - p0 > P.hub_segment) ? p0 + P.hub_segment : pe;
-    constexpr uint32_t kGroups = kThreads / T;
-    float z = 0.f;
-    if (live) {
-      // edge group g takes p0 + g, p0 + g + kGroups, ...: kUnroll of them in flight at once
-      for (uint64_t p = p0 + threadIdx.x / T; p < p1; p += kGroups * kUnroll) {  // 64-bit: no wrap
-        float v[kUnroll];
+    const uint32_t p1 = (pe - ps > P.hub_segment) ? ps + P.hub_segment : pe;
+    const uint32_t p0 = ps + (threadIdx.x / T) * kSegPerGroup;
+    constexpr uint32_t kIdLoads = (kSegPerGroup + T - 1) / T;
+    uint32_t c[kIdLoads];
 #pragma unroll
-        for (uint32_t r = 0; r < kUnroll; ++r) {
-          const uint64_t q = p + r * kGroups;
-          v[r] = q < p1 ? P.cin[(uint64_t)P.col[q] * P.k + j] : 0.f;
-        }
-#pragma unroll
-        for (uint32_t r = 0; r < kUnroll; ++r)
-          if (p + r * kGroups < p1) z += v[r];
-      }
+    for (uint32_t t = 0; t < kIdLoads; ++t) {
+      const uint32_t pp = p0 + t * T + vl;
+      c[t] = (t * T + vl < kSegPerGroup && pp < p1) ? P.col[pp] : 0u;
     }
+    float v[kSegPerGroup];
+#pragma unroll
+    for (uint32_t t = 0; t < kSegPerGroup; ++t) {
+      const uint32_t u = (uint32_t)__shfl((int)c[t / T], (int)(t % T), T);
+      v[t] = (live && p0 + t < p1) ? P.cin[(uint64_t)u * P.k + j] : 0.f;
+    }
+    float z = 0.f;
+#pragma unroll
+    for (uint32_t t = 0; t < kSegPerGroup; ++t)
+      if (p0 + t < p1) z += v[t];
     hub_red[threadIdx.x] = z;
     __syncthreads();
     if (threadIdx.x < (uint32_t)T) {
+      constexpr uint32_t kGroups = kThreads / T;
       float t = 0.f;
       for (uint32_t g = 0; g < kGroups; ++g) t += hub_red[g * T + threadIdx.x];
       if (j < P.k) P.hub_part[(uint64_t)s * P.k + j] = t;
@@ -279,6 +335,26 @@ __global__ __launch_bounds__(kThreads) void unpermute_rows_kernel(uint32_t n, ui
 
 inline uint32_t grid1(uint64_t m) { return (uint32_t)ceil_div(m ? m : 1, kThreads); }
 
+// light CSC: row i keeps its in-edges when in-degree <= light_max, else none
+__global__ __launch_bounds__(kThreads) void light_len_kernel(uint32_t n, const uint32_t* __restrict__ rp,
+                                                             uint32_t light_max, uint32_t* __restrict__ len) {
+  const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
+  if (i < n) {
+    const uint32_t d = rp[i + 1] - rp[i];
+    len[i] = d <= light_max ? d : 0u;
+  }
+}
+
+__global__ __launch_bounds__(kThreads) void light_copy_kernel(uint32_t n, const uint32_t* __restrict__ rp,
+                                                              const uint32_t* __restrict__ col,
+                                                              const uint32_t* __restrict__ lrp,
+                                                              uint32_t* __restrict__ lcol) {
+  const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t d = lrp[i + 1] - lrp[i];
+  for (uint32_t t = 0; t < d; ++t) lcol[lrp[i] + t] = col[rp[i] + t];
+}
+
 }  // namespace
 
 // Host driver; tile = k-tile width (8, 16, 32 or 64).
@@ -299,11 +375,23 @@ void ppr_run(nwg_ctx* ctx, nwg_graph* g, uint32_t k, const std::vector<uint32_t>
   // that no lane group walks more than 32 edges of a row; rebuilt only when the plan
   // (hence the active CSC) or the k-tile changes.
   const uint32_t light_max = 64;
-  const uint32_t seg_edges = (kThreads / tile) * 32;
+  const uint32_t seg_edges = (kThreads / tile) * kSegPerGroup;
   if (st.seg_version != g->plan_version || st.seg_tile != tile) {
     st.seg_version = 0;
     build_segments(s, a.ptr, n, light_max + 1, seg_edges, st.heavy_rows, st.heavy_seg_ptr, st.heavy_seg_row,
                    &st.n_heavy, &st.n_segs);
+    DevBuf<uint32_t> len(n);
+    st.lrp.alloc((uint64_t)n + 1);
+    light_len_kernel<<<grid1(n), kThreads, 0, s>>>(n, a.ptr, light_max, len.p);
+    NWG_LAUNCH_CHECK();
+    exclusive_scan_u32(s, len.p, st.lrp.p, n);
+    uint32_t n_light = 0;
+    NWG_HIP(hipMemcpyAsync(&n_light, st.lrp.p + n, 4, hipMemcpyDeviceToHost, s));
+    NWG_HIP(hipStreamSynchronize(s));
+    st.lcol.alloc(std::max<uint32_t>(n_light, 1));
+    light_copy_kernel<<<grid1(n), kThreads, 0, s>>>(n, a.ptr, a.idx, st.lrp.p, st.lcol.p);
+    NWG_LAUNCH_CHECK();
+    NWG_HIP(hipStreamSynchronize(s));
     st.seg_version = g->plan_version;
     st.seg_tile = tile;
   }
@@ -355,7 +443,8 @@ void ppr_run(nwg_ctx* ctx, nwg_graph* g, uint32_t k, const std::vector<uint32_t>
   NWG_LAUNCH_CHECK();
 
   const uint32_t tiles = (uint32_t)ceil_div(k, tile);
-  const uint32_t row_blocks = (uint32_t)ceil_div(n, kThreads / tile);
+  const uint32_t rows_per_group = tile >= 16 ? 8u : 4u;  // PprRows<tile>
+  const uint32_t row_blocks = (uint32_t)ceil_div(n, (kThreads / tile) * rows_per_group);
   const uint64_t nk = (uint64_t)n * k;
   st.c0.ensure(nk);
   st.c1.ensure(nk);
@@ -370,6 +459,8 @@ void ppr_run(nwg_ctx* ctx, nwg_graph* g, uint32_t k, const std::vector<uint32_t>
   PprParams P{};
   P.rp = a.ptr;
   P.col = a.idx;
+  P.lrp = st.lrp.p;
+  P.lcol = st.lcol.p;
   P.degflag = st.degflag.p;
   P.seed_row = st.seed_row.p;
   P.seed_vec = st.seed_vec.p;
