@@ -27,6 +27,14 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
+
+# HBM bytes of one sweep measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE (each scaled by
+# what the counter reads for a stream copy of known size), for the configurations profiled in
+# profiles/: (workload, scale, binned, relabel) -> (bytes per sweep, profile file)
+MEASURED_TRAFFIC = {
+    ("rmat", 25, 1, 1): (7529360139, "profiles/pr_binned_sweep_rmat25.md"),
+    ("rmat", 25, 0, 1): (23300197580, "profiles/pr_sweep_rmat25_relabel.md"),
+}
 METRIC = "GTEPS per PR iteration + % HBM roofline; batched PPR/adjoint vs plain"
 
 
@@ -167,6 +175,16 @@ def run_plain(args, ctx, grid=False):
     log(f"sweep {sweep_ms:.4f} ms -> {achieved:.1f} GB/s algorithmic ({achieved / HBM_PEAK_GBS:.3f} of peak)")
 
     value = nnz * sweeps / (ms_per_step * 1e-3) / 1e9
+    if plan.get("binned"):
+        kernel_desc = "one sweep (pr_chunk_kernel + pr_bin_kernel + pr_bin_combine_kernel)"
+    else:
+        kernel_desc = ("one sweep (pr_tile_kernel + pr_group_kernel<4..64> + pr_sgroup_kernel<4..64> + pr_hubseg_kernel"
+                       " + pr_finish_kernel, those the plan uses)")
+    traffic, traffic_src = args.traffic, "--traffic" if args.traffic else None
+    if traffic is None and not grid and args.edge_factor == 16:
+        hit = MEASURED_TRAFFIC.get(("rmat", args.scale, int(plan.get("binned", 0)), int(plan.get("relabel", 0))))
+        if hit:
+            traffic, traffic_src = hit[0], f"rocprofv3 FETCH_SIZE + WRITE_SIZE, {hit[1]}"
     out = base_line(args, round(value, 3), "GTEPS", ms_per_step, "f32", {
         "workload": workload,
         "n": n,
@@ -183,9 +201,9 @@ def run_plain(args, ctx, grid=False):
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
         "frac": round(achieved / HBM_PEAK_GBS, 4),
-        "traffic": args.traffic,
-        "kernel": ("one sweep (pr_tile_kernel + pr_group_kernel<4..64> + pr_sgroup_kernel<4..64> + pr_hubseg_kernel"
-                   " + pr_finish_kernel, those the plan uses)"),
+        "traffic": traffic,
+        "traffic_source": traffic_src,
+        "kernel": kernel_desc,
         "kernel_ms": round(sweep_ms, 5),
         "algorithmic_bytes": b_iter,
     }

@@ -41,6 +41,8 @@ constexpr uint32_t kItem2Entries = 1u << 18;  // entries per bin-pass item
 constexpr uint32_t kItem1Entries = 1u << 19;  // entries per chunk-pass item
 constexpr uint32_t kPadSrc = 0xFFFFu;           // source offset of a pad entry (value 0)
 constexpr uint32_t kNoSlot = 0xFFFFFFFFu;     // bin-pass item of a bin that is one item
+constexpr uint32_t kGroups1 = 4;              // groups of 8 entries in flight per lane: chunk pass
+constexpr uint32_t kGroups2 = 2;              // and bin pass (more would cost it its second workgroup per CU)
 
 // Row sums are kept as integer-valued doubles in units of 2^-52: each contribution is
 // rounded to the nearest unit (ties to even), and as a row sum z <= 1 stays below 2^53
@@ -93,11 +95,11 @@ __global__ __launch_bounds__(kBT) void pr_chunk_kernel(BinParams P) {
   const uint32_t u0 = it.x << kChunkBits;
   for (uint32_t q = threadIdx.x; q < kChunkSrc; q += kBT) cst[q] = (u0 + q < P.n) ? P.contrib[u0 + q] : 0.f;
   __syncthreads();
-  for (uint32_t g0 = 0; g0 < m; g0 += 2 * kBT) {
-    uint32_t p[2];
-    uint4 uu[2];
+  for (uint32_t g0 = 0; g0 < m; g0 += kGroups1 * kBT) {
+    uint32_t p[kGroups1];
+    uint4 uu[kGroups1];
 #pragma unroll
-    for (uint32_t h = 0; h < 2; ++h) {
+    for (uint32_t h = 0; h < kGroups1; ++h) {
       const uint32_t g = g0 + h * kBT + threadIdx.x;
       if (g < m) {
         uint32_t lo = 0, hi = ns - 1;  // last tile j with sflat[j] <= g
@@ -111,7 +113,7 @@ __global__ __launch_bounds__(kBT) void pr_chunk_kernel(BinParams P) {
       }
     }
 #pragma unroll
-    for (uint32_t h = 0; h < 2; ++h) {
+    for (uint32_t h = 0; h < kGroups1; ++h) {
       if (g0 + h * kBT + threadIdx.x >= m) continue;
       const uint32_t w[4] = {uu[h].x, uu[h].y, uu[h].z, uu[h].w};
       float o[8];
@@ -160,17 +162,17 @@ __global__ __launch_bounds__(kBT) void pr_bin_kernel(BinParams P) {
   for (uint32_t q = threadIdx.x; q < kBinRows; q += kBT) lacc[q] = 0.0;
   __syncthreads();
   // A wave takes 512 consecutive entries (8 per lane, from an 8-aligned start: the arrays
-  // are padded), two such blocks in flight; the loop bounds are uniform over the wave so
+  // are padded), kGroups2 such blocks in flight; the loop bounds are uniform over the wave so
   // every lane takes part in the votes of wave_flush.  Runs of one row are added in
   // registers; a lane's last run goes through wave_flush.
   const uint32_t w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  for (uint64_t wq = (uint64_t)(it.y & ~7u) + 512u * w; wq < it.z; wq += 2u * 512u * (kBT / 64)) {
-    uint4 rw[2];
-    float4 va[2], vb[2];
+  for (uint64_t wq = (uint64_t)(it.y & ~7u) + 512u * w; wq < it.z; wq += kGroups2 * 512u * (kBT / 64)) {
+    uint4 rw[kGroups2];
+    float4 va[kGroups2], vb[kGroups2];
 #pragma unroll
-    for (uint32_t h = 0; h < 2; ++h) {
+    for (uint32_t h = 0; h < kGroups2; ++h) {
       // unconditional loads (clamped into the item's first group when past its end, then
-      // masked below) so that both groups' loads are issued before any is waited on
+      // masked below) so that all groups' loads are issued before any is waited on
       const uint64_t q = wq + h * 512u * (kBT / 64) + 8u * lane;
       const uint64_t qc = q < it.z ? q : (uint64_t)(it.y & ~7u);
       rw[h] = *reinterpret_cast<const uint4*>(P.brow + qc);
@@ -178,7 +180,7 @@ __global__ __launch_bounds__(kBT) void pr_bin_kernel(BinParams P) {
       vb[h] = *reinterpret_cast<const float4*>(P.vals + qc + 4);
     }
 #pragma unroll
-    for (uint32_t h = 0; h < 2; ++h) {
+    for (uint32_t h = 0; h < kGroups2; ++h) {
       const uint64_t q = wq + h * 512u * (kBT / 64) + 8u * lane;
       uint32_t cur = kNoRow;
       double run = 0.0;
