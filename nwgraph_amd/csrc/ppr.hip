@@ -5,8 +5,10 @@
 // base (1-d)/N replaced by (1-d) v_j, v_j uniform over a small seed set, and its own
 // stopping rule (first sweep with L1 change < threshold).  Scores and contributions
 // are row-major N x k, so one in-edge u of row i gathers the contiguous k-float
-// segment C[u][0..k): a T-lane slice of a wavefront (T = k-tile, 8/16/32/64) loads
-// it in one coalesced access while the column-id stream is shared by all k vectors.
+// T-float slice C[tile][u][0..T) of the k-tile being served (T = 8/16/32/64): a T-lane
+// slice of a wavefront loads it in one coalesced access.  Contributions are tile-major and
+// workgroup b serves tile b % tiles, so with tiles dividing 8 each XCD's L2 caches one
+// tile's contributions (T floats per source) instead of all k.
 //
 // Work split per sweep (one launch + one finishing launch):
 //   row workgroups   (256/T) lane groups x 8 (T = 8: 4) consecutive rows x one k-tile; a
@@ -35,6 +37,11 @@ constexpr int kThreads = 256;
 constexpr uint32_t kSeedFlag = 0x80000000u;
 // gathers a lane issues before it adds them (in edge order): memory-level parallelism
 constexpr uint32_t kUnroll = 8;
+// (heavy row, vector) items per finishing workgroup, and the vectors whose L1 changes that
+// workgroup first adds in LDS
+constexpr uint32_t kFinItems = 4096;
+constexpr uint32_t kFinMaxK = 256;
+constexpr uint32_t kFoldRows = 1024;  // row-workgroup L1 partials folded per finishing workgroup
 // edges of a heavy-row segment per lane group (all their gathers are in flight at once)
 constexpr uint32_t kSegPerGroup = 32;
 // consecutive light rows per lane group (lanes 0..kR of the group hold their row pointers)
@@ -57,6 +64,7 @@ struct PprParams {
   float* cout;
   float* x;
   uint32_t n, k;
+  uint32_t tiles, kt_shift;  // k-tiles and log2 of their width
   float damp;
   uint32_t light_max, hub_segment;  // rows with in-degree <= light_max take the row path
   uint32_t row_blocks;       // workgroups of rows per k-tile
@@ -89,14 +97,19 @@ __device__ __forceinline__ float seed_base(const PprParams& P, uint32_t i, uint3
   return b;
 }
 
+// Contributions are stored tile-major, [tile][row][T] (T = 1 << kt_shift vectors per tile),
+// so that the gathers of one k-tile touch only that tile's array; scores stay row-major.
+__device__ __forceinline__ uint64_t cidx(const PprParams& P, uint32_t i, uint32_t j) {
+  return (((uint64_t)(j >> P.kt_shift) * P.n + i) << P.kt_shift) + (j & ((1u << P.kt_shift) - 1));
+}
+
 // update of row i, vector j from its sum z, its degree/seed word df and x_old xo
 __device__ __forceinline__ double ppr_finish_row(const PprParams& P, uint32_t i, uint32_t j, float z, uint32_t df,
                                                  float xo) {
   const float b = (df & kSeedFlag) ? seed_base(P, i, j) : 0.f;
-  const uint64_t o = (uint64_t)i * P.k + j;
   const float xn = __builtin_fmaf(P.damp, z, b);
-  P.x[o] = xn;
-  P.cout[o] = xn / (float)(df & ~kSeedFlag);
+  P.x[(uint64_t)i * P.k + j] = xn;
+  P.cout[cidx(P, i, j)] = xn / (float)(df & ~kSeedFlag);
   return (double)fabsf(xn - xo);
 }
 
@@ -126,7 +139,9 @@ __global__ __launch_bounds__(kThreads) void ppr_sweep_kernel(PprParams P) {
   __shared__ float hub_red[kThreads];
   __shared__ float zbuf[kThreads * PprRows<T>::value];
   __shared__ int any_live;
-  const uint32_t tile = blockIdx.y;
+  // workgroup b serves k-tile b % tiles: with tiles dividing 8, every XCD (workgroup b runs
+  // on XCD b % 8) serves one k-tile, and its L2 caches that tile's contributions only
+  const uint32_t tile = blockIdx.x % P.tiles;
   const uint32_t vl = threadIdx.x % T;
   const uint32_t j = tile * T + vl;
   const bool live = j < P.k && !vec_done(P, j);
@@ -135,7 +150,8 @@ __global__ __launch_bounds__(kThreads) void ppr_sweep_kernel(PprParams P) {
   if (live) any_live = 1;
   __syncthreads();
   if (!any_live) return;
-  const uint32_t b = blockIdx.x;
+  const uint32_t b = blockIdx.x / P.tiles;
+  const float* __restrict__ ctile = P.cin + (uint64_t)tile * P.n * T;
   if (b < P.row_blocks) {
     // Light rows: each T-lane group takes kR consecutive rows and walks their in-edges as
     // one flat list over the compacted light CSC (heavy rows are empty there), T column
@@ -165,7 +181,7 @@ __global__ __launch_bounds__(kThreads) void ppr_sweep_kernel(PprParams P) {
 #pragma unroll
           for (uint32_t t = 0; t < kUnroll; ++t) {
             const uint32_t u = (uint32_t)__shfl((int)c, (int)(q0 + t), T);
-            v[t] = (live && q0 + t < m) ? P.cin[(uint64_t)u * P.k + j] : 0.f;
+            v[t] = (live && q0 + t < m) ? ctile[(uint64_t)u * T + vl] : 0.f;
           }
 #pragma unroll
           for (uint32_t t = 0; t < kUnroll; ++t) {
@@ -226,7 +242,7 @@ __global__ __launch_bounds__(kThreads) void ppr_sweep_kernel(PprParams P) {
 #pragma unroll
     for (uint32_t t = 0; t < kSegPerGroup; ++t) {
       const uint32_t u = (uint32_t)__shfl((int)c[t / T], (int)(t % T), T);
-      v[t] = (live && p0 + t < p1) ? P.cin[(uint64_t)u * P.k + j] : 0.f;
+      v[t] = (live && p0 + t < p1) ? ctile[(uint64_t)u * T + vl] : 0.f;
     }
     float z = 0.f;
 #pragma unroll
@@ -248,26 +264,48 @@ __global__ __launch_bounds__(kThreads) void ppr_sweep_kernel(PprParams P) {
 template <int T>
 __global__ __launch_bounds__(kThreads) void ppr_finish_kernel(PprParams P, uint32_t tiles) {
   __shared__ double red[kThreads];
-  // part 1: hub rows, one thread per (hub row, vector)
-  const uint64_t t = (uint64_t)blockIdx.x * kThreads + threadIdx.x;
+  __shared__ unsigned long long errj[kFinMaxK];
+  // part 1: heavy rows, kFinItems (heavy row, vector) items per workgroup; their L1 changes
+  // are added per vector in LDS (fixed point: order independent), then one atomic per vector
+  for (uint32_t q = threadIdx.x; q < kFinMaxK; q += kThreads) errj[q] = 0ull;
+  __syncthreads();
   const uint64_t hub_items = (uint64_t)P.n_hub_rows * P.k;
-  const uint32_t j = (uint32_t)(t % P.k);
-  double e = 0.0;
-  if (t < hub_items && !vec_done(P, j)) {
+  for (uint32_t m = threadIdx.x; m < kFinItems; m += kThreads) {
+    const uint64_t t = (uint64_t)blockIdx.x * kFinItems + m;
+    if (t >= hub_items) break;
+    const uint32_t j = (uint32_t)(t % P.k);
+    if (vec_done(P, j)) continue;
     const uint32_t h = (uint32_t)(t / P.k);
+    // the row's segment partials, added in segment order with 8 loads in flight (a hub row
+    // has hundreds of segments)
     float z = 0.f;
-    for (uint32_t s = P.seg_ptr[h]; s < P.seg_ptr[h + 1]; ++s) z += P.hub_part[(uint64_t)s * P.k + j];
-    e = ppr_update(P, P.hub_rows[h], j, z);
+    const uint32_t s1 = P.seg_ptr[h + 1];
+    for (uint32_t s = P.seg_ptr[h]; s < s1; s += 8) {
+      float v[8];
+#pragma unroll
+      for (uint32_t r = 0; r < 8; ++r) v[r] = s + r < s1 ? P.hub_part[(uint64_t)(s + r) * P.k + j] : 0.f;
+#pragma unroll
+      for (uint32_t r = 0; r < 8; ++r)
+        if (s + r < s1) z += v[r];
+    }
+    const double e = ppr_update(P, P.hub_rows[h], j, z);
+    if (e > 0.0) {
+      if (j < kFinMaxK) atomicAdd(&errj[j], err_to_fixed(e));
+      else atomicAdd(P.acc + (uint64_t)P.iter * P.k + j, err_to_fixed(e));
+    }
   }
-  if (t < hub_items && e > 0.0) atomicAdd(P.acc + (uint64_t)P.iter * P.k + j, err_to_fixed(e));
-  // part 2: fold blk_err rows; this workgroup folds 32 workgroup rows of every tile
+  __syncthreads();
+  for (uint32_t q = threadIdx.x; q < kFinMaxK && q < P.k; q += kThreads)
+    if (errj[q]) atomicAdd(P.acc + (uint64_t)P.iter * P.k + q, errj[q]);
+  // part 2: fold blk_err rows; this workgroup folds kFoldRows workgroup rows of every tile
+  // (few workgroups, so few atomics land on each vector's accumulator)
   const uint64_t rows_total = (uint64_t)P.row_blocks;
-  const uint64_t r0 = (uint64_t)blockIdx.x * 32;
+  const uint64_t r0 = (uint64_t)blockIdx.x * kFoldRows;
   for (uint32_t tl = 0; tl < tiles; ++tl) {
     double acc = 0.0;
     const uint32_t vl = threadIdx.x % T;
     const uint32_t slot = threadIdx.x / T;  // kThreads/T slots stride over the 32 rows
-    for (uint64_t r = r0 + slot; r < r0 + 32 && r < rows_total; r += kThreads / T)
+    for (uint64_t r = r0 + slot; r < r0 + kFoldRows && r < rows_total; r += kThreads / T)
       acc += P.blk_err[((uint64_t)tl * P.row_blocks + r) * T + vl];
     red[threadIdx.x] = acc;
     __syncthreads();
@@ -282,14 +320,14 @@ __global__ __launch_bounds__(kThreads) void ppr_finish_kernel(PprParams P, uint3
   }
 }
 
-__global__ __launch_bounds__(kThreads) void ppr_init_kernel(uint32_t n, uint32_t k, const uint32_t* __restrict__ deg,
-                                                            float init, float* __restrict__ x,
-                                                            float* __restrict__ c0) {
+__global__ __launch_bounds__(kThreads) void ppr_init_kernel(uint32_t n, uint32_t k, uint32_t kt_shift,
+                                                            const uint32_t* __restrict__ deg, float init,
+                                                            float* __restrict__ x, float* __restrict__ c0) {
   const uint64_t t = (uint64_t)blockIdx.x * kThreads + threadIdx.x;
   if (t < (uint64_t)n * k) {
-    const uint32_t i = (uint32_t)(t / k);
+    const uint32_t i = (uint32_t)(t / k), j = (uint32_t)(t % k);
     x[t] = init;
-    c0[t] = init / (float)deg[i];
+    c0[(((uint64_t)(j >> kt_shift) * n + i) << kt_shift) + (j & ((1u << kt_shift) - 1))] = init / (float)deg[i];
   }
 }
 
@@ -446,8 +484,9 @@ void ppr_run(nwg_ctx* ctx, nwg_graph* g, uint32_t k, const std::vector<uint32_t>
   const uint32_t rows_per_group = tile >= 16 ? 8u : 4u;  // PprRows<tile>
   const uint32_t row_blocks = (uint32_t)ceil_div(n, (kThreads / tile) * rows_per_group);
   const uint64_t nk = (uint64_t)n * k;
-  st.c0.ensure(nk);
-  st.c1.ensure(nk);
+  const uint64_t nc = (uint64_t)tiles * n * tile;  // tile-major contributions, last tile padded
+  st.c0.ensure(nc);
+  st.c1.ensure(nc);
   if (a.order) st.x.ensure(nk);
   float* x = a.order ? st.x.p : scores_dev;
   st.hub_part.ensure((uint64_t)std::max<uint32_t>(n_segs, 1) * k);
@@ -469,6 +508,8 @@ void ppr_run(nwg_ctx* ctx, nwg_graph* g, uint32_t k, const std::vector<uint32_t>
   P.x = x;
   P.n = n;
   P.k = k;
+  P.tiles = tiles;
+  P.kt_shift = (uint32_t)__builtin_ctz(tile);
   P.damp = damping;
   P.light_max = light_max;
   P.hub_segment = seg_edges;
@@ -485,12 +526,14 @@ void ppr_run(nwg_ctx* ctx, nwg_graph* g, uint32_t k, const std::vector<uint32_t>
 
   const float init = (float)(1.0 / n);
   NWG_HIP(hipEventRecord(ctx->ev0, s));
-  ppr_init_kernel<<<grid1(nk), kThreads, 0, s>>>(n, k, a.deg, init, x, c0p);
+  ppr_init_kernel<<<grid1(nk), kThreads, 0, s>>>(n, k, P.kt_shift, a.deg, init, x, c0p);
   NWG_LAUNCH_CHECK();
   ppr_zero_acc_kernel<<<grid1(max_iters * k), kThreads, 0, s>>>(st.acc.p, max_iters * k);
   NWG_LAUNCH_CHECK();
-  const dim3 grid(row_blocks + n_segs, tiles);
-  const uint64_t fin_a = ceil_div((uint64_t)n_heavy * k, kThreads), fin_b = ceil_div(row_blocks, 32);
+  const uint64_t grid64 = (uint64_t)(row_blocks + n_segs) * tiles;
+  NWG_REQUIRE(grid64 < (1ull << 31), "batched solve too large for one launch");
+  const uint32_t grid = (uint32_t)grid64;
+  const uint64_t fin_a = ceil_div((uint64_t)n_heavy * k, kFinItems), fin_b = ceil_div(row_blocks, kFoldRows);
   const uint32_t fin = (uint32_t)std::max<uint64_t>(std::max(fin_a, fin_b), 1);
   for (uint64_t it = 0; it < max_iters; ++it) {
     P.iter = (uint32_t)it;
