@@ -15,10 +15,9 @@
 //               reference's update (page_rank.hpp:133-135): x = fma(d, z, base),
 //               |x - x_old| into the sweep's L1 change, contrib = x / deg.
 //
-// Row sums are exact sums of the contributions rounded to units of 2^-52 (integer-valued
-// doubles, see quantize), so they do not depend on how the entries are split or ordered:
-// results are deterministic, and z is within a few 2^-52 of the exact sum of the float
-// contributions.  The L1 change is
+// Row sums are exact fixed-point sums of the contributions (units of 2^-60, see quantize), so
+// they do not depend on how the entries are split or ordered: results are deterministic,
+// and z is the float rounding of the (nearly) exact sum of the float contributions.  The L1 change is
 // accumulated in fixed point as in the pull sweep (device_utils.hpp), and every
 // workgroup first checks the previous sweep's change against the threshold
 // (page_rank.hpp:144-146), so a captured loop stops working after convergence.
@@ -44,13 +43,36 @@ constexpr uint32_t kNoSlot = 0xFFFFFFFFu;     // bin-pass item of a bin that is 
 constexpr uint32_t kGroups1 = 4;              // groups of 8 entries in flight per lane: chunk pass
 constexpr uint32_t kGroups2 = 2;              // and bin pass (more would cost it its second workgroup per CU)
 
-// Row sums are kept as integer-valued doubles in units of 2^-52: each contribution is
-// rounded to the nearest unit (ties to even), and as a row sum z <= 1 stays below 2^53
-// units, every add is exact: the sum does not depend on the order or the split of the
-// entries.  (z <= sum of all x <= 1: x starts summing to 1 and a sweep maps a vector of
-// sum <= 1 to one of sum <= 1.)
-__device__ __forceinline__ double quantize(float c) { return __builtin_rint((double)c * 0x1p52); }
-__device__ __forceinline__ float dequantize(double v) { return (float)(v * 0x1p-52); }
+// Row sums are exact 64-bit fixed-point sums in units of 2^-60.  A float contribution
+// c = m * 2^(e-150) (m the 24-bit significand) is converted by shifting m: exact for
+// c >= 2^-37, rounded to the nearest unit (ties away from zero) below that.  As a row sum
+// z <= 1 stays below 2^60 units, integer adds never overflow and the sum does not depend on
+// the order or the split of the entries.  (z <= sum of all x <= 1: x starts summing to 1
+// and a sweep maps a vector of sum <= 1 to one of sum <= 1.)  No FP64 arithmetic is used.
+__device__ __forceinline__ unsigned long long quantize(float c) {
+  const uint32_t b = __float_as_uint(c);  // c >= 0 and finite
+  const uint32_t e = b >> 23;
+  const unsigned long long m = (b & 0x7FFFFFu) | (e ? 0x800000u : 0u);
+  const int sh = (int)(e ? e : 1u) - 90;  // c * 2^60 = m * 2^sh
+  if (sh >= 0) return m << sh;
+  if (sh <= -25) return 0ull;  // below half a unit
+  return (m + (1ull << (-sh - 1))) >> -sh;
+}
+// v * 2^-60 rounded to the nearest float (ties to even), with integer operations only
+__device__ __forceinline__ float dequantize(unsigned long long v) {
+  if (v == 0ull) return 0.f;
+  const int lz = __clzll((long long)v);
+  const unsigned long long nv = v << lz;  // bit 63 set
+  uint32_t mant = (uint32_t)(nv >> 40);   // 24 significant bits
+  const unsigned long long rest = nv & ((1ull << 40) - 1), half = 1ull << 39;
+  if (rest > half || (rest == half && (mant & 1u))) ++mant;
+  int ex = 63 - lz - 60 - 23;  // value = mant * 2^ex
+  if (mant == (1u << 24)) {
+    mant >>= 1;
+    ++ex;
+  }
+  return __builtin_ldexpf((float)mant, ex);  // exact: mant < 2^24, result a normal float
+}
 
 __device__ __forceinline__ bool converged_before(const BinParams& P) {
   return P.iter > 0 && fixed_to_err(P.acc[P.iter - 1]) < P.threshold;
@@ -130,19 +152,19 @@ __global__ __launch_bounds__(kBT) void pr_chunk_kernel(BinParams P) {
 
 constexpr uint32_t kNoRow = 0xFFFFFFFFu;
 
-// Adds each lane's run (row cur, quantized sum run; cur == kNoRow: none) into lacc.  When
+// Adds each lane's run (row cur, fixed-point sum run; cur == kNoRow: none) into lacc.  When
 // every run in the wave belongs to one row (a hub row filling the wave), the runs are summed
 // across the wave first and one lane issues the atomic, instead of 64 atomics on one address.
-// The values are integers below 2^53, so every add is exact and the order does not matter.
-// All lanes of the wave must call it together.
-__device__ __forceinline__ void wave_flush(double* lacc, uint32_t cur, double run, uint32_t lane) {
+// Integer adds: the order does not matter.  All lanes of the wave must call it together.
+__device__ __forceinline__ void wave_flush(unsigned long long* lacc, uint32_t cur, unsigned long long run,
+                                           uint32_t lane) {
   const bool has = cur != kNoRow;
   const unsigned long long m = __ballot(has);
   if (!m) return;
   const int first = __builtin_ctzll(m);
   const uint32_t row0 = (uint32_t)__shfl((int)cur, first);
   if (__ballot(has && cur != row0) == 0ull) {
-    double t = has ? run : 0.0;
+    unsigned long long t = has ? run : 0ull;
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) t += __shfl_xor(t, o);
     if ((int)lane == first) atomicAdd(&lacc[row0], t);
@@ -155,11 +177,11 @@ __device__ __forceinline__ void wave_flush(double* lacc, uint32_t cur, double ru
 // its rows here, an item of a split bin stores its partial rows in its slot of P.part
 __global__ __launch_bounds__(kBT) void pr_bin_kernel(BinParams P) {
   if (converged_before(P)) return;
-  __shared__ double lacc[kBinRows];
+  __shared__ unsigned long long lacc[kBinRows];
   __shared__ double red[kBT / 64];
   const uint4 it = P.items2[blockIdx.x];  // {bin, first entry, end entry, partial slot or kNoSlot}
   const uint32_t b = it.x;
-  for (uint32_t q = threadIdx.x; q < kBinRows; q += kBT) lacc[q] = 0.0;
+  for (uint32_t q = threadIdx.x; q < kBinRows; q += kBT) lacc[q] = 0ull;
   __syncthreads();
   // A wave takes 512 consecutive entries (8 per lane, from an 8-aligned start: the arrays
   // are padded), kGroups2 such blocks in flight; the loop bounds are uniform over the wave so
@@ -183,7 +205,7 @@ __global__ __launch_bounds__(kBT) void pr_bin_kernel(BinParams P) {
     for (uint32_t h = 0; h < kGroups2; ++h) {
       const uint64_t q = wq + h * 512u * (kBT / 64) + 8u * lane;
       uint32_t cur = kNoRow;
-      double run = 0.0;
+      unsigned long long run = 0ull;
       if (q < it.z) {
         const uint32_t rows[8] = {rw[h].x & 0xFFFFu, rw[h].x >> 16, rw[h].y & 0xFFFFu, rw[h].y >> 16,
                                   rw[h].z & 0xFFFFu, rw[h].z >> 16, rw[h].w & 0xFFFFu, rw[h].w >> 16};
@@ -192,7 +214,7 @@ __global__ __launch_bounds__(kBT) void pr_bin_kernel(BinParams P) {
         for (uint32_t r = 0; r < 8; ++r) {
           const uint64_t e = q + r;
           if (e < it.y || e >= it.z) continue;
-          const double f = quantize(vs[r]);
+          const unsigned long long f = quantize(vs[r]);
           if (rows[r] == cur) {
             run += f;
           } else {
@@ -208,7 +230,7 @@ __global__ __launch_bounds__(kBT) void pr_bin_kernel(BinParams P) {
   __syncthreads();
   const uint32_t r0 = b << kBinBits;
   if (it.w != kNoSlot) {
-    double* dst = P.part + (uint64_t)it.w * kBinRows;
+    unsigned long long* dst = P.part + (uint64_t)it.w * kBinRows;
     for (uint32_t v = threadIdx.x; v < kBinRows; v += kBT) dst[v] = lacc[v];
     return;
   }
@@ -228,8 +250,8 @@ __global__ __launch_bounds__(256) void pr_bin_combine_kernel(BinParams P) {
   const uint32_t i = (it.x << kBinBits) + v;
   double e = 0.0;
   if (i < P.n) {
-    const double* src = P.part + (uint64_t)it.y * kBinRows + v;
-    double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+    const unsigned long long* src = P.part + (uint64_t)it.y * kBinRows + v;
+    unsigned long long s0 = 0ull, s1 = 0ull, s2 = 0ull, s3 = 0ull;
     uint32_t k = 0;
     for (; k + 4 <= it.z; k += 4) {
       s0 += src[(uint64_t)k * kBinRows];

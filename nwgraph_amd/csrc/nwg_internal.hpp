@@ -142,7 +142,7 @@ struct BinParams {
   const uint4* items3;
   const uint32_t* seg_start;
   const uint32_t* seg_flat;
-  double* part;              // n_slots << kBinBits partial row sums of the split bins' items
+  unsigned long long* part;  // n_slots << kBinBits partial row sums (fixed point) of the split bins' items
   const uint32_t* deg;
   float* x;
   float* contrib;
@@ -217,7 +217,7 @@ struct PrState {
   DevBuf<double> blk_err;          // per-workgroup L1 partials of one sweep
   DevBuf<float> spart;             // kSlices x n_srows partial sums of the sliced rows
   DevBuf<float> bvals;             // binned sweep: one value per entry
-  DevBuf<double> bpart;            // binned sweep: partial rows of split bins
+  DevBuf<unsigned long long> bpart;  // binned sweep: partial rows of split bins
   Captured cap;
 };
 
