@@ -107,8 +107,19 @@ __global__ __launch_bounds__(kThreads) void wbase_kernel(uint32_t n, const float
   if (i < n) base[i] = v ? (float)(omd * (double)v[i]) : (float)(omd / (double)n);
 }
 
-// Per-row sums over the CSR of v[q] = a[perm[q]] * (b ? b[perm[q]] : 1), accumulated in
-// double in CSR order (deterministic), on the same LDS work items as the sweep.
+// Per-row sums over the CSR of v[q] = a[perm[q]] * (b ? b[perm[q]] : 1), in CSR order
+// (deterministic), on the same LDS work items as the sweep.  FP32 with Kahan compensation:
+// the error stays within a few ulp of the sum whatever the row length.
+struct Kahan {
+  float s = 0.f, c = 0.f;
+  __device__ __forceinline__ void add(float v) {
+    const float y = v - c;
+    const float t = s + y;
+    c = (t - s) - y;
+    s = t;
+  }
+};
+
 __global__ __launch_bounds__(kThreads) void csr_rowsum_kernel(const uint32_t* __restrict__ rp,
                                                               const uint32_t* __restrict__ perm,
                                                               const float* __restrict__ a,
@@ -116,31 +127,34 @@ __global__ __launch_bounds__(kThreads) void csr_rowsum_kernel(const uint32_t* __
                                                               const uint32_t* __restrict__ item_r0,
                                                               const uint32_t* __restrict__ item_r1, uint32_t n_items,
                                                               float* __restrict__ out) {
-  __shared__ double vals[kItemNnz];
-  __shared__ double red_d[kThreads / 64];
+  __shared__ float vals[kItemNnz];
+  __shared__ float red_f[kThreads / 64];
   for (uint32_t it = blockIdx.x; it < n_items; it += gridDim.x) {
     const uint32_t r0 = item_r0[it], r1 = item_r1[it];
     const uint32_t p0 = rp[r0], p1 = rp[r1];
     if (p1 - p0 > kItemNnz) {
-      // a single long row: strided partial sums, fixed-tree reduction
-      double acc = 0.0;
+      // a single long row: strided compensated partial sums, fixed-tree reduction of the
+      // partials and of their compensations
+      Kahan k;
       for (uint32_t q = p0 + threadIdx.x; q < p1; q += kThreads) {
         const uint32_t e = perm[q];
-        acc += (double)a[e] * (b ? (double)b[e] : 1.0);
+        k.add(b ? a[e] * b[e] : a[e]);
       }
-      acc = block_sum<kThreads>(acc, red_d);
-      if (threadIdx.x == 0) out[r0] = (float)acc;
+      const float hi = block_sum<kThreads>(k.s, red_f);
+      __syncthreads();
+      const float lo = block_sum<kThreads>(-k.c, red_f);
+      if (threadIdx.x == 0) out[r0] = hi + lo;
     } else {
       for (uint32_t q = threadIdx.x; q < p1 - p0; q += kThreads) {
         const uint32_t e = perm[p0 + q];
-        vals[q] = (double)a[e] * (b ? (double)b[e] : 1.0);
+        vals[q] = b ? a[e] * b[e] : a[e];
       }
       __syncthreads();
       const uint32_t u = r0 + threadIdx.x;
       if (u < r1) {
-        double acc = 0.0;
-        for (uint32_t q = rp[u] - p0; q < rp[u + 1] - p0; ++q) acc += vals[q];
-        out[u] = (float)acc;
+        Kahan k;
+        for (uint32_t q = rp[u] - p0; q < rp[u + 1] - p0; ++q) k.add(vals[q]);
+        out[u] = k.s - k.c;
       }
     }
     __syncthreads();
