@@ -215,18 +215,27 @@ def test_pr_driver_karate(tmp_path):
     assert all(int(r.split()[2]) == 4 for r in rows)
 
 
-def test_edge_cases(ctx):
+def with_plan(g, plan):
+    if plan:
+        p = g.get_plan()
+        p.update(plan)
+        g.set_plan(**p)
+    return g
+
+
+@pytest.mark.parametrize("plan", [{}, {"binned": 1}, {"binned": 1, "relabel": 1}])
+def test_edge_cases(ctx, plan):
     # no edges: every score becomes base after the first sweep
-    g = ctx.graph_from_edges(5, np.zeros(0, np.uint32), np.zeros(0, np.uint32))
+    g = with_plan(ctx.graph_from_edges(5, np.zeros(0, np.uint32), np.zeros(0, np.uint32)), plan)
     x, info = g.pagerank(D, 1e-4, 20)
     base = np.float32((1.0 - np.float64(np.float32(D))) / 5)
     assert np.all(x == base) and info["iterations"] == 2
     # one vertex with a self loop removed by cleaning
-    g1 = ctx.graph_from_edges(1, np.array([0], np.uint32), np.array([0], np.uint32))
+    g1 = with_plan(ctx.graph_from_edges(1, np.array([0], np.uint32), np.array([0], np.uint32)), plan)
     x1, _ = g1.pagerank(D, 1e-4, 20)
     assert g1.nnz == 0 and x1[0] == np.float32((1.0 - np.float64(np.float32(D))) / 1)
     # max_iters = 0 leaves the initial scores (page_rank.hpp:100-104)
-    g2 = ctx.rmat(10, 8, seed=1)
+    g2 = with_plan(ctx.rmat(10, 8, seed=1), plan)
     x2, info2 = g2.pagerank(D, 1e-4, 0)
     assert info2["iterations"] == 0 and np.all(x2 == np.float32(1.0 / 1024))
     # a huge threshold stops after one sweep
@@ -236,6 +245,18 @@ def test_edge_cases(ctx):
     assert info3["iterations"] == it == 1 and np.max(np.abs(x3 - ref) / ref) < 1e-6
     # dangling vertices lose mass: sum of scores stays below 1 (no redistribution)
     assert x3.sum() < 1.0
+    # a star into vertex 0 and a path, n not a multiple of any bin or chunk size, with
+    # isolated vertices: rows of every length from 0 to 40000
+    n = 70001
+    src = np.concatenate([np.arange(1, 40001), np.arange(40001, 69999)]).astype(np.uint32)
+    dst = np.concatenate([np.zeros(40000), np.arange(40002, 70000)]).astype(np.uint32)
+    g4 = with_plan(ctx.graph_from_edges(n, src, dst), plan)
+    ptr, idx, deg = g4.csc()
+    x4, info4 = g4.pagerank(D, 0.0, 30)
+    # checked against FP64 Jacobi: the float oracle's sequential 40000-term sum into vertex 0
+    # drifts by ~1e-5, the device's tree or exact sums do not
+    x64, _ = O.fp64_jacobi(n, ptr, idx, deg, damping=float(np.float32(D)), tol=0.0, max_iters=30)
+    assert info4["iterations"] == 30 and np.max(np.abs(x4 - x64)) <= 1e-6
 
 
 def test_repeat_calls_bit_identical_and_timing_mode(ctx):
