@@ -155,11 +155,12 @@ def run_plain(args, ctx, grid=False):
     buf = torch.empty(n, dtype=torch.float32, device="cuda")
     torch.cuda.synchronize()
     for _ in range(args.warmup):
-        g.pagerank_device(buf.data_ptr(), None, 0.85, args.tol, iters)
+        g.pagerank_device(buf.data_ptr(), None, 0.85, args.tol, iters, inplace=args.inplace)
     log(f"warmup {args.warmup} done")
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    infos = [g.pagerank_device(buf.data_ptr(), None, 0.85, args.tol, iters) for _ in range(args.steps)]
+    infos = [g.pagerank_device(buf.data_ptr(), None, 0.85, args.tol, iters, inplace=args.inplace)
+             for _ in range(args.steps)]
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     ms_per_step = wall * 1e3 / args.steps
@@ -168,21 +169,23 @@ def run_plain(args, ctx, grid=False):
     log(f"timed {args.steps} steps: {ms_per_step:.3f} ms/step wall, {ev_ms:.3f} ms/step events, {sweeps} sweeps")
 
     # live per-sweep kernel timing (HIP events on the library's stream around each sweep's launches)
-    kinfo = [g.pagerank_device(buf.data_ptr(), None, 0.85, args.tol, iters, kernel_timing=True) for _ in range(3)]
+    kinfo = [g.pagerank_device(buf.data_ptr(), None, 0.85, args.tol, iters, kernel_timing=True, inplace=args.inplace)
+             for _ in range(3)]
     sweep_ms = float(np.median([k["sweep_ms_avg"] for k in kinfo]))
     b_iter = bytes_per_iter(n, nnz)
     achieved = b_iter / (sweep_ms * 1e-3) / 1e9
     log(f"sweep {sweep_ms:.4f} ms -> {achieved:.1f} GB/s algorithmic ({achieved / HBM_PEAK_GBS:.3f} of peak)")
 
     value = nnz * sweeps / (ms_per_step * 1e-3) / 1e9
-    if plan.get("binned"):
+    binned = bool(plan.get("binned")) and not args.inplace  # the in-place schedule runs the pull kernels
+    if binned:
         kernel_desc = "one sweep (pr_chunk_kernel + pr_bin_kernel + pr_bin_combine_kernel)"
     else:
         kernel_desc = ("one sweep (pr_tile_kernel + pr_group_kernel<4..64> + pr_sgroup_kernel<4..64> + pr_hubseg_kernel"
                        " + pr_finish_kernel, those the plan uses)")
     traffic, traffic_src = args.traffic, "--traffic" if args.traffic else None
     if traffic is None and not grid and args.edge_factor == 16:
-        hit = MEASURED_TRAFFIC.get(("rmat", args.scale, int(plan.get("binned", 0)), int(plan.get("relabel", 0))))
+        hit = MEASURED_TRAFFIC.get(("rmat", args.scale, int(binned), int(plan.get("relabel", 0))))
         if hit:
             traffic, traffic_src = hit[0], f"rocprofv3 FETCH_SIZE + WRITE_SIZE, {hit[1]}"
     out = base_line(args, round(value, 3), "GTEPS", ms_per_step, "f32", {
@@ -193,6 +196,7 @@ def run_plain(args, ctx, grid=False):
         "ms_per_sweep_events": round(ev_ms / max(sweeps, 1), 5),
         "build_s": round(build_s, 3),
         "plan": plan,
+        "schedule": "in-place (v11)" if args.inplace else "Jacobi (v10)",
         "parallelism": "single GPU",
     })
     out["roofline"] = {
@@ -397,6 +401,8 @@ def main():
     ap.add_argument("--no-autotune", dest="autotune", action="store_false",
                     help="keep the default sweep plan instead of autotuning it before the timed region")
     ap.add_argument("--plan", default="", help="k=v,... overrides of the sweep plan")
+    ap.add_argument("--inplace", action="store_true",
+                    help="plain/grid: the reference's in-place (v11) schedule instead of Jacobi (v10)")
     ap.add_argument("--traffic", type=float, default=None, help="measured HBM bytes per sweep (rocprofv3 PMC)")
     args = ap.parse_args()
     args.iters_given = args.iters is not None

@@ -160,6 +160,16 @@ int nwg_generate_ppr_seeds(uint32_t n, uint32_t k, uint32_t s, uint64_t seed, ui
 int nwg_graph_pagerank(nwg_ctx* ctx, nwg_graph* g, float* scores, const float* teleport,
                        float damping, float threshold, uint64_t max_iters, int mem_kind,
                        nwg_pr_info* info);
+/* Same solve with the reference's own in-place schedule: nw::graph::page_rank
+ * (page_rank.hpp:121-147) updates outgoing_contrib inside the sweep (:135), so rows later
+ * in a sweep see earlier rows' new contributions.  Here one contribution array is updated
+ * during the sweep by the pull kernels (the plan's binned option is ignored): rows see
+ * whatever neighbours have already written.  Like the reference's multi-threaded run the
+ * result depends on the schedule, so it is not reproducible bit for bit; it converges to
+ * the same fixed point, usually in fewer sweeps on high-diameter graphs. */
+int nwg_graph_pagerank_inplace(nwg_ctx* ctx, nwg_graph* g, float* scores, const float* teleport,
+                               float damping, float threshold, uint64_t max_iters, int mem_kind,
+                               nwg_pr_info* info);
 /* Scheduling plan of the sweep: set explicitly, read back, or autotune. */
 int nwg_graph_set_plan(nwg_ctx* ctx, nwg_graph* g, const nwg_plan_params* p);
 int nwg_graph_get_plan(const nwg_graph* g, nwg_plan_params* p);

@@ -303,8 +303,8 @@ int nwg_generate_grid(nwg_ctx* ctx, uint32_t width, uint32_t keep_per10000, uint
 }
 
 // ---------------------------------------------------------------- PageRank
-int nwg_graph_pagerank(nwg_ctx* ctx, nwg_graph* g, float* scores, const float* teleport, float damping,
-                       float threshold, uint64_t max_iters, int mem_kind, nwg_pr_info* info) {
+static int graph_pagerank(nwg_ctx* ctx, nwg_graph* g, float* scores, const float* teleport, float damping,
+                          float threshold, uint64_t max_iters, int mem_kind, nwg_pr_info* info, bool inplace) {
   if (!ctx) return NWG_EINVAL;
   return guarded(ctx, [&] {
     check_graph(ctx, g);
@@ -323,12 +323,22 @@ int nwg_graph_pagerank(nwg_ctx* ctx, nwg_graph* g, float* scores, const float* t
       out_dev.alloc(g->n);
       sc = out_dev.p;
     }
-    nwg::pagerank_run(ctx, g, tele, damping, threshold, max_iters, info, sc);
+    nwg::pagerank_run(ctx, g, tele, damping, threshold, max_iters, info, sc, inplace);
     if (mem_kind == NWG_MEM_HOST) {
       NWG_HIP(hipMemcpyAsync(scores, out_dev.p, (size_t)g->n * 4, hipMemcpyDeviceToHost, s));
       NWG_HIP(hipStreamSynchronize(s));
     }
   });
+}
+
+int nwg_graph_pagerank(nwg_ctx* ctx, nwg_graph* g, float* scores, const float* teleport, float damping,
+                       float threshold, uint64_t max_iters, int mem_kind, nwg_pr_info* info) {
+  return graph_pagerank(ctx, g, scores, teleport, damping, threshold, max_iters, mem_kind, info, false);
+}
+
+int nwg_graph_pagerank_inplace(nwg_ctx* ctx, nwg_graph* g, float* scores, const float* teleport, float damping,
+                               float threshold, uint64_t max_iters, int mem_kind, nwg_pr_info* info) {
+  return graph_pagerank(ctx, g, scores, teleport, damping, threshold, max_iters, mem_kind, info, true);
 }
 
 int nwg_pagerank_f32(nwg_ctx* ctx, uint32_t n, uint64_t nnz, const uint32_t* csc_ptr, const uint32_t* csc_idx,

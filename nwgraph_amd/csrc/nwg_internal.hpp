@@ -319,8 +319,10 @@ nwg_plan_params default_plan_params();
 // binned sweep (binned.hip)
 void binned_build(hipStream_t s, const ActiveCsc& a, uint32_t n, uint64_t nnz, BinnedLayout& L);
 void binned_enqueue_sweep(hipStream_t s, const BinParams& P);
+// inplace: one contribution array updated during the sweep (the reference's v11 schedule,
+// page_rank.hpp:135), pull kernels only; else Jacobi (v10) with double-buffered contributions
 void pagerank_run(nwg_ctx* ctx, nwg_graph* g, const float* teleport_dev, float damping, float threshold,
-                  uint64_t max_iters, nwg_pr_info* info, float* scores_dev);
+                  uint64_t max_iters, nwg_pr_info* info, float* scores_dev, bool inplace = false);
 
 // ---- batched personalised PageRank (ppr.hip) ----
 void ppr_run(nwg_ctx* ctx, nwg_graph* g, uint32_t k, const std::vector<uint32_t>& seed_ptr,

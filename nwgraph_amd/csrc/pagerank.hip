@@ -629,7 +629,7 @@ static void enqueue_sweep(hipStream_t s, const SweepParams& P) {
 }
 
 static void enqueue_solve(hipStream_t s, nwg_graph* g, PrState& st, SweepParams P, uint64_t iters,
-                          hipEvent_t* ev) {
+                          hipEvent_t* ev, bool inplace) {
   const uint32_t n = g->n;
   const float init = (float)(1.0 / n);  // page_rank.hpp:93
   pr_init_kernel<<<blocks_for(std::max<uint64_t>(n, iters)), kThreads, 0, s>>>(n, P.deg, init, st.x.p,
@@ -637,8 +637,9 @@ static void enqueue_solve(hipStream_t s, nwg_graph* g, PrState& st, SweepParams 
   NWG_LAUNCH_CHECK();
   const uint32_t nt = g->plan.params.stream_hints;
   const BinnedLayout& L = g->plan.bl;
+  const bool binned = g->plan.params.binned && !inplace;  // the in-place schedule is a pull sweep
   BinParams B{};
-  if (g->plan.params.binned) {
+  if (binned) {
     // contrib is one buffer here: the chunk pass of sweep k reads it before the bin pass
     // of sweep k rewrites it
     B.brow = L.brow.p;
@@ -665,10 +666,12 @@ static void enqueue_solve(hipStream_t s, nwg_graph* g, PrState& st, SweepParams 
   }
   for (uint64_t k = 0; k < iters; ++k) {
     P.iter = (uint32_t)k;
-    P.cin = (k & 1) ? st.contrib1.p : st.contrib0.p;
-    P.cout = (k & 1) ? st.contrib0.p : st.contrib1.p;
+    // in place (page_rank.hpp:135): one contribution array, each row's new contribution is
+    // visible to the rows gathered after it; else double-buffered (Jacobi)
+    P.cin = inplace ? st.contrib0.p : (k & 1) ? st.contrib1.p : st.contrib0.p;
+    P.cout = inplace ? st.contrib0.p : (k & 1) ? st.contrib0.p : st.contrib1.p;
     if (ev) NWG_HIP(hipEventRecord(ev[2 * k], s));
-    if (g->plan.params.binned) {
+    if (binned) {
       B.iter = (uint32_t)k;
       binned_enqueue_sweep(s, B);
     } else if (nt == 2) {
@@ -683,7 +686,7 @@ static void enqueue_solve(hipStream_t s, nwg_graph* g, PrState& st, SweepParams 
 }
 
 void pagerank_run(nwg_ctx* ctx, nwg_graph* g, const float* teleport_dev, float damping, float threshold,
-                  uint64_t max_iters, nwg_pr_info* info, float* scores_dev) {
+                  uint64_t max_iters, nwg_pr_info* info, float* scores_dev, bool inplace) {
   hipStream_t s = ctx->stream;
   NWG_REQUIRE(max_iters < (1ull << 31), "max_iters too large");
   if (!g->plan.built) plan_build(s, g, default_plan_params());
@@ -697,7 +700,7 @@ void pagerank_run(nwg_ctx* ctx, nwg_graph* g, const float* teleport_dev, float d
   st.hub_part.ensure(std::max<uint32_t>(g->plan.n_hub_segs, 1));
   st.blk_err.ensure(std::max<uint64_t>(sweep_workgroups_before_hubs(g->plan), 1));
   st.spart.ensure(std::max<uint64_t>((uint64_t)kSlices * g->plan.n_srows, 1));
-  if (g->plan.params.binned) {
+  if (g->plan.params.binned && !inplace) {
     const BinnedLayout& L = g->plan.bl;
     st.bvals.ensure(L.padded);
     st.bpart.ensure(std::max<uint64_t>((uint64_t)L.n_slots << kBinBits, 1));
@@ -731,12 +734,13 @@ void pagerank_run(nwg_ctx* ctx, nwg_graph* g, const float* teleport_dev, float d
     key.iters = max_iters;
     key.damping = damping;
     key.threshold = threshold;
+    key.extra = inplace ? 1u : 0u;
     if (!st.cap.valid || !(st.cap.key == key)) {
       st.cap.reset();
       hipStream_t cs = ctx->capture_stream;
       NWG_HIP(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
       try {
-        enqueue_solve(cs, g, st, P, max_iters, nullptr);
+        enqueue_solve(cs, g, st, P, max_iters, nullptr, inplace);
       } catch (...) {
         hipGraph_t tmp = nullptr;
         (void)hipStreamEndCapture(cs, &tmp);
@@ -756,7 +760,7 @@ void pagerank_run(nwg_ctx* ctx, nwg_graph* g, const float* teleport_dev, float d
     for (auto& e : evs) NWG_HIP(hipEventCreate(&e));
     NWG_HIP(hipEventRecord(ctx->ev0, s));
     try {
-      enqueue_solve(s, g, st, P, max_iters, evs.data());
+      enqueue_solve(s, g, st, P, max_iters, evs.data(), inplace);
     } catch (...) {
       for (auto& e : evs) (void)hipEventDestroy(e);
       throw;

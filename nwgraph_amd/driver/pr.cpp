@@ -1,16 +1,19 @@
 // pr.cpp -- PageRank benchmark driver over libnwgraph_hip, the counterpart of the
 // reference's bench/gapbs/pr.cpp (usage pr.cpp:16-34, main pr.cpp:54-177).
 //
-//   pr [--version ID...] -f FILE... [-i NUM] [-t NUM] [-n NUM] [-dvV] [--log FILE] [--log-header] [THREADS]...
+//   pr [--version ID...] -f FILE... [-i NUM] [-t NUM] [-n NUM] [-dvV] [--log FILE] [--log-header] [--autotune] [THREADS]...
 //
 // FILE is a Matrix Market file, an NW binary edge list (load_graph,
 // bench/gapbs/common.hpp:82-101), or a synthetic spec generated on the device:
 //   rmat:SCALE[:EDGE_FACTOR[:SEED]]   RMAT, A/B/C/D = .57/.19/.19/.05, cleaned
 //   grid:WIDTH[:SEED]                 4-neighbour lattice (p=0.85) + 0.5% long-range edges
-// Versions 10 and 11 both run the GPU sweep (Jacobi schedule of page_rank_v10; v11's
-// in-place update order is a CPU schedule artefact).  Defaults follow the reference:
-// version 11, -i 20 (0 means 1), -t 1e-4, -n 1 (0 means 1).  THREADS are accepted and
-// reported like the reference's (which only uses them for TBB); the GPU ignores them.
+// Version 10 runs the Jacobi schedule of page_rank_v10 (nwg_graph_pagerank, deterministic);
+// version 11 runs the reference's in-place schedule (nwg_graph_pagerank_inplace: one
+// contribution array updated during the sweep, schedule-dependent like the reference's
+// TBB run).  --autotune picks the sweep plan first (nwg_graph_autotune).  Defaults follow
+// the reference: version 11, -i 20 (0 means 1), -t 1e-4, -n 1 (0 means 1).  THREADS are
+// accepted and reported like the reference's (which only uses them for TBB); the GPU
+// ignores them.
 #include <nwgraph_hip.h>
 
 #include <algorithm>
@@ -37,7 +40,7 @@ struct Args {
   long max_iters = 20;
   float tolerance = 1e-4f;
   long trials = 1;
-  bool debug = false, verify = false, verbose = false, log_header = false;
+  bool debug = false, verify = false, verbose = false, log_header = false, autotune = false;
   std::string log_file;
   std::vector<long> threads;
 };
@@ -47,7 +50,7 @@ struct Args {
                "  Usage:\n"
                "      pr (-h | --help)\n"
                "      pr [--version ID...] -f FILE... [-i NUM] [-t NUM] [-n NUM] [-dvV] [--log FILE] [--log-header] "
-               "[THREADS]...\n\n"
+               "[--autotune] [THREADS]...\n\n"
                "  Options:\n"
                "      -h, --help          show this screen\n"
                "      --version ID        algorithm version to run [default: 11]\n"
@@ -59,7 +62,9 @@ struct Args {
                "      --log-header        add a header to the log file\n"
                "      -d, --debug         run in debug mode\n"
                "      -v, --verify        verify results\n"
-               "      -V, --verbose       run in verbose mode\n";
+               "      -V, --verbose       run in verbose mode\n"
+               "      --autotune          choose the sweep plan on the device first (not in the reference)\n"
+               "  Versions: 10 = Jacobi schedule (reproducible), 11 = in-place schedule (the reference's default)\n";
   std::exit(rc);
 }
 
@@ -82,6 +87,7 @@ Args parse(int argc, char** argv) {
     else if (s == "-n") a.trials = std::stol(need("-n"));
     else if (s == "--log") a.log_file = need("--log");
     else if (s == "--log-header") a.log_header = true;
+    else if (s == "--autotune") a.autotune = true;
     else if (s.size() > 1 && s[0] == '-' && s[1] != '-' && !std::isdigit((unsigned char)s[1])) {
       for (size_t k = 1; k < s.size(); ++k) {
         if (s[k] == 'd') a.debug = true;
@@ -182,6 +188,13 @@ int main(int argc, char** argv) {
     nwg_graph_info(g, &n, &nnz);
     if (args.verbose) std::cout << "% adjacency<1>: " << n << " vertices, " << nnz << " edges\n";
     std::cout << "# [ degrees ]: built on the device with the adjacency" << std::endl;
+    if (args.autotune) {
+      nwg_plan_params plan{};
+      check(ctx, nwg_graph_autotune(ctx, g, &plan), "nwg_graph_autotune");
+      if (args.verbose)
+        std::cout << "% plan: binned " << plan.binned << " relabel " << plan.relabel << " short_max " << plan.short_max
+                  << " group_max " << plan.group_max << " hub_min " << plan.hub_min << "\n";
+    }
     if (args.debug) {
       std::vector<uint32_t> ptr(n + 1);
       check(ctx, nwg_graph_get_csc(ctx, g, ptr.data(), nullptr, nullptr), "nwg_graph_get_csc");
@@ -201,10 +214,12 @@ int main(int argc, char** argv) {
           info.error_trace = trace.data();
           info.error_trace_len = trace.size();
           const auto t0 = std::chrono::steady_clock::now();
+          // version 11 is the reference's in-place schedule, version 10 its Jacobi twin
+          const auto solve = id == 11 ? nwg_graph_pagerank_inplace : nwg_graph_pagerank;
           check(ctx,
-                nwg_graph_pagerank(ctx, g, rankings.data(), nullptr, 0.85f, args.tolerance, (uint64_t)args.max_iters,
-                                   NWG_MEM_HOST, &info),
-                "nwg_graph_pagerank");
+                solve(ctx, g, rankings.data(), nullptr, 0.85f, args.tolerance, (uint64_t)args.max_iters, NWG_MEM_HOST,
+                      &info),
+                id == 11 ? "nwg_graph_pagerank_inplace" : "nwg_graph_pagerank");
           const double sec = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
           nw::graph::hip::detail::trace_row("iter", "error", "time", "outgoing");
           for (uint64_t k = 0; k < info.iterations; ++k)

@@ -191,24 +191,26 @@ class DeviceGraph:
             "error_trace": trace[:it].copy(),
         }
 
-    def pagerank(self, damping=0.85, threshold=1e-4, max_iters=20, teleport=None, kernel_timing=False):
-        """Plain (or personalised) PageRank; returns (scores float32[n], info dict)."""
+    def pagerank(self, damping=0.85, threshold=1e-4, max_iters=20, teleport=None, kernel_timing=False,
+                 inplace=False):
+        """Plain (or personalised) PageRank; returns (scores float32[n], info dict).  inplace=True
+        runs the reference's in-place (v11) schedule, which is not reproducible bit for bit."""
         scores = np.empty(self.n, np.float32)
         tele = None if teleport is None else np.ascontiguousarray(teleport, np.float32)
         info, trace = self._info(max_iters, kernel_timing)
-        self.ctx.check(self.ctx.lib.nwg_graph_pagerank(self.ctx.handle, self.handle, _ptr(scores), _ptr(tele),
-                                                       float(damping), float(threshold), int(max_iters),
-                                                       NWG_MEM_HOST, ctypes.byref(info)))
+        fn = self.ctx.lib.nwg_graph_pagerank_inplace if inplace else self.ctx.lib.nwg_graph_pagerank
+        self.ctx.check(fn(self.ctx.handle, self.handle, _ptr(scores), _ptr(tele), float(damping), float(threshold),
+                          int(max_iters), NWG_MEM_HOST, ctypes.byref(info)))
         return scores, self._info_dict(info, trace)
 
     def pagerank_device(self, scores_ptr, teleport_ptr=None, damping=0.85, threshold=1e-4, max_iters=20,
-                        kernel_timing=False):
+                        kernel_timing=False, inplace=False):
         """PageRank into a device buffer (integer device pointer); returns the info dict."""
         info, trace = self._info(max_iters, kernel_timing)
-        self.ctx.check(self.ctx.lib.nwg_graph_pagerank(self.ctx.handle, self.handle, ctypes.c_void_p(scores_ptr),
-                                                       ctypes.c_void_p(teleport_ptr) if teleport_ptr else None,
-                                                       float(damping), float(threshold), int(max_iters),
-                                                       NWG_MEM_DEVICE, ctypes.byref(info)))
+        fn = self.ctx.lib.nwg_graph_pagerank_inplace if inplace else self.ctx.lib.nwg_graph_pagerank
+        self.ctx.check(fn(self.ctx.handle, self.handle, ctypes.c_void_p(scores_ptr),
+                          ctypes.c_void_p(teleport_ptr) if teleport_ptr else None, float(damping), float(threshold),
+                          int(max_iters), NWG_MEM_DEVICE, ctypes.byref(info)))
         return self._info_dict(info, trace)
 
     # ---- batched personalised PageRank -----------------------------------------

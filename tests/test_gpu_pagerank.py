@@ -349,3 +349,33 @@ def test_binned_sweep_grid_and_personalised(ctx):
     v = np.zeros(n, np.float32)
     v[ids] = 1.0 / 8
     check_against_oracle(g, n, ind, col, deg, teleport=v)
+
+
+@pytest.mark.parametrize("kind", ["grid", "rmat"])
+def test_inplace_schedule_reaches_the_fixed_point(ctx, kind):
+    """The reference's in-place (v11) schedule: not reproducible bit for bit, so it is checked
+    through what does not depend on the schedule: it reaches the same FP64 fixed point as the
+    Jacobi schedule, its reported L1 change is below the threshold at the stop, and on the
+    high-diameter grid it takes no more sweeps than Jacobi."""
+    if kind == "grid":
+        g = ctx.grid(256, seed=5)
+        n, ind, col, deg = O.grid_csc(256, seed=5)
+    else:
+        g = ctx.rmat(14, 16, seed=2)
+        n, ind, col, deg = O.rmat_csc(14, 16, seed=2)
+    x64, _ = O.fp64_jacobi(n, ind, col, deg, damping=float(np.float32(D)))
+    xj, ij = g.pagerank(D, 1e-7, 300)
+    xi, ii = g.pagerank(D, 1e-7, 300, inplace=True)
+    assert ii["iterations"] < 300 and ii["final_error"] < 1e-7
+    assert np.max(np.abs(xj - x64)) <= 1e-6 and np.max(np.abs(xi - x64)) <= 1e-6
+    if kind == "grid":
+        assert ii["iterations"] <= ij["iterations"]
+    # personalised teleport, and a binned plan (the in-place schedule always runs the pull kernels)
+    p = g.get_plan()
+    p.update(binned=1)
+    g.set_plan(**p)
+    v = np.zeros(n, np.float32)
+    v[O.ppr_sources(n, 1, 8, seed=3)[0]] = 1.0 / 8
+    xv, iv = g.pagerank(D, 1e-7, 300, teleport=v, inplace=True)
+    x64v, _ = O.fp64_jacobi(n, ind, col, deg, v=v, damping=float(np.float32(D)))
+    assert iv["final_error"] < 1e-7 and np.max(np.abs(xv - x64v)) <= 1e-6
