@@ -379,3 +379,20 @@ def test_inplace_schedule_reaches_the_fixed_point(ctx, kind):
     xv, iv = g.pagerank(D, 1e-7, 300, teleport=v, inplace=True)
     x64v, _ = O.fp64_jacobi(n, ind, col, deg, v=v, damping=float(np.float32(D)))
     assert iv["final_error"] < 1e-7 and np.max(np.abs(xv - x64v)) <= 1e-6
+
+
+@pytest.mark.parametrize("plan", [{}, {"binned": 1}, {"binned": 1, "relabel": 1}])
+def test_uncleaned_graph_with_duplicates_and_loops(ctx, plan):
+    """adjacency<1> as pr.cpp builds it keeps duplicate edges and self-loops (build.hpp:209-232):
+    every plan sums each copy (a duplicate source appears twice in its row)."""
+    rng = np.random.default_rng(12)
+    n = 40000
+    s = rng.integers(0, n, 600000).astype(np.uint32)
+    d = (rng.integers(0, n, 600000) // 7).astype(np.uint32)  # skewed in-degrees
+    s[:5000] = d[:5000]  # self-loops
+    s[5000:15000] = s[15000:25000]
+    d[5000:15000] = d[15000:25000]  # duplicates
+    g = with_plan(ctx.graph_from_edges(n, s, d, clean=False), plan)
+    ind, col = O.csc(n, s, d)
+    deg = O.out_degrees(n, ind, col)
+    check_against_oracle(g, n, ind, col, deg)
