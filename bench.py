@@ -285,6 +285,8 @@ def run_multi(args, ctx):
     graphs = [ctx.rmat(int(s), args.edge_factor, seed=1000 + q) for q, s in enumerate(scales)]
     mg = ctx.multigraph(graphs)
     log(f"{mg.ngraphs} graphs, scales 10..18: n={mg.n} nnz={mg.nnz}")
+    binned = mg.autotune() if args.autotune else 0
+    log(f"schedule: {'binned sweep of the union' if binned else 'persistent item kernel'}")
     for _ in range(args.warmup):
         mg.pagerank(0.85, args.tol, args.iters)
     t0 = time.perf_counter()
@@ -297,9 +299,10 @@ def run_multi(args, ctx):
     value = edge_sweeps / (np.mean(ev) * 1e-3) / 1e9
     out = base_line(args, round(value, 3), "GTEPS (sum over graphs, device time)", ms, "f32", {
         "workload": f"{mg.ngraphs} independent RMAT graphs, scales U{{10..18}} (seed {args.seed}), ef "
-                    f"{args.edge_factor}, one persistent launch per sweep in one HIP graph, per-graph tol "
+                    f"{args.edge_factor}, all sweeps in one HIP graph, per-graph tol "
                     f"{args.tol}, max {args.iters}",
         "n": mg.n, "nnz": mg.nnz, "ms_device": round(float(np.mean(ev)), 3),
+        "schedule": "binned" if binned else "items",
         "iterations_min_max": [int(its.min()), int(its.max())],
     })
     mg.close()

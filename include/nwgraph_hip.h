@@ -205,6 +205,13 @@ int nwg_multigraph_info(const nwg_multigraph* mg, uint32_t* ngraphs, uint64_t* t
 int nwg_multigraph_pagerank(nwg_ctx* ctx, nwg_multigraph* mg, float* scores, float damping,
                             float threshold, uint64_t max_iters, int mem_kind,
                             uint64_t* iterations, double* final_error, nwg_pr_info* info);
+/* Schedule of the union's sweeps: only plan->binned is read (0: one persistent kernel per
+ * sweep over row items; 1: the binned sweep over a tiled copy of the union, rows of a
+ * converged graph frozen).  No counterpart in the reference (it has no batched solve). */
+int nwg_multigraph_set_plan(nwg_ctx* ctx, nwg_multigraph* mg, const nwg_plan_params* plan);
+/* Times both schedules over a few sweeps and keeps the faster; *binned_out (nullable)
+ * receives the choice. */
+int nwg_multigraph_autotune(nwg_ctx* ctx, nwg_multigraph* mg, uint32_t* binned_out);
 
 /* ------------------------------------------- weighted PageRank and adjoint */
 /* Edge weights in CSC order (weight of edge col[p] -> row).  With normalize=1 each

@@ -103,6 +103,8 @@ _SIGNATURES = {
     "nwg_ppr_autotune": (c_int, [c_void_p, c_void_p, c_u32, c_void_p, c_void_p, P_u32]),
     "nwg_multigraph_create":(c_int, [c_void_p, c_u32, ctypes.POINTER(c_void_p), ctypes.POINTER(c_void_p)]),
     "nwg_multigraph_destroy": (None, [c_void_p]),
+    "nwg_multigraph_set_plan": (c_int, [c_void_p, c_void_p, ctypes.POINTER(PlanParams)]),
+    "nwg_multigraph_autotune": (c_int, [c_void_p, c_void_p, P_u32]),
     "nwg_multigraph_info": (c_int, [c_void_p, P_u32, P_u64, P_u64]),
     "nwg_multigraph_pagerank": (c_int, [c_void_p, c_void_p, c_void_p, c_float, c_float, c_u64, c_int, c_void_p,
                                         c_void_p, ctypes.POINTER(PrInfo)]),

@@ -74,9 +74,16 @@ __device__ __forceinline__ float dequantize(unsigned long long v) {
   return __builtin_ldexpf((float)mant, ex);  // exact: mant < 2^24, result a normal float
 }
 
+// One graph: every workgroup of sweep k exits when sweep k-1's change was below the
+// threshold.  Several graphs (row_graph set): the passes always run and each graph's rows
+// are frozen once that graph has converged (graph_frozen).
 __device__ __forceinline__ bool converged_before(const BinParams& P) {
-  return P.iter > 0 && fixed_to_err(P.acc[P.iter - 1]) < P.threshold;
+  return !P.row_graph && P.iter > 0 && fixed_to_err(P.acc[P.iter - 1]) < P.threshold;
 }
+__device__ __forceinline__ bool graph_frozen(const BinParams& P, uint32_t g) {
+  return P.iter > 0 && fixed_to_err(P.acc[(uint64_t)(P.iter - 1) * P.ngraphs + g]) < P.threshold;
+}
+
 
 // the reference's row update (page_rank.hpp:133-135) -> |x_new - x_old|
 __device__ __forceinline__ double finish_row(const BinParams& P, uint32_t i, float z) {
@@ -88,16 +95,47 @@ __device__ __forceinline__ double finish_row(const BinParams& P, uint32_t i, flo
   return (double)fabsf(xn - xo);
 }
 
-__device__ __forceinline__ double block_sum_d(double v, double* red /* kBT/64 */) {
-  v = wave_sum(v);
-  const uint32_t w = threadIdx.x >> 6;
-  if ((threadIdx.x & 63) == 0) red[w] = v;
+constexpr uint32_t kGraphSlots = 64;  // graphs whose L1 changes a workgroup first adds in LDS
+
+// Finishes rows r0 + v for v = t, t + nt, ... < nrows (z(v) their sums) of a workgroup with
+// nt threads, and accounts their L1 changes.  One graph: a block sum, one atomic into
+// acc[iter].  Several graphs: each row's change, in fixed point, goes to an LDS slot of its
+// graph (graphs g0 .. g0 + kGraphSlots - 1, g0 the graph of row r0; the union's rows are
+// grouped by graph) or straight to acc, then one atomic per graph; rows of converged graphs
+// are left as they are.  red: nt / 64 doubles, gerr: kGraphSlots.
+template <int NT, class Z>
+__device__ __forceinline__ void finish_rows(const BinParams& P, uint32_t r0, uint32_t nrows, Z z, double* red,
+                                            unsigned long long* gerr) {
+  if (!P.row_graph) {
+    double e = 0.0;
+    for (uint32_t v = threadIdx.x; v < nrows; v += NT) e += finish_row(P, r0 + v, z(v));
+    e = wave_sum(e);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = e;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      double t = 0.0;
+      for (uint32_t q = 0; q < NT / 64; ++q) t += red[q];
+      if (t > 0.0) atomicAdd(P.acc + P.iter, err_to_fixed(t));
+    }
+    return;
+  }
+  for (uint32_t q = threadIdx.x; q < kGraphSlots; q += NT) gerr[q] = 0ull;
   __syncthreads();
-  double t = 0.0;
-  if (threadIdx.x == 0)
-    for (uint32_t q = 0; q < kBT / 64; ++q) t += red[q];
-  return t;  // valid in thread 0
+  const uint32_t g0 = nrows ? P.row_graph[r0] : 0u;
+  for (uint32_t v = threadIdx.x; v < nrows; v += NT) {
+    const uint32_t g = P.row_graph[r0 + v];
+    if (graph_frozen(P, g)) continue;
+    const double e = finish_row(P, r0 + v, z(v));
+    if (e > 0.0) {
+      if (g - g0 < kGraphSlots) atomicAdd(&gerr[g - g0], err_to_fixed(e));
+      else atomicAdd(P.acc + (uint64_t)P.iter * P.ngraphs + g, err_to_fixed(e));
+    }
+  }
+  __syncthreads();
+  for (uint32_t q = threadIdx.x; q < kGraphSlots; q += NT)
+    if (gerr[q]) atomicAdd(P.acc + (uint64_t)P.iter * P.ngraphs + g0 + q, gerr[q]);
 }
+
 
 // chunk pass: vals[e] = contrib[src(e)] over the tiles of one item.  Tiles are padded to
 // multiples of 8 entries (pads have source kPadSrc and get 0), so a lane moves 8 entries
@@ -179,6 +217,7 @@ __global__ __launch_bounds__(kBT) void pr_bin_kernel(BinParams P) {
   if (converged_before(P)) return;
   __shared__ unsigned long long lacc[kBinRows];
   __shared__ double red[kBT / 64];
+  __shared__ unsigned long long gerr[kGraphSlots];
   const uint4 it = P.items2[blockIdx.x];  // {bin, first entry, end entry, partial slot or kNoSlot}
   const uint32_t b = it.x;
   for (uint32_t q = threadIdx.x; q < kBinRows; q += kBT) lacc[q] = 0ull;
@@ -234,10 +273,8 @@ __global__ __launch_bounds__(kBT) void pr_bin_kernel(BinParams P) {
     for (uint32_t v = threadIdx.x; v < kBinRows; v += kBT) dst[v] = lacc[v];
     return;
   }
-  double e = 0.0;
-  for (uint32_t v = threadIdx.x; v < kBinRows && r0 + v < P.n; v += kBT) e += finish_row(P, r0 + v, dequantize(lacc[v]));
-  e = block_sum_d(e, red);
-  if (threadIdx.x == 0 && e > 0.0) atomicAdd(P.acc + P.iter, err_to_fixed(e));
+  const uint32_t nrows = r0 < P.n ? min(kBinRows, P.n - r0) : 0u;
+  finish_rows<kBT>(P, r0, nrows, [&](uint32_t v) { return dequantize(lacc[v]); }, red, gerr);
 }
 
 // rows of the bins split over several bin-pass items: add the items' partial rows (exact,
@@ -245,30 +282,27 @@ __global__ __launch_bounds__(kBT) void pr_bin_kernel(BinParams P) {
 __global__ __launch_bounds__(256) void pr_bin_combine_kernel(BinParams P) {
   if (converged_before(P)) return;
   __shared__ double red[4];
+  __shared__ unsigned long long gerr[kGraphSlots];
   const uint4 it = P.items3[blockIdx.y];  // {bin, first slot, slots}
-  const uint32_t v = blockIdx.x * 256 + threadIdx.x;
-  const uint32_t i = (it.x << kBinBits) + v;
-  double e = 0.0;
-  if (i < P.n) {
-    const unsigned long long* src = P.part + (uint64_t)it.y * kBinRows + v;
-    unsigned long long s0 = 0ull, s1 = 0ull, s2 = 0ull, s3 = 0ull;
-    uint32_t k = 0;
-    for (; k + 4 <= it.z; k += 4) {
-      s0 += src[(uint64_t)k * kBinRows];
-      s1 += src[(uint64_t)(k + 1) * kBinRows];
-      s2 += src[(uint64_t)(k + 2) * kBinRows];
-      s3 += src[(uint64_t)(k + 3) * kBinRows];
-    }
-    for (; k < it.z; ++k) s0 += src[(uint64_t)k * kBinRows];
-    e = finish_row(P, i, dequantize((s0 + s1) + (s2 + s3)));
-  }
-  e = wave_sum(e);
-  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = e;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    const double t = (red[0] + red[1]) + (red[2] + red[3]);
-    if (t > 0.0) atomicAdd(P.acc + P.iter, err_to_fixed(t));
-  }
+  const uint32_t v0 = blockIdx.x * 256;
+  const uint32_t r0 = (it.x << kBinBits) + v0;
+  const uint32_t nrows = r0 < P.n ? min(256u, P.n - r0) : 0u;
+  const unsigned long long* src = P.part + (uint64_t)it.y * kBinRows + v0;
+  finish_rows<256>(
+      P, r0, nrows,
+      [&](uint32_t v) {
+        unsigned long long s0 = 0ull, s1 = 0ull, s2 = 0ull, s3 = 0ull;
+        uint32_t k = 0;
+        for (; k + 4 <= it.z; k += 4) {
+          s0 += src[(uint64_t)k * kBinRows + v];
+          s1 += src[(uint64_t)(k + 1) * kBinRows + v];
+          s2 += src[(uint64_t)(k + 2) * kBinRows + v];
+          s3 += src[(uint64_t)(k + 3) * kBinRows + v];
+        }
+        for (; k < it.z; ++k) s0 += src[(uint64_t)k * kBinRows + v];
+        return dequantize((s0 + s1) + (s2 + s3));
+      },
+      red, gerr);
 }
 
 // ---- layout construction ----

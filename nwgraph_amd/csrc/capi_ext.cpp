@@ -117,6 +117,43 @@ int nwg_multigraph_pagerank(nwg_ctx* ctx, nwg_multigraph* mg, float* scores, flo
   });
 }
 
+int nwg_multigraph_set_plan(nwg_ctx* ctx, nwg_multigraph* mg, const nwg_plan_params* plan) {
+  if (!ctx) return NWG_EINVAL;
+  return guarded(ctx, [&] {
+    NWG_REQUIRE(mg != nullptr && plan != nullptr, "multigraph and plan must not be null");
+    NWG_REQUIRE(plan->binned <= 1, "plan: binned must be 0 or 1");
+    DeviceGuard dg(ctx->device);
+    nwg::multigraph_set_binned(ctx, mg, plan->binned != 0);
+  });
+}
+
+int nwg_multigraph_autotune(nwg_ctx* ctx, nwg_multigraph* mg, uint32_t* binned_out) {
+  if (!ctx) return NWG_EINVAL;
+  return guarded(ctx, [&] {
+    NWG_REQUIRE(mg != nullptr, "multigraph is null");
+    DeviceGuard dg(ctx->device);
+    const uint64_t nv = nwg::multigraph_vertices(mg);
+    nwg::DevBuf<float> out(nv ? nv : 1);
+    double best_ms = 1e30;
+    uint32_t best = 0;
+    for (uint32_t b = 0; b <= 1; ++b) {
+      nwg::multigraph_set_binned(ctx, mg, b != 0);
+      double ms = 1e30;
+      for (int r = 0; r < 2; ++r) {  // the first run of a schedule also captures its graph
+        nwg_pr_info info{};
+        nwg::multigraph_run(ctx, mg, 0.85f, -1.0f, 4, out.p, nullptr, nullptr, &info);
+        ms = info.ms_total;
+      }
+      if (ms < best_ms) {
+        best_ms = ms;
+        best = b;
+      }
+    }
+    nwg::multigraph_set_binned(ctx, mg, best != 0);
+    if (binned_out) *binned_out = best;
+  });
+}
+
 int nwg_graph_set_weights(nwg_ctx* ctx, nwg_graph* g, const float* w_csc, int normalize, int mem_kind) {
   if (!ctx) return NWG_EINVAL;
   return guarded(ctx, [&] {

@@ -124,6 +124,20 @@ __global__ __launch_bounds__(kThreads) void multi_copy_kernel(uint32_t n, uint64
   if (t < nnz) col_all[eoff + t] = col[t] + voff;
 }
 
+// graph of every row of the union
+__global__ __launch_bounds__(kThreads) void row_graph_kernel(uint32_t n, uint32_t voff, uint32_t g,
+                                                             uint32_t* __restrict__ row_graph) {
+  const uint32_t t = blockIdx.x * kThreads + threadIdx.x;
+  if (t < n) row_graph[voff + t] = g;
+}
+
+// per-row base (1-d)/N_g of the binned sweep
+__global__ __launch_bounds__(kThreads) void row_base_kernel(uint64_t nv, const uint32_t* __restrict__ row_graph,
+                                                            const float* __restrict__ gbase, float* __restrict__ base) {
+  const uint64_t t = (uint64_t)blockIdx.x * kThreads + threadIdx.x;
+  if (t < nv) base[t] = gbase[row_graph[t]];
+}
+
 inline uint32_t grid1(uint64_t m) { return (uint32_t)ceil_div(m ? m : 1, kThreads); }
 
 }  // namespace
@@ -141,6 +155,12 @@ struct nwg_multigraph {
   nwg::DevBuf<unsigned long long> acc;
   uint32_t n_items = 0;
   nwg::Captured cap;
+  // binned sweep of the union (plan option binned)
+  bool binned = false;
+  nwg::BinnedLayout bl;
+  nwg::DevBuf<uint32_t> row_graph;
+  nwg::DevBuf<float> row_base, bvals;
+  nwg::DevBuf<unsigned long long> bpart;
 };
 
 namespace nwg {
@@ -209,6 +229,27 @@ nwg_multigraph* multigraph_create(nwg_ctx* ctx, uint32_t ngraphs, nwg_graph* con
   return mg.release();
 }
 
+// Selects the schedule of the union's sweeps: the items of multi_sweep_kernel, or the binned
+// sweep (binned.hip) over a tiled copy of the union with a graph id per row.
+void multigraph_set_binned(nwg_ctx* ctx, nwg_multigraph* mg, bool binned) {
+  hipStream_t s = ctx->stream;
+  if (binned && !mg->bl.built) {
+    mg->row_graph.alloc(mg->nv ? mg->nv : 1);
+    for (uint32_t g = 0; g < mg->ngraphs; ++g) {
+      const uint32_t n = (uint32_t)(mg->voff[g + 1] - mg->voff[g]);
+      if (n) row_graph_kernel<<<grid1(n), kThreads, 0, s>>>(n, (uint32_t)mg->voff[g], g, mg->row_graph.p);
+    }
+    NWG_LAUNCH_CHECK();
+    binned_build(s, ActiveCsc{mg->rp.p, mg->col.p, mg->deg.p, nullptr}, (uint32_t)mg->nv, mg->ne, mg->bl);
+    mg->row_base.alloc(mg->nv ? mg->nv : 1);
+    mg->bvals.alloc(mg->bl.padded);
+    mg->bpart.alloc(std::max<uint64_t>((uint64_t)mg->bl.n_slots << kBinBits, 1));
+    NWG_HIP(hipStreamSynchronize(s));
+  }
+  if (mg->binned != binned) mg->cap.reset();
+  mg->binned = binned;
+}
+
 void multigraph_run(nwg_ctx* ctx, nwg_multigraph* mg, float damping, float threshold, uint64_t max_iters,
                     float* scores_dev, uint64_t* iterations, double* final_error, nwg_pr_info* info) {
   hipStream_t s = ctx->stream;
@@ -240,12 +281,42 @@ void multigraph_run(nwg_ctx* ctx, nwg_multigraph* mg, float damping, float thres
   P.acc = mg->acc.p;
   P.threshold = (double)threshold;
 
+  BinParams B{};
+  if (mg->binned) {
+    row_base_kernel<<<grid1(mg->nv), kThreads, 0, s>>>(mg->nv, mg->row_graph.p, mg->gbase.p, mg->row_base.p);
+    NWG_LAUNCH_CHECK();
+    const BinnedLayout& L = mg->bl;
+    B.brow = L.brow.p;
+    B.bsrc = L.bsrc.p;
+    B.vals = mg->bvals.p;
+    B.items2 = L.items2.p;
+    B.items1 = L.items1.p;
+    B.items3 = L.items3.p;
+    B.seg_start = L.seg_start.p;
+    B.seg_flat = L.seg_flat.p;
+    B.part = mg->bpart.p;
+    B.deg = mg->deg.p;
+    B.x = mg->x.p;
+    B.contrib = mg->c0.p;  // one buffer: the chunk pass reads it before the bin pass rewrites it
+    B.basev = mg->row_base.p;
+    B.damp = damping;
+    B.n = (uint32_t)mg->nv;
+    B.n_items2 = L.n_items2;
+    B.n_items1 = L.n_items1;
+    B.n_items3 = L.n_items3;
+    B.acc = mg->acc.p;
+    B.threshold = (double)threshold;
+    B.row_graph = mg->row_graph.p;
+    B.ngraphs = G;
+  }
+
   CaptureKey key;
   key.ptrs[0] = mg->acc.p;
+  key.ptrs[1] = mg->binned ? mg->bvals.p : nullptr;
   key.iters = max_iters;
   key.damping = damping;
   key.threshold = threshold;
-  key.extra = grid;
+  key.extra = grid | (mg->binned ? 0x80000000u : 0u);
   if (!mg->cap.valid || !(mg->cap.key == key)) {
     mg->cap.reset();
     hipStream_t cs = ctx->capture_stream;
@@ -253,6 +324,11 @@ void multigraph_run(nwg_ctx* ctx, nwg_multigraph* mg, float damping, float thres
     multi_init_kernel<<<grid1(std::max<uint64_t>(mg->nv, max_iters * G)), kThreads, 0, cs>>>(
         mg->nv, mg->deg.p, mg->vinit.p, mg->x.p, mg->c0.p, mg->acc.p, max_iters * G);
     for (uint64_t k = 0; k < max_iters; ++k) {
+      if (mg->binned) {
+        B.iter = (uint32_t)k;
+        binned_enqueue_sweep(cs, B);
+        continue;
+      }
       P.iter = (uint32_t)k;
       P.cin = (k & 1) ? mg->c1.p : mg->c0.p;
       P.cout = (k & 1) ? mg->c0.p : mg->c1.p;

@@ -151,9 +151,11 @@ struct BinParams {
   float damp;
   uint32_t n;
   uint32_t n_items2, n_items1, n_items3;
-  unsigned long long* acc;
+  unsigned long long* acc;    // one graph: iters; several graphs: iters x ngraphs
   uint32_t iter;
   double threshold;
+  const uint32_t* row_graph;  // several graphs in one union: graph of every row (null: one graph)
+  uint32_t ngraphs;
 };
 
 // Identity of a captured sweep loop: re-capture when any of it changes.
@@ -334,6 +336,7 @@ nwg_multigraph* multigraph_create(nwg_ctx* ctx, uint32_t ngraphs, nwg_graph* con
 void multigraph_run(nwg_ctx* ctx, nwg_multigraph* mg, float damping, float threshold, uint64_t max_iters,
                     float* scores_dev, uint64_t* iterations, double* final_error, nwg_pr_info* info);
 uint64_t multigraph_vertices(const nwg_multigraph* mg);
+void multigraph_set_binned(nwg_ctx* ctx, nwg_multigraph* mg, bool binned);
 
 // ---- weighted PageRank and adjoint (wpr.hip) ----
 void build_items(hipStream_t s, const uint32_t* rp_dev, uint32_t n, DevBuf<uint32_t>& r0, DevBuf<uint32_t>& r1,

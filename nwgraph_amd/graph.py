@@ -325,6 +325,19 @@ class MultiGraph:
         self.ngraphs, self.n, self.nnz = ng.value, nv.value, ne.value
         self.offsets = np.concatenate([[0], np.cumsum([g.n for g in self.graphs])]).astype(np.int64)
 
+    def set_plan(self, binned):
+        """Schedule of the union's sweeps: the per-item persistent kernel (binned=0) or the
+        binned sweep of the union (binned=1)."""
+        p = PlanParams()
+        p.binned = int(binned)
+        self.ctx.check(self.ctx.lib.nwg_multigraph_set_plan(self.ctx.handle, self.handle, ctypes.byref(p)))
+
+    def autotune(self):
+        """Times both schedules and keeps the faster; returns 1 if the binned sweep won."""
+        b = ctypes.c_uint32(0)
+        self.ctx.check(self.ctx.lib.nwg_multigraph_autotune(self.ctx.handle, self.handle, ctypes.byref(b)))
+        return int(b.value)
+
     def pagerank(self, damping=0.85, threshold=1e-4, max_iters=20):
         scores = np.empty(self.n, np.float32)
         its = np.zeros(self.ngraphs, np.uint64)

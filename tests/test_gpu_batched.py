@@ -102,7 +102,10 @@ def test_ppr_rejects_bad_seeds(ctx):
         g.ppr([np.array([g.n + 3], np.uint32)])  # out of range
 
 
-def test_multigraph_matches_per_graph_oracle(ctx):
+@pytest.mark.parametrize("binned", [0, 1])
+def test_multigraph_matches_per_graph_oracle(ctx, binned):
+    """24 graphs of scales 4..12; with binned=1 one 2^13-row bin holds many graphs (more
+    than the 64 a workgroup counts in LDS), each frozen once converged."""
     rng = np.random.default_rng(11)
     scales = rng.integers(4, 13, 24)
     graphs, refs = [], []
@@ -111,6 +114,7 @@ def test_multigraph_matches_per_graph_oracle(ctx):
         refs.append(O.rmat_csc(int(sc), 16, seed=100 + q))
     mg = ctx.multigraph(graphs)
     assert mg.ngraphs == len(graphs) and mg.n == sum(g.n for g in graphs)
+    mg.set_plan(binned)
     X, its, errs, info = mg.pagerank(D, 1e-4, 50)
     converged = 0
     for q, (n, ind, col, deg) in enumerate(refs):
@@ -123,7 +127,8 @@ def test_multigraph_matches_per_graph_oracle(ctx):
     assert converged > 0  # per-graph stopping is exercised
 
 
-def test_multigraph_with_hub_items_and_repeat(ctx):
+@pytest.mark.parametrize("binned", [0, 1])
+def test_multigraph_with_hub_items_and_repeat(ctx, binned):
     # a star whose centre has 5999 in-edges (more than one LDS work item holds) plus a
     # ring, next to an RMAT graph and a grid
     n0 = 6000
@@ -136,6 +141,7 @@ def test_multigraph_with_hub_items_and_repeat(ctx):
     refs = [(n0, ind0, col0, O.out_degrees(n0, ind0, col0)), O.rmat_csc(9, 8, seed=2), O.grid_csc(64, seed=3)]
     assert int(np.diff(ind0.astype(np.int64)).max()) > 4096
     mg = ctx.multigraph(graphs)
+    mg.set_plan(binned)
     X1, its1, _, _ = mg.pagerank(D, 1e-4, 100)
     X2, its2, _, _ = mg.pagerank(D, 1e-4, 100)
     assert np.array_equal(X1, X2) and np.array_equal(its1, its2)
@@ -154,3 +160,14 @@ def test_multigraph_with_hub_items_and_repeat(ctx):
         gpu_err = np.max(np.abs(X1[mg.offsets[q]:mg.offsets[q + 1]] - x64k))
         ref_err = np.max(np.abs(ref - x64k))
         assert gpu_err <= 1e-6 and gpu_err <= ref_err + 1e-7, (gpu_err, ref_err)
+
+
+def test_multigraph_autotune_keeps_results(ctx):
+    graphs = [ctx.rmat(sc, 16, seed=40 + sc) for sc in (8, 11, 14, 16)]
+    mg = ctx.multigraph(graphs)
+    X0, its0, _, _ = mg.pagerank(D, 0.0, 20)  # fixed sweep count: no stopping decision to differ
+    choice = mg.autotune()
+    assert choice in (0, 1)
+    X1, its1, _, _ = mg.pagerank(D, 0.0, 20)
+    assert np.all(its0 == 20) and np.all(its1 == 20)
+    assert np.max(np.abs(X0 - X1) / np.maximum(X0, 1e-30)) < 1e-5  # summation order only

@@ -119,11 +119,13 @@ def test_pagerank_rmat_matches_v10(ctx, scale):
     x, info, ref = check_against_oracle(g, n, ind, col, deg)
     # relative agreement far inside the 1e-6 absolute gate (FP32 summation order only)
     assert np.max(np.abs(x - ref) / np.maximum(ref, 1e-30)) < 1e-4
-    # report-only gap to the sequential in-place (v11, one thread) schedule and to FP64
+    # both the device and the reference's float run against FP64 Jacobi at the same sweep
+    # count (same damping and start): within 1e-6 (measured gaps are ~1e-8)
+    x64, _ = O.fp64_jacobi(n, ind, col, deg, damping=float(np.float32(D)), tol=0.0, max_iters=info["iterations"])
+    assert np.max(np.abs(x - x64)) <= 1e-6 and np.max(np.abs(ref - x64)) <= 1e-6
+    # report-only gap to the sequential in-place (v11, one thread) schedule
     v11, it11, _ = O.pr_v11(n, ind, col, deg)
     assert np.max(np.abs(x - v11)) < 5e-5
-    x64, _ = O.fp64_jacobi(n, ind, col, deg)
-    assert np.max(np.abs(x - x64)) < 1e-3
 
 
 @pytest.mark.parametrize(
