@@ -77,8 +77,21 @@ __device__ __forceinline__ float dequantize(unsigned long long v) {
 // One graph: every workgroup of sweep k exits when sweep k-1's change was below the
 // threshold.  Several graphs (row_graph set): the passes always run and each graph's rows
 // are frozen once that graph has converged (graph_frozen).
+// One graph's change of sweep k is the sum of its kBinAccLanes lane counters; every wave
+// loads and sums them (exactly, in fixed point), so all waves of a workgroup agree.
 __device__ __forceinline__ bool converged_before(const BinParams& P) {
-  return !P.row_graph && P.iter > 0 && fixed_to_err(P.acc[P.iter - 1]) < P.threshold;
+  static_assert(kBinAccLanes == 64, "one lane counter per wavefront lane");
+  if (P.row_graph || P.iter == 0) return false;
+  unsigned long long v =
+      P.acc[(uint64_t)(P.iter - 1) * kBinAccStride + (uint64_t)(threadIdx.x & 63u) * kBinAccPad];
+#pragma unroll
+  for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m, 64);
+  return fixed_to_err(v) < P.threshold;
+}
+// the lane counter a workgroup adds one graph's change of this sweep into
+__device__ __forceinline__ unsigned long long* residual_lane(const BinParams& P) {
+  const uint32_t b = blockIdx.x + blockIdx.y * gridDim.x;
+  return P.acc + (uint64_t)P.iter * kBinAccStride + (uint64_t)(b % kBinAccLanes) * kBinAccPad;
 }
 __device__ __forceinline__ bool graph_frozen(const BinParams& P, uint32_t g) {
   return P.iter > 0 && fixed_to_err(P.acc[(uint64_t)(P.iter - 1) * P.ngraphs + g]) < P.threshold;
@@ -98,8 +111,8 @@ __device__ __forceinline__ double finish_row(const BinParams& P, uint32_t i, flo
 constexpr uint32_t kGraphSlots = 64;  // graphs whose L1 changes a workgroup first adds in LDS
 
 // Finishes rows r0 + v for v = t, t + nt, ... < nrows (z(v) their sums) of a workgroup with
-// nt threads, and accounts their L1 changes.  One graph: a block sum, one atomic into
-// acc[iter].  Several graphs: each row's change, in fixed point, goes to an LDS slot of its
+// nt threads, and accounts their L1 changes.  One graph: a block sum, one atomic into a
+// lane counter of the sweep.  Several graphs: each row's change, in fixed point, goes to an LDS slot of its
 // graph (graphs g0 .. g0 + kGraphSlots - 1, g0 the graph of row r0; the union's rows are
 // grouped by graph) or straight to acc, then one atomic per graph; rows of converged graphs
 // are left as they are.  red: nt / 64 doubles, gerr: kGraphSlots.
@@ -115,7 +128,7 @@ __device__ __forceinline__ void finish_rows(const BinParams& P, uint32_t r0, uin
     if (threadIdx.x == 0) {
       double t = 0.0;
       for (uint32_t q = 0; q < NT / 64; ++q) t += red[q];
-      if (t > 0.0) atomicAdd(P.acc + P.iter, err_to_fixed(t));
+      if (t > 0.0) atomicAdd(residual_lane(P), err_to_fixed(t));
     }
     return;
   }
@@ -278,29 +291,41 @@ __global__ __launch_bounds__(kBT) void pr_bin_kernel(BinParams P) {
 }
 
 // rows of the bins split over several bin-pass items: add the items' partial rows (exact,
-// so in any order) and finish the rows; grid (kBinRows / 256, split bins)
+// so in any order) and finish the rows.  A workgroup (one combine unit) takes 256 / L rows
+// of one split bin, each row's slots shared by L lanes (slot k on lane k mod L); the host
+// picks L per bin from its slot count, so a bin with hundreds of slots is read by many
+// workgroups at once and a bin with a few by few.  One workgroup per unit.
 __global__ __launch_bounds__(256) void pr_bin_combine_kernel(BinParams P) {
   if (converged_before(P)) return;
   __shared__ double red[4];
   __shared__ unsigned long long gerr[kGraphSlots];
-  const uint4 it = P.items3[blockIdx.y];  // {bin, first slot, slots}
-  const uint32_t v0 = blockIdx.x * 256;
+  __shared__ unsigned long long psum[256];
+  const uint4 it = P.items3[blockIdx.x];  // {bin, first slot, slots, first row | log2 L << 16}
+  const uint32_t lanes = 1u << (it.w >> 16), rows = 256u >> (it.w >> 16);
+  const uint32_t v0 = it.w & 0xFFFFu;
   const uint32_t r0 = (it.x << kBinBits) + v0;
-  const uint32_t nrows = r0 < P.n ? min(256u, P.n - r0) : 0u;
-  const unsigned long long* src = P.part + (uint64_t)it.y * kBinRows + v0;
+  const uint32_t nrows = r0 < P.n ? min(rows, P.n - r0) : 0u;
+  {
+    const uint32_t row = threadIdx.x % rows, lane = threadIdx.x / rows;
+    const unsigned long long* src = P.part + (uint64_t)it.y * kBinRows + v0 + row;
+    unsigned long long s0 = 0ull, s1 = 0ull, s2 = 0ull, s3 = 0ull;
+    uint32_t k = lane;
+    for (; k + 3 * lanes < it.z; k += 4 * lanes) {
+      s0 += src[(uint64_t)k * kBinRows];
+      s1 += src[(uint64_t)(k + lanes) * kBinRows];
+      s2 += src[(uint64_t)(k + 2 * lanes) * kBinRows];
+      s3 += src[(uint64_t)(k + 3 * lanes) * kBinRows];
+    }
+    for (; k < it.z; k += lanes) s0 += src[(uint64_t)k * kBinRows];
+    psum[threadIdx.x] = (s0 + s1) + (s2 + s3);
+  }
+  __syncthreads();
   finish_rows<256>(
       P, r0, nrows,
       [&](uint32_t v) {
-        unsigned long long s0 = 0ull, s1 = 0ull, s2 = 0ull, s3 = 0ull;
-        uint32_t k = 0;
-        for (; k + 4 <= it.z; k += 4) {
-          s0 += src[(uint64_t)k * kBinRows + v];
-          s1 += src[(uint64_t)(k + 1) * kBinRows + v];
-          s2 += src[(uint64_t)(k + 2) * kBinRows + v];
-          s3 += src[(uint64_t)(k + 3) * kBinRows + v];
-        }
-        for (; k < it.z; ++k) s0 += src[(uint64_t)k * kBinRows + v];
-        return dequantize((s0 + s1) + (s2 + s3));
+        unsigned long long t = 0ull;
+        for (uint32_t q = 0; q < lanes; ++q) t += psum[q * rows + v];
+        return dequantize(t);
       },
       red, gerr);
 }
@@ -368,6 +393,21 @@ __global__ __launch_bounds__(256) void tile_ptr_kernel(uint64_t nnz, const uint6
 
 inline uint32_t grid256(uint64_t m) { return (uint32_t)ceil_div(m ? m : 1, 256); }
 
+__global__ __launch_bounds__(256) void residual_clear_kernel(unsigned long long* __restrict__ lanes, uint64_t words) {
+  const uint64_t t = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+  if (t < words) lanes[t] = 0ull;
+}
+
+// acc[k] = sum of sweep k's lane counters (exact integer sum); one thread per sweep
+__global__ __launch_bounds__(256) void residual_fold_kernel(const unsigned long long* __restrict__ lanes,
+                                                            unsigned long long* __restrict__ acc, uint64_t iters) {
+  const uint64_t k = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+  if (k >= iters) return;
+  unsigned long long t = 0ull;
+  for (uint32_t l = 0; l < kBinAccLanes; ++l) t += lanes[k * kBinAccStride + (uint64_t)l * kBinAccPad];
+  acc[k] = t;
+}
+
 }  // namespace
 
 void binned_build(hipStream_t s, const ActiveCsc& a, uint32_t n, uint64_t nnz, BinnedLayout& L) {
@@ -426,7 +466,11 @@ void binned_build(hipStream_t s, const ActiveCsc& a, uint32_t n, uint64_t nnz, B
   for (uint32_t b = 0; b < L.n_bins; ++b) {
     const uint32_t p0 = tp[(uint64_t)b * C], p1 = tp[(uint64_t)(b + 1) * C];
     const uint32_t cnt = (uint32_t)std::max<uint64_t>(ceil_div(p1 - p0, kItem2Entries), 1);
-    if (cnt > 1) it3.push_back(make_uint4(b, slots, cnt, 0));
+    if (cnt > 1) {
+      // combine units: 256 / L rows each, L lanes per row sharing its cnt slots
+      const uint32_t lg = cnt > 32 ? 3u : cnt > 8 ? 2u : 0u;
+      for (uint32_t v0 = 0; v0 < kBinRows; v0 += 256u >> lg) it3.push_back(make_uint4(b, slots, cnt, v0 | (lg << 16)));
+    }
     for (uint32_t q = 0; q < cnt; ++q) {
       const uint32_t a0 = p0 + (uint32_t)((uint64_t)(p1 - p0) * q / cnt);
       const uint32_t a1 = p0 + (uint32_t)((uint64_t)(p1 - p0) * (q + 1) / cnt);
@@ -482,7 +526,20 @@ void binned_build(hipStream_t s, const ActiveCsc& a, uint32_t n, uint64_t nnz, B
 void binned_enqueue_sweep(hipStream_t s, const BinParams& P) {
   if (P.n_items1) pr_chunk_kernel<<<P.n_items1, kBT, 0, s>>>(P);
   pr_bin_kernel<<<P.n_items2, kBT, 0, s>>>(P);
-  if (P.n_items3) pr_bin_combine_kernel<<<dim3(kBinRows / 256, P.n_items3), 256, 0, s>>>(P);
+  if (P.n_items3) pr_bin_combine_kernel<<<P.n_items3, 256, 0, s>>>(P);
+  NWG_LAUNCH_CHECK();
+}
+
+void binned_clear_residuals(hipStream_t s, unsigned long long* lanes, uint64_t iters) {
+  const uint64_t words = iters * kBinAccStride;
+  if (!words) return;
+  residual_clear_kernel<<<(uint32_t)((words + 255) / 256), 256, 0, s>>>(lanes, words);
+  NWG_LAUNCH_CHECK();
+}
+
+void binned_fold_residuals(hipStream_t s, const unsigned long long* lanes, unsigned long long* acc, uint64_t iters) {
+  if (!iters) return;
+  residual_fold_kernel<<<(uint32_t)((iters + 255) / 256), 256, 0, s>>>(lanes, acc, iters);
   NWG_LAUNCH_CHECK();
 }
 

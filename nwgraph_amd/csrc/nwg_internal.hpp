@@ -96,13 +96,19 @@ __host__ __device__ inline uint32_t slice_of(uint32_t u) { return (u >> kSliceSh
 constexpr int kBinBits = 13;    // rows per bin: 8192 int64 accumulators = 64 KiB of LDS (2 workgroups per CU)
 constexpr int kChunkBits = 15;  // sources per chunk: 32768 floats = 128 KiB of LDS
 constexpr uint32_t kBinMaxSegs = 2048;  // tiles of one chunk-pass work item
+// One graph's per-sweep L1 change is added by thousands of workgroups; they spread their
+// atomics over kBinAccLanes counters kBinAccPad words apart (one address would serialise
+// them), which every workgroup sums when it checks convergence.
+constexpr uint32_t kBinAccLanes = 64;
+constexpr uint32_t kBinAccPad = 8;
+constexpr uint32_t kBinAccStride = kBinAccLanes * kBinAccPad;  // words per sweep
 struct BinnedLayout {
   bool built = false;
   uint32_t n_bins = 0, n_chunks = 0, n_items2 = 0, n_items1 = 0, n_items3 = 0, n_segs = 0, n_slots = 0;
   uint64_t nnz = 0, padded = 0;  // entries, and the length of the per-entry arrays
   DevBuf<uint16_t> brow, bsrc;   // row & (2^kBinBits - 1), source & (2^kChunkBits - 1)
   DevBuf<uint4> items2;          // bin pass: {bin, first entry, end entry, partial slot (or none)}
-  DevBuf<uint4> items3;          // combine pass: {split bin, first slot, slots}
+  DevBuf<uint4> items3;          // combine pass units: {split bin, first slot, slots, first row | log2 lanes << 16}
   DevBuf<uint4> items1;          // chunk pass: {chunk, first tile, end tile, entries}
   DevBuf<uint32_t> seg_start;    // non-empty tiles of each chunk: first entry
   DevBuf<uint32_t> seg_flat;     // and offset inside their chunk-pass item
@@ -151,7 +157,7 @@ struct BinParams {
   float damp;
   uint32_t n;
   uint32_t n_items2, n_items1, n_items3;
-  unsigned long long* acc;    // one graph: iters; several graphs: iters x ngraphs
+  unsigned long long* acc;    // one graph: iters x kBinAccStride (lanes); several graphs: iters x ngraphs
   uint32_t iter;
   double threshold;
   const uint32_t* row_graph;  // several graphs in one union: graph of every row (null: one graph)
@@ -220,6 +226,7 @@ struct PrState {
   DevBuf<float> spart;             // kSlices x n_srows partial sums of the sliced rows
   DevBuf<float> bvals;             // binned sweep: one value per entry
   DevBuf<unsigned long long> bpart;  // binned sweep: partial rows of split bins
+  DevBuf<unsigned long long> bacc;   // binned sweep: per-sweep L1 change, kBinAccLanes lanes
   Captured cap;
 };
 
@@ -321,6 +328,9 @@ nwg_plan_params default_plan_params();
 // binned sweep (binned.hip)
 void binned_build(hipStream_t s, const ActiveCsc& a, uint32_t n, uint64_t nnz, BinnedLayout& L);
 void binned_enqueue_sweep(hipStream_t s, const BinParams& P);
+// lane counters of iters sweeps cleared / summed into acc[0 .. iters)
+void binned_clear_residuals(hipStream_t s, unsigned long long* lanes, uint64_t iters);
+void binned_fold_residuals(hipStream_t s, const unsigned long long* lanes, unsigned long long* acc, uint64_t iters);
 // inplace: one contribution array updated during the sweep (the reference's v11 schedule,
 // page_rank.hpp:135), pull kernels only; else Jacobi (v10) with double-buffered contributions
 void pagerank_run(nwg_ctx* ctx, nwg_graph* g, const float* teleport_dev, float damping, float threshold,

@@ -661,8 +661,9 @@ static void enqueue_solve(hipStream_t s, nwg_graph* g, PrState& st, SweepParams 
     B.n = n;
     B.n_items2 = L.n_items2;
     B.n_items1 = L.n_items1;
-    B.acc = P.acc;
+    B.acc = st.bacc.p;
     B.threshold = P.threshold;
+    binned_clear_residuals(s, st.bacc.p, iters);
   }
   for (uint64_t k = 0; k < iters; ++k) {
     P.iter = (uint32_t)k;
@@ -683,6 +684,7 @@ static void enqueue_solve(hipStream_t s, nwg_graph* g, PrState& st, SweepParams 
     }
     if (ev) NWG_HIP(hipEventRecord(ev[2 * k + 1], s));
   }
+  if (binned) binned_fold_residuals(s, st.bacc.p, st.acc.p, iters);
 }
 
 void pagerank_run(nwg_ctx* ctx, nwg_graph* g, const float* teleport_dev, float damping, float threshold,
@@ -704,6 +706,7 @@ void pagerank_run(nwg_ctx* ctx, nwg_graph* g, const float* teleport_dev, float d
     const BinnedLayout& L = g->plan.bl;
     st.bvals.ensure(L.padded);
     st.bpart.ensure(std::max<uint64_t>((uint64_t)L.n_slots << kBinBits, 1));
+    st.bacc.ensure(std::max<uint64_t>(max_iters, 1) * kBinAccStride);
   }
   const bool personalised = teleport_dev != nullptr;
   if (personalised) {
@@ -730,6 +733,7 @@ void pagerank_run(nwg_ctx* ctx, nwg_graph* g, const float* teleport_dev, float d
     key.ptrs[8] = st.spart.p;
     key.ptrs[9] = st.bvals.p;
     key.ptrs[10] = st.bpart.p;
+    key.ptrs[11] = st.bacc.p;
     key.plan_version = g->plan_version;
     key.iters = max_iters;
     key.damping = damping;
