@@ -115,13 +115,23 @@ constexpr uint32_t kGraphSlots = 64;  // graphs whose L1 changes a workgroup fir
 // lane counter of the sweep.  Several graphs: each row's change, in fixed point, goes to an LDS slot of its
 // graph (graphs g0 .. g0 + kGraphSlots - 1, g0 the graph of row r0; the union's rows are
 // grouped by graph) or straight to acc, then one atomic per graph; rows of converged graphs
-// are left as they are.  red: nt / 64 doubles, gerr: kGraphSlots.
-template <int NT, class Z>
+// are left as they are.  red: nt / 64 doubles, gerr: kGraphSlots.  nrows <= MAXR; the row
+// loop has a fixed trip count (MAXR / NT, unrolled) so a thread's row loads are all in
+// flight together.
+// MULTI: whether P.row_graph is set (the kernels are instantiated per case, so the one-graph
+// kernels carry no per-graph code).
+template <int NT, int MAXR, bool MULTI, class Z>
 __device__ __forceinline__ void finish_rows(const BinParams& P, uint32_t r0, uint32_t nrows, Z z, double* red,
                                             unsigned long long* gerr) {
-  if (!P.row_graph) {
+  static_assert(MAXR % NT == 0, "whole rounds of rows");
+  if constexpr (!MULTI) {
+    (void)gerr;
     double e = 0.0;
-    for (uint32_t v = threadIdx.x; v < nrows; v += NT) e += finish_row(P, r0 + v, z(v));
+#pragma unroll
+    for (uint32_t k = 0; k < MAXR / NT; ++k) {
+      const uint32_t v = threadIdx.x + k * NT;
+      if (v < nrows) e += finish_row(P, r0 + v, z(v));
+    }
     e = wave_sum(e);
     if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = e;
     __syncthreads();
@@ -130,23 +140,26 @@ __device__ __forceinline__ void finish_rows(const BinParams& P, uint32_t r0, uin
       for (uint32_t q = 0; q < NT / 64; ++q) t += red[q];
       if (t > 0.0) atomicAdd(residual_lane(P), err_to_fixed(t));
     }
-    return;
-  }
-  for (uint32_t q = threadIdx.x; q < kGraphSlots; q += NT) gerr[q] = 0ull;
-  __syncthreads();
-  const uint32_t g0 = nrows ? P.row_graph[r0] : 0u;
-  for (uint32_t v = threadIdx.x; v < nrows; v += NT) {
-    const uint32_t g = P.row_graph[r0 + v];
-    if (graph_frozen(P, g)) continue;
-    const double e = finish_row(P, r0 + v, z(v));
-    if (e > 0.0) {
-      if (g - g0 < kGraphSlots) atomicAdd(&gerr[g - g0], err_to_fixed(e));
-      else atomicAdd(P.acc + (uint64_t)P.iter * P.ngraphs + g, err_to_fixed(e));
+  } else {
+    for (uint32_t q = threadIdx.x; q < kGraphSlots; q += NT) gerr[q] = 0ull;
+    __syncthreads();
+    const uint32_t g0 = nrows ? P.row_graph[r0] : 0u;
+#pragma unroll
+    for (uint32_t k = 0; k < MAXR / NT; ++k) {
+      const uint32_t v = threadIdx.x + k * NT;
+      if (v >= nrows) continue;
+      const uint32_t g = P.row_graph[r0 + v];
+      if (graph_frozen(P, g)) continue;
+      const double e = finish_row(P, r0 + v, z(v));
+      if (e > 0.0) {
+        if (g - g0 < kGraphSlots) atomicAdd(&gerr[g - g0], err_to_fixed(e));
+        else atomicAdd(P.acc + (uint64_t)P.iter * P.ngraphs + g, err_to_fixed(e));
+      }
     }
+    __syncthreads();
+    for (uint32_t q = threadIdx.x; q < kGraphSlots; q += NT)
+      if (gerr[q]) atomicAdd(P.acc + (uint64_t)P.iter * P.ngraphs + g0 + q, gerr[q]);
   }
-  __syncthreads();
-  for (uint32_t q = threadIdx.x; q < kGraphSlots; q += NT)
-    if (gerr[q]) atomicAdd(P.acc + (uint64_t)P.iter * P.ngraphs + g0 + q, gerr[q]);
 }
 
 
@@ -226,11 +239,12 @@ __device__ __forceinline__ void wave_flush(unsigned long long* lacc, uint32_t cu
 
 // bin pass: row sums of one entry range of one bin in LDS; a bin that is one item finishes
 // its rows here, an item of a split bin stores its partial rows in its slot of P.part
+template <bool MULTI>
 __global__ __launch_bounds__(kBT) void pr_bin_kernel(BinParams P) {
   if (converged_before(P)) return;
   __shared__ unsigned long long lacc[kBinRows];
   __shared__ double red[kBT / 64];
-  __shared__ unsigned long long gerr[kGraphSlots];
+  __shared__ unsigned long long gerr[MULTI ? kGraphSlots : 1];
   const uint4 it = P.items2[blockIdx.x];  // {bin, first entry, end entry, partial slot or kNoSlot}
   const uint32_t b = it.x;
   for (uint32_t q = threadIdx.x; q < kBinRows; q += kBT) lacc[q] = 0ull;
@@ -287,7 +301,7 @@ __global__ __launch_bounds__(kBT) void pr_bin_kernel(BinParams P) {
     return;
   }
   const uint32_t nrows = r0 < P.n ? min(kBinRows, P.n - r0) : 0u;
-  finish_rows<kBT>(P, r0, nrows, [&](uint32_t v) { return dequantize(lacc[v]); }, red, gerr);
+  finish_rows<kBT, kBinRows, MULTI>(P, r0, nrows, [&](uint32_t v) { return dequantize(lacc[v]); }, red, gerr);
 }
 
 // rows of the bins split over several bin-pass items: add the items' partial rows (exact,
@@ -295,10 +309,11 @@ __global__ __launch_bounds__(kBT) void pr_bin_kernel(BinParams P) {
 // of one split bin, each row's slots shared by L lanes (slot k on lane k mod L); the host
 // picks L per bin from its slot count, so a bin with hundreds of slots is read by many
 // workgroups at once and a bin with a few by few.  One workgroup per unit.
+template <bool MULTI>
 __global__ __launch_bounds__(256) void pr_bin_combine_kernel(BinParams P) {
   if (converged_before(P)) return;
   __shared__ double red[4];
-  __shared__ unsigned long long gerr[kGraphSlots];
+  __shared__ unsigned long long gerr[MULTI ? kGraphSlots : 1];
   __shared__ unsigned long long psum[256];
   const uint4 it = P.items3[blockIdx.x];  // {bin, first slot, slots, first row | log2 L << 16}
   const uint32_t lanes = 1u << (it.w >> 16), rows = 256u >> (it.w >> 16);
@@ -320,7 +335,7 @@ __global__ __launch_bounds__(256) void pr_bin_combine_kernel(BinParams P) {
     psum[threadIdx.x] = (s0 + s1) + (s2 + s3);
   }
   __syncthreads();
-  finish_rows<256>(
+  finish_rows<256, 256, MULTI>(
       P, r0, nrows,
       [&](uint32_t v) {
         unsigned long long t = 0ull;
@@ -525,8 +540,13 @@ void binned_build(hipStream_t s, const ActiveCsc& a, uint32_t n, uint64_t nnz, B
 
 void binned_enqueue_sweep(hipStream_t s, const BinParams& P) {
   if (P.n_items1) pr_chunk_kernel<<<P.n_items1, kBT, 0, s>>>(P);
-  pr_bin_kernel<<<P.n_items2, kBT, 0, s>>>(P);
-  if (P.n_items3) pr_bin_combine_kernel<<<P.n_items3, 256, 0, s>>>(P);
+  if (P.row_graph) {
+    pr_bin_kernel<true><<<P.n_items2, kBT, 0, s>>>(P);
+    if (P.n_items3) pr_bin_combine_kernel<true><<<P.n_items3, 256, 0, s>>>(P);
+  } else {
+    pr_bin_kernel<false><<<P.n_items2, kBT, 0, s>>>(P);
+    if (P.n_items3) pr_bin_combine_kernel<false><<<P.n_items3, 256, 0, s>>>(P);
+  }
   NWG_LAUNCH_CHECK();
 }
 
