@@ -37,7 +37,7 @@ constexpr int kBT = 1024;  // threads of a bin / chunk pass workgroup (16 waves,
 constexpr uint32_t kBinRows = 1u << kBinBits;
 constexpr uint32_t kChunkSrc = 1u << kChunkBits;
 constexpr uint32_t kItem2Entries = 1u << 18;  // entries per bin-pass item
-constexpr uint32_t kItem1Entries = 1u << 19;  // entries per chunk-pass item
+constexpr uint32_t kItem1Entries = 1u << 18;  // entries per chunk-pass item
 constexpr uint32_t kPadSrc = 0xFFFFu;           // source offset of a pad entry (value 0)
 constexpr uint32_t kNoSlot = 0xFFFFFFFFu;     // bin-pass item of a bin that is one item
 constexpr uint32_t kGroups1 = 4;              // groups of 8 entries in flight per lane: chunk pass
@@ -168,7 +168,8 @@ __device__ __forceinline__ void finish_rows(const BinParams& P, uint32_t r0, uin
 // with one 16-byte id load and two 16-byte value stores.
 __global__ __launch_bounds__(kBT) void pr_chunk_kernel(BinParams P) {
   if (converged_before(P)) return;
-  __shared__ float cst[kChunkSrc];
+  __shared__ float4 cst4[kChunkSrc / 4];
+  float* cst = reinterpret_cast<float*>(cst4);
   __shared__ uint32_t sstart[kBinMaxSegs];
   __shared__ uint32_t sflat[kBinMaxSegs + 1];
   const uint4 it = P.items1[blockIdx.x];  // {chunk, first tile, end tile, groups of 8 entries}
@@ -178,8 +179,20 @@ __global__ __launch_bounds__(kBT) void pr_chunk_kernel(BinParams P) {
     sflat[k] = P.seg_flat[it.y + k];
   }
   if (threadIdx.x == 0) sflat[ns] = m;
+  // stage the chunk's contributions: every thread's 16-byte loads issued before any is
+  // waited on (one load at a time would leave the workgroup waiting on memory latency)
   const uint32_t u0 = it.x << kChunkBits;
-  for (uint32_t q = threadIdx.x; q < kChunkSrc; q += kBT) cst[q] = (u0 + q < P.n) ? P.contrib[u0 + q] : 0.f;
+  if ((uint64_t)u0 + kChunkSrc <= P.n) {
+    constexpr uint32_t kV = kChunkSrc / 4 / kBT;
+    const float4* src = reinterpret_cast<const float4*>(P.contrib + u0);
+    float4 v[kV];
+#pragma unroll
+    for (uint32_t k = 0; k < kV; ++k) v[k] = src[threadIdx.x + k * kBT];
+#pragma unroll
+    for (uint32_t k = 0; k < kV; ++k) cst4[threadIdx.x + k * kBT] = v[k];
+  } else {
+    for (uint32_t q = threadIdx.x; q < kChunkSrc; q += kBT) cst[q] = (u0 + q < P.n) ? P.contrib[u0 + q] : 0.f;
+  }
   __syncthreads();
   for (uint32_t g0 = 0; g0 < m; g0 += kGroups1 * kBT) {
     uint32_t p[kGroups1];
@@ -517,6 +530,9 @@ void binned_build(hipStream_t s, const ActiveCsc& a, uint32_t n, uint64_t nnz, B
     }
     if ((uint32_t)sst.size() > first) it1.push_back(make_uint4(c, first, (uint32_t)sst.size(), flat));
   }
+  // largest items first: with one workgroup per CU the pass ends when its last item does,
+  // so the small items fill in at the end
+  std::stable_sort(it1.begin(), it1.end(), [](const uint4& x, const uint4& y) { return x.w > y.w; });
   L.n_items2 = (uint32_t)it2.size();
   L.n_items3 = (uint32_t)it3.size();
   L.n_slots = slots;
