@@ -104,7 +104,7 @@ __device__ __forceinline__ double finish_row(const BinParams& P, uint32_t i, flo
   const float xo = P.x[i];
   const float xn = __builtin_fmaf(P.damp, z, b);
   P.x[i] = xn;
-  P.contrib[i] = xn / (float)P.deg[i];
+  P.cout[i] = xn / (float)P.deg[i];
   return (double)fabsf(xn - xo);
 }
 
@@ -184,14 +184,14 @@ __global__ __launch_bounds__(kBT) void pr_chunk_kernel(BinParams P) {
   const uint32_t u0 = it.x << kChunkBits;
   if ((uint64_t)u0 + kChunkSrc <= P.n) {
     constexpr uint32_t kV = kChunkSrc / 4 / kBT;
-    const float4* src = reinterpret_cast<const float4*>(P.contrib + u0);
+    const float4* src = reinterpret_cast<const float4*>(P.cin + u0);
     float4 v[kV];
 #pragma unroll
     for (uint32_t k = 0; k < kV; ++k) v[k] = src[threadIdx.x + k * kBT];
 #pragma unroll
     for (uint32_t k = 0; k < kV; ++k) cst4[threadIdx.x + k * kBT] = v[k];
   } else {
-    for (uint32_t q = threadIdx.x; q < kChunkSrc; q += kBT) cst[q] = (u0 + q < P.n) ? P.contrib[u0 + q] : 0.f;
+    for (uint32_t q = threadIdx.x; q < kChunkSrc; q += kBT) cst[q] = (u0 + q < P.n) ? P.cin[u0 + q] : 0.f;
   }
   __syncthreads();
   for (uint32_t g0 = 0; g0 < m; g0 += kGroups1 * kBT) {
@@ -250,49 +250,57 @@ __device__ __forceinline__ void wave_flush(unsigned long long* lacc, uint32_t cu
   }
 }
 
-// bin pass: row sums of one entry range of one bin in LDS; a bin that is one item finishes
-// its rows here, an item of a split bin stores its partial rows in its slot of P.part
-template <bool MULTI>
-__global__ __launch_bounds__(kBT) void pr_bin_kernel(BinParams P) {
-  if (converged_before(P)) return;
-  __shared__ unsigned long long lacc[kBinRows];
-  __shared__ double red[kBT / 64];
-  __shared__ unsigned long long gerr[MULTI ? kGraphSlots : 1];
-  const uint4 it = P.items2[blockIdx.x];  // {bin, first entry, end entry, partial slot or kNoSlot}
-  const uint32_t b = it.x;
-  for (uint32_t q = threadIdx.x; q < kBinRows; q += kBT) lacc[q] = 0ull;
-  __syncthreads();
-  // A wave takes 512 consecutive entries (8 per lane, from an 8-aligned start: the arrays
-  // are padded), kGroups2 such blocks in flight; the loop bounds are uniform over the wave so
-  // every lane takes part in the votes of wave_flush.  Runs of one row are added in
-  // registers; a lane's last run goes through wave_flush.
+// Adds the entries [e0, e1) of one bin into lacc.  A wave takes 512 consecutive entries (8
+// per lane, from an 8-aligned start: the arrays are padded), kGroups2 such blocks in flight;
+// the loop bounds are uniform over the wave so every lane takes part in the votes of
+// wave_flush.  Runs of one row are added in registers; a lane's last run goes through
+// wave_flush.  HOT: the entries lie in a hot tile and their values are hot[source] (pads,
+// source kPadSrc, add 0); else they are read from P.vals.
+template <bool HOT>
+__device__ __forceinline__ void bin_range(const BinParams& P, uint64_t e0, uint64_t e1, unsigned long long* lacc,
+                                          const float* hot) {
   const uint32_t w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  for (uint64_t wq = (uint64_t)(it.y & ~7u) + 512u * w; wq < it.z; wq += kGroups2 * 512u * (kBT / 64)) {
-    uint4 rw[kGroups2];
+  for (uint64_t wq = (e0 & ~7ull) + 512u * w; wq < e1; wq += kGroups2 * 512u * (kBT / 64)) {
+    uint4 rw[kGroups2], sw[kGroups2];
     float4 va[kGroups2], vb[kGroups2];
 #pragma unroll
     for (uint32_t h = 0; h < kGroups2; ++h) {
-      // unconditional loads (clamped into the item's first group when past its end, then
+      // unconditional loads (clamped into the range's first group when past its end, then
       // masked below) so that all groups' loads are issued before any is waited on
       const uint64_t q = wq + h * 512u * (kBT / 64) + 8u * lane;
-      const uint64_t qc = q < it.z ? q : (uint64_t)(it.y & ~7u);
+      const uint64_t qc = q < e1 ? q : (e0 & ~7ull);
       rw[h] = *reinterpret_cast<const uint4*>(P.brow + qc);
-      va[h] = *reinterpret_cast<const float4*>(P.vals + qc);
-      vb[h] = *reinterpret_cast<const float4*>(P.vals + qc + 4);
+      if constexpr (HOT) {
+        sw[h] = *reinterpret_cast<const uint4*>(P.bsrc + qc);
+      } else {
+        va[h] = *reinterpret_cast<const float4*>(P.vals + qc);
+        vb[h] = *reinterpret_cast<const float4*>(P.vals + qc + 4);
+      }
     }
 #pragma unroll
     for (uint32_t h = 0; h < kGroups2; ++h) {
       const uint64_t q = wq + h * 512u * (kBT / 64) + 8u * lane;
       uint32_t cur = kNoRow;
       unsigned long long run = 0ull;
-      if (q < it.z) {
+      if (q < e1) {
         const uint32_t rows[8] = {rw[h].x & 0xFFFFu, rw[h].x >> 16, rw[h].y & 0xFFFFu, rw[h].y >> 16,
                                   rw[h].z & 0xFFFFu, rw[h].z >> 16, rw[h].w & 0xFFFFu, rw[h].w >> 16};
-        const float vs[8] = {va[h].x, va[h].y, va[h].z, va[h].w, vb[h].x, vb[h].y, vb[h].z, vb[h].w};
+        float vs[8];
+        if constexpr (HOT) {
+          const uint32_t sv[4] = {sw[h].x, sw[h].y, sw[h].z, sw[h].w};
+#pragma unroll
+          for (uint32_t t = 0; t < 8; ++t) {
+            const uint32_t u = (sv[t >> 1] >> (16 * (t & 1))) & 0xFFFFu;
+            vs[t] = u == kPadSrc ? 0.f : hot[u];
+          }
+        } else {
+          vs[0] = va[h].x; vs[1] = va[h].y; vs[2] = va[h].z; vs[3] = va[h].w;
+          vs[4] = vb[h].x; vs[5] = vb[h].y; vs[6] = vb[h].z; vs[7] = vb[h].w;
+        }
 #pragma unroll
         for (uint32_t r = 0; r < 8; ++r) {
           const uint64_t e = q + r;
-          if (e < it.y || e >= it.z) continue;
+          if (e < e0 || e >= e1) continue;
           const unsigned long long f = quantize(vs[r]);
           if (rows[r] == cur) {
             run += f;
@@ -306,6 +314,47 @@ __global__ __launch_bounds__(kBT) void pr_bin_kernel(BinParams P) {
       wave_flush(lacc, cur, run, lane);
     }
   }
+}
+
+// bin pass: row sums of one entry range of one bin in LDS; a bin that is one item finishes
+// its rows here, an item of a split bin stores its partial rows in its slot of P.part.
+// The range's part inside the bin's hot tile (a prefix of the bin) reads its values from
+// the hot contributions staged in LDS.
+template <bool MULTI>
+__global__ __launch_bounds__(kBT) void pr_bin_kernel(BinParams P) {
+  if (converged_before(P)) return;
+  __shared__ unsigned long long lacc[kBinRows];
+  __shared__ double red[kBT / 64];
+  __shared__ unsigned long long gerr[MULTI ? kGraphSlots : 1];
+  __shared__ float hot[MULTI ? 1 : kHotSrc];
+  const uint4 it = P.items2[blockIdx.x];  // {bin, first entry, end entry, partial slot or kNoSlot}
+  const uint32_t b = it.x;
+  for (uint32_t q = threadIdx.x; q < kBinRows; q += kBT) lacc[q] = 0ull;
+  uint32_t he = it.y;  // end of the hot part of [it.y, it.z)
+  if constexpr (!MULTI) {
+    if (P.n_hot) {
+      he = min(max(P.hot_end[b], it.y), it.z);
+      if (he > it.y) {
+        constexpr uint32_t kR = (kHotSrc + kBT - 1) / kBT;
+        float v[kR];
+#pragma unroll
+        for (uint32_t k = 0; k < kR; ++k) {
+          const uint32_t q = threadIdx.x + k * kBT;
+          v[k] = q < P.n_hot ? P.cin[q] : 0.f;
+        }
+#pragma unroll
+        for (uint32_t k = 0; k < kR; ++k) {
+          const uint32_t q = threadIdx.x + k * kBT;
+          if (q < kHotSrc) hot[q] = v[k];
+        }
+      }
+    }
+  }
+  __syncthreads();
+  if constexpr (!MULTI) {
+    if (he > it.y) bin_range<true>(P, it.y, he, lacc, hot);
+  }
+  bin_range<false>(P, he, it.z, lacc, hot);
   __syncthreads();
   const uint32_t r0 = b << kBinBits;
   if (it.w != kNoSlot) {
@@ -360,9 +409,11 @@ __global__ __launch_bounds__(256) void pr_bin_combine_kernel(BinParams P) {
 
 // ---- layout construction ----
 // tile key of CSC position p (row v, source u) and its packed low bits
+// (tile column: 0 for a source below n_hot, else 1 + chunk; without hot tiles the chunk)
 __global__ __launch_bounds__(256) void bin_key_kernel(uint32_t n, uint64_t nnz, const uint32_t* __restrict__ rp,
-                                                      const uint32_t* __restrict__ col, uint32_t n_chunks,
-                                                      uint64_t* __restrict__ keys, uint32_t* __restrict__ low) {
+                                                      const uint32_t* __restrict__ col, uint32_t n_cols,
+                                                      uint32_t n_hot, uint64_t* __restrict__ keys,
+                                                      uint32_t* __restrict__ low) {
   const uint64_t p = (uint64_t)blockIdx.x * 256 + threadIdx.x;
   if (p >= nnz) return;
   uint32_t lo = 0, hi = n;  // row of p: first r with rp[r + 1] > p
@@ -372,8 +423,10 @@ __global__ __launch_bounds__(256) void bin_key_kernel(uint32_t n, uint64_t nnz, 
     else hi = mid;
   }
   const uint32_t u = col[p];
-  keys[p] = (uint64_t)(lo >> kBinBits) * n_chunks + (u >> kChunkBits);
-  low[p] = ((lo & (kBinRows - 1)) << 16) | (u & (kChunkSrc - 1));
+  const bool hot = u < n_hot;
+  const uint32_t c = hot ? 0u : (u >> kChunkBits) + (n_hot ? 1u : 0u);
+  keys[p] = (uint64_t)(lo >> kBinBits) * n_cols + c;
+  low[p] = ((lo & (kBinRows - 1)) << 16) | (hot ? u : (u & (kChunkSrc - 1)));
 }
 
 // sorted entry q of tile t goes to padded position ptp[t] + (q - tp[t])
@@ -438,19 +491,22 @@ __global__ __launch_bounds__(256) void residual_fold_kernel(const unsigned long 
 
 }  // namespace
 
-void binned_build(hipStream_t s, const ActiveCsc& a, uint32_t n, uint64_t nnz, BinnedLayout& L) {
+void binned_build(hipStream_t s, const ActiveCsc& a, uint32_t n, uint64_t nnz, BinnedLayout& L, bool hot) {
   NWG_REQUIRE(nnz < (1ull << 32) - 16, "binned layout needs fewer than 2^32 - 16 entries");
   L = BinnedLayout();
   L.n_bins = (uint32_t)ceil_div(n, kBinRows);
   L.n_chunks = (uint32_t)ceil_div(n, kChunkSrc);
+  L.n_hot = hot ? std::min<uint32_t>(kHotSrc, n) : 0u;
   L.nnz = nnz;
-  const uint64_t n_tiles = (uint64_t)L.n_bins * L.n_chunks;
+  // tile columns per bin: the hot tile (if any), then one per chunk
+  const uint32_t hc = L.n_hot ? 1u : 0u, TC = L.n_chunks + hc;
+  const uint64_t n_tiles = (uint64_t)L.n_bins * TC;
   std::vector<uint32_t> tp(n_tiles + 1, 0);  // becomes the padded tile pointers
   uint64_t total = 0;
   if (nnz) {
     DevBuf<uint64_t> keys(nnz), keys_tmp(nnz);
     DevBuf<uint32_t> low(nnz), low_tmp(nnz);
-    bin_key_kernel<<<grid256(nnz), 256, 0, s>>>(n, nnz, a.ptr, a.idx, L.n_chunks, keys.p, low.p);
+    bin_key_kernel<<<grid256(nnz), 256, 0, s>>>(n, nnz, a.ptr, a.idx, TC, L.n_hot, keys.p, low.p);
     NWG_LAUNCH_CHECK();
     int kb = 1;
     while (kb < 64 && (1ull << kb) < n_tiles) ++kb;
@@ -487,12 +543,17 @@ void binned_build(hipStream_t s, const ActiveCsc& a, uint32_t n, uint64_t nnz, B
     NWG_HIP(hipMemsetAsync(L.bsrc.p, 0xFF, L.padded * 2, s));
   }
   const uint32_t C = L.n_chunks;
+  // hot tile ends (the hot tile is the first of its bin)
+  std::vector<uint32_t> he(L.n_bins);
+  for (uint32_t b = 0; b < L.n_bins; ++b) he[b] = tp[(uint64_t)b * TC + hc];
+  L.hot_end.alloc(std::max<size_t>(he.size(), 1));
+  if (!he.empty()) NWG_HIP(hipMemcpyAsync(L.hot_end.p, he.data(), he.size() * 4, hipMemcpyHostToDevice, s));
   // bin pass items: every bin, cut into ranges of at most kItem2Entries entries; the items
   // of a split bin get consecutive partial slots, and the bin a combine item
   std::vector<uint4> it2, it3;
   uint32_t slots = 0;
   for (uint32_t b = 0; b < L.n_bins; ++b) {
-    const uint32_t p0 = tp[(uint64_t)b * C], p1 = tp[(uint64_t)(b + 1) * C];
+    const uint32_t p0 = tp[(uint64_t)b * TC], p1 = tp[(uint64_t)(b + 1) * TC];
     const uint32_t cnt = (uint32_t)std::max<uint64_t>(ceil_div(p1 - p0, kItem2Entries), 1);
     if (cnt > 1) {
       // combine units: 256 / L rows each, L lanes per row sharing its cnt slots
@@ -513,7 +574,7 @@ void binned_build(hipStream_t s, const ActiveCsc& a, uint32_t n, uint64_t nnz, B
   for (uint32_t c = 0; c < C; ++c) {
     uint32_t first = (uint32_t)sst.size(), flat = 0;
     for (uint32_t b = 0; b < L.n_bins; ++b) {
-      const uint32_t t0 = tp[(uint64_t)b * C + c], t1 = tp[(uint64_t)b * C + c + 1];
+      const uint32_t t0 = tp[(uint64_t)b * TC + hc + c], t1 = tp[(uint64_t)b * TC + hc + c + 1];
       // a long tile is cut into pieces of at most kItem1Entries entries
       for (uint32_t q0 = t0; q0 < t1; q0 += kItem1Entries) {
         const uint32_t len = (std::min<uint32_t>(t1 - q0, kItem1Entries)) / 8;

@@ -240,7 +240,8 @@ void multigraph_set_binned(nwg_ctx* ctx, nwg_multigraph* mg, bool binned) {
       if (n) row_graph_kernel<<<grid1(n), kThreads, 0, s>>>(n, (uint32_t)mg->voff[g], g, mg->row_graph.p);
     }
     NWG_LAUNCH_CHECK();
-    binned_build(s, ActiveCsc{mg->rp.p, mg->col.p, mg->deg.p, nullptr}, (uint32_t)mg->nv, mg->ne, mg->bl);
+    // no hot tiles: the union's first ids are one graph's, not the union's hubs
+    binned_build(s, ActiveCsc{mg->rp.p, mg->col.p, mg->deg.p, nullptr}, (uint32_t)mg->nv, mg->ne, mg->bl, false);
     mg->row_base.alloc(mg->nv ? mg->nv : 1);
     mg->bvals.alloc(mg->bl.padded);
     mg->bpart.alloc(std::max<uint64_t>((uint64_t)mg->bl.n_slots << kBinBits, 1));
@@ -297,7 +298,9 @@ void multigraph_run(nwg_ctx* ctx, nwg_multigraph* mg, float damping, float thres
     B.part = mg->bpart.p;
     B.deg = mg->deg.p;
     B.x = mg->x.p;
-    B.contrib = mg->c0.p;  // one buffer: the chunk pass reads it before the bin pass rewrites it
+    // one buffer: the chunk pass reads it before the bin pass rewrites it (no hot tiles read it there)
+    B.cin = mg->c0.p;
+    B.cout = mg->c0.p;
     B.basev = mg->row_base.p;
     B.damp = damping;
     B.n = (uint32_t)mg->nv;

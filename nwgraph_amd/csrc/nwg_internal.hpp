@@ -96,6 +96,11 @@ __host__ __device__ inline uint32_t slice_of(uint32_t u) { return (u >> kSliceSh
 constexpr int kBinBits = 13;    // rows per bin: 8192 int64 accumulators = 64 KiB of LDS (2 workgroups per CU)
 constexpr int kChunkBits = 15;  // sources per chunk: 32768 floats = 128 KiB of LDS
 constexpr uint32_t kBinMaxSegs = 2048;  // tiles of one chunk-pass work item
+// Hot tiles: a bin's entries whose source is below kHotSrc (the highest out-degrees in a
+// degree-ordered numbering) skip the value array: the bin pass reads them from the
+// contributions staged in its LDS next to the row accumulators (kHotSrc floats keep two
+// workgroups per CU).
+constexpr uint32_t kHotSrc = 3968;
 // One graph's per-sweep L1 change is added by thousands of workgroups; they spread their
 // atomics over kBinAccLanes counters kBinAccPad words apart (one address would serialise
 // them), which every workgroup sums when it checks convergence.
@@ -105,8 +110,10 @@ constexpr uint32_t kBinAccStride = kBinAccLanes * kBinAccPad;  // words per swee
 struct BinnedLayout {
   bool built = false;
   uint32_t n_bins = 0, n_chunks = 0, n_items2 = 0, n_items1 = 0, n_items3 = 0, n_segs = 0, n_slots = 0;
+  uint32_t n_hot = 0;            // sources [0, n_hot) form each bin's hot tile (0: no hot tiles)
   uint64_t nnz = 0, padded = 0;  // entries, and the length of the per-entry arrays
-  DevBuf<uint16_t> brow, bsrc;   // row & (2^kBinBits - 1), source & (2^kChunkBits - 1)
+  DevBuf<uint16_t> brow, bsrc;   // row & (2^kBinBits - 1); source & (2^kChunkBits - 1), or the source in a hot tile
+  DevBuf<uint32_t> hot_end;      // per bin: end of its hot tile (its first tile)
   DevBuf<uint4> items2;          // bin pass: {bin, first entry, end entry, partial slot (or none)}
   DevBuf<uint4> items3;          // combine pass units: {split bin, first slot, slots, first row | log2 lanes << 16}
   DevBuf<uint4> items1;          // chunk pass: {chunk, first tile, end tile, entries}
@@ -151,7 +158,10 @@ struct BinParams {
   unsigned long long* part;  // n_slots << kBinBits partial row sums (fixed point) of the split bins' items
   const uint32_t* deg;
   float* x;
-  float* contrib;
+  const float* cin;   // contributions of the previous sweep (read by the chunk pass and the hot tiles)
+  float* cout;        // contributions written by the row updates (may equal cin without hot tiles)
+  const uint32_t* hot_end;
+  uint32_t n_hot;
   const float* basev;
   float base;
   float damp;
@@ -326,7 +336,7 @@ void build_segments(hipStream_t s, const uint32_t* rp, uint32_t n, uint32_t min_
                     uint32_t* nsegs);
 nwg_plan_params default_plan_params();
 // binned sweep (binned.hip)
-void binned_build(hipStream_t s, const ActiveCsc& a, uint32_t n, uint64_t nnz, BinnedLayout& L);
+void binned_build(hipStream_t s, const ActiveCsc& a, uint32_t n, uint64_t nnz, BinnedLayout& L, bool hot);
 void binned_enqueue_sweep(hipStream_t s, const BinParams& P);
 // lane counters of iters sweeps cleared / summed into acc[0 .. iters)
 void binned_clear_residuals(hipStream_t s, unsigned long long* lanes, uint64_t iters);

@@ -531,7 +531,7 @@ void plan_build(hipStream_t s, nwg_graph* g, const nwg_plan_params& pp) {
     pl.scol.release();
   }
   pl.n_tiles = (uint32_t)ceil_div(n, kThreads);
-  if (pp.binned) binned_build(s, g->active(pp.relabel != 0), n, g->nnz, pl.bl);
+  if (pp.binned) binned_build(s, g->active(pp.relabel != 0), n, g->nnz, pl.bl, true);
   NWG_HIP(hipStreamSynchronize(s));
   pl.built = true;
   g->plan = std::move(pl);
@@ -640,8 +640,8 @@ static void enqueue_solve(hipStream_t s, nwg_graph* g, PrState& st, SweepParams 
   const bool binned = g->plan.params.binned && !inplace;  // the in-place schedule is a pull sweep
   BinParams B{};
   if (binned) {
-    // contrib is one buffer here: the chunk pass of sweep k reads it before the bin pass
-    // of sweep k rewrites it
+    // contributions double-buffered like the pull sweep's: the hot tiles of the bin pass
+    // read the previous sweep's while the row updates write the next
     B.brow = L.brow.p;
     B.bsrc = L.bsrc.p;
     B.vals = st.bvals.p;
@@ -654,7 +654,8 @@ static void enqueue_solve(hipStream_t s, nwg_graph* g, PrState& st, SweepParams 
     B.n_items3 = L.n_items3;
     B.deg = P.deg;
     B.x = P.x;
-    B.contrib = st.contrib0.p;
+    B.hot_end = L.hot_end.p;
+    B.n_hot = L.n_hot;
     B.basev = P.basev;
     B.base = P.base;
     B.damp = P.damp;
@@ -674,6 +675,8 @@ static void enqueue_solve(hipStream_t s, nwg_graph* g, PrState& st, SweepParams 
     if (ev) NWG_HIP(hipEventRecord(ev[2 * k], s));
     if (binned) {
       B.iter = (uint32_t)k;
+      B.cin = P.cin;
+      B.cout = P.cout;
       binned_enqueue_sweep(s, B);
     } else if (nt == 2) {
       enqueue_sweep<2>(s, P);
