@@ -41,7 +41,8 @@ constexpr uint32_t kUnroll = 8;
 // workgroup first adds in LDS
 constexpr uint32_t kFinItems = 4096;
 constexpr uint32_t kFinMaxK = 256;
-constexpr uint32_t kFoldRows = 1024;  // row-workgroup L1 partials folded per finishing workgroup
+constexpr uint32_t kFoldRows = 128;  // row-workgroup L1 partials folded per finishing workgroup
+constexpr uint32_t kFinU = 4;        // heavy-row items a finishing thread has in flight
 // edges of a heavy-row segment per lane group (all their gathers are in flight at once)
 constexpr uint32_t kSegPerGroup = 32;
 // consecutive light rows per lane group (lanes 0..kR of the group hold their row pointers)
@@ -111,10 +112,6 @@ __device__ __forceinline__ double ppr_finish_row(const PprParams& P, uint32_t i,
   P.x[(uint64_t)i * P.k + j] = xn;
   P.cout[cidx(P, i, j)] = xn / (float)(df & ~kSeedFlag);
   return (double)fabsf(xn - xo);
-}
-
-__device__ __forceinline__ double ppr_update(const PprParams& P, uint32_t i, uint32_t j, float z) {
-  return ppr_finish_row(P, i, j, z, P.degflag[i], P.x[(uint64_t)i * P.k + j]);
 }
 
 // per-vector sum over the 64/T row slots of a wave, then over the 4 waves; result for
@@ -270,28 +267,50 @@ __global__ __launch_bounds__(kThreads) void ppr_finish_kernel(PprParams P, uint3
   for (uint32_t q = threadIdx.x; q < kFinMaxK; q += kThreads) errj[q] = 0ull;
   __syncthreads();
   const uint64_t hub_items = (uint64_t)P.n_hub_rows * P.k;
-  for (uint32_t m = threadIdx.x; m < kFinItems; m += kThreads) {
-    const uint64_t t = (uint64_t)blockIdx.x * kFinItems + m;
-    if (t >= hub_items) break;
-    const uint32_t j = (uint32_t)(t % P.k);
-    if (vec_done(P, j)) continue;
-    const uint32_t h = (uint32_t)(t / P.k);
-    // the row's segment partials, added in segment order with 8 loads in flight (a hub row
-    // has hundreds of segments)
-    float z = 0.f;
-    const uint32_t s1 = P.seg_ptr[h + 1];
-    for (uint32_t s = P.seg_ptr[h]; s < s1; s += 8) {
-      float v[8];
+  // kFinU items per thread at a time, every item's first loads (segment bounds, then the
+  // first partial, degree word and old score) issued before any is used
+  for (uint32_t m0 = 0; m0 < kFinItems; m0 += kFinU * kThreads) {
+    bool ok[kFinU];
+    uint32_t jv[kFinU], row[kFinU], s0[kFinU], s1[kFinU], df[kFinU];
+    float z[kFinU], xo[kFinU];
 #pragma unroll
-      for (uint32_t r = 0; r < 8; ++r) v[r] = s + r < s1 ? P.hub_part[(uint64_t)(s + r) * P.k + j] : 0.f;
-#pragma unroll
-      for (uint32_t r = 0; r < 8; ++r)
-        if (s + r < s1) z += v[r];
+    for (uint32_t u = 0; u < kFinU; ++u) {
+      const uint64_t t = (uint64_t)blockIdx.x * kFinItems + m0 + u * kThreads + threadIdx.x;
+      jv[u] = (uint32_t)(t % P.k);
+      ok[u] = t < hub_items && !vec_done(P, jv[u]);
+      if (ok[u]) {
+        const uint32_t h = (uint32_t)(t / P.k);
+        s0[u] = P.seg_ptr[h];
+        s1[u] = P.seg_ptr[h + 1];  // > s0: a heavy row has at least one segment
+        row[u] = P.hub_rows[h];
+      }
     }
-    const double e = ppr_update(P, P.hub_rows[h], j, z);
-    if (e > 0.0) {
-      if (j < kFinMaxK) atomicAdd(&errj[j], err_to_fixed(e));
-      else atomicAdd(P.acc + (uint64_t)P.iter * P.k + j, err_to_fixed(e));
+#pragma unroll
+    for (uint32_t u = 0; u < kFinU; ++u) {
+      if (ok[u]) {
+        z[u] = P.hub_part[(uint64_t)s0[u] * P.k + jv[u]];
+        df[u] = P.degflag[row[u]];
+        xo[u] = P.x[(uint64_t)row[u] * P.k + jv[u]];
+      }
+    }
+#pragma unroll
+    for (uint32_t u = 0; u < kFinU; ++u) {
+      if (!ok[u]) continue;
+      // the other segment partials (rows of many segments), added in segment order with 8
+      // loads in flight
+      for (uint32_t s = s0[u] + 1; s < s1[u]; s += 8) {
+        float v[8];
+#pragma unroll
+        for (uint32_t r = 0; r < 8; ++r) v[r] = s + r < s1[u] ? P.hub_part[(uint64_t)(s + r) * P.k + jv[u]] : 0.f;
+#pragma unroll
+        for (uint32_t r = 0; r < 8; ++r)
+          if (s + r < s1[u]) z[u] += v[r];
+      }
+      const double e = ppr_finish_row(P, row[u], jv[u], z[u], df[u], xo[u]);
+      if (e > 0.0) {
+        if (jv[u] < kFinMaxK) atomicAdd(&errj[jv[u]], err_to_fixed(e));
+        else atomicAdd(P.acc + (uint64_t)P.iter * P.k + jv[u], err_to_fixed(e));
+      }
     }
   }
   __syncthreads();
@@ -302,12 +321,16 @@ __global__ __launch_bounds__(kThreads) void ppr_finish_kernel(PprParams P, uint3
   const uint64_t rows_total = (uint64_t)P.row_blocks;
   const uint64_t r0 = (uint64_t)blockIdx.x * kFoldRows;
   for (uint32_t tl = 0; tl < tiles; ++tl) {
-    double acc = 0.0;
     const uint32_t vl = threadIdx.x % T;
-    const uint32_t slot = threadIdx.x / T;  // kThreads/T slots stride over the 32 rows
-    for (uint64_t r = r0 + slot; r < r0 + kFoldRows && r < rows_total; r += kThreads / T)
-      acc += P.blk_err[((uint64_t)tl * P.row_blocks + r) * T + vl];
-    red[threadIdx.x] = acc;
+    const uint32_t slot = threadIdx.x / T;  // kThreads/T slots stride over the kFoldRows rows
+    constexpr uint32_t kS = kThreads / T, kPer = kFoldRows / kS;
+    double a[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (uint32_t q = 0; q < kPer; ++q) {
+      const uint64_t r = r0 + slot + (uint64_t)q * kS;
+      if (r < rows_total) a[q & 3] += P.blk_err[((uint64_t)tl * P.row_blocks + r) * T + vl];
+    }
+    red[threadIdx.x] = (a[0] + a[1]) + (a[2] + a[3]);
     __syncthreads();
     if (threadIdx.x < (uint32_t)T) {
       double sum = 0.0;
