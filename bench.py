@@ -263,6 +263,15 @@ def run_ppr(args, ctx):
         seq_dev += si["ms_total"]
     torch.cuda.synchronize()
     ms_seq = (time.perf_counter() - t0) * 1e3
+    # 64 sequential plain solves (uniform teleport) on the same graph and plan
+    t0 = time.perf_counter()
+    plain_its, plain_dev = [], 0.0
+    for j in range(k):
+        si = g.pagerank_device(x1.data_ptr(), 0, 0.85, args.tol, args.iters)
+        plain_its.append(si["iterations"])
+        plain_dev += si["ms_total"]
+    torch.cuda.synchronize()
+    ms_plain = (time.perf_counter() - t0) * 1e3
     vec_sweeps = int(np.sum(its))
     value = nnz * vec_sweeps / (ms_batched * 1e-3) / 1e9
     out = base_line(args, round(value, 3), "GTEPS (vector-sweeps x edges / s)", ms_batched, "f32", {
@@ -273,6 +282,10 @@ def run_ppr(args, ctx):
         "ms_batched": round(ms_batched, 3), "ms_64_sequential": round(ms_seq, 3),
         "device_ms_batched": round(info["ms_total"], 3), "device_ms_64_sequential": round(seq_dev, 3),
         "speedup_vs_sequential": round(ms_seq / ms_batched, 3),
+        "plain_solve_sweeps": int(np.sum(plain_its)), "ms_64_sequential_plain": round(ms_plain, 3),
+        "device_ms_64_sequential_plain": round(plain_dev, 3),
+        "speedup_vs_sequential_plain": round(ms_plain / ms_batched, 3),
+        "useful_vector_sweep_fraction": round(vec_sweeps / (k * max(1, int(np.max(its)))), 3),
     })
     g.close()
     return out

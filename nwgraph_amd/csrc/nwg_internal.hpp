@@ -244,9 +244,13 @@ struct PrState {
 // CSC (rebuilt when the plan or the k-tile changes) and grow-only buffers.
 struct PprState {
   uint64_t seg_version = 0;  // plan_version the segments were built for (0 = none)
-  uint32_t seg_tile = 0;
+  uint32_t seg_tile = 0, seg_slices = 0;
   uint32_t n_heavy = 0, n_segs = 0;
   DevBuf<uint32_t> heavy_rows, heavy_seg_ptr, heavy_seg_row;
+  // heavy rows' in-edges regrouped by (row, source slice); segments [start, end) of it,
+  // numbered row by row; the segment ids of each slice, slice after slice
+  DevBuf<uint32_t> hcol, seg_start, seg_end, slice_segs;
+  std::vector<uint32_t> slice_off;
   DevBuf<uint32_t> lrp, lcol;  // light rows' in-edges, compacted (heavy rows empty)
   DevBuf<uint32_t> degflag, seed_row, seed_vec, ids, ids_new;
   DevBuf<float> seed_base, c0, c1, x, hub_part;

@@ -39,7 +39,7 @@ constexpr uint32_t kSeedFlag = 0x80000000u;
 constexpr uint32_t kUnroll = 8;
 // (heavy row, vector) items per finishing workgroup, and the vectors whose L1 changes that
 // workgroup first adds in LDS
-constexpr uint32_t kFinItems = 4096;
+constexpr uint32_t kFinItems = 2048;
 constexpr uint32_t kFinMaxK = 256;
 constexpr uint32_t kFoldRows = 128;  // row-workgroup L1 partials folded per finishing workgroup
 constexpr uint32_t kFinU = 4;        // heavy-row items a finishing thread has in flight
@@ -73,6 +73,13 @@ struct PprParams {
   const uint32_t* hub_rows;
   const uint32_t* seg_ptr;
   const uint32_t* seg_row;
+  const uint32_t* hcol;        // heavy rows' in-edges regrouped by (row, source slice)
+  const uint32_t* seg_start;   // segment s covers hcol[seg_start[s], seg_end[s])
+  const uint32_t* seg_end;
+  const uint32_t* slice_segs;  // segment ids of slice x: slice_segs[slice_off[x] .. slice_off[x + 1])
+  uint32_t slice_off[9];
+  uint32_t n_slices;           // source slices per k-tile; tiles * n_slices <= 8 workgroup classes
+  uint32_t light_blocks_pad;   // row workgroups of all k-tiles, rounded up to a multiple of 8
   float* hub_part;           // n_hub_segs x k
   double* blk_err;           // (row_blocks * tiles) x T
   unsigned long long* acc;   // iters x k
@@ -136,9 +143,26 @@ __global__ __launch_bounds__(kThreads) void ppr_sweep_kernel(PprParams P) {
   __shared__ float hub_red[kThreads];
   __shared__ float zbuf[kThreads * PprRows<T>::value];
   __shared__ int any_live;
-  // workgroup b serves k-tile b % tiles: with tiles dividing 8, every XCD (workgroup b runs
-  // on XCD b % 8) serves one k-tile, and its L2 caches that tile's contributions only
-  const uint32_t tile = blockIdx.x % P.tiles;
+  // Row workgroup b serves k-tile b % tiles: with tiles dividing 8, every XCD (workgroup b
+  // runs on XCD b % 8) serves one k-tile, and its L2 caches that tile's contributions only.
+  // Heavy-segment workgroups start at a multiple of 8; workgroup class c = (b - start) % 8
+  // (its XCD) serves k-tile c % tiles and source slice c / tiles, so that each of the XCDs
+  // of one k-tile caches only its slice of the sources.
+  const bool heavy = blockIdx.x >= P.light_blocks_pad;
+  uint32_t tile, b, seg = 0;
+  if (!heavy) {
+    tile = blockIdx.x % P.tiles;
+    b = blockIdx.x / P.tiles;
+    if (b >= P.row_blocks) return;  // padding up to the heavy workgroups
+  } else {
+    const uint32_t hb = blockIdx.x - P.light_blocks_pad, cls = hb % 8u;
+    if (cls >= P.tiles * P.n_slices) return;
+    tile = cls % P.tiles;
+    const uint32_t x = cls / P.tiles, q = hb / 8u;
+    if (q >= P.slice_off[x + 1] - P.slice_off[x]) return;
+    seg = P.slice_segs[P.slice_off[x] + q];
+    b = 0;
+  }
   const uint32_t vl = threadIdx.x % T;
   const uint32_t j = tile * T + vl;
   const bool live = j < P.k && !vec_done(P, j);
@@ -147,9 +171,8 @@ __global__ __launch_bounds__(kThreads) void ppr_sweep_kernel(PprParams P) {
   if (live) any_live = 1;
   __syncthreads();
   if (!any_live) return;
-  const uint32_t b = blockIdx.x / P.tiles;
   const float* __restrict__ ctile = P.cin + (uint64_t)tile * P.n * T;
-  if (b < P.row_blocks) {
+  if (!heavy) {
     // Light rows: each T-lane group takes kR consecutive rows and walks their in-edges as
     // one flat list over the compacted light CSC (heavy rows are empty there), T column
     // ids per coalesced load and kUnroll gathers in flight; each row's sum (in edge order)
@@ -218,22 +241,19 @@ __global__ __launch_bounds__(kThreads) void ppr_sweep_kernel(PprParams P) {
     e = slot_reduce<T>(e, red);
     if (threadIdx.x < (uint32_t)T) P.blk_err[((uint64_t)tile * P.row_blocks + b) * T + threadIdx.x] = e;
   } else {
-    // heavy segment s of one k-tile: edge group g takes the contiguous kSegPerGroup edges
-    // [p0 + g*kSegPerGroup, ...), loads their ids in one pass and issues all the gathers
-    // before adding them in edge order; the group partials are added in group order
-    const uint32_t s = b - P.row_blocks;
-    const uint32_t h = P.seg_row[s];
-    const uint32_t i = P.hub_rows[h];
-    const uint32_t ps = P.rp[i] + (s - P.seg_ptr[h]) * P.hub_segment;
-    const uint32_t pe = P.rp[i + 1];
-    const uint32_t p1 = (pe - ps > P.hub_segment) ? ps + P.hub_segment : pe;
+    // heavy segment s of one k-tile (at most hub_segment edges of one row and one source
+    // slice): edge group g takes the contiguous kSegPerGroup edges [p0 + g*kSegPerGroup, ...),
+    // loads their ids in one pass and issues all the gathers before adding them in edge
+    // order; the group partials are added in group order
+    const uint32_t s = seg;
+    const uint32_t ps = P.seg_start[s], p1 = P.seg_end[s];
     const uint32_t p0 = ps + (threadIdx.x / T) * kSegPerGroup;
     constexpr uint32_t kIdLoads = (kSegPerGroup + T - 1) / T;
     uint32_t c[kIdLoads];
 #pragma unroll
     for (uint32_t t = 0; t < kIdLoads; ++t) {
       const uint32_t pp = p0 + t * T + vl;
-      c[t] = (t * T + vl < kSegPerGroup && pp < p1) ? P.col[pp] : 0u;
+      c[t] = (t * T + vl < kSegPerGroup && pp < p1) ? P.hcol[pp] : 0u;
     }
     float v[kSegPerGroup];
 #pragma unroll
@@ -296,14 +316,14 @@ __global__ __launch_bounds__(kThreads) void ppr_finish_kernel(PprParams P, uint3
 #pragma unroll
     for (uint32_t u = 0; u < kFinU; ++u) {
       if (!ok[u]) continue;
-      // the other segment partials (rows of many segments), added in segment order with 8
+      // the other segment partials (rows of many segments), added in segment order with 32
       // loads in flight
-      for (uint32_t s = s0[u] + 1; s < s1[u]; s += 8) {
-        float v[8];
+      for (uint32_t s = s0[u] + 1; s < s1[u]; s += 32) {
+        float v[32];
 #pragma unroll
-        for (uint32_t r = 0; r < 8; ++r) v[r] = s + r < s1[u] ? P.hub_part[(uint64_t)(s + r) * P.k + jv[u]] : 0.f;
+        for (uint32_t r = 0; r < 32; ++r) v[r] = s + r < s1[u] ? P.hub_part[(uint64_t)(s + r) * P.k + jv[u]] : 0.f;
 #pragma unroll
-        for (uint32_t r = 0; r < 8; ++r)
+        for (uint32_t r = 0; r < 32; ++r)
           if (s + r < s1[u]) z[u] += v[r];
       }
       const double e = ppr_finish_row(P, row[u], jv[u], z[u], df[u], xo[u]);
@@ -396,6 +416,52 @@ __global__ __launch_bounds__(kThreads) void unpermute_rows_kernel(uint32_t n, ui
 
 inline uint32_t grid1(uint64_t m) { return (uint32_t)ceil_div(m ? m : 1, kThreads); }
 
+__global__ __launch_bounds__(kThreads) void heavy_deg_kernel(uint32_t nh, const uint32_t* __restrict__ rows,
+                                                             const uint32_t* __restrict__ rp,
+                                                             uint32_t* __restrict__ deg) {
+  const uint32_t h = blockIdx.x * kThreads + threadIdx.x;
+  if (h < nh) deg[h] = rp[rows[h] + 1] - rp[rows[h]];
+}
+
+// heavy edge e (row h's edges at hoff[h] ..): key (h, slice), payload the source.  A row
+// of at least big edges is split by source slice; a shorter one keeps its edges together
+// in slice h mod nsl (a slice of its own would leave its segments mostly empty).
+__global__ __launch_bounds__(kThreads) void heavy_key_kernel(uint64_t eh, uint32_t nh,
+                                                             const uint32_t* __restrict__ hoff,
+                                                             const uint32_t* __restrict__ rows,
+                                                             const uint32_t* __restrict__ rp,
+                                                             const uint32_t* __restrict__ col, uint32_t sh,
+                                                             uint32_t nsl, uint32_t big,
+                                                             uint64_t* __restrict__ keys,
+                                                             uint32_t* __restrict__ vals) {
+  const uint64_t e = (uint64_t)blockIdx.x * kThreads + threadIdx.x;
+  if (e >= eh) return;
+  uint32_t lo = 0, hi = nh - 1;  // last h with hoff[h] <= e
+  while (lo < hi) {
+    const uint32_t mid = (lo + hi + 1) >> 1;
+    if (hoff[mid] <= e) lo = mid;
+    else hi = mid - 1;
+  }
+  const uint32_t u = col[rp[rows[lo]] + (uint32_t)(e - hoff[lo])];
+  const uint32_t x = hoff[lo + 1] - hoff[lo] >= big ? (u >> sh) % nsl : lo % nsl;
+  keys[e] = ((uint64_t)lo << 3) | x;
+  vals[e] = u;
+}
+
+// gp[t] = first sorted position with key >= t, t in [0, ng]
+__global__ __launch_bounds__(kThreads) void group_ptr_kernel(uint64_t eh, const uint64_t* __restrict__ keys,
+                                                             uint64_t ng, uint32_t* __restrict__ gp) {
+  const uint64_t t = (uint64_t)blockIdx.x * kThreads + threadIdx.x;
+  if (t > ng) return;
+  uint64_t lo = 0, hi = eh;
+  while (lo < hi) {
+    const uint64_t mid = (lo + hi) >> 1;
+    if (keys[mid] < t) lo = mid + 1;
+    else hi = mid;
+  }
+  gp[t] = (uint32_t)lo;
+}
+
 // light CSC: row i keeps its in-edges when in-degree <= light_max, else none
 __global__ __launch_bounds__(kThreads) void light_len_kernel(uint32_t n, const uint32_t* __restrict__ rp,
                                                              uint32_t light_max, uint32_t* __restrict__ len) {
@@ -418,6 +484,89 @@ __global__ __launch_bounds__(kThreads) void light_copy_kernel(uint32_t n, const 
 
 }  // namespace
 
+// Heavy rows (in-degree >= min_deg): their in-edges regrouped by (row, source slice), the
+// slice of source u being (u >> sh) mod nsl with sh chosen so that the T-float
+// contributions of one 128-byte line fall in one slice (rows of fewer than nsl * seg
+// edges stay whole, in slice h mod nsl); then cut into segments of at most seg edges of
+// one (row, slice), numbered row by row (so a row's partials are added in a fixed order),
+// and listed per slice.
+static void build_heavy_slices(hipStream_t s, const ActiveCsc& a, uint32_t n, uint32_t min_deg, uint32_t seg,
+                               uint32_t tile, uint32_t nsl, PprState& st) {
+  DevBuf<uint32_t> unused_ptr, unused_row;
+  uint32_t nh = 0, ns0 = 0;
+  build_segments(s, a.ptr, n, min_deg, seg, st.heavy_rows, unused_ptr, unused_row, &nh, &ns0);
+  const uint32_t sh = tile >= 32 ? 0u : tile == 16 ? 1u : 2u;
+  std::vector<uint32_t> gp_h((uint64_t)nh * 8 + 1, 0);
+  uint64_t eh = 0;
+  if (nh) {
+    DevBuf<uint32_t> deg(nh), hoff((uint64_t)nh + 1);
+    heavy_deg_kernel<<<grid1(nh), kThreads, 0, s>>>(nh, st.heavy_rows.p, a.ptr, deg.p);
+    NWG_LAUNCH_CHECK();
+    exclusive_scan_u32(s, deg.p, hoff.p, nh);
+    uint32_t eh32 = 0;
+    NWG_HIP(hipMemcpyAsync(&eh32, hoff.p + nh, 4, hipMemcpyDeviceToHost, s));
+    NWG_HIP(hipStreamSynchronize(s));
+    eh = eh32;
+    st.hcol.alloc(std::max<uint64_t>(eh, 1));
+    if (eh) {
+      DevBuf<uint64_t> keys(eh), keys_tmp(eh);
+      DevBuf<uint32_t> vals_tmp(eh);
+      heavy_key_kernel<<<(uint32_t)ceil_div(eh, kThreads), kThreads, 0, s>>>(eh, nh, hoff.p, st.heavy_rows.p, a.ptr,
+                                                                             a.idx, sh, nsl, nsl * seg, keys.p,
+                                                                             st.hcol.p);
+      NWG_LAUNCH_CHECK();
+      int kb = 3;
+      while (kb < 64 && (1ull << kb) < (uint64_t)nh * 8) ++kb;
+      radix_sort_pairs(s, keys.p, keys_tmp.p, st.hcol.p, vals_tmp.p, eh, kb);  // stable: sources stay ascending
+      DevBuf<uint32_t> gp((uint64_t)nh * 8 + 1);
+      group_ptr_kernel<<<grid1((uint64_t)nh * 8 + 1), kThreads, 0, s>>>(eh, keys.p, (uint64_t)nh * 8, gp.p);
+      NWG_LAUNCH_CHECK();
+      NWG_HIP(hipMemcpyAsync(gp_h.data(), gp.p, gp_h.size() * 4, hipMemcpyDeviceToHost, s));
+      NWG_HIP(hipStreamSynchronize(s));
+    }
+  } else {
+    st.hcol.alloc(1);
+  }
+  std::vector<uint32_t> sptr((uint64_t)nh + 1), sstart, send, srow;
+  std::vector<std::vector<uint32_t>> lists(nsl);
+  for (uint32_t h = 0; h < nh; ++h) {
+    sptr[h] = (uint32_t)sstart.size();
+    for (uint32_t x = 0; x < nsl; ++x) {
+      const uint32_t g0 = gp_h[(uint64_t)h * 8 + x], g1 = gp_h[(uint64_t)h * 8 + x + 1];
+      for (uint32_t p = g0; p < g1; p += seg) {
+        lists[x].push_back((uint32_t)sstart.size());
+        sstart.push_back(p);
+        send.push_back(std::min(p + seg, g1));
+        srow.push_back(h);
+      }
+    }
+  }
+  sptr[nh] = (uint32_t)sstart.size();
+  std::vector<uint32_t> flat;
+  st.slice_off.assign(nsl + 1, 0);
+  for (uint32_t x = 0; x < nsl; ++x) {
+    st.slice_off[x] = (uint32_t)flat.size();
+    flat.insert(flat.end(), lists[x].begin(), lists[x].end());
+  }
+  st.slice_off[nsl] = (uint32_t)flat.size();
+  const uint32_t ns = (uint32_t)sstart.size();
+  st.heavy_seg_ptr.alloc((uint64_t)nh + 1);
+  st.heavy_seg_row.alloc(std::max<uint32_t>(ns, 1));
+  st.seg_start.alloc(std::max<uint32_t>(ns, 1));
+  st.seg_end.alloc(std::max<uint32_t>(ns, 1));
+  st.slice_segs.alloc(std::max<uint32_t>(ns, 1));
+  NWG_HIP(hipMemcpyAsync(st.heavy_seg_ptr.p, sptr.data(), sptr.size() * 4, hipMemcpyHostToDevice, s));
+  if (ns) {
+    NWG_HIP(hipMemcpyAsync(st.heavy_seg_row.p, srow.data(), ns * 4ull, hipMemcpyHostToDevice, s));
+    NWG_HIP(hipMemcpyAsync(st.seg_start.p, sstart.data(), ns * 4ull, hipMemcpyHostToDevice, s));
+    NWG_HIP(hipMemcpyAsync(st.seg_end.p, send.data(), ns * 4ull, hipMemcpyHostToDevice, s));
+    NWG_HIP(hipMemcpyAsync(st.slice_segs.p, flat.data(), ns * 4ull, hipMemcpyHostToDevice, s));
+  }
+  NWG_HIP(hipStreamSynchronize(s));
+  st.n_heavy = nh;
+  st.n_segs = ns;
+}
+
 // Host driver; tile = k-tile width (8, 16, 32 or 64).
 void ppr_run(nwg_ctx* ctx, nwg_graph* g, uint32_t k, const std::vector<uint32_t>& seed_ptr,
              const std::vector<uint32_t>& seeds, float damping, float threshold, uint64_t max_iters, uint32_t tile,
@@ -437,10 +586,12 @@ void ppr_run(nwg_ctx* ctx, nwg_graph* g, uint32_t k, const std::vector<uint32_t>
   // (hence the active CSC) or the k-tile changes.
   const uint32_t light_max = 64;
   const uint32_t seg_edges = (kThreads / tile) * kSegPerGroup;
-  if (st.seg_version != g->plan_version || st.seg_tile != tile) {
+  // source slices per k-tile: the 8 XCDs are shared out as tiles x slices
+  const uint32_t tiles_k = (uint32_t)ceil_div(k, tile);
+  const uint32_t nsl = (8 % tiles_k == 0) ? 8 / tiles_k : 1;
+  if (st.seg_version != g->plan_version || st.seg_tile != tile || st.seg_slices != nsl) {
     st.seg_version = 0;
-    build_segments(s, a.ptr, n, light_max + 1, seg_edges, st.heavy_rows, st.heavy_seg_ptr, st.heavy_seg_row,
-                   &st.n_heavy, &st.n_segs);
+    build_heavy_slices(s, a, n, light_max + 1, seg_edges, tile, nsl, st);
     DevBuf<uint32_t> len(n);
     st.lrp.alloc((uint64_t)n + 1);
     light_len_kernel<<<grid1(n), kThreads, 0, s>>>(n, a.ptr, light_max, len.p);
@@ -455,6 +606,7 @@ void ppr_run(nwg_ctx* ctx, nwg_graph* g, uint32_t k, const std::vector<uint32_t>
     NWG_HIP(hipStreamSynchronize(s));
     st.seg_version = g->plan_version;
     st.seg_tile = tile;
+    st.seg_slices = nsl;
   }
   const uint32_t n_heavy = st.n_heavy, n_segs = st.n_segs;
   // seed entries sorted by (row, vector); rows in the active numbering
@@ -546,6 +698,15 @@ void ppr_run(nwg_ctx* ctx, nwg_graph* g, uint32_t k, const std::vector<uint32_t>
   P.blk_err = st.blk_err.p;
   P.acc = st.acc.p;
   P.threshold = (double)threshold;
+  P.hcol = st.hcol.p;
+  P.seg_start = st.seg_start.p;
+  P.seg_end = st.seg_end.p;
+  P.slice_segs = st.slice_segs.p;
+  P.n_slices = nsl;
+  uint32_t max_slice = 0;
+  for (uint32_t x = 0; x <= nsl; ++x) P.slice_off[x] = st.slice_off[x];
+  for (uint32_t x = 0; x < nsl; ++x) max_slice = std::max(max_slice, st.slice_off[x + 1] - st.slice_off[x]);
+  P.light_blocks_pad = (uint32_t)(ceil_div((uint64_t)row_blocks * tiles, 8) * 8);
 
   const float init = (float)(1.0 / n);
   NWG_HIP(hipEventRecord(ctx->ev0, s));
@@ -553,7 +714,7 @@ void ppr_run(nwg_ctx* ctx, nwg_graph* g, uint32_t k, const std::vector<uint32_t>
   NWG_LAUNCH_CHECK();
   ppr_zero_acc_kernel<<<grid1(max_iters * k), kThreads, 0, s>>>(st.acc.p, max_iters * k);
   NWG_LAUNCH_CHECK();
-  const uint64_t grid64 = (uint64_t)(row_blocks + n_segs) * tiles;
+  const uint64_t grid64 = (uint64_t)P.light_blocks_pad + 8ull * max_slice;
   NWG_REQUIRE(grid64 < (1ull << 31), "batched solve too large for one launch");
   const uint32_t grid = (uint32_t)grid64;
   const uint64_t fin_a = ceil_div((uint64_t)n_heavy * k, kFinItems), fin_b = ceil_div(row_blocks, kFoldRows);
