@@ -248,6 +248,13 @@ int nwg_read_mm(const char* path, uint32_t* n, uint64_t* ne, uint32_t** src, uin
 /* NW binary edge list (edge_list::serialize/deserialize, edge_list.hpp:237-280). */
 int nwg_read_nw(const char* path, uint32_t* n, uint64_t* ne, uint32_t** src, uint32_t** dst);
 int nwg_write_nw(const char* path, uint32_t n, uint64_t ne, const uint32_t* src, const uint32_t* dst);
+/* Binary adjacency (index_adjacency<idx>::serialize / deserialize, adjacency.hpp:176-191,
+ * containers/compressed.hpp:377-413, graph_base.hpp:72-87, soa.hpp:333-375), the
+ * attribute-free uint32 form: n + 1 row offsets ptr and nnz ids idx (a CSC for idx = 1, a
+ * CSR for idx = 0).  The reader checks the layout, offsets and ids and mallocs *ptr (n + 1)
+ * and *idx (nnz); free with nwg_free. */
+int nwg_write_adjacency(const char* path, uint32_t n, uint64_t nnz, const uint32_t* ptr, const uint32_t* idx);
+int nwg_read_adjacency(const char* path, uint32_t* n, uint64_t* nnz, uint32_t** ptr, uint32_t** idx);
 void nwg_free(void* p);
 
 #ifdef __cplusplus

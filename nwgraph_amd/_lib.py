@@ -119,6 +119,8 @@ _SIGNATURES = {
     "nwg_read_mm":(c_int, [ctypes.c_char_p, P_u32, P_u64, ctypes.POINTER(P_u32), ctypes.POINTER(P_u32)]),
     "nwg_read_nw": (c_int, [ctypes.c_char_p, P_u32, P_u64, ctypes.POINTER(P_u32), ctypes.POINTER(P_u32)]),
     "nwg_write_nw": (c_int, [ctypes.c_char_p, c_u32, c_u64, c_void_p, c_void_p]),
+    "nwg_read_adjacency": (c_int, [ctypes.c_char_p, P_u32, P_u64, ctypes.POINTER(P_u32), ctypes.POINTER(P_u32)]),
+    "nwg_write_adjacency": (c_int, [ctypes.c_char_p, c_u32, c_u64, c_void_p, c_void_p]),
     "nwg_free": (None, [c_void_p]),
 }
 

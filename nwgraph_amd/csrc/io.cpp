@@ -1,5 +1,6 @@
 // io.cpp -- graph input formats of the reference's PageRank driver (load_graph,
-// bench/gapbs/common.hpp:82-101): Matrix Market text and the NW binary edge list.
+// bench/gapbs/common.hpp:82-101): Matrix Market text and the NW binary edge list; and
+// the binary form of an adjacency (index_adjacency::serialize / deserialize).
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -142,6 +143,84 @@ int nwg_write_nw(const char* path, uint32_t n, uint64_t ne, const uint32_t* src,
     if (ne) out.write(reinterpret_cast<const char*>(cols[a]), (std::streamsize)(ne * 4));
   }
   return out ? NWG_OK : NWG_EFORMAT;
+}
+
+// Adjacency (index_adjacency<idx>::serialize, adjacency.hpp:176-191): the graph base
+// (graph_base.hpp:72-87: size_t N, bool is_open), then the indexed struct of arrays
+// (containers/compressed.hpp:377-413): char magic[34] "NW GRAPH indexed_struct_of_arrays",
+// 8 bytes written from its 4-byte index_t N_ (the upper 4 are whatever follows N_ in
+// memory: written as 0 here and ignored when read; an adjacency built from an edge list,
+// common.hpp:103-108, holds N_ = n + 1), size_t n + 1, size_t sizeof(index_t), the n + 1
+// row offsets, then the struct of arrays (soa.hpp:350-356: size_t entries, size_t arrays,
+// and per array size_t count, size_t element size, raw data).  Only the attribute-free
+// uint32 form (adjacency<idx>, defaults.hpp:23-24) is read and written.
+static const char kAdjMagic[34] = "NW GRAPH indexed_struct_of_arrays";
+
+int nwg_write_adjacency(const char* path, uint32_t n, uint64_t nnz, const uint32_t* ptr, const uint32_t* idx) {
+  if (!path || !ptr || (!idx && nnz)) return NWG_EINVAL;
+  if ((uint64_t)ptr[n] != nnz || ptr[0] != 0) return NWG_EINVAL;
+  std::ofstream out(path, std::ios::binary);
+  if (!out) return NWG_EINVAL;
+  const uint64_t N = n, st = (uint64_t)n + 1, es = 4, narr = 1;
+  const char open_flag = 0;
+  const uint32_t n_field[2] = {n + 1u, 0u};
+  out.write(reinterpret_cast<const char*>(&N), 8);
+  out.write(&open_flag, 1);
+  out.write(kAdjMagic, 34);
+  out.write(reinterpret_cast<const char*>(n_field), 8);
+  out.write(reinterpret_cast<const char*>(&st), 8);
+  out.write(reinterpret_cast<const char*>(&es), 8);
+  out.write(reinterpret_cast<const char*>(ptr), (std::streamsize)(st * 4));
+  out.write(reinterpret_cast<const char*>(&nnz), 8);
+  out.write(reinterpret_cast<const char*>(&narr), 8);
+  out.write(reinterpret_cast<const char*>(&nnz), 8);
+  out.write(reinterpret_cast<const char*>(&es), 8);
+  if (nnz) out.write(reinterpret_cast<const char*>(idx), (std::streamsize)(nnz * 4));
+  return out ? NWG_OK : NWG_EFORMAT;
+}
+
+int nwg_read_adjacency(const char* path, uint32_t* n, uint64_t* nnz, uint32_t** ptr, uint32_t** idx) {
+  if (!path || !n || !nnz || !ptr || !idx) return NWG_EINVAL;
+  std::ifstream in(path, std::ios::binary);
+  if (!in) return NWG_EINVAL;
+  uint64_t N = 0, nfield = 0, st = 0, es = 0, ne = 0, narr = 0, cnt = 0, es2 = 0;
+  char open_flag = 0, magic[34];
+  in.read(reinterpret_cast<char*>(&N), 8);
+  in.read(&open_flag, 1);
+  in.read(magic, 34);
+  if (!in || std::memcmp(magic, kAdjMagic, 34) != 0) return NWG_EFORMAT;
+  in.read(reinterpret_cast<char*>(&nfield), 8);
+  in.read(reinterpret_cast<char*>(&st), 8);
+  in.read(reinterpret_cast<char*>(&es), 8);
+  if (!in || es != 4 || st != N + 1 || N > 0xFFFFFFFFull) return NWG_EFORMAT;
+  std::vector<uint32_t> rp(st);
+  in.read(reinterpret_cast<char*>(rp.data()), (std::streamsize)(st * 4));
+  in.read(reinterpret_cast<char*>(&ne), 8);
+  in.read(reinterpret_cast<char*>(&narr), 8);
+  in.read(reinterpret_cast<char*>(&cnt), 8);
+  in.read(reinterpret_cast<char*>(&es2), 8);
+  if (!in || narr != 1 || cnt != ne || es2 != 4 || rp[0] != 0 || rp[N] != ne) return NWG_EFORMAT;
+  for (uint64_t i = 0; i < N; ++i)
+    if (rp[i + 1] < rp[i]) return NWG_EFORMAT;
+  std::vector<uint32_t> col(ne);
+  if (ne) in.read(reinterpret_cast<char*>(col.data()), (std::streamsize)(ne * 4));
+  if (!in) return NWG_EFORMAT;
+  for (uint64_t q = 0; q < ne; ++q)
+    if (col[q] >= N) return NWG_EFORMAT;
+  uint32_t* a = static_cast<uint32_t*>(std::malloc(st * sizeof(uint32_t)));
+  uint32_t* b = static_cast<uint32_t*>(std::malloc((ne ? ne : 1) * sizeof(uint32_t)));
+  if (!a || !b) {
+    std::free(a);
+    std::free(b);
+    return NWG_ENOMEM;
+  }
+  std::memcpy(a, rp.data(), st * 4);
+  if (ne) std::memcpy(b, col.data(), ne * 4);
+  *n = (uint32_t)N;
+  *nnz = ne;
+  *ptr = a;
+  *idx = b;
+  return NWG_OK;
 }
 
 void nwg_free(void* p) { std::free(p); }

@@ -410,6 +410,36 @@ class Adjacency:
     def num_edges(self):
         return int(self.to_be_indexed_.size)
 
+    def serialize(self, outfile_name):
+        """index_adjacency::serialize (adjacency.hpp:176-180): the binary form the reference
+        writes, through nwg_write_adjacency."""
+        lib = _lib.load()
+        rc = lib.nwg_write_adjacency(str(outfile_name).encode(), self.size(), self.num_edges(), _ptr(self.indices_),
+                                     _ptr(self.to_be_indexed_))
+        if rc != 0:
+            raise NwgError(rc, f"cannot write {outfile_name}")
+
+    def deserialize(self, infile_name):
+        """index_adjacency::deserialize (adjacency.hpp:187-191): replaces this adjacency's
+        arrays with the file's, through nwg_read_adjacency (which validates the layout)."""
+        lib = _lib.load()
+        n = ctypes.c_uint32()
+        ne = ctypes.c_uint64()
+        pp = ctypes.POINTER(ctypes.c_uint32)()
+        ip = ctypes.POINTER(ctypes.c_uint32)()
+        rc = lib.nwg_read_adjacency(str(infile_name).encode(), ctypes.byref(n), ctypes.byref(ne), ctypes.byref(pp),
+                                    ctypes.byref(ip))
+        if rc != 0:
+            raise NwgError(rc, f"cannot read {infile_name}")
+        try:
+            self.indices_ = np.ctypeslib.as_array(pp, shape=(n.value + 1,)).copy()
+            m = ne.value
+            self.to_be_indexed_ = np.ctypeslib.as_array(ip, shape=(m,)).copy() if m else np.empty(0, np.uint32)
+        finally:
+            lib.nwg_free(ctypes.cast(pp, ctypes.c_void_p))
+            lib.nwg_free(ctypes.cast(ip, ctypes.c_void_p))
+        return self
+
 
 def build_adjacency(el, clean=False, ctx=None):
     """build_adjacency<1> (bench/gapbs/common.hpp:103-108) on the device.  clean=False keeps

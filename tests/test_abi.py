@@ -116,3 +116,79 @@ def test_reader_rejects_bad_files(tmp_path):
     q.write_bytes(b"not a graph")
     with pytest.raises(NwgError):
         nwgraph_amd.read_nw(q)
+
+
+def _adjacency_bytes(n, ptr, idx, n_field=None, junk=0):
+    """The bytes index_adjacency<1>::serialize writes (adjacency.hpp:176-180): graph base
+    (graph_base.hpp:72-75: size_t N, bool is_open), indexed struct of arrays
+    (compressed.hpp:377-387: 34-byte magic, 8 bytes from the 4-byte index_t N_, size_t N+1,
+    size_t 4, offsets), struct of arrays (soa.hpp:333-356: size_t entries, size_t arrays, then
+    size_t count, size_t element size and the ids)."""
+    ptr = np.asarray(ptr, np.uint32)
+    idx = np.asarray(idx, np.uint32)
+    nf = n + 1 if n_field is None else n_field
+    b = np.uint64(n).tobytes() + b"\0"
+    b += b"NW GRAPH indexed_struct_of_arrays\0"
+    b += np.uint32(nf).tobytes() + np.uint32(junk).tobytes()
+    b += np.uint64(n + 1).tobytes() + np.uint64(4).tobytes() + ptr.tobytes()
+    b += np.uint64(idx.size).tobytes() + np.uint64(1).tobytes()
+    b += np.uint64(idx.size).tobytes() + np.uint64(4).tobytes() + idx.tobytes()
+    return b
+
+
+def test_adjacency_serialize_layout(tmp_path):
+    import nwgraph_amd
+
+    ptr, idx = [0, 1, 3, 3, 4], [2, 0, 3, 1]
+    a = nwgraph_amd.Adjacency(ptr, idx)
+    p = tmp_path / "a.adj"
+    a.serialize(p)
+    assert p.read_bytes() == _adjacency_bytes(4, ptr, idx)
+    # an empty graph and a graph without edges
+    for n, pp, ii in ((0, [0], []), (3, [0, 0, 0, 0], [])):
+        q = tmp_path / f"e{n}.adj"
+        nwgraph_amd.Adjacency(pp, ii).serialize(q)
+        assert q.read_bytes() == _adjacency_bytes(n, pp, ii)
+        b = nwgraph_amd.Adjacency([0], []).deserialize(q)
+        assert b.size() == n and b.num_edges() == 0
+
+
+def test_adjacency_roundtrip_and_reference_files(tmp_path):
+    import nwgraph_amd
+    import pyoracle as O
+
+    n, s, d = O.read_mm(os.path.join(GOLDEN, "karate.mtx"))
+    ind, col = O.csc(n, *O.clean_edges(s, d))
+    a = nwgraph_amd.Adjacency(ind, col)
+    p = tmp_path / "k.adj"
+    a.serialize(p)
+    b = nwgraph_amd.Adjacency([0], []).deserialize(p)
+    assert b.size() == n and np.array_equal(b.indices_, ind) and np.array_equal(b.to_be_indexed_, col)
+    # a file as the reference writes it for an adjacency made with the (N, M) constructor
+    # (N_ = N) and with memory garbage in the upper half of the 8-byte N_ field
+    q = tmp_path / "ref.adj"
+    q.write_bytes(_adjacency_bytes(n, ind, col, n_field=n, junk=0xDEADBEEF))
+    c = nwgraph_amd.Adjacency([0], []).deserialize(q)
+    assert np.array_equal(c.indices_, ind) and np.array_equal(c.to_be_indexed_, col)
+
+
+def test_adjacency_reader_rejects_bad_files(tmp_path):
+    import nwgraph_amd
+    from nwgraph_amd import NwgError
+
+    ptr, idx = [0, 1, 3, 3, 4], [2, 0, 3, 1]
+    good = _adjacency_bytes(4, ptr, idx)
+    bad = {
+        "magic": good[:9] + b"NW GRAPH indexed_struct_of_arrayZ\0" + good[43:],
+        "truncated": good[:-3],
+        "id out of range": _adjacency_bytes(4, ptr, [2, 0, 4, 1]),
+        "offsets decrease": _adjacency_bytes(4, [0, 3, 1, 3, 4], idx),
+        "offsets end": _adjacency_bytes(4, [0, 1, 3, 3, 3], idx[:4]),
+    }
+    for name, data in bad.items():
+        q = tmp_path / "bad.adj"
+        q.write_bytes(data)
+        with pytest.raises(NwgError):
+            nwgraph_amd.Adjacency([0], []).deserialize(q)
+    with pytest.raises(NwgError):  # the writer checks the offsets against the ids
+        nwgraph_amd.Adjacency([0, 1, 3], [0]).serialize(tmp_path / "w.adj")
