@@ -6,14 +6,18 @@
 // over a static, tiled copy of the CSC (layout in nwg_internal.hpp, BinnedLayout):
 //
 //   chunk pass  work item = one source chunk (2^15 sources) and a run of its tiles:
-//               stage contrib[chunk] in LDS, then vals[e] = contrib[src(e)] for every
-//               entry e of those tiles (2-byte source offsets in, 4-byte values out)
-//   bin pass    work item = a contiguous entry range of one bin (2^13 rows): add
-//               vals[e] into LDS row accumulators (4 + 2 bytes in per entry); a bin that
-//               is one item finishes its rows, the items of a split bin store partial
-//               rows that the combine pass adds and finishes.  Finishing a row is the
-//               reference's update (page_rank.hpp:133-135): x = fma(d, z, base),
+//               stage contrib[chunk] in LDS, then vals[p] = contrib[src(p)] for every
+//               pair p of those tiles (2-byte source offsets in, 4-byte values out)
+//   bin pass    work item = a contiguous entry range of one bin (2^13 rows): add each
+//               entry's pair value into LDS row accumulators (2 bytes per entry and 4 per
+//               pair in); a bin that is one item finishes its rows, the items of a split
+//               bin store partial rows that the combine pass adds and finishes.  Finishing
+//               a row is the reference's update (page_rank.hpp:133-135): x = fma(d, z, base),
 //               |x - x_old| into the sweep's L1 change, contrib = x / deg.
+//
+// A pair is (bin, source): the entries of one bin with one source.  At RMAT-25 with degree
+// ordering there are 0.43 pairs per entry outside the hot tiles, so moving values per pair
+// instead of per entry takes the streamed bytes from 12 to about 6 per entry.
 //
 // Row sums are exact fixed-point sums of the contributions (units of 2^-60, see quantize), so
 // they do not depend on how the entries are split or ordered: results are deterministic,
@@ -37,26 +41,28 @@ constexpr int kBT = 1024;  // threads of a bin / chunk pass workgroup (16 waves,
 constexpr uint32_t kBinRows = 1u << kBinBits;
 constexpr uint32_t kChunkSrc = 1u << kChunkBits;
 constexpr uint32_t kItem2Entries = 1u << 18;  // entries per bin-pass item
-constexpr uint32_t kItem1Entries = 1u << 18;  // entries per chunk-pass item
-constexpr uint32_t kPadSrc = 0xFFFFu;           // source offset of a pad entry (value 0)
+constexpr uint32_t kItem1Pairs = 1u << 18;    // pairs per chunk-pass item
+constexpr uint32_t kPadSrc = 0xFFFFu;           // source offset of a pad pair (value 0)
+constexpr uint32_t kRowMask = kBinRows - 1;
 constexpr uint32_t kNoSlot = 0xFFFFFFFFu;     // bin-pass item of a bin that is one item
 constexpr uint32_t kGroups1 = 4;              // groups of 8 entries in flight per lane: chunk pass
 constexpr uint32_t kGroups2 = 2;              // and bin pass (more would cost it its second workgroup per CU)
 
 // Row sums are exact 64-bit fixed-point sums in units of 2^-60.  A float contribution
-// c = m * 2^(e-150) (m the 24-bit significand) is converted by shifting m: exact for
-// c >= 2^-37, rounded to the nearest unit (ties away from zero) below that.  As a row sum
-// z <= 1 stays below 2^60 units, integer adds never overflow and the sum does not depend on
-// the order or the split of the entries.  (z <= sum of all x <= 1: x starts summing to 1
-// and a sweep maps a vector of sum <= 1 to one of sum <= 1.)  No FP64 arithmetic is used.
+// c in [0, 1] is converted with float operations that are all exact: s = c * 2^28,
+// h = floor(s) (h <= 2^28; (float)h is exact), r = s - h (the fraction bits of s), and
+// l = floor(r * 2^32); the fixed-point value h * 2^32 + l is c * 2^60 truncated to an
+// integer, which is exact for c >= 2^-37 (every significand bit at or above 2^-60).  As a
+// row sum z <= 1 stays below 2^60 units, integer adds never overflow and the sum does not
+// depend on the order or the split of the entries.  (z <= sum of all x <= 1: x starts
+// summing to 1 and a sweep maps a vector of sum <= 1 to one of sum <= 1.)  No FP64
+// arithmetic is used.
 __device__ __forceinline__ unsigned long long quantize(float c) {
-  const uint32_t b = __float_as_uint(c);  // c >= 0 and finite
-  const uint32_t e = b >> 23;
-  const unsigned long long m = (b & 0x7FFFFFu) | (e ? 0x800000u : 0u);
-  const int sh = (int)(e ? e : 1u) - 90;  // c * 2^60 = m * 2^sh
-  if (sh >= 0) return m << sh;
-  if (sh <= -25) return 0ull;  // below half a unit
-  return (m + (1ull << (-sh - 1))) >> -sh;
+  const float sc = c * 268435456.f;  // c * 2^28
+  const uint32_t h = (uint32_t)sc;
+  const float r = sc - (float)h;
+  const uint32_t l = (uint32_t)(r * 4294967296.f);  // floor(r * 2^32) < 2^32
+  return ((unsigned long long)h << 32) | l;
 }
 // v * 2^-60 rounded to the nearest float (ties to even), with integer operations only
 __device__ __forceinline__ float dequantize(unsigned long long v) {
@@ -163,16 +169,16 @@ __device__ __forceinline__ void finish_rows(const BinParams& P, uint32_t r0, uin
 }
 
 
-// chunk pass: vals[e] = contrib[src(e)] over the tiles of one item.  Tiles are padded to
-// multiples of 8 entries (pads have source kPadSrc and get 0), so a lane moves 8 entries
-// with one 16-byte id load and two 16-byte value stores.
+// chunk pass: vals[p] = contrib[src(p)] over the pairs of the tiles of one item.  Tiles are
+// padded to multiples of 8 pairs (pads have source kPadSrc and get 0), so a lane moves 8
+// pairs with one 16-byte id load and two 16-byte value stores.
 __global__ __launch_bounds__(kBT) void pr_chunk_kernel(BinParams P) {
   if (converged_before(P)) return;
   __shared__ float4 cst4[kChunkSrc / 4];
   float* cst = reinterpret_cast<float*>(cst4);
   __shared__ uint32_t sstart[kBinMaxSegs];
   __shared__ uint32_t sflat[kBinMaxSegs + 1];
-  const uint4 it = P.items1[blockIdx.x];  // {chunk, first tile, end tile, groups of 8 entries}
+  const uint4 it = P.items1[blockIdx.x];  // {chunk, first tile, end tile, groups of 8 pairs}
   const uint32_t ns = it.z - it.y, m = it.w;
   for (uint32_t k = threadIdx.x; k < ns; k += kBT) {
     sstart[k] = P.seg_start[it.y + k];
@@ -227,91 +233,71 @@ __global__ __launch_bounds__(kBT) void pr_chunk_kernel(BinParams P) {
   }
 }
 
-constexpr uint32_t kNoRow = 0xFFFFFFFFu;
-
-// Adds each lane's run (row cur, fixed-point sum run; cur == kNoRow: none) into lacc.  When
-// every run in the wave belongs to one row (a hub row filling the wave), the runs are summed
-// across the wave first and one lane issues the atomic, instead of 64 atomics on one address.
-// Integer adds: the order does not matter.  All lanes of the wave must call it together.
-__device__ __forceinline__ void wave_flush(unsigned long long* lacc, uint32_t cur, unsigned long long run,
-                                           uint32_t lane) {
-  const bool has = cur != kNoRow;
-  const unsigned long long m = __ballot(has);
-  if (!m) return;
-  const int first = __builtin_ctzll(m);
-  const uint32_t row0 = (uint32_t)__shfl((int)cur, first);
-  if (__ballot(has && cur != row0) == 0ull) {
-    unsigned long long t = has ? run : 0ull;
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) t += __shfl_xor(t, o);
-    if ((int)lane == first) atomicAdd(&lacc[row0], t);
-  } else if (has) {
-    atomicAdd(&lacc[cur], run);
-  }
-}
-
-// Adds the entries [e0, e1) of one bin into lacc.  A wave takes 512 consecutive entries (8
-// per lane, from an 8-aligned start: the arrays are padded), kGroups2 such blocks in flight;
-// the loop bounds are uniform over the wave so every lane takes part in the votes of
-// wave_flush.  Runs of one row are added in registers; a lane's last run goes through
-// wave_flush.  HOT: the entries lie in a hot tile and their values are hot[source] (pads,
-// source kPadSrc, add 0); else they are read from P.vals.
+// Adds the entries [e0, e1) of one bin into lacc; e0 and e1 are block boundaries (the layout
+// aligns bins and hot tiles).  The range is walked in blocks of kBinBlock entries (8 per
+// lane), one block per wave, kGroups2 blocks in flight.  The pair of an entry is the block's
+// first pair (bpid) plus the pair starts after the block's first entry up to the entry: the
+// lanes count them with one ballot per entry slot.  HOT: the entries lie in a hot tile and a
+// pair's value is hot[source] (pads, source kPadSrc, add 0); else it is P.vals[pair].
+// Entries are ordered by source, so one row rarely repeats inside a lane or an instruction:
+// each entry is one LDS add, with no run tracking.
 template <bool HOT>
-__device__ __forceinline__ void bin_range(const BinParams& P, uint64_t e0, uint64_t e1, unsigned long long* lacc,
+__device__ __forceinline__ void bin_range(const BinParams& P, uint32_t e0, uint32_t e1, unsigned long long* lacc,
                                           const float* hot) {
   const uint32_t w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  for (uint64_t wq = (e0 & ~7ull) + 512u * w; wq < e1; wq += kGroups2 * 512u * (kBT / 64)) {
-    uint4 rw[kGroups2], sw[kGroups2];
-    float4 va[kGroups2], vb[kGroups2];
+  constexpr uint32_t kWaves = kBT / 64;
+  // values are addressed by 32-bit byte offsets (the layout keeps pairs below 2^30)
+  const char* vbase = reinterpret_cast<const char*>(P.vals);
+  const char* sbase = reinterpret_cast<const char*>(P.bsrc);
+  const uint32_t ke = e1 / kBinBlock;
+  for (uint32_t k0 = e0 / kBinBlock + w; k0 < ke; k0 += kGroups2 * kWaves) {
+    uint4 rw[kGroups2];
+    uint32_t pb[kGroups2];
 #pragma unroll
     for (uint32_t h = 0; h < kGroups2; ++h) {
-      // unconditional loads (clamped into the range's first group when past its end, then
-      // masked below) so that all groups' loads are issued before any is waited on
-      const uint64_t q = wq + h * 512u * (kBT / 64) + 8u * lane;
-      const uint64_t qc = q < e1 ? q : (e0 & ~7ull);
-      rw[h] = *reinterpret_cast<const uint4*>(P.brow + qc);
-      if constexpr (HOT) {
-        sw[h] = *reinterpret_cast<const uint4*>(P.bsrc + qc);
-      } else {
-        va[h] = *reinterpret_cast<const float4*>(P.vals + qc);
-        vb[h] = *reinterpret_cast<const float4*>(P.vals + qc + 4);
+      // a block past the range re-reads block k0 (in bounds) and is skipped below, so that
+      // all blocks' loads are issued before any is waited on
+      const uint32_t k = k0 + h * kWaves;
+      const uint32_t kc = k < ke ? k : k0;
+      rw[h] = *reinterpret_cast<const uint4*>(P.brow + kc * kBinBlock + 8u * lane);
+      pb[h] = P.bpid[kc];
+    }
+    uint32_t rows[kGroups2][8];
+    float vs[kGroups2][8];
+#pragma unroll
+    for (uint32_t h = 0; h < kGroups2; ++h) {
+      const uint32_t wd[4] = {rw[h].x, rw[h].y, rw[h].z, rw[h].w};
+      uint32_t fb[8];
+#pragma unroll
+      for (uint32_t t = 0; t < 8; ++t) {
+        rows[h][t] = (wd[t >> 1] >> (16 * (t & 1))) & kRowMask;
+        fb[t] = (wd[t >> 1] >> (16 * (t & 1) + 15)) & 1u;
+      }
+      if (lane == 0) fb[0] = 0;  // the block's first entry is in pair bpid by definition
+      // pair starts in the lanes below this one, all slots
+      uint32_t pid = pb[h];
+#pragma unroll
+      for (uint32_t t = 0; t < 8; ++t) {
+        const unsigned long long m = __ballot(fb[t]);
+        pid = __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, pid));
+      }
+#pragma unroll
+      for (uint32_t t = 0; t < 8; ++t) {
+        pid += fb[t];
+        if constexpr (HOT) {
+          const uint32_t u = *reinterpret_cast<const uint16_t*>(sbase + (pid << 1));
+          const float v = hot[min(u, kHotSrc - 1u)];
+          vs[h][t] = u < kHotSrc ? v : 0.f;
+        } else {
+          vs[h][t] = *reinterpret_cast<const float*>(vbase + (pid << 2));
+        }
       }
     }
 #pragma unroll
     for (uint32_t h = 0; h < kGroups2; ++h) {
-      const uint64_t q = wq + h * 512u * (kBT / 64) + 8u * lane;
-      uint32_t cur = kNoRow;
-      unsigned long long run = 0ull;
-      if (q < e1) {
-        const uint32_t rows[8] = {rw[h].x & 0xFFFFu, rw[h].x >> 16, rw[h].y & 0xFFFFu, rw[h].y >> 16,
-                                  rw[h].z & 0xFFFFu, rw[h].z >> 16, rw[h].w & 0xFFFFu, rw[h].w >> 16};
-        float vs[8];
-        if constexpr (HOT) {
-          const uint32_t sv[4] = {sw[h].x, sw[h].y, sw[h].z, sw[h].w};
+      if (k0 + h * kWaves >= ke) continue;  // wave-uniform
 #pragma unroll
-          for (uint32_t t = 0; t < 8; ++t) {
-            const uint32_t u = (sv[t >> 1] >> (16 * (t & 1))) & 0xFFFFu;
-            vs[t] = u == kPadSrc ? 0.f : hot[u];
-          }
-        } else {
-          vs[0] = va[h].x; vs[1] = va[h].y; vs[2] = va[h].z; vs[3] = va[h].w;
-          vs[4] = vb[h].x; vs[5] = vb[h].y; vs[6] = vb[h].z; vs[7] = vb[h].w;
-        }
-#pragma unroll
-        for (uint32_t r = 0; r < 8; ++r) {
-          const uint64_t e = q + r;
-          if (e < e0 || e >= e1) continue;
-          const unsigned long long f = quantize(vs[r]);
-          if (rows[r] == cur) {
-            run += f;
-          } else {
-            if (cur != kNoRow) atomicAdd(&lacc[cur], run);
-            cur = rows[r];
-            run = f;
-          }
-        }
-      }
-      wave_flush(lacc, cur, run, lane);
+      for (uint32_t t = 0; t < 8; ++t) atomicAdd(&lacc[rows[h][t]], quantize(vs[h][t]));
     }
   }
 }
@@ -408,12 +394,11 @@ __global__ __launch_bounds__(256) void pr_bin_combine_kernel(BinParams P) {
 }
 
 // ---- layout construction ----
-// tile key of CSC position p (row v, source u) and its packed low bits
-// (tile column: 0 for a source below n_hot, else 1 + chunk; without hot tiles the chunk)
+// sort key of CSC position p (row v, source u): (bin of v, u), and v's offset in its bin.
+// A stable sort by this key orders each bin by source, then row (CSC order).
 __global__ __launch_bounds__(256) void bin_key_kernel(uint32_t n, uint64_t nnz, const uint32_t* __restrict__ rp,
-                                                      const uint32_t* __restrict__ col, uint32_t n_cols,
-                                                      uint32_t n_hot, uint64_t* __restrict__ keys,
-                                                      uint32_t* __restrict__ low) {
+                                                      const uint32_t* __restrict__ col, int ub,
+                                                      uint64_t* __restrict__ keys, uint32_t* __restrict__ low) {
   const uint64_t p = (uint64_t)blockIdx.x * 256 + threadIdx.x;
   if (p >= nnz) return;
   uint32_t lo = 0, hi = n;  // row of p: first r with rp[r + 1] > p
@@ -422,54 +407,90 @@ __global__ __launch_bounds__(256) void bin_key_kernel(uint32_t n, uint64_t nnz, 
     if (rp[mid + 1] <= p) lo = mid + 1;
     else hi = mid;
   }
-  const uint32_t u = col[p];
-  const bool hot = u < n_hot;
-  const uint32_t c = hot ? 0u : (u >> kChunkBits) + (n_hot ? 1u : 0u);
-  keys[p] = (uint64_t)(lo >> kBinBits) * n_cols + c;
-  low[p] = ((lo & (kBinRows - 1)) << 16) | (hot ? u : (u & (kChunkSrc - 1)));
+  keys[p] = ((uint64_t)(lo >> kBinBits) << ub) | col[p];
+  low[p] = lo & kRowMask;
 }
 
-// sorted entry q of tile t goes to padded position ptp[t] + (q - tp[t])
-__global__ __launch_bounds__(256) void bin_place_kernel(uint64_t nnz, const uint64_t* __restrict__ keys,
-                                                        const uint32_t* __restrict__ low,
-                                                        const uint32_t* __restrict__ tp,
-                                                        const uint32_t* __restrict__ ptp,
-                                                        uint16_t* __restrict__ brow, uint16_t* __restrict__ bsrc) {
+// tile of a sort key (tile column: 0 for a source below n_hot, else 1 + chunk; without hot
+// tiles the chunk)
+__device__ __forceinline__ uint64_t tile_of_key(uint64_t key, int ub, uint32_t TC, uint32_t n_hot) {
+  const uint32_t u = (uint32_t)(key & ((1ull << ub) - 1));
+  const uint32_t c = u < n_hot ? 0u : (u >> kChunkBits) + (n_hot ? 1u : 0u);
+  return (key >> ub) * TC + c;
+}
+
+// flag[q] = 1 when sorted entry q starts a pair (its key differs from the previous one's)
+__global__ __launch_bounds__(256) void pair_flag_kernel(uint64_t nnz, const uint64_t* __restrict__ keys,
+                                                        uint32_t* __restrict__ flag) {
   const uint64_t q = (uint64_t)blockIdx.x * 256 + threadIdx.x;
   if (q >= nnz) return;
-  const uint64_t t = keys[q];
-  const uint64_t np = ptp[t] + (q - tp[t]);
-  brow[np] = (uint16_t)(low[q] >> 16);
-  bsrc[np] = (uint16_t)(low[q] & 0xFFFFu);
+  flag[q] = (q == 0 || keys[q] != keys[q - 1]) ? 1u : 0u;
 }
 
-// pads of tile t: row of the tile's last entry (so they join its run), source kPadSrc
-__global__ __launch_bounds__(256) void bin_pad_kernel(uint64_t n_tiles, const uint32_t* __restrict__ tp,
-                                                      const uint32_t* __restrict__ ptp, uint16_t* __restrict__ brow,
-                                                      uint16_t* __restrict__ bsrc) {
-  const uint64_t t = (uint64_t)blockIdx.x * 256 + threadIdx.x;
-  if (t >= n_tiles) return;
-  const uint32_t cnt = tp[t + 1] - tp[t];
-  if (!cnt) return;
-  const uint32_t last = ptp[t] + cnt - 1;
-  for (uint32_t r = cnt; r < ptp[t + 1] - ptp[t]; ++r) {
-    brow[ptp[t] + r] = brow[last];
-    bsrc[ptp[t] + r] = (uint16_t)kPadSrc;
-  }
-}
-
-// tile_ptr[t] = first sorted position with key >= t, for t in [0, n_tiles]
-__global__ __launch_bounds__(256) void tile_ptr_kernel(uint64_t nnz, const uint64_t* __restrict__ keys,
-                                                       uint64_t n_tiles, uint32_t* __restrict__ tile_ptr) {
+// tp[t] = first sorted position whose tile is >= t, tpp[t] = pairs before it (pex: exclusive
+// scan of the pair flags, nnz + 1 long), for t in [0, n_tiles]
+__global__ __launch_bounds__(256) void tile_ptr_kernel(uint64_t nnz, const uint64_t* __restrict__ keys, int ub,
+                                                       uint32_t TC, uint32_t n_hot, uint64_t n_tiles,
+                                                       const uint32_t* __restrict__ pex, uint32_t* __restrict__ tp,
+                                                       uint32_t* __restrict__ tpp) {
   const uint64_t t = (uint64_t)blockIdx.x * 256 + threadIdx.x;
   if (t > n_tiles) return;
   uint64_t lo = 0, hi = nnz;
   while (lo < hi) {
     const uint64_t mid = (lo + hi) >> 1;
-    if (keys[mid] < t) lo = mid + 1;
+    if (tile_of_key(keys[mid], ub, TC, n_hot) < t) lo = mid + 1;
     else hi = mid;
   }
-  tile_ptr[t] = (uint32_t)lo;
+  tp[t] = (uint32_t)lo;
+  tpp[t] = pex[lo];
+}
+
+// sorted entry q of tile t goes to entry etp[t] + (q - tp[t]); its pair, numbered inside the
+// tile from tpp[t], to pair ptp[t] + ...; the first entry of a kBinBlock block records its pair
+__global__ __launch_bounds__(256) void bin_place_kernel(uint64_t nnz, const uint64_t* __restrict__ keys,
+                                                        const uint32_t* __restrict__ low,
+                                                        const uint32_t* __restrict__ flag,
+                                                        const uint32_t* __restrict__ pex, int ub, uint32_t TC,
+                                                        uint32_t n_hot, const uint32_t* __restrict__ tp,
+                                                        const uint32_t* __restrict__ tpp,
+                                                        const uint32_t* __restrict__ etp,
+                                                        const uint32_t* __restrict__ ptp,
+                                                        uint16_t* __restrict__ brow, uint16_t* __restrict__ bsrc,
+                                                        uint32_t* __restrict__ bpid) {
+  const uint64_t q = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+  if (q >= nnz) return;
+  const uint64_t key = keys[q];
+  const uint64_t t = tile_of_key(key, ub, TC, n_hot);
+  const uint32_t f = flag[q];
+  const uint32_t ne = etp[t] + (uint32_t)(q - tp[t]);
+  const uint32_t np = ptp[t] + (pex[q] + f - 1u - tpp[t]);
+  brow[ne] = (uint16_t)(low[q] | (f ? kPairStart : 0u));
+  if (f) {
+    const uint32_t u = (uint32_t)(key & ((1ull << ub) - 1));
+    bsrc[np] = (uint16_t)(u < n_hot ? u : (u & (kChunkSrc - 1)));
+  }
+  if (ne % kBinBlock == 0) bpid[ne / kBinBlock] = np;
+}
+
+// pads of tile t: the tile's pairs are padded to a multiple of 8 with pairs of source
+// kPadSrc (value 0), each given one entry (row 0, pair start) after the tile's entries.
+// Entries that fill a tile up to a block boundary stay as cleared (row 0, no pair start):
+// they join the last pad pair, and no block starts among them.
+__global__ __launch_bounds__(256) void bin_pad_kernel(uint64_t n_tiles, const uint32_t* __restrict__ tp,
+                                                      const uint32_t* __restrict__ tpp,
+                                                      const uint32_t* __restrict__ etp,
+                                                      const uint32_t* __restrict__ ptp, uint16_t* __restrict__ brow,
+                                                      uint16_t* __restrict__ bsrc, uint32_t* __restrict__ bpid) {
+  const uint64_t t = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+  if (t >= n_tiles) return;
+  const uint32_t ne = tp[t + 1] - tp[t], np = tpp[t + 1] - tpp[t];
+  const uint32_t pads = (ptp[t + 1] - ptp[t]) - np;
+  for (uint32_t j = 0; j < pads; ++j) {
+    const uint32_t e = etp[t] + ne + j, p = ptp[t] + np + j;
+    brow[e] = (uint16_t)kPairStart;
+    bsrc[p] = (uint16_t)kPadSrc;
+    if (e % kBinBlock == 0) bpid[e / kBinBlock] = p;
+  }
 }
 
 inline uint32_t grid256(uint64_t m) { return (uint32_t)ceil_div(m ? m : 1, 256); }
@@ -492,7 +513,7 @@ __global__ __launch_bounds__(256) void residual_fold_kernel(const unsigned long 
 }  // namespace
 
 void binned_build(hipStream_t s, const ActiveCsc& a, uint32_t n, uint64_t nnz, BinnedLayout& L, bool hot) {
-  NWG_REQUIRE(nnz < (1ull << 32) - 16, "binned layout needs fewer than 2^32 - 16 entries");
+  NWG_REQUIRE(nnz < (1ull << 31), "binned layout needs fewer than 2^31 entries");
   L = BinnedLayout();
   L.n_bins = (uint32_t)ceil_div(n, kBinRows);
   L.n_chunks = (uint32_t)ceil_div(n, kChunkSrc);
@@ -501,85 +522,124 @@ void binned_build(hipStream_t s, const ActiveCsc& a, uint32_t n, uint64_t nnz, B
   // tile columns per bin: the hot tile (if any), then one per chunk
   const uint32_t hc = L.n_hot ? 1u : 0u, TC = L.n_chunks + hc;
   const uint64_t n_tiles = (uint64_t)L.n_bins * TC;
-  std::vector<uint32_t> tp(n_tiles + 1, 0);  // becomes the padded tile pointers
-  uint64_t total = 0;
+  // etp / ptp: each tile's first entry and first pair in the padded layout
+  std::vector<uint32_t> etp(n_tiles + 1, 0), ptp(n_tiles + 1, 0);
+  uint64_t etotal = 0, ptotal = 0;
   if (nnz) {
+    int ub = 1;  // bits of a source id
+    while (ub < 32 && (1ull << ub) < n) ++ub;
     DevBuf<uint64_t> keys(nnz), keys_tmp(nnz);
     DevBuf<uint32_t> low(nnz), low_tmp(nnz);
-    bin_key_kernel<<<grid256(nnz), 256, 0, s>>>(n, nnz, a.ptr, a.idx, TC, L.n_hot, keys.p, low.p);
+    bin_key_kernel<<<grid256(nnz), 256, 0, s>>>(n, nnz, a.ptr, a.idx, ub, keys.p, low.p);
     NWG_LAUNCH_CHECK();
-    int kb = 1;
-    while (kb < 64 && (1ull << kb) < n_tiles) ++kb;
-    // stable: entries of one tile keep their CSC order (row, then source)
+    int kb = ub;
+    while (kb < 64 && (1ull << (kb - ub)) < L.n_bins) ++kb;
+    // stable: the entries of one (bin, source) keep their CSC order (row ascending)
     radix_sort_pairs(s, keys.p, keys_tmp.p, low.p, low_tmp.p, nnz, kb);
-    DevBuf<uint32_t> d_tp(n_tiles + 1), d_ptp(n_tiles + 1);
-    tile_ptr_kernel<<<grid256(n_tiles + 1), 256, 0, s>>>(nnz, keys.p, n_tiles, d_tp.p);
+    // the sort's temporaries now hold the pair flags and their exclusive scan
+    uint32_t* flag = low_tmp.p;
+    uint32_t* pex = reinterpret_cast<uint32_t*>(keys_tmp.p);  // nnz + 1 <= 2 nnz words
+    pair_flag_kernel<<<grid256(nnz), 256, 0, s>>>(nnz, keys.p, flag);
     NWG_LAUNCH_CHECK();
+    exclusive_scan_u32(s, flag, pex, nnz);
+    DevBuf<uint32_t> d_tp(n_tiles + 1), d_tpp(n_tiles + 1), d_etp(n_tiles + 1), d_ptp(n_tiles + 1);
+    tile_ptr_kernel<<<grid256(n_tiles + 1), 256, 0, s>>>(nnz, keys.p, ub, TC, L.n_hot, n_tiles, pex, d_tp.p, d_tpp.p);
+    NWG_LAUNCH_CHECK();
+    std::vector<uint32_t> tp(n_tiles + 1), tpp(n_tiles + 1);
     NWG_HIP(hipMemcpyAsync(tp.data(), d_tp.p, (n_tiles + 1) * 4, hipMemcpyDeviceToHost, s));
+    NWG_HIP(hipMemcpyAsync(tpp.data(), d_tpp.p, (n_tiles + 1) * 4, hipMemcpyDeviceToHost, s));
     NWG_HIP(hipStreamSynchronize(s));
-    std::vector<uint32_t> ptp(n_tiles + 1, 0);
+    // A bin's hot tile and the bin itself end on a kBinBlock boundary, so every bin-pass range
+    // is made of whole blocks: the entries that fill up to the boundary join the tile's last
+    // pad pair (value 0; the tile gets 8 pad pairs if it has none).
     for (uint64_t t = 0; t < n_tiles; ++t) {
-      total += (uint64_t)(tp[t + 1] - tp[t] + 7) / 8 * 8;
-      NWG_REQUIRE(total < (1ull << 32) - 16, "binned layout needs fewer than 2^32 - 16 padded entries");
-      ptp[t + 1] = (uint32_t)total;
+      const uint64_t np = tpp[t + 1] - tpp[t];
+      uint64_t pp = (np + 7) / 8 * 8;
+      uint64_t cur = etotal + (tp[t + 1] - tp[t]) + (pp - np);
+      const uint64_t col = t % TC;
+      if (((hc && col == 0) || col == TC - 1) && cur % kBinBlock) {
+        if (pp == np) {
+          pp += 8;
+          cur += 8;
+        }
+        cur = (cur + kBinBlock - 1) / kBinBlock * kBinBlock;
+      }
+      ptotal += pp;
+      etotal = cur;
+      // 32-bit entry indices with room for the blocks a bin-pass loop looks past its range,
+      // and 32-bit byte offsets of values
+      NWG_REQUIRE(etotal < (1ull << 31) && ptotal < (1ull << 30), "binned layout needs fewer than 2^31 padded entries");
+      ptp[t + 1] = (uint32_t)ptotal;
+      etp[t + 1] = (uint32_t)etotal;
     }
-    L.padded = total + 8;
+    L.padded = (etotal + kBinBlock - 1) / kBinBlock * kBinBlock + kBinBlock;
+    L.pairs = ptotal + 8;
     L.brow.alloc(L.padded);
-    L.bsrc.alloc(L.padded);
+    L.bsrc.alloc(L.pairs);
+    L.bpid.alloc(L.padded / kBinBlock);
     NWG_HIP(hipMemsetAsync(L.brow.p, 0, L.padded * 2, s));
-    NWG_HIP(hipMemsetAsync(L.bsrc.p, 0xFF, L.padded * 2, s));
+    NWG_HIP(hipMemsetAsync(L.bsrc.p, 0xFF, L.pairs * 2, s));
+    NWG_HIP(hipMemsetAsync(L.bpid.p, 0, L.padded / kBinBlock * 4, s));
+    NWG_HIP(hipMemcpyAsync(d_etp.p, etp.data(), (n_tiles + 1) * 4, hipMemcpyHostToDevice, s));
     NWG_HIP(hipMemcpyAsync(d_ptp.p, ptp.data(), (n_tiles + 1) * 4, hipMemcpyHostToDevice, s));
-    bin_place_kernel<<<grid256(nnz), 256, 0, s>>>(nnz, keys.p, low.p, d_tp.p, d_ptp.p, L.brow.p, L.bsrc.p);
+    bin_place_kernel<<<grid256(nnz), 256, 0, s>>>(nnz, keys.p, low.p, flag, pex, ub, TC, L.n_hot, d_tp.p, d_tpp.p,
+                                                  d_etp.p, d_ptp.p, L.brow.p, L.bsrc.p, L.bpid.p);
     NWG_LAUNCH_CHECK();
-    bin_pad_kernel<<<grid256(n_tiles), 256, 0, s>>>(n_tiles, d_tp.p, d_ptp.p, L.brow.p, L.bsrc.p);
+    bin_pad_kernel<<<grid256(n_tiles), 256, 0, s>>>(n_tiles, d_tp.p, d_tpp.p, d_etp.p, d_ptp.p, L.brow.p, L.bsrc.p,
+                                                    L.bpid.p);
     NWG_LAUNCH_CHECK();
     NWG_HIP(hipStreamSynchronize(s));
-    tp.swap(ptp);
   } else {
-    L.padded = 8;
+    L.padded = kBinBlock;
+    L.pairs = 8;
     L.brow.alloc(L.padded);
-    L.bsrc.alloc(L.padded);
+    L.bsrc.alloc(L.pairs);
+    L.bpid.alloc(1);
     NWG_HIP(hipMemsetAsync(L.brow.p, 0, L.padded * 2, s));
-    NWG_HIP(hipMemsetAsync(L.bsrc.p, 0xFF, L.padded * 2, s));
+    NWG_HIP(hipMemsetAsync(L.bsrc.p, 0xFF, L.pairs * 2, s));
+    NWG_HIP(hipMemsetAsync(L.bpid.p, 0, 4, s));
   }
   const uint32_t C = L.n_chunks;
   // hot tile ends (the hot tile is the first of its bin)
   std::vector<uint32_t> he(L.n_bins);
-  for (uint32_t b = 0; b < L.n_bins; ++b) he[b] = tp[(uint64_t)b * TC + hc];
+  for (uint32_t b = 0; b < L.n_bins; ++b) he[b] = etp[(uint64_t)b * TC + hc];
   L.hot_end.alloc(std::max<size_t>(he.size(), 1));
   if (!he.empty()) NWG_HIP(hipMemcpyAsync(L.hot_end.p, he.data(), he.size() * 4, hipMemcpyHostToDevice, s));
-  // bin pass items: every bin, cut into ranges of at most kItem2Entries entries; the items
-  // of a split bin get consecutive partial slots, and the bin a combine item
+  // bin pass items: every bin, cut into ranges of at most kItem2Entries entries (inner cuts on
+  // block boundaries, so no block is walked by two items of one bin); the items of a split bin
+  // get consecutive partial slots, and the bin combine units
   std::vector<uint4> it2, it3;
   uint32_t slots = 0;
   for (uint32_t b = 0; b < L.n_bins; ++b) {
-    const uint32_t p0 = tp[(uint64_t)b * TC], p1 = tp[(uint64_t)(b + 1) * TC];
+    const uint32_t p0 = etp[(uint64_t)b * TC], p1 = etp[(uint64_t)(b + 1) * TC];
     const uint32_t cnt = (uint32_t)std::max<uint64_t>(ceil_div(p1 - p0, kItem2Entries), 1);
     if (cnt > 1) {
       // combine units: 256 / L rows each, L lanes per row sharing its cnt slots
       const uint32_t lg = cnt > 32 ? 3u : cnt > 8 ? 2u : 0u;
       for (uint32_t v0 = 0; v0 < kBinRows; v0 += 256u >> lg) it3.push_back(make_uint4(b, slots, cnt, v0 | (lg << 16)));
     }
-    for (uint32_t q = 0; q < cnt; ++q) {
-      const uint32_t a0 = p0 + (uint32_t)((uint64_t)(p1 - p0) * q / cnt);
-      const uint32_t a1 = p0 + (uint32_t)((uint64_t)(p1 - p0) * (q + 1) / cnt);
-      it2.push_back(make_uint4(b, a0, a1, cnt > 1 ? slots++ : kNoSlot));
-    }
+    auto cut = [&](uint32_t q) -> uint32_t {
+      if (q == 0) return p0;
+      if (q == cnt) return p1;
+      const uint32_t c = p0 + (uint32_t)((uint64_t)(p1 - p0) * q / cnt);
+      return std::max(p0, c / kBinBlock * kBinBlock);
+    };
+    for (uint32_t q = 0; q < cnt; ++q) it2.push_back(make_uint4(b, cut(q), cut(q + 1), cnt > 1 ? slots++ : kNoSlot));
   }
   // chunk pass items: the non-empty tiles of each chunk (long tiles cut into pieces), cut
-  // at kItem1Entries entries or kBinMaxSegs pieces; offsets and sizes in groups of 8
-  // entries (every padded tile, and kItem1Entries, is a whole number of groups)
+  // at kItem1Pairs pairs or kBinMaxSegs pieces; offsets and sizes in groups of 8 pairs
+  // (every padded tile, and kItem1Pairs, is a whole number of groups)
   std::vector<uint4> it1;
   std::vector<uint32_t> sst, sfl;
   for (uint32_t c = 0; c < C; ++c) {
     uint32_t first = (uint32_t)sst.size(), flat = 0;
     for (uint32_t b = 0; b < L.n_bins; ++b) {
-      const uint32_t t0 = tp[(uint64_t)b * TC + hc + c], t1 = tp[(uint64_t)b * TC + hc + c + 1];
-      // a long tile is cut into pieces of at most kItem1Entries entries
-      for (uint32_t q0 = t0; q0 < t1; q0 += kItem1Entries) {
-        const uint32_t len = (std::min<uint32_t>(t1 - q0, kItem1Entries)) / 8;
+      const uint32_t t0 = ptp[(uint64_t)b * TC + hc + c], t1 = ptp[(uint64_t)b * TC + hc + c + 1];
+      // a long tile is cut into pieces of at most kItem1Pairs pairs
+      for (uint32_t q0 = t0; q0 < t1; q0 += kItem1Pairs) {
+        const uint32_t len = (std::min<uint32_t>(t1 - q0, kItem1Pairs)) / 8;
         if ((uint32_t)sst.size() > first &&
-            (flat + len > kItem1Entries / 8 || (uint32_t)sst.size() - first >= kBinMaxSegs)) {
+            (flat + len > kItem1Pairs / 8 || (uint32_t)sst.size() - first >= kBinMaxSegs)) {
           it1.push_back(make_uint4(c, first, (uint32_t)sst.size(), flat));
           first = (uint32_t)sst.size();
           flat = 0;

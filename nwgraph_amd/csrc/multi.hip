@@ -243,7 +243,7 @@ void multigraph_set_binned(nwg_ctx* ctx, nwg_multigraph* mg, bool binned) {
     // no hot tiles: the union's first ids are one graph's, not the union's hubs
     binned_build(s, ActiveCsc{mg->rp.p, mg->col.p, mg->deg.p, nullptr}, (uint32_t)mg->nv, mg->ne, mg->bl, false);
     mg->row_base.alloc(mg->nv ? mg->nv : 1);
-    mg->bvals.alloc(mg->bl.padded);
+    mg->bvals.alloc(mg->bl.pairs);
     mg->bpart.alloc(std::max<uint64_t>((uint64_t)mg->bl.n_slots << kBinBits, 1));
     NWG_HIP(hipStreamSynchronize(s));
   }
@@ -289,6 +289,7 @@ void multigraph_run(nwg_ctx* ctx, nwg_multigraph* mg, float damping, float thres
     const BinnedLayout& L = mg->bl;
     B.brow = L.brow.p;
     B.bsrc = L.bsrc.p;
+    B.bpid = L.bpid.p;
     B.vals = mg->bvals.p;
     B.items2 = L.items2.p;
     B.items1 = L.items1.p;

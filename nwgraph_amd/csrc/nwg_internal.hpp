@@ -89,13 +89,18 @@ __host__ __device__ inline uint32_t slice_of(uint32_t u) { return (u >> kSliceSh
 // Binned sweep layout (propagation blocking) of the active CSC.  Rows form bins of
 // 2^kBinBits, sources form chunks of 2^kChunkBits; every CSC entry (row v <- source u)
 // belongs to tile (bin v >> kBinBits, chunk u >> kChunkBits).  Entries are stored tile
-// by tile, bin-major, each tile in CSC order; per entry only the low bits of its row and
-// source are kept.  The chunk pass writes contrib[u] into every entry of a chunk (from
-// the chunk staged in LDS), the bin pass streams each bin and adds its entries into LDS
-// row accumulators.
+// by tile, bin-major, each tile ordered by source, then row.  The entries of one bin that
+// share a source form a pair (bin, source): the value of a pair is written and read once,
+// however many rows of the bin the source feeds.  Per entry the layout keeps the row's
+// low bits and a bit that marks the first entry of a pair; per pair, the source's low
+// bits.  The chunk pass writes contrib[u] into every pair of a chunk (from the chunk
+// staged in LDS), the bin pass streams each bin's entries, finds each entry's pair by
+// counting pair starts, and adds the pair's value into LDS row accumulators.
 constexpr int kBinBits = 13;    // rows per bin: 8192 int64 accumulators = 64 KiB of LDS (2 workgroups per CU)
 constexpr int kChunkBits = 15;  // sources per chunk: 32768 floats = 128 KiB of LDS
 constexpr uint32_t kBinMaxSegs = 2048;  // tiles of one chunk-pass work item
+constexpr uint32_t kBinBlock = 512;     // entries of one bin-pass wave block (8 per lane); pair index kept per block
+constexpr uint32_t kPairStart = 0x8000u;  // entry flag: first entry of its pair
 // Hot tiles: a bin's entries whose source is below kHotSrc (the highest out-degrees in a
 // degree-ordered numbering) skip the value array: the bin pass reads them from the
 // contributions staged in its LDS next to the row accumulators (kHotSrc floats keep two
@@ -111,13 +116,16 @@ struct BinnedLayout {
   bool built = false;
   uint32_t n_bins = 0, n_chunks = 0, n_items2 = 0, n_items1 = 0, n_items3 = 0, n_segs = 0, n_slots = 0;
   uint32_t n_hot = 0;            // sources [0, n_hot) form each bin's hot tile (0: no hot tiles)
-  uint64_t nnz = 0, padded = 0;  // entries, and the length of the per-entry arrays
-  DevBuf<uint16_t> brow, bsrc;   // row & (2^kBinBits - 1); source & (2^kChunkBits - 1), or the source in a hot tile
+  uint64_t nnz = 0, padded = 0;  // entries, and the length of the per-entry array (pads included)
+  uint64_t pairs = 0;            // length of the per-pair arrays (tiles padded to 8 pairs)
+  DevBuf<uint16_t> brow;         // per entry: row & (2^kBinBits - 1), | kPairStart on a pair's first entry
+  DevBuf<uint16_t> bsrc;         // per pair: source & (2^kChunkBits - 1), or the source in a hot tile
+  DevBuf<uint32_t> bpid;         // per kBinBlock entries: pair of the block's first entry
   DevBuf<uint32_t> hot_end;      // per bin: end of its hot tile (its first tile)
   DevBuf<uint4> items2;          // bin pass: {bin, first entry, end entry, partial slot (or none)}
   DevBuf<uint4> items3;          // combine pass units: {split bin, first slot, slots, first row | log2 lanes << 16}
-  DevBuf<uint4> items1;          // chunk pass: {chunk, first tile, end tile, entries}
-  DevBuf<uint32_t> seg_start;    // non-empty tiles of each chunk: first entry
+  DevBuf<uint4> items1;          // chunk pass: {chunk, first tile, end tile, groups of 8 pairs}
+  DevBuf<uint32_t> seg_start;    // non-empty tiles of each chunk: first pair
   DevBuf<uint32_t> seg_flat;     // and offset inside their chunk-pass item
 };
 
@@ -149,7 +157,8 @@ struct Plan {
 struct BinParams {
   const uint16_t* brow;
   const uint16_t* bsrc;
-  float* vals;
+  const uint32_t* bpid;
+  float* vals;  // one value per pair
   const uint4* items2;
   const uint4* items1;
   const uint4* items3;
@@ -234,7 +243,7 @@ struct PrState {
   DevBuf<float> hub_part;          // per hub segment partial sums
   DevBuf<double> blk_err;          // per-workgroup L1 partials of one sweep
   DevBuf<float> spart;             // kSlices x n_srows partial sums of the sliced rows
-  DevBuf<float> bvals;             // binned sweep: one value per entry
+  DevBuf<float> bvals;             // binned sweep: one value per (bin, source) pair
   DevBuf<unsigned long long> bpart;  // binned sweep: partial rows of split bins
   DevBuf<unsigned long long> bacc;   // binned sweep: per-sweep L1 change, kBinAccLanes lanes
   Captured cap;

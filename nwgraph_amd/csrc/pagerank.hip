@@ -644,6 +644,7 @@ static void enqueue_solve(hipStream_t s, nwg_graph* g, PrState& st, SweepParams 
     // read the previous sweep's while the row updates write the next
     B.brow = L.brow.p;
     B.bsrc = L.bsrc.p;
+    B.bpid = L.bpid.p;
     B.vals = st.bvals.p;
     B.items2 = L.items2.p;
     B.items1 = L.items1.p;
@@ -707,7 +708,7 @@ void pagerank_run(nwg_ctx* ctx, nwg_graph* g, const float* teleport_dev, float d
   st.spart.ensure(std::max<uint64_t>((uint64_t)kSlices * g->plan.n_srows, 1));
   if (g->plan.params.binned && !inplace) {
     const BinnedLayout& L = g->plan.bl;
-    st.bvals.ensure(L.padded);
+    st.bvals.ensure(L.pairs);
     st.bpart.ensure(std::max<uint64_t>((uint64_t)L.n_slots << kBinBits, 1));
     st.bacc.ensure(std::max<uint64_t>(max_iters, 1) * kBinAccStride);
   }
