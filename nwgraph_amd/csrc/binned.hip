@@ -627,19 +627,36 @@ void binned_build(hipStream_t s, const ActiveCsc& a, uint32_t n, uint64_t nnz, B
     for (uint32_t q = 0; q < cnt; ++q) it2.push_back(make_uint4(b, cut(q), cut(q + 1), cnt > 1 ? slots++ : kNoSlot));
   }
   // chunk pass items: the non-empty tiles of each chunk (long tiles cut into pieces), cut
-  // at kItem1Pairs pairs or kBinMaxSegs pieces; offsets and sizes in groups of 8 pairs
-  // (every padded tile, and kItem1Pairs, is a whole number of groups)
+  // at cap1 pairs or kBinMaxSegs pieces; offsets and sizes in groups of 8 pairs (every
+  // padded tile, and cap1, is a whole number of groups)
+  // Item size: kItem1Pairs, or smaller when the graph has fewer than 4 chunks per CU, so that
+  // every CU gets about 4 items (an item stages its whole chunk, from L2 or MALL when a chunk
+  // is cut into several).
+  uint32_t cap1 = kItem1Pairs;
+  {
+    int dev = 0, cus = 0;
+    NWG_HIP(hipGetDevice(&dev));
+    NWG_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    const uint64_t want = 4ull * (uint64_t)std::max(cus, 1);
+    if (C < want) {
+      uint64_t chunk_pairs = 0;
+      for (uint32_t b = 0; b < L.n_bins; ++b)
+        chunk_pairs += ptp[(uint64_t)(b + 1) * TC] - ptp[(uint64_t)b * TC + hc];
+      const uint64_t per = (ceil_div(chunk_pairs, want) + 7) / 8 * 8;
+      cap1 = (uint32_t)std::min<uint64_t>(kItem1Pairs, std::max<uint64_t>(per, 8192));
+    }
+  }
   std::vector<uint4> it1;
   std::vector<uint32_t> sst, sfl;
   for (uint32_t c = 0; c < C; ++c) {
     uint32_t first = (uint32_t)sst.size(), flat = 0;
     for (uint32_t b = 0; b < L.n_bins; ++b) {
       const uint32_t t0 = ptp[(uint64_t)b * TC + hc + c], t1 = ptp[(uint64_t)b * TC + hc + c + 1];
-      // a long tile is cut into pieces of at most kItem1Pairs pairs
-      for (uint32_t q0 = t0; q0 < t1; q0 += kItem1Pairs) {
-        const uint32_t len = (std::min<uint32_t>(t1 - q0, kItem1Pairs)) / 8;
+      // a long tile is cut into pieces of at most cap1 pairs
+      for (uint32_t q0 = t0; q0 < t1; q0 += cap1) {
+        const uint32_t len = (std::min<uint32_t>(t1 - q0, cap1)) / 8;
         if ((uint32_t)sst.size() > first &&
-            (flat + len > kItem1Pairs / 8 || (uint32_t)sst.size() - first >= kBinMaxSegs)) {
+            (flat + len > cap1 / 8 || (uint32_t)sst.size() - first >= kBinMaxSegs)) {
           it1.push_back(make_uint4(c, first, (uint32_t)sst.size(), flat));
           first = (uint32_t)sst.size();
           flat = 0;
