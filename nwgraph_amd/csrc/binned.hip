@@ -40,7 +40,7 @@ namespace {
 constexpr int kBT = 1024;  // threads of a bin / chunk pass workgroup (16 waves, one workgroup per CU)
 constexpr uint32_t kBinRows = 1u << kBinBits;
 constexpr uint32_t kChunkSrc = 1u << kChunkBits;
-constexpr uint32_t kItem2Entries = 1u << 18;  // entries per bin-pass item
+constexpr uint32_t kItem2Log = 19;  // log2 of the most entries of a bin-pass item
 constexpr uint32_t kItem1Pairs = 1u << 18;    // pairs per chunk-pass item
 constexpr uint32_t kPadSrc = 0xFFFFu;           // source offset of a pad pair (value 0)
 constexpr uint32_t kRowMask = kBinRows - 1;
@@ -605,14 +605,31 @@ void binned_build(hipStream_t s, const ActiveCsc& a, uint32_t n, uint64_t nnz, B
   for (uint32_t b = 0; b < L.n_bins; ++b) he[b] = etp[(uint64_t)b * TC + hc];
   L.hot_end.alloc(std::max<size_t>(he.size(), 1));
   if (!he.empty()) NWG_HIP(hipMemcpyAsync(L.hot_end.p, he.data(), he.size() * 4, hipMemcpyHostToDevice, s));
-  // bin pass items: every bin, cut into ranges of at most kItem2Entries entries (inner cuts on
+  // bin pass items: every bin, cut into ranges of at most cap2 entries (inner cuts on
   // block boundaries, so no block is walked by two items of one bin); the items of a split bin
   // get consecutive partial slots, and the bin combine units
+  // Item size: the power of two nearest to entries / (4 x CUs), between 2^15 and
+  // 2^kItem2Log.  Big graphs take big items (fewer split bins, so fewer partial rows to
+  // store and combine); small ones take small items, so that their heavy bins spread over
+  // all CUs.  (Measured on MI355X: 2^19 best at RMAT-25, 2^18 for the 256-graph union,
+  // 2^17 at RMAT-23, 2^16 at RMAT-22 and below.)
+  int dev = 0, cus = 0;
+  NWG_HIP(hipGetDevice(&dev));
+  NWG_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+  uint32_t cap2 = 1u << 15;
+  {
+    const uint64_t q = etotal / (4ull * (uint64_t)std::max(cus, 1));
+    uint32_t lg = 0;
+    while (lg < 63 && (2ull << lg) <= q) ++lg;  // floor(log2 q)
+    if (q && (double)q >= 1.4142 * (double)(1ull << lg)) ++lg;
+    lg = std::min<uint32_t>(std::max<uint32_t>(lg, 15u), kItem2Log);
+    cap2 = 1u << lg;
+  }
   std::vector<uint4> it2, it3;
   uint32_t slots = 0;
   for (uint32_t b = 0; b < L.n_bins; ++b) {
     const uint32_t p0 = etp[(uint64_t)b * TC], p1 = etp[(uint64_t)(b + 1) * TC];
-    const uint32_t cnt = (uint32_t)std::max<uint64_t>(ceil_div(p1 - p0, kItem2Entries), 1);
+    const uint32_t cnt = (uint32_t)std::max<uint64_t>(ceil_div(p1 - p0, cap2), 1);
     if (cnt > 1) {
       // combine units: 256 / L rows each, L lanes per row sharing its cnt slots
       const uint32_t lg = cnt > 32 ? 3u : cnt > 8 ? 2u : 0u;
@@ -634,9 +651,6 @@ void binned_build(hipStream_t s, const ActiveCsc& a, uint32_t n, uint64_t nnz, B
   // is cut into several).
   uint32_t cap1 = kItem1Pairs;
   {
-    int dev = 0, cus = 0;
-    NWG_HIP(hipGetDevice(&dev));
-    NWG_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
     const uint64_t want = 4ull * (uint64_t)std::max(cus, 1);
     if (C < want) {
       uint64_t chunk_pairs = 0;
