@@ -1,8 +1,10 @@
 // autotune.cpp -- picks the sweep plan (degree-bin cutoffs, hub split, cache hints)
 // of a graph by timing candidate plans on the device with HIP events.
 //
-// Search: coordinate descent from the default plan, one parameter at a time, each
-// candidate timed over a few sweeps (launched one by one, each bracketed by events).
+// Search: coordinate descent over the pull sweep's parameters from the default plan, one
+// parameter at a time, each candidate timed over a few sweeps (launched one by one, each
+// bracketed by events); then the binned sweep in both numberings; then the best pull plan
+// and the best binned plan timed again over more sweeps, and the faster one kept.
 // Results are cached per (log2 in-degree histogram, n, nnz, device) for the process
 // and, if NWG_TUNE_CACHE names a file, appended there and reloaded by later processes.
 #include <hip/hip_runtime.h>
@@ -119,16 +121,17 @@ nwg_plan_params autotune_plan(nwg_ctx* ctx, nwg_graph* g) {
       return it->second;
     }
   }
-  auto eval = [&](const nwg_plan_params& p) -> double {
+  auto eval_n = [&](const nwg_plan_params& p, int reps, uint64_t sweeps) -> double {
     try {
       plan_build(s, g, p);
     } catch (const Error&) {
       return 1e30;  // invalid combination
     }
     double best = 1e30;
-    for (int r = 0; r < 2; ++r) best = std::min(best, time_plan_sweeps(ctx, g, 4));
+    for (int r = 0; r < reps; ++r) best = std::min(best, time_plan_sweeps(ctx, g, sweeps));
     return best;
   };
+  auto eval = [&](const nwg_plan_params& p) -> double { return eval_n(p, 2, 4); };
   nwg_plan_params best = default_plan_params();
   double best_ms = eval(best);
   auto try_field = [&](uint32_t nwg_plan_params::*field, std::initializer_list<uint32_t> values) {
@@ -143,20 +146,26 @@ nwg_plan_params autotune_plan(nwg_ctx* ctx, nwg_graph* g) {
       }
     }
   };
+  // the pull sweep: numbering, then its row-class cutoffs, hub split, slicing and hints
+  best.binned = 0;
   try_field(&nwg_plan_params::relabel, {0, 1});
-  try_field(&nwg_plan_params::binned, {0, 1});
-  if (best.binned) {
-    // the binned sweep ignores the row-class cutoffs; only the numbering matters
-    try_field(&nwg_plan_params::relabel, {0, 1});
-  } else {
-    try_field(&nwg_plan_params::stream_hints, {0, 1, 2});
-    try_field(&nwg_plan_params::short_max, {0, 2, 4, 8, 16, 32});
-    try_field(&nwg_plan_params::group_max, {16, 32, 64, 128, 256, 512, 1024});
-    try_field(&nwg_plan_params::hub_min, {2048, 4096, 8192, 16384, 32768, 65536, 262144});
-    try_field(&nwg_plan_params::hub_segment, {1024, 2048, 4096, 8192, 16384});
-    try_field(&nwg_plan_params::slice_min, {0, 33, 65, 129, 257, 1025});
-    try_field(&nwg_plan_params::short_max, {0, 2, 4, 8, 16, 32});
-  }
+  try_field(&nwg_plan_params::stream_hints, {0, 1, 2});
+  try_field(&nwg_plan_params::short_max, {0, 2, 4, 8, 16, 32});
+  try_field(&nwg_plan_params::group_max, {16, 32, 64, 128, 256, 512, 1024});
+  try_field(&nwg_plan_params::hub_min, {2048, 4096, 8192, 16384, 32768, 65536, 262144});
+  try_field(&nwg_plan_params::hub_segment, {1024, 2048, 4096, 8192, 16384});
+  try_field(&nwg_plan_params::slice_min, {0, 33, 65, 129, 257, 1025});
+  try_field(&nwg_plan_params::short_max, {0, 2, 4, 8, 16, 32});
+  const nwg_plan_params pull = best;
+  // the binned sweep ignores the row-class cutoffs; only the numbering matters
+  nwg_plan_params binned = pull, b0 = pull;
+  binned.binned = b0.binned = 1;
+  binned.relabel = 1;
+  b0.relabel = 0;
+  if (eval(b0) < eval(binned)) binned = b0;
+  // the two winners, timed again over more sweeps
+  const double t_pull = eval_n(pull, 3, 8), t_binned = eval_n(binned, 3, 8);
+  best = t_binned < t_pull ? binned : pull;
   plan_build(s, g, best);
   {
     std::lock_guard<std::mutex> lk(g_cache_mu);

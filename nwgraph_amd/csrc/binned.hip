@@ -646,12 +646,14 @@ void binned_build(hipStream_t s, const ActiveCsc& a, uint32_t n, uint64_t nnz, B
   // chunk pass items: the non-empty tiles of each chunk (long tiles cut into pieces), cut
   // at cap1 pairs or kBinMaxSegs pieces; offsets and sizes in groups of 8 pairs (every
   // padded tile, and cap1, is a whole number of groups)
-  // Item size: kItem1Pairs, or smaller when the graph has fewer than 4 chunks per CU, so that
-  // every CU gets about 4 items (an item stages its whole chunk, from L2 or MALL when a chunk
-  // is cut into several).
+  // Item size: kItem1Pairs, or smaller when the graph has fewer than 2 chunks per CU, so that
+  // every CU gets about 2 items.  An item stages its whole chunk (from L2 or MALL when a chunk
+  // is cut into several), so more, smaller items cost staging: measured on MI355X, 2 items
+  // per CU beat 4 on the 4096^2 grid (0.179 vs 0.194 ms per sweep) and on RMAT-22 (0.131 vs
+  // 0.137), and beat whole chunks on RMAT-22 (0.162) and RMAT-20 (0.050 vs 0.083).
   uint32_t cap1 = kItem1Pairs;
   {
-    const uint64_t want = 4ull * (uint64_t)std::max(cus, 1);
+    const uint64_t want = 2ull * (uint64_t)std::max(cus, 1);
     if (C < want) {
       uint64_t chunk_pairs = 0;
       for (uint32_t b = 0; b < L.n_bins; ++b)

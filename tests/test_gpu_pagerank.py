@@ -323,6 +323,23 @@ def test_full_size_sliced_plan_rmat22(ctx):
     assert np.max(np.abs(x_b - x_def) / np.maximum(x_def, 1e-30)) < 1e-5
 
 
+@pytest.mark.parametrize("kind", ["rmat", "grid"])
+def test_autotuned_plan_matches_v10(ctx, kind):
+    """The autotuner's plan (pull or binned, whichever timed faster) keeps the v10 gate."""
+    if kind == "rmat":
+        g = ctx.rmat(17, 16, seed=4)
+        n, ind, col, deg = O.rmat_csc(17, 16, seed=4)
+        iters = 20
+    else:
+        g = ctx.grid(512, seed=4)
+        n, ind, col, deg = O.grid_csc(512, seed=4)
+        iters = 100
+    plan = g.autotune()
+    assert plan["binned"] in (0, 1) and plan["relabel"] in (0, 1)
+    assert g.get_plan() == plan
+    check_against_oracle(g, n, ind, col, deg, max_iters=iters)
+
+
 @pytest.mark.parametrize("relabel", [0, 1])
 def test_binned_sweep_many_bins_matches_v10(ctx, relabel):
     """Binned sweep at RMAT-18: 16 bins of 2^14 rows, 8 chunks of 2^15 sources, the hot
