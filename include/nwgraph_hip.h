@@ -170,6 +170,19 @@ int nwg_graph_pagerank(nwg_ctx* ctx, nwg_graph* g, float* scores, const float* t
 int nwg_graph_pagerank_inplace(nwg_ctx* ctx, nwg_graph* g, float* scores, const float* teleport,
                                float damping, float threshold, uint64_t max_iters, int mem_kind,
                                nwg_pr_info* info);
+/* The reference's delta (residual) PageRank, page_rank_v13
+ * (include/nwgraph/experimental/algorithms/page_rank.hpp:560-625, pr --version 13).  Every
+ * vertex starts with residual 1 - d and score 0.  An iteration absorbs each residual above
+ * `threshold` into the score and sends r * d / outdeg to the out-neighbours (:580-594); then
+ * every residual becomes the sum of what the vertex received (:597-612), so residuals at or
+ * below the threshold that were not absorbed are dropped.  The loop ends after the first
+ * iteration in which no vertex with out-edges pushed (:616-620), or after max_iters.  Scores
+ * are not normalised (they sum to about N, not 1).  The pull phase runs on the plan's pull
+ * kernels (the binned option is ignored); the loop is one captured HIP graph that stops
+ * working after the last iteration without a host round trip.  info->iterations counts the
+ * iterations executed; the error trace is 0, as the reference traces it (:615). */
+int nwg_graph_pagerank_delta(nwg_ctx* ctx, nwg_graph* g, float* scores, float damping,
+                             float threshold, uint64_t max_iters, int mem_kind, nwg_pr_info* info);
 /* Scheduling plan of the sweep: set explicitly, read back, or autotune. */
 int nwg_graph_set_plan(nwg_ctx* ctx, nwg_graph* g, const nwg_plan_params* p);
 int nwg_graph_get_plan(const nwg_graph* g, nwg_plan_params* p);

@@ -95,6 +95,8 @@ _SIGNATURES = {
                                    ctypes.POINTER(PrInfo)]),
     "nwg_graph_pagerank_inplace": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_float, c_float, c_u64, c_int,
                                            ctypes.POINTER(PrInfo)]),
+    "nwg_graph_pagerank_delta": (c_int, [c_void_p, c_void_p, c_void_p, c_float, c_float, c_u64, c_int,
+                                         ctypes.POINTER(PrInfo)]),
     "nwg_graph_set_plan": (c_int, [c_void_p, c_void_p, ctypes.POINTER(PlanParams)]),
     "nwg_graph_get_plan": (c_int, [c_void_p, ctypes.POINTER(PlanParams)]),
     "nwg_graph_autotune": (c_int, [c_void_p, c_void_p, ctypes.POINTER(PlanParams)]),

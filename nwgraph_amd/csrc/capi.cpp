@@ -304,7 +304,8 @@ int nwg_generate_grid(nwg_ctx* ctx, uint32_t width, uint32_t keep_per10000, uint
 
 // ---------------------------------------------------------------- PageRank
 static int graph_pagerank(nwg_ctx* ctx, nwg_graph* g, float* scores, const float* teleport, float damping,
-                          float threshold, uint64_t max_iters, int mem_kind, nwg_pr_info* info, bool inplace) {
+                          float threshold, uint64_t max_iters, int mem_kind, nwg_pr_info* info,
+                          nwg::PrSchedule sched) {
   if (!ctx) return NWG_EINVAL;
   return guarded(ctx, [&] {
     check_graph(ctx, g);
@@ -323,7 +324,7 @@ static int graph_pagerank(nwg_ctx* ctx, nwg_graph* g, float* scores, const float
       out_dev.alloc(g->n);
       sc = out_dev.p;
     }
-    nwg::pagerank_run(ctx, g, tele, damping, threshold, max_iters, info, sc, inplace);
+    nwg::pagerank_run(ctx, g, tele, damping, threshold, max_iters, info, sc, sched);
     if (mem_kind == NWG_MEM_HOST) {
       NWG_HIP(hipMemcpyAsync(scores, out_dev.p, (size_t)g->n * 4, hipMemcpyDeviceToHost, s));
       NWG_HIP(hipStreamSynchronize(s));
@@ -333,12 +334,20 @@ static int graph_pagerank(nwg_ctx* ctx, nwg_graph* g, float* scores, const float
 
 int nwg_graph_pagerank(nwg_ctx* ctx, nwg_graph* g, float* scores, const float* teleport, float damping,
                        float threshold, uint64_t max_iters, int mem_kind, nwg_pr_info* info) {
-  return graph_pagerank(ctx, g, scores, teleport, damping, threshold, max_iters, mem_kind, info, false);
+  return graph_pagerank(ctx, g, scores, teleport, damping, threshold, max_iters, mem_kind, info,
+                        nwg::PrSchedule::kJacobi);
 }
 
 int nwg_graph_pagerank_inplace(nwg_ctx* ctx, nwg_graph* g, float* scores, const float* teleport, float damping,
                                float threshold, uint64_t max_iters, int mem_kind, nwg_pr_info* info) {
-  return graph_pagerank(ctx, g, scores, teleport, damping, threshold, max_iters, mem_kind, info, true);
+  return graph_pagerank(ctx, g, scores, teleport, damping, threshold, max_iters, mem_kind, info,
+                        nwg::PrSchedule::kInPlace);
+}
+
+int nwg_graph_pagerank_delta(nwg_ctx* ctx, nwg_graph* g, float* scores, float damping, float threshold,
+                             uint64_t max_iters, int mem_kind, nwg_pr_info* info) {
+  return graph_pagerank(ctx, g, scores, nullptr, damping, threshold, max_iters, mem_kind, info,
+                        nwg::PrSchedule::kDelta);
 }
 
 int nwg_pagerank_f32(nwg_ctx* ctx, uint32_t n, uint64_t nnz, const uint32_t* csc_ptr, const uint32_t* csc_idx,

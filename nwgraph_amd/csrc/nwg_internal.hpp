@@ -354,10 +354,16 @@ void binned_enqueue_sweep(hipStream_t s, const BinParams& P);
 // lane counters of iters sweeps cleared / summed into acc[0 .. iters)
 void binned_clear_residuals(hipStream_t s, unsigned long long* lanes, uint64_t iters);
 void binned_fold_residuals(hipStream_t s, const unsigned long long* lanes, unsigned long long* acc, uint64_t iters);
-// inplace: one contribution array updated during the sweep (the reference's v11 schedule,
-// page_rank.hpp:135), pull kernels only; else Jacobi (v10) with double-buffered contributions
+// Schedules of pagerank_run:
+//   kJacobi   page_rank_v10: double-buffered contributions (pull sweep or binned sweep)
+//   kInPlace  page_rank (v11): one contribution array updated during the sweep
+//             (page_rank.hpp:135), pull kernels only
+//   kDelta    page_rank_v13 (experimental/algorithms/page_rank.hpp:560-625): residuals
+//             pushed as deltas, pulled by the pull kernels; un-normalised scores, no teleport
+enum class PrSchedule : uint32_t { kJacobi = 0, kInPlace = 1, kDelta = 2 };
 void pagerank_run(nwg_ctx* ctx, nwg_graph* g, const float* teleport_dev, float damping, float threshold,
-                  uint64_t max_iters, nwg_pr_info* info, float* scores_dev, bool inplace = false);
+                  uint64_t max_iters, nwg_pr_info* info, float* scores_dev,
+                  PrSchedule sched = PrSchedule::kJacobi);
 
 // ---- batched personalised PageRank (ppr.hip) ----
 void ppr_run(nwg_ctx* ctx, nwg_graph* g, uint32_t k, const std::vector<uint32_t>& seed_ptr,

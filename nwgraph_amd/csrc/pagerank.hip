@@ -74,6 +74,9 @@ struct SweepParams {
   unsigned long long* acc;
   uint32_t iter;
   double threshold;
+  // delta schedule (page_rank_v13): cin holds the deltas, cout the residuals, x the scores;
+  // acc[k] is nonzero when some vertex pushed in iteration k
+  uint32_t delta;
 };
 
 // entries of blk_err reduced by one workgroup of the finishing kernel
@@ -123,11 +126,21 @@ __device__ __forceinline__ float gather_sum(const SweepParams& P, uint32_t p0, u
 }
 
 __device__ __forceinline__ bool converged_before(const SweepParams& P) {
-  return P.iter > 0 && fixed_to_err(P.acc[P.iter - 1]) < P.threshold;
+  if (P.iter == 0) return false;
+  // delta schedule: the loop ends after an iteration in which nothing was pushed
+  // (experimental/algorithms/page_rank.hpp:616-620)
+  if (P.delta) return P.acc[P.iter - 1] == 0ull;
+  return fixed_to_err(P.acc[P.iter - 1]) < P.threshold;
 }
 
 template <int NT>
 __device__ __forceinline__ double update_row(const SweepParams& P, uint32_t i, float z) {
+  if (P.delta) {
+    // delta schedule, pull phase (experimental/algorithms/page_rank.hpp:597-612): the row's
+    // residual becomes the sum of its in-neighbours' deltas (all >= 0), or 0
+    st<NT>(P.cout + i, z > 0.f ? z : 0.f);
+    return 0.0;
+  }
   const float b = P.basev ? ld<NT>(P.basev + i) : P.base;
   const float xo = ld<NT>(P.x + i);
   const float xn = __builtin_fmaf(P.damp, z, b);
@@ -253,6 +266,48 @@ __global__ __launch_bounds__(kThreads) void pr_init_kernel(uint32_t n, const uin
     c0[i] = init / (float)deg[i];
   }
   if (i < iters) acc[i] = 0ull;
+}
+
+// delta schedule (page_rank_v13, experimental/algorithms/page_rank.hpp:570-577): scores 0,
+// deltas 0, every residual (float)(1.0 - d)
+__global__ __launch_bounds__(kThreads) void pr_delta_init_kernel(uint32_t n, float r0, float* __restrict__ x,
+                                                                 float* __restrict__ dlt, float* __restrict__ res,
+                                                                 unsigned long long* __restrict__ acc,
+                                                                 uint64_t iters) {
+  const uint64_t i = (uint64_t)blockIdx.x * kThreads + threadIdx.x;
+  if (i < n) {
+    x[i] = 0.f;
+    dlt[i] = 0.f;
+    res[i] = r0;
+  }
+  if (i < iters) acc[i] = 0ull;
+}
+
+// delta schedule, outgoing phase (experimental/algorithms/page_rank.hpp:580-594): a residual
+// above the threshold is absorbed into the score and, for a vertex with out-edges, turned
+// into the delta r * d / deg its out-neighbours receive.  A workgroup in which some vertex
+// pushed stores 1 in acc[iter] (plain stores of one value: only zero / nonzero matters).
+__global__ __launch_bounds__(kThreads) void pr_delta_push_kernel(SweepParams P, float* __restrict__ res,
+                                                                 float* __restrict__ dlt) {
+  if (converged_before(P)) return;
+  const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
+  const float thr = (float)P.threshold;
+  bool pushed = false;
+  if (i < P.n) {
+    float dv = 0.f;
+    const float r = res[i];
+    if (r > thr) {
+      P.x[i] += r;
+      res[i] = 0.f;
+      const uint32_t dg = P.deg[i];
+      if (dg > 0) {
+        dv = r * P.damp / (float)dg;
+        pushed = true;
+      }
+    }
+    dlt[i] = dv;
+  }
+  if (__syncthreads_or(pushed) && threadIdx.x == 0) P.acc[P.iter] = 1ull;
 }
 
 // personalised base (1-d) * v rounded once from double; with a relabelled CSC row i
@@ -629,13 +684,34 @@ static void enqueue_sweep(hipStream_t s, const SweepParams& P) {
 }
 
 static void enqueue_solve(hipStream_t s, nwg_graph* g, PrState& st, SweepParams P, uint64_t iters,
-                          hipEvent_t* ev, bool inplace) {
+                          hipEvent_t* ev, PrSchedule sched) {
   const uint32_t n = g->n;
+  const uint32_t nt = g->plan.params.stream_hints;
+  if (sched == PrSchedule::kDelta) {
+    // page_rank_v13: contrib0 holds the deltas, contrib1 the residuals, x the scores
+    P.delta = 1;
+    P.cin = st.contrib0.p;
+    P.cout = st.contrib1.p;
+    pr_delta_init_kernel<<<blocks_for(std::max<uint64_t>(n, iters)), kThreads, 0, s>>>(
+        n, (float)(1.0 - (double)P.damp), st.x.p, st.contrib0.p, st.contrib1.p, st.acc.p, iters);
+    NWG_LAUNCH_CHECK();
+    for (uint64_t k = 0; k < iters; ++k) {
+      P.iter = (uint32_t)k;
+      if (ev) NWG_HIP(hipEventRecord(ev[2 * k], s));
+      pr_delta_push_kernel<<<blocks_for(n), kThreads, 0, s>>>(P, st.contrib1.p, st.contrib0.p);
+      NWG_LAUNCH_CHECK();
+      if (nt == 2) enqueue_sweep<2>(s, P);
+      else if (nt == 1) enqueue_sweep<1>(s, P);
+      else enqueue_sweep<0>(s, P);
+      if (ev) NWG_HIP(hipEventRecord(ev[2 * k + 1], s));
+    }
+    return;
+  }
+  const bool inplace = sched == PrSchedule::kInPlace;
   const float init = (float)(1.0 / n);  // page_rank.hpp:93
   pr_init_kernel<<<blocks_for(std::max<uint64_t>(n, iters)), kThreads, 0, s>>>(n, P.deg, init, st.x.p,
                                                                                st.contrib0.p, st.acc.p, iters);
   NWG_LAUNCH_CHECK();
-  const uint32_t nt = g->plan.params.stream_hints;
   const BinnedLayout& L = g->plan.bl;
   const bool binned = g->plan.params.binned && !inplace;  // the in-place schedule is a pull sweep
   BinParams B{};
@@ -692,7 +768,8 @@ static void enqueue_solve(hipStream_t s, nwg_graph* g, PrState& st, SweepParams 
 }
 
 void pagerank_run(nwg_ctx* ctx, nwg_graph* g, const float* teleport_dev, float damping, float threshold,
-                  uint64_t max_iters, nwg_pr_info* info, float* scores_dev, bool inplace) {
+                  uint64_t max_iters, nwg_pr_info* info, float* scores_dev, PrSchedule sched) {
+  NWG_REQUIRE(sched != PrSchedule::kDelta || teleport_dev == nullptr, "the delta schedule takes no teleport vector");
   hipStream_t s = ctx->stream;
   NWG_REQUIRE(max_iters < (1ull << 31), "max_iters too large");
   if (!g->plan.built) plan_build(s, g, default_plan_params());
@@ -706,7 +783,7 @@ void pagerank_run(nwg_ctx* ctx, nwg_graph* g, const float* teleport_dev, float d
   st.hub_part.ensure(std::max<uint32_t>(g->plan.n_hub_segs, 1));
   st.blk_err.ensure(std::max<uint64_t>(sweep_workgroups_before_hubs(g->plan), 1));
   st.spart.ensure(std::max<uint64_t>((uint64_t)kSlices * g->plan.n_srows, 1));
-  if (g->plan.params.binned && !inplace) {
+  if (g->plan.params.binned && sched == PrSchedule::kJacobi) {
     const BinnedLayout& L = g->plan.bl;
     st.bvals.ensure(L.pairs);
     st.bpart.ensure(std::max<uint64_t>((uint64_t)L.n_slots << kBinBits, 1));
@@ -742,13 +819,13 @@ void pagerank_run(nwg_ctx* ctx, nwg_graph* g, const float* teleport_dev, float d
     key.iters = max_iters;
     key.damping = damping;
     key.threshold = threshold;
-    key.extra = inplace ? 1u : 0u;
+    key.extra = (uint32_t)sched;
     if (!st.cap.valid || !(st.cap.key == key)) {
       st.cap.reset();
       hipStream_t cs = ctx->capture_stream;
       NWG_HIP(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
       try {
-        enqueue_solve(cs, g, st, P, max_iters, nullptr, inplace);
+        enqueue_solve(cs, g, st, P, max_iters, nullptr, sched);
       } catch (...) {
         hipGraph_t tmp = nullptr;
         (void)hipStreamEndCapture(cs, &tmp);
@@ -768,7 +845,7 @@ void pagerank_run(nwg_ctx* ctx, nwg_graph* g, const float* teleport_dev, float d
     for (auto& e : evs) NWG_HIP(hipEventCreate(&e));
     NWG_HIP(hipEventRecord(ctx->ev0, s));
     try {
-      enqueue_solve(s, g, st, P, max_iters, evs.data(), inplace);
+      enqueue_solve(s, g, st, P, max_iters, evs.data(), sched);
     } catch (...) {
       for (auto& e : evs) (void)hipEventDestroy(e);
       throw;
@@ -791,6 +868,15 @@ void pagerank_run(nwg_ctx* ctx, nwg_graph* g, const float* teleport_dev, float d
   uint64_t iters = max_iters;
   double last = 0.0;
   for (uint64_t k = 0; k < max_iters; ++k) {
+    if (sched == PrSchedule::kDelta) {
+      // the reference traces error 0 for this schedule (page_rank.hpp:615); the loop ends
+      // after the first iteration that pushed nothing
+      if (acc[k] == 0ull) {
+        iters = k + 1;
+        break;
+      }
+      continue;
+    }
     const double e = fixed_to_err(acc[k]);
     last = e;
     if (e < (double)threshold) {
@@ -814,7 +900,8 @@ void pagerank_run(nwg_ctx* ctx, nwg_graph* g, const float* teleport_dev, float d
     info->ms_per_iter = iters ? ms_total / (double)iters : 0.0;
     info->sweep_ms_avg = sweep_ms;
     if (info->error_trace)
-      for (uint64_t k = 0; k < iters && k < info->error_trace_len; ++k) info->error_trace[k] = fixed_to_err(acc[k]);
+      for (uint64_t k = 0; k < iters && k < info->error_trace_len; ++k)
+        info->error_trace[k] = sched == PrSchedule::kDelta ? 0.0 : fixed_to_err(acc[k]);
   }
 }
 

@@ -10,7 +10,9 @@
 // Version 10 runs the Jacobi schedule of page_rank_v10 (nwg_graph_pagerank, deterministic);
 // version 11 runs the reference's in-place schedule (nwg_graph_pagerank_inplace: one
 // contribution array updated during the sweep, schedule-dependent like the reference's
-// TBB run).  --autotune picks the sweep plan first (nwg_graph_autotune).  Defaults follow
+// TBB run); version 13 the delta (residual) PageRank of page_rank_v13
+// (nwg_graph_pagerank_delta, un-normalised scores).  --autotune picks the sweep plan first
+// (nwg_graph_autotune).  Defaults follow
 // the reference: version 11, -i 20 (0 means 1), -t 1e-4, -n 1 (0 means 1).  THREADS are
 // accepted and reported like the reference's (which only uses them for TBB); the GPU
 // ignores them.
@@ -206,7 +208,7 @@ int main(int argc, char** argv) {
     for (long thread : args.threads) {
       for (long id : args.versions) {
         for (long trial = 0; trial < args.trials; ++trial) {
-          if (id != 10 && id != 11) {
+          if (id != 10 && id != 11 && id != 13) {
             std::cerr << "Unknown version id " << id << std::endl;
             continue;
           }
@@ -214,12 +216,20 @@ int main(int argc, char** argv) {
           info.error_trace = trace.data();
           info.error_trace_len = trace.size();
           const auto t0 = std::chrono::steady_clock::now();
-          // version 11 is the reference's in-place schedule, version 10 its Jacobi twin
-          const auto solve = id == 11 ? nwg_graph_pagerank_inplace : nwg_graph_pagerank;
-          check(ctx,
-                solve(ctx, g, rankings.data(), nullptr, 0.85f, args.tolerance, (uint64_t)args.max_iters, NWG_MEM_HOST,
-                      &info),
-                id == 11 ? "nwg_graph_pagerank_inplace" : "nwg_graph_pagerank");
+          // version 11 is the reference's in-place schedule, version 10 its Jacobi twin,
+          // version 13 the delta (residual) PageRank with un-normalised scores
+          if (id == 13) {
+            check(ctx,
+                  nwg_graph_pagerank_delta(ctx, g, rankings.data(), 0.85f, args.tolerance, (uint64_t)args.max_iters,
+                                           NWG_MEM_HOST, &info),
+                  "nwg_graph_pagerank_delta");
+          } else {
+            const auto solve = id == 11 ? nwg_graph_pagerank_inplace : nwg_graph_pagerank;
+            check(ctx,
+                  solve(ctx, g, rankings.data(), nullptr, 0.85f, args.tolerance, (uint64_t)args.max_iters,
+                        NWG_MEM_HOST, &info),
+                  id == 11 ? "nwg_graph_pagerank_inplace" : "nwg_graph_pagerank");
+          }
           const double sec = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
           nw::graph::hip::detail::trace_row("iter", "error", "time", "outgoing");
           for (uint64_t k = 0; k < info.iterations; ++k)

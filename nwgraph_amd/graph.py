@@ -213,6 +213,16 @@ class DeviceGraph:
                           int(max_iters), NWG_MEM_DEVICE, ctypes.byref(info)))
         return self._info_dict(info, trace)
 
+    def pagerank_delta(self, damping=0.85, threshold=1e-4, max_iters=20):
+        """The reference's delta PageRank (page_rank_v13, pr --version 13): un-normalised
+        scores (about N in total); returns (scores float32[n], info dict)."""
+        scores = np.empty(self.n, np.float32)
+        info, trace = self._info(max_iters)
+        self.ctx.check(self.ctx.lib.nwg_graph_pagerank_delta(self.ctx.handle, self.handle, _ptr(scores),
+                                                             float(damping), float(threshold), int(max_iters),
+                                                             NWG_MEM_HOST, ctypes.byref(info)))
+        return scores, self._info_dict(info, trace)
+
     # ---- batched personalised PageRank -----------------------------------------
     @staticmethod
     def _seed_csr(seeds):

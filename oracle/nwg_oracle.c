@@ -17,6 +17,10 @@
  *   nwo_fill_sorted    reference KATs adjacency_test.cpp:26-45,240-257 and a
  *                      scipy.sparse CSC/CSR conversion of the same edges.
  *   nwo_out_degrees    numpy bincount of the sources.
+ *   nwo_pr_v13         the karate golden values (scores / N, threshold 1e-9); with threshold
+ *                      0 the dense Neumann partial sums sum_t (d P^T)^t (1-d) 1 (numpy
+ *                      matrix powers); the directed cycle's closed form (residual
+ *                      (1-d) d^t, absorbed while above the threshold)
  *   nwo_pr_v1          the reference's karate golden values (page_rank_test.cpp:71-75,
  *                      0.5 % relative), scipy dense solve.
  *   nwo_pr_v10         closed forms (cycle / regular digraph -> 1/N; isolated vertex ->
@@ -412,6 +416,52 @@ uint64_t nwo_pr_v1(uint32_t n, const uint32_t* indices, const uint32_t* col, con
     if (error < threshold) break;
   }
   free(contrib);
+  return iter;
+}
+
+uint64_t nwo_pr_v13(uint32_t n, const uint32_t* indices, const uint32_t* col, const uint32_t* deg, float* pr,
+                    float damping, float threshold, uint64_t max_iters, uint64_t* pushed) {
+  /* experimental/page_rank.hpp:560-625 (page_rank_v13, Real = float).  Every vertex starts
+   * with residual 1 - d (:572-576: the double 1.0 - d stored as float).  An iteration first
+   * absorbs each residual above the threshold into the score and turns it into a delta
+   * r * d / deg (:580-594), then sets every row's residual to the sum of its in-neighbours'
+   * positive deltas, or 0 (:597-612): residuals at or below the threshold that were not
+   * absorbed are dropped.  The loop stops after the first iteration in which no vertex with
+   * out-edges pushed (:616-620). */
+  float* delta = (float*)malloc((uint64_t)(n ? n : 1) * sizeof(float));
+  float* residual = (float*)malloc((uint64_t)(n ? n : 1) * sizeof(float));
+  for (uint32_t i = 0; i < n; ++i) {
+    pr[i] = 0.0f;
+    delta[i] = 0.0f;
+    residual[i] = (float)(1.0 - (double)damping);
+  }
+  uint64_t iter = 0;
+  while (iter < max_iters) {
+    uint64_t changed = 0;
+    for (uint32_t src = 0; src < n; ++src) {
+      delta[src] = 0.0f;
+      if (residual[src] > threshold) {
+        const float old = residual[src];
+        pr[src] += old;
+        residual[src] = 0.0f;
+        if (deg[src] > 0) {
+          delta[src] = old * damping / (float)deg[src];
+          ++changed;
+        }
+      }
+    }
+    for (uint32_t i = 0; i < n; ++i) {
+      float sum = 0.0f;
+      for (uint64_t p = indices[i]; p < indices[i + 1]; ++p)
+        if (delta[col[p]] > 0.0f) sum += delta[col[p]];
+      residual[i] = sum > 0.0f ? sum : 0.0f;
+    }
+    if (pushed) pushed[iter] = changed;
+    ++iter;
+    if (!changed) break;
+  }
+  free(delta);
+  free(residual);
   return iter;
 }
 

@@ -10,6 +10,7 @@
  *   page_rank (v11) ........ include/nwgraph/algorithms/page_rank.hpp:89-148
  *   page_rank_v10 (Jacobi) . include/nwgraph/experimental/algorithms/page_rank.hpp:377-432
  *   page_rank_v1 ........... include/nwgraph/experimental/algorithms/page_rank.hpp:82-119
+ *   page_rank_v13 (delta) .. include/nwgraph/experimental/algorithms/page_rank.hpp:560-625
  *   fill_directed<idx> ..... include/nwgraph/build.hpp:209-232
  *   lexical_sort_by/uniq/remove_self_loops  include/nwgraph/build.hpp:57-70,392-410
  *   sort_to_be_indexed ..... include/nwgraph/containers/compressed.hpp:491-502
@@ -85,6 +86,11 @@ uint64_t nwo_pr_v10(uint32_t n, const uint32_t* indices, const uint32_t* col, co
 /* v1: sequential Jacobi in double (the function the karate golden values pin). */
 uint64_t nwo_pr_v1(uint32_t n, const uint32_t* indices, const uint32_t* col, const uint32_t* deg,
                    double* pr, double damping, double threshold, uint64_t max_iters);
+/* v13: delta (residual) PageRank in float, un-normalised scores (sum over vertices ~ N).
+ * Returns the iterations executed; pushed[k] (optional) gets the number of vertices
+ * whose residual was above the threshold and had out-edges in iteration k. */
+uint64_t nwo_pr_v13(uint32_t n, const uint32_t* indices, const uint32_t* col, const uint32_t* deg,
+                    float* pr, float damping, float threshold, uint64_t max_iters, uint64_t* pushed);
 
 /* ---- FP64 references ---- */
 /* Jacobi in double on x = (1-d) v + d A x, A_iu = w(u->i) (w NULL => 1/outdeg(u));

@@ -38,6 +38,7 @@ def load():
         "nwo_pr_v11_parallel": (U64, [U32, V, V, V, V, F, F, U64, I, V, V]),
         "nwo_pr_v10": (U64, [U32, V, V, V, V, F, F, U64, V, I, V, V]),
         "nwo_pr_v1": (U64, [U32, V, V, V, V, D, D, U64]),
+        "nwo_pr_v13": (U64, [U32, V, V, V, V, F, F, U64, V]),
         "nwo_fp64_jacobi": (U64, [U32, V, V, V, V, V, D, D, U64, V]),
         "nwo_fp64_adjoint": (U64, [U32, V, V, V, V, V, D, D, U64, V]),
         "nwo_read_mm": (I, [ctypes.c_char_p, V, V, V, V]),
@@ -172,6 +173,14 @@ def pr_v10(n, ind, col, deg, damping=0.85, threshold=1e-4, max_iters=20, telepor
     it = load().nwo_pr_v10(n, _p(_u32(ind)), _p(_u32(col)), _p(_u32(deg)), _p(pr), damping, threshold, max_iters,
                            _p(tel), nthreads, _p(tr), _p(sec))
     return pr, int(it), tr[:it].copy(), sec[:it].copy()
+
+
+def pr_v13(n, ind, col, deg, damping=0.85, threshold=1e-4, max_iters=20):
+    pr = np.empty(n, np.float32)
+    pushed = np.zeros(max(max_iters, 1), np.uint64)
+    it = load().nwo_pr_v13(n, _p(_u32(ind)), _p(_u32(col)), _p(_u32(deg)), _p(pr), damping, threshold, max_iters,
+                           _p(pushed))
+    return pr, int(it), pushed[:it].copy()
 
 
 def pr_v1(n, ind, col, deg, damping=0.85, threshold=1e-4, max_iters=100):

@@ -21,7 +21,8 @@ def declared_functions():
 def test_header_declares_expected_surface():
     names = declared_functions()
     for required in ("nwg_pagerank_f32", "nwg_graph_pagerank", "nwg_graph_build_from_edges", "nwg_ppr_batched",
-                     "nwg_multigraph_pagerank", "nwg_wpr_forward", "nwg_wpr_backward", "nwg_graph_autotune"):
+                     "nwg_multigraph_pagerank", "nwg_wpr_forward", "nwg_wpr_backward", "nwg_graph_autotune",
+                     "nwg_graph_pagerank_inplace", "nwg_graph_pagerank_delta"):
         assert required in names
 
 

@@ -323,6 +323,57 @@ def test_full_size_sliced_plan_rmat22(ctx):
     assert np.max(np.abs(x_b - x_def) / np.maximum(x_def, 1e-30)) < 1e-5
 
 
+@pytest.mark.parametrize("kind,plan", [("rmat", {}), ("rmat", {"relabel": 1}), ("rmat", {"binned": 1}),
+                                       ("grid", {}), ("rmat16", {})])
+def test_delta_schedule_matches_v13(ctx, kind, plan):
+    """pr --version 13 (page_rank_v13): the push kernel plus the pull kernels against the
+    oracle's restatement.  Pull sums differ from the oracle's sequential float sums only in
+    order; a residual that lands within rounding of the threshold can be absorbed by one and
+    not the other, so the gate is: same iteration count, and all but 0.1 % of the scores
+    within 1e-5 relative, none off by more than 10 x the threshold."""
+    thr = 1e-4
+    if kind == "grid":
+        g = ctx.grid(128, seed=2)
+        n, ind, col, deg = O.grid_csc(128, seed=2)
+        iters = 100
+    else:
+        sc = 16 if kind == "rmat16" else 13
+        g = ctx.rmat(sc, 16, seed=2)
+        n, ind, col, deg = O.rmat_csc(sc, 16, seed=2)
+        iters = 60
+    if plan:
+        p = g.get_plan()
+        p.update(plan)
+        g.set_plan(**p)
+    x, info = g.pagerank_delta(D, thr, iters)
+    ref, it, pushed = O.pr_v13(n, ind, col, deg, D, thr, iters)
+    assert info["iterations"] == it
+    assert np.all(np.asarray(info["error_trace"]) == 0.0)
+    diff = np.abs(x.astype(np.float64) - ref)
+    assert np.mean(diff > 1e-5 * np.abs(ref) + 1e-7) <= 1e-3
+    assert diff.max() <= 10 * thr
+    # un-normalised: the scores sum to about N (a little less: dangling vertices and dropped
+    # residuals lose mass)
+    assert 0.3 * n < x.sum() <= 1.0001 * n
+    # deterministic: a second call is bit-identical
+    x2, info2 = g.pagerank_delta(D, thr, iters)
+    assert np.array_equal(x, x2) and info2["iterations"] == info["iterations"]
+
+
+def test_delta_schedule_cycle_stops_like_the_closed_form(ctx):
+    """A directed cycle: every residual is (1-d) d^t; the device stops after the first
+    iteration that pushes nothing (t = 31 at threshold 1e-3), like the oracle."""
+    n = 1000
+    s = np.arange(n, dtype=np.uint32)
+    d = ((s + 1) % n).astype(np.uint32)
+    g = ctx.graph_from_edges(n, s, d)
+    x, info = g.pagerank_delta(D, 1e-3, 100)
+    ind, col = O.csc(n, s, d)
+    ref, it, _ = O.pr_v13(n, ind, col, O.out_degrees(n, ind, col), D, 1e-3, 100)
+    assert info["iterations"] == it == 32
+    assert np.allclose(x, ref, rtol=1e-6)
+
+
 @pytest.mark.parametrize("kind", ["rmat", "grid"])
 def test_autotuned_plan_matches_v10(ctx, kind):
     """The autotuner's plan (pull or binned, whichever timed faster) keeps the v10 gate."""

@@ -184,6 +184,71 @@ def test_v10_float_karate_close_to_golden():
     assert np.max(np.abs(pr - pr1)) < 1e-6
 
 
+def test_v13_karate_golden():
+    """v13 (delta PageRank, experimental/algorithms/page_rank.hpp:560-625) keeps un-normalised
+    scores: with a tiny threshold, scores / N reach the reference's karate golden values."""
+    g, n, ind, col, deg = karate_csc()
+    pr, it, pushed = O.pr_v13(n, ind, col, deg, 0.85, 1e-9, 1000)
+    ans = np.array(g["pagerank"])
+    assert 50 < it < 1000 and pushed[-1] == 0 and np.all(pushed[:-1] > 0)
+    assert np.all(np.abs(pr / n - ans) <= g["rel_tolerance"] * np.abs(ans))
+
+
+@pytest.mark.parametrize("iters", [1, 2, 5, 12])
+def test_v13_threshold_zero_is_the_neumann_partial_sum(iters):
+    """With threshold 0 every positive residual is absorbed, so after k iterations the scores
+    are sum_{t<k} (d A)^t (1-d) 1 (A[i,u] = 1/outdeg(u) per edge u->i): dense matrix powers."""
+    n, ind, col, deg = O.rmat_csc(8, 8, seed=5)
+    pr, it, _ = O.pr_v13(n, ind, col, deg, 0.85, 0.0, iters)
+    assert it == iters
+    d = float(np.float32(0.85))
+    A = O.dense_matrix(n, ind, col, deg)
+    A[:, deg == 0] = 0.0
+    r = np.full(n, float(np.float32(1.0 - d)))
+    acc = np.zeros(n)
+    for _ in range(iters):
+        acc += r
+        r = d * (A @ r)
+    assert np.allclose(pr, acc, rtol=2e-5, atol=1e-7)
+
+
+def test_v13_cycle_closed_form():
+    """On a directed cycle every residual is (1-d) d^t at iteration t; it is absorbed while
+    above the threshold, and the loop stops after the first iteration that absorbs nothing."""
+    n, thr = 100, 1e-3
+    s = np.arange(n, dtype=np.uint32)
+    ind, col = O.csc(n, s, ((s + 1) % n).astype(np.uint32))
+    deg = O.out_degrees(n, ind, col)
+    pr, it, pushed = O.pr_v13(n, ind, col, deg, 0.85, thr, 100)
+    d = float(np.float32(0.85))
+    rt = [(1.0 - d) * d ** t for t in range(100)]
+    stop = next(t for t, r in enumerate(rt) if r <= thr)
+    assert it == stop + 1 and np.all(pushed[:stop] == n) and pushed[stop] == 0
+    assert np.allclose(pr, sum(rt[:stop]), rtol=1e-5)
+
+
+def test_v13_drops_unabsorbed_residuals():
+    """The pull phase overwrites every residual (page_rank.hpp:608-611), so a residual at or
+    below the threshold that was not absorbed is lost.  Vertices: a=0 (out-degree 20, to b
+    and 19 sinks), b=1 (a sink), y=2 -> z=3, z (out-degree 10, to b and 9 sinks); threshold
+    0.02.  Iteration 0: all absorb r0 = 1 - d; b then holds r0 d/20 + r0 d/10 = 0.0191.
+    Iteration 1: b keeps it un-absorbed; z absorbs y's r0 d and pushes r0 d^2/10 = 0.0108,
+    which replaces b's 0.0191 (kept, it would have summed to 0.0300 and been absorbed).
+    Iteration 2 pushes nothing, so the loop stops after 3 iterations with pr[b] = r0 and
+    pr[z] = r0 (1 + d)."""
+    s_ = [0] * 20 + [3] * 10 + [2]
+    d_ = [1] + list(range(4, 23)) + [1] + list(range(23, 32)) + [3]
+    n = 32
+    ind, col = O.csc(n, np.array(s_, np.uint32), np.array(d_, np.uint32))
+    deg = O.out_degrees(n, ind, col)
+    assert deg[0] == 20 and deg[3] == 10 and deg[2] == 1 and deg[1] == 0
+    pr, it, pushed = O.pr_v13(n, ind, col, deg, 0.85, 0.02, 20)
+    r0 = np.float32(1.0 - float(np.float32(0.85)))
+    assert it == 3 and list(pushed) == [3, 1, 0]
+    assert pr[1] == r0
+    assert np.isclose(pr[3], float(r0) * (1.0 + float(np.float32(0.85))), rtol=1e-6)
+
+
 def test_v10_regular_digraph_is_uniform():
     # a directed cycle: every vertex has in/out degree 1, PageRank = 1/N exactly each sweep
     n = 1000
