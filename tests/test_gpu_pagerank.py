@@ -301,6 +301,38 @@ def test_full_size_properties_rmat22(ctx):
     assert abs(i20["error_trace"][-1] - np.abs(x20.astype(np.float64) - x19).sum()) < 1e-9
 
 
+@pytest.mark.parametrize("kind", ["rmat25", "grid4096"])
+def test_full_size_properties_binned(ctx, kind):
+    """The benchmark's full-size inputs on the binned sweep (the plan the autotuner picks for
+    both): one more sweep equals the FP64 update of the previous iterate on sampled rows
+    (the 100 highest in-degree rows included), mass balance over all edges, the L1 trace,
+    and a repeat that is bit-identical."""
+    g = ctx.rmat(25, 16, seed=1) if kind == "rmat25" else ctx.grid(4096, seed=1)
+    p = g.get_plan()
+    p.update(binned=1, relabel=1 if kind == "rmat25" else 0)
+    g.set_plan(**p)
+    ptr, idx, deg = g.csc()
+    n = g.n
+    assert int(deg.sum(dtype=np.uint64)) == g.nnz == idx.size
+    x19, i19 = g.pagerank(D, 0.0, 19)
+    x20, i20 = g.pagerank(D, 0.0, 20)
+    assert i19["iterations"] == 19 and i20["iterations"] == 20
+    c = x19.astype(np.float64) / np.where(deg > 0, deg, 1)
+    base = np.float64(np.float32((1.0 - np.float64(np.float32(D))) / n))
+    dd = np.float64(np.float32(D))
+    rng = np.random.default_rng(0)
+    sample = np.concatenate([rng.integers(0, n, 4000), np.argsort(np.diff(ptr.astype(np.int64)))[-100:]])
+    for i in sample:
+        want = base + dd * c[idx[ptr[i]:ptr[i + 1]]].sum()
+        assert abs(x20[i] - want) <= 1e-6 * max(want, 1e-12) + 1e-12, (int(i), float(x20[i]), want)
+    lhs = x20.astype(np.float64).sum()
+    rhs = n * base + dd * x19.astype(np.float64)[deg > 0].sum()
+    assert abs(lhs - rhs) < 1e-5 * rhs
+    assert abs(i20["error_trace"][-1] - np.abs(x20.astype(np.float64) - x19).sum()) < 1e-6 * i20["error_trace"][-1]
+    x20b, _ = g.pagerank(D, 0.0, 20)
+    assert np.array_equal(x20, x20b)
+
+
 def test_full_size_sliced_plan_rmat22(ctx):
     """At RMAT-22 the degree-ordered, source-sliced plan gives the default plan's scores
     (only the FP32 summation order of the sliced rows differs) and leaves the CSC as built."""
