@@ -183,7 +183,9 @@ int nwg_graph_pagerank_inplace(nwg_ctx* ctx, nwg_graph* g, float* scores, const 
  * iterations executed; the error trace is 0, as the reference traces it (:615). */
 int nwg_graph_pagerank_delta(nwg_ctx* ctx, nwg_graph* g, float* scores, float damping,
                              float threshold, uint64_t max_iters, int mem_kind, nwg_pr_info* info);
-/* Scheduling plan of the sweep: set explicitly, read back, or autotune. */
+/* Scheduling plan of the sweep: set explicitly, read back, or autotune.  Without either, the
+ * first PageRank solve of a graph builds a plan by size: the binned sweep from 2^18 edges,
+ * on the degree-ordered copy from 2^23 edges, else the pull sweep (get_plan reports it). */
 int nwg_graph_set_plan(nwg_ctx* ctx, nwg_graph* g, const nwg_plan_params* p);
 int nwg_graph_get_plan(const nwg_graph* g, nwg_plan_params* p);
 /* Time candidate plans on this graph with HIP events and keep the fastest; results are

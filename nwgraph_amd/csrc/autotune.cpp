@@ -132,7 +132,7 @@ nwg_plan_params autotune_plan(nwg_ctx* ctx, nwg_graph* g) {
     return best;
   };
   auto eval = [&](const nwg_plan_params& p) -> double { return eval_n(p, 2, 4); };
-  nwg_plan_params best = default_plan_params();
+  nwg_plan_params best = default_plan_params();  // the pull sweep, original numbering
   double best_ms = eval(best);
   auto try_field = [&](uint32_t nwg_plan_params::*field, std::initializer_list<uint32_t> values) {
     for (uint32_t v : values) {
@@ -147,7 +147,6 @@ nwg_plan_params autotune_plan(nwg_ctx* ctx, nwg_graph* g) {
     }
   };
   // the pull sweep: numbering, then its row-class cutoffs, hub split, slicing and hints
-  best.binned = 0;
   try_field(&nwg_plan_params::relabel, {0, 1});
   try_field(&nwg_plan_params::stream_hints, {0, 1, 2});
   try_field(&nwg_plan_params::short_max, {0, 2, 4, 8, 16, 32});

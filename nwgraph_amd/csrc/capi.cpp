@@ -374,7 +374,7 @@ int nwg_graph_set_plan(nwg_ctx* ctx, nwg_graph* g, const nwg_plan_params* p) {
 
 int nwg_graph_get_plan(const nwg_graph* g, nwg_plan_params* p) {
   if (!g || !p) return NWG_EINVAL;
-  *p = g->plan.built ? g->plan.params : nwg::default_plan_params();
+  *p = g->plan.built ? g->plan.params : nwg::default_plan_for(g);
   return NWG_OK;
 }
 

@@ -348,6 +348,8 @@ void build_segments(hipStream_t s, const uint32_t* rp, uint32_t n, uint32_t min_
                     DevBuf<uint32_t>& rows, DevBuf<uint32_t>& seg_ptr, DevBuf<uint32_t>& seg_row, uint32_t* nrows,
                     uint32_t* nsegs);
 nwg_plan_params default_plan_params();
+// the plan a PageRank solve of g builds on first use (binned sweep and degree ordering by size)
+nwg_plan_params default_plan_for(const nwg_graph* g);
 // binned sweep (binned.hip)
 void binned_build(hipStream_t s, const ActiveCsc& a, uint32_t n, uint64_t nnz, BinnedLayout& L, bool hot);
 void binned_enqueue_sweep(hipStream_t s, const BinParams& P);

@@ -340,7 +340,7 @@ def test_full_size_sliced_plan_rmat22(ctx):
     ptr0, idx0, _ = g.csc()
     x_def, i_def = g.pagerank(D, 1e-4, 20)
     p = g.get_plan()
-    p.update(short_max=4, group_max=128, hub_min=65536, hub_segment=8192, slice_min=65, relabel=1)
+    p.update(short_max=4, group_max=128, hub_min=65536, hub_segment=8192, slice_min=65, relabel=1, binned=0)
     g.set_plan(**p)
     x_sl, i_sl = g.pagerank(D, 1e-4, 20)
     assert i_sl["iterations"] == i_def["iterations"]
