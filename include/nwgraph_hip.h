@@ -92,7 +92,10 @@ const char* nwg_last_error(const nwg_ctx* ctx);
  * (n floats) is written in place like the reference's pre-sized page_rank vector.
  * Semantics: Jacobi sweeps (the v10 twin, experimental/algorithms/page_rank.hpp:377-432)
  * with the reference's scalars; stops after the first sweep whose L1 change is
- * below threshold, or after max_iters sweeps. */
+ * below threshold, or after max_iters sweeps.  The graph lives for this call only, so it
+ * runs the pull sweep, which needs no layout built; a graph solved more than once is
+ * faster resident (nwg_graph_create_csc + nwg_graph_pagerank), where the binned layout is
+ * built once. */
 int nwg_pagerank_f32(nwg_ctx* ctx, uint32_t n, uint64_t nnz, const uint32_t* csc_ptr /*[host]*/,
                      const uint32_t* csc_idx /*[host]*/, const uint32_t* outdeg /*[host] or NULL*/,
                      float* scores /*[host] n*/, float damping, float threshold, uint64_t max_iters,

@@ -357,7 +357,12 @@ int nwg_pagerank_f32(nwg_ctx* ctx, uint32_t n, uint64_t nnz, const uint32_t* csc
   nwg_graph* g = nullptr;
   int rc = nwg_graph_create_csc(ctx, n, nnz, csc_ptr, csc_idx, outdeg, NWG_MEM_HOST, &g);
   if (rc != NWG_OK) return rc;
-  rc = nwg_graph_pagerank(ctx, g, scores, nullptr, damping, threshold, max_iters, NWG_MEM_HOST, info);
+  // A graph made for one solve takes the pull sweep on its own numbering: it has nothing to
+  // build, while the binned layout and the degree-ordered copy cost more than they save in
+  // one solve of a few tens of sweeps (RMAT-25: ~160 ms to build, ~115 ms saved over 20).
+  const nwg_plan_params pull = nwg::default_plan_params();
+  rc = nwg_graph_set_plan(ctx, g, &pull);
+  if (rc == NWG_OK) rc = nwg_graph_pagerank(ctx, g, scores, nullptr, damping, threshold, max_iters, NWG_MEM_HOST, info);
   nwg_graph_destroy(g);
   return rc;
 }
