@@ -244,7 +244,9 @@ __global__ __launch_bounds__(kBT) void pr_chunk_kernel(BinParams P) {
 template <bool HOT>
 __device__ __forceinline__ void bin_range(const BinParams& P, uint32_t e0, uint32_t e1, unsigned long long* lacc,
                                           const float* hot) {
-  const uint32_t w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  // the wave index is uniform over the wave: in a scalar register, the block indices and the
+  // blocks' first pairs stay scalar too
+  const uint32_t w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
   constexpr uint32_t kWaves = kBT / 64;
   // values are addressed by 32-bit byte offsets (the layout keeps pairs below 2^30)
   const char* vbase = reinterpret_cast<const char*>(P.vals);
