@@ -501,9 +501,11 @@ nwg_plan_params default_plan_params() {
 // edge factor 16): the binned sweep beats the pull sweep from 2^16 vertices (1M edges: 0.023
 // vs 0.048 ms per sweep) up (RMAT-20: 0.049 vs 0.164; RMAT-25: 0.94 vs 3.65), and on the
 // 4096^2 grid; degree ordering pays from RMAT-19 (8M edges) on and costs a little below.
+// The binned layout indexes entries and pairs with 32 bits (below 2^31 entries and 2^30
+// pairs with pads), so larger graphs keep the pull sweep.
 nwg_plan_params default_plan_for(const nwg_graph* g) {
   nwg_plan_params p = default_plan_params();
-  if (g->nnz >= (1ull << 18)) p.binned = 1;
+  if (g->nnz >= (1ull << 18) && g->nnz < (1ull << 30) - (1ull << 26)) p.binned = 1;
   if (g->nnz >= (1ull << 23)) p.relabel = 1;
   return p;
 }
