@@ -166,10 +166,11 @@ int nwg_graph_pagerank(nwg_ctx* ctx, nwg_graph* g, float* scores, const float* t
 /* Same solve with the reference's own in-place schedule: nw::graph::page_rank
  * (page_rank.hpp:121-147) updates outgoing_contrib inside the sweep (:135), so rows later
  * in a sweep see earlier rows' new contributions.  Here one contribution array is updated
- * during the sweep by the pull kernels (the plan's binned option is ignored): rows see
- * whatever neighbours have already written.  Like the reference's multi-threaded run the
- * result depends on the schedule, so it is not reproducible bit for bit; it converges to
- * the same fixed point, usually in fewer sweeps on high-diameter graphs. */
+ * during the sweep.  With the pull kernels, rows see whatever neighbours have already
+ * written.  With the binned sweep (plan option binned), each sweep's non-hot contributions
+ * are staged before any row is finished, so updates reach other rows within the sweep only
+ * through the hot sources.  Like the reference's multi-threaded run the result depends on
+ * the schedule, so it is not reproducible bit for bit; it converges to the same fixed point. */
 int nwg_graph_pagerank_inplace(nwg_ctx* ctx, nwg_graph* g, float* scores, const float* teleport,
                                float damping, float threshold, uint64_t max_iters, int mem_kind,
                                nwg_pr_info* info);

@@ -726,7 +726,11 @@ static void enqueue_solve(hipStream_t s, nwg_graph* g, PrState& st, SweepParams 
                                                                                st.contrib0.p, st.acc.p, iters);
   NWG_LAUNCH_CHECK();
   const BinnedLayout& L = g->plan.bl;
-  const bool binned = g->plan.params.binned && !inplace;  // the in-place schedule is a pull sweep
+  // In place, the binned sweep reads and writes one contribution array: its chunk pass stages
+  // every non-hot contribution before any row of the sweep is finished, so updates made in
+  // the sweep reach other rows only through the hot sources, which each bin-pass work item
+  // stages when it starts.
+  const bool binned = g->plan.params.binned != 0;
   BinParams B{};
   if (binned) {
     // contributions double-buffered like the pull sweep's: the hot tiles of the bin pass
@@ -796,7 +800,7 @@ void pagerank_run(nwg_ctx* ctx, nwg_graph* g, const float* teleport_dev, float d
   st.hub_part.ensure(std::max<uint32_t>(g->plan.n_hub_segs, 1));
   st.blk_err.ensure(std::max<uint64_t>(sweep_workgroups_before_hubs(g->plan), 1));
   st.spart.ensure(std::max<uint64_t>((uint64_t)kSlices * g->plan.n_srows, 1));
-  if (g->plan.params.binned && sched == PrSchedule::kJacobi) {
+  if (g->plan.params.binned && sched != PrSchedule::kDelta) {
     const BinnedLayout& L = g->plan.bl;
     st.bvals.ensure(L.pairs);
     st.bpart.ensure(std::max<uint64_t>((uint64_t)L.n_slots << kBinBits, 1));

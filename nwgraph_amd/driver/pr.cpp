@@ -9,8 +9,8 @@
 //   grid:WIDTH[:SEED]                 4-neighbour lattice (p=0.85) + 0.5% long-range edges
 // Version 10 runs the Jacobi schedule of page_rank_v10 (nwg_graph_pagerank, deterministic);
 // version 11 runs the reference's in-place schedule (nwg_graph_pagerank_inplace: one
-// contribution array updated during the sweep, schedule-dependent like the reference's
-// TBB run); version 13 the delta (residual) PageRank of page_rank_v13
+// contribution array updated during the sweep, on the pull kernels or the binned sweep,
+// schedule-dependent like the reference's TBB run); version 13 the delta (residual) PageRank of page_rank_v13
 // (nwg_graph_pagerank_delta, un-normalised scores).  --autotune picks the sweep plan first
 // (nwg_graph_autotune).  Defaults follow
 // the reference: version 11, -i 20 (0 means 1), -t 1e-4, -n 1 (0 means 1).  THREADS are

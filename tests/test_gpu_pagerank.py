@@ -472,7 +472,7 @@ def test_inplace_schedule_reaches_the_fixed_point(ctx, kind):
     assert np.max(np.abs(xj - x64)) <= 1e-6 and np.max(np.abs(xi - x64)) <= 1e-6
     if kind == "grid":
         assert ii["iterations"] <= ij["iterations"]
-    # personalised teleport, and a binned plan (the in-place schedule always runs the pull kernels)
+    # personalised teleport, and the binned sweep in place (one contribution array)
     p = g.get_plan()
     p.update(binned=1)
     g.set_plan(**p)
