@@ -286,10 +286,13 @@ __global__ __launch_bounds__(kThreads) void pr_delta_init_kernel(uint32_t n, flo
 // delta schedule, outgoing phase (experimental/algorithms/page_rank.hpp:580-594): a residual
 // above the threshold is absorbed into the score and, for a vertex with out-edges, turned
 // into the delta r * d / deg its out-neighbours receive.  A workgroup in which some vertex
-// pushed stores 1 in acc[iter] (plain stores of one value: only zero / nonzero matters).
+// pushed stores 1 in flags[iter * stride] (plain stores of one value: only zero / nonzero
+// matters); the kernel does nothing after an iteration whose flag stayed 0.
 __global__ __launch_bounds__(kThreads) void pr_delta_push_kernel(SweepParams P, float* __restrict__ res,
-                                                                 float* __restrict__ dlt) {
-  if (converged_before(P)) return;
+                                                                 float* __restrict__ dlt,
+                                                                 unsigned long long* __restrict__ flags,
+                                                                 uint32_t stride) {
+  if (P.iter > 0 && flags[(uint64_t)(P.iter - 1) * stride] == 0ull) return;
   const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
   const float thr = (float)P.threshold;
   bool pushed = false;
@@ -307,7 +310,7 @@ __global__ __launch_bounds__(kThreads) void pr_delta_push_kernel(SweepParams P, 
     }
     dlt[i] = dv;
   }
-  if (__syncthreads_or(pushed) && threadIdx.x == 0) P.acc[P.iter] = 1ull;
+  if (__syncthreads_or(pushed) && threadIdx.x == 0) flags[(uint64_t)P.iter * stride] = 1ull;
 }
 
 // personalised base (1-d) * v rounded once from double; with a relabelled CSC row i
@@ -696,26 +699,79 @@ static void enqueue_sweep(hipStream_t s, const SweepParams& P) {
   NWG_LAUNCH_CHECK();
 }
 
+// Device parameters of the binned sweep over g's plan (contribution arrays set per sweep).
+static BinParams make_bin_params(nwg_graph* g, PrState& st, const SweepParams& P) {
+  const BinnedLayout& L = g->plan.bl;
+  BinParams B{};
+  B.brow = L.brow.p;
+  B.bsrc = L.bsrc.p;
+  B.bpid = L.bpid.p;
+  B.vals = st.bvals.p;
+  B.items2 = L.items2.p;
+  B.items1 = L.items1.p;
+  B.seg_start = L.seg_start.p;
+  B.seg_flat = L.seg_flat.p;
+  B.part = st.bpart.p;
+  B.items3 = L.items3.p;
+  B.n_items3 = L.n_items3;
+  B.deg = P.deg;
+  B.x = P.x;
+  B.hot_end = L.hot_end.p;
+  B.n_hot = L.n_hot;
+  B.basev = P.basev;
+  B.base = P.base;
+  B.damp = P.damp;
+  B.n = g->n;
+  B.n_items2 = L.n_items2;
+  B.n_items1 = L.n_items1;
+  B.acc = st.bacc.p;
+  B.threshold = P.threshold;
+  return B;
+}
+
 static void enqueue_solve(hipStream_t s, nwg_graph* g, PrState& st, SweepParams P, uint64_t iters,
                           hipEvent_t* ev, PrSchedule sched) {
   const uint32_t n = g->n;
   const uint32_t nt = g->plan.params.stream_hints;
+  // In place, the binned sweep reads and writes one contribution array: its chunk pass stages
+  // every non-hot contribution before any row of the sweep is finished, so updates made in
+  // the sweep reach other rows only through the hot sources, which each bin-pass work item
+  // stages when it starts.  The delta schedule keeps the pull kernels: the binned sweep's
+  // exact fixed-point row sums cover [0, 16), enough for PageRank (a row sum is at most the
+  // total score, 1) but not for un-normalised residuals, which reach thousands on hub rows.
+  const bool binned = g->plan.params.binned != 0 && sched != PrSchedule::kDelta;
+  BinParams B{};
+  if (binned) {
+    B = make_bin_params(g, st, P);
+    binned_clear_residuals(s, st.bacc.p, iters);
+  }
+  auto sweep = [&](const float* cin, float* cout) {
+    if (binned) {
+      B.iter = P.iter;
+      B.cin = cin;
+      B.cout = cout;
+      binned_enqueue_sweep(s, B);
+      return;
+    }
+    P.cin = cin;
+    P.cout = cout;
+    if (nt == 2) enqueue_sweep<2>(s, P);
+    else if (nt == 1) enqueue_sweep<1>(s, P);
+    else enqueue_sweep<0>(s, P);
+  };
   if (sched == PrSchedule::kDelta) {
-    // page_rank_v13: contrib0 holds the deltas, contrib1 the residuals, x the scores
+    // page_rank_v13: contrib0 holds the deltas, contrib1 the residuals, x the scores; the
+    // push kernel's flags go to acc, where the pull kernels' stopping test reads them
     P.delta = 1;
-    P.cin = st.contrib0.p;
-    P.cout = st.contrib1.p;
     pr_delta_init_kernel<<<blocks_for(std::max<uint64_t>(n, iters)), kThreads, 0, s>>>(
         n, (float)(1.0 - (double)P.damp), st.x.p, st.contrib0.p, st.contrib1.p, st.acc.p, iters);
     NWG_LAUNCH_CHECK();
     for (uint64_t k = 0; k < iters; ++k) {
       P.iter = (uint32_t)k;
       if (ev) NWG_HIP(hipEventRecord(ev[2 * k], s));
-      pr_delta_push_kernel<<<blocks_for(n), kThreads, 0, s>>>(P, st.contrib1.p, st.contrib0.p);
+      pr_delta_push_kernel<<<blocks_for(n), kThreads, 0, s>>>(P, st.contrib1.p, st.contrib0.p, st.acc.p, 1u);
       NWG_LAUNCH_CHECK();
-      if (nt == 2) enqueue_sweep<2>(s, P);
-      else if (nt == 1) enqueue_sweep<1>(s, P);
-      else enqueue_sweep<0>(s, P);
+      sweep(st.contrib0.p, st.contrib1.p);
       if (ev) NWG_HIP(hipEventRecord(ev[2 * k + 1], s));
     }
     return;
@@ -725,60 +781,14 @@ static void enqueue_solve(hipStream_t s, nwg_graph* g, PrState& st, SweepParams 
   pr_init_kernel<<<blocks_for(std::max<uint64_t>(n, iters)), kThreads, 0, s>>>(n, P.deg, init, st.x.p,
                                                                                st.contrib0.p, st.acc.p, iters);
   NWG_LAUNCH_CHECK();
-  const BinnedLayout& L = g->plan.bl;
-  // In place, the binned sweep reads and writes one contribution array: its chunk pass stages
-  // every non-hot contribution before any row of the sweep is finished, so updates made in
-  // the sweep reach other rows only through the hot sources, which each bin-pass work item
-  // stages when it starts.
-  const bool binned = g->plan.params.binned != 0;
-  BinParams B{};
-  if (binned) {
-    // contributions double-buffered like the pull sweep's: the hot tiles of the bin pass
-    // read the previous sweep's while the row updates write the next
-    B.brow = L.brow.p;
-    B.bsrc = L.bsrc.p;
-    B.bpid = L.bpid.p;
-    B.vals = st.bvals.p;
-    B.items2 = L.items2.p;
-    B.items1 = L.items1.p;
-    B.seg_start = L.seg_start.p;
-    B.seg_flat = L.seg_flat.p;
-    B.part = st.bpart.p;
-    B.items3 = L.items3.p;
-    B.n_items3 = L.n_items3;
-    B.deg = P.deg;
-    B.x = P.x;
-    B.hot_end = L.hot_end.p;
-    B.n_hot = L.n_hot;
-    B.basev = P.basev;
-    B.base = P.base;
-    B.damp = P.damp;
-    B.n = n;
-    B.n_items2 = L.n_items2;
-    B.n_items1 = L.n_items1;
-    B.acc = st.bacc.p;
-    B.threshold = P.threshold;
-    binned_clear_residuals(s, st.bacc.p, iters);
-  }
   for (uint64_t k = 0; k < iters; ++k) {
     P.iter = (uint32_t)k;
+    if (ev) NWG_HIP(hipEventRecord(ev[2 * k], s));
     // in place (page_rank.hpp:135): one contribution array, each row's new contribution is
     // visible to the rows gathered after it; else double-buffered (Jacobi)
-    P.cin = inplace ? st.contrib0.p : (k & 1) ? st.contrib1.p : st.contrib0.p;
-    P.cout = inplace ? st.contrib0.p : (k & 1) ? st.contrib0.p : st.contrib1.p;
-    if (ev) NWG_HIP(hipEventRecord(ev[2 * k], s));
-    if (binned) {
-      B.iter = (uint32_t)k;
-      B.cin = P.cin;
-      B.cout = P.cout;
-      binned_enqueue_sweep(s, B);
-    } else if (nt == 2) {
-      enqueue_sweep<2>(s, P);
-    } else if (nt == 1) {
-      enqueue_sweep<1>(s, P);
-    } else {
-      enqueue_sweep<0>(s, P);
-    }
+    if (inplace) sweep(st.contrib0.p, st.contrib0.p);
+    else if (k & 1) sweep(st.contrib1.p, st.contrib0.p);
+    else sweep(st.contrib0.p, st.contrib1.p);
     if (ev) NWG_HIP(hipEventRecord(ev[2 * k + 1], s));
   }
   if (binned) binned_fold_residuals(s, st.bacc.p, st.acc.p, iters);
