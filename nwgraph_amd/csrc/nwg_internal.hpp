@@ -262,7 +262,8 @@ struct PprState {
   std::vector<uint32_t> slice_off;
   DevBuf<uint32_t> lrp, lcol;  // light rows' in-edges, compacted (heavy rows empty)
   DevBuf<uint32_t> degflag, seed_row, seed_vec, ids, ids_new;
-  DevBuf<float> seed_base, c0, c1, x, hub_part;
+  DevBuf<uint32_t> its;  // sweeps each vector executed (read by the final-scores kernel)
+  DevBuf<float> seed_base, c0, c1, hub_part;
   DevBuf<double> blk_err;
   DevBuf<unsigned long long> acc;
 };

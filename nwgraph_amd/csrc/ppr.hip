@@ -3,12 +3,14 @@
 // Each vector j is the reference's PageRank loop (page_rank_v10 Jacobi schedule,
 // include/nwgraph/experimental/algorithms/page_rank.hpp:377-432) with the uniform
 // base (1-d)/N replaced by (1-d) v_j, v_j uniform over a small seed set, and its own
-// stopping rule (first sweep with L1 change < threshold).  Scores and contributions
-// are row-major N x k, so one in-edge u of row i gathers the contiguous k-float
-// T-float slice C[tile][u][0..T) of the k-tile being served (T = 8/16/32/64): a T-lane
-// slice of a wavefront loads it in one coalesced access.  Contributions are tile-major and
-// workgroup b serves tile b % tiles, so with tiles dividing 8 each XCD's L2 caches one
-// tile's contributions (T floats per source) instead of all k.
+// stopping rule (first sweep with L1 change < threshold).  Contributions are tile-major,
+// so one in-edge u of row i gathers the contiguous T-float slice C[tile][u][0..T) of the
+// k-tile being served (T = 8/16/32/64): a T-lane slice of a wavefront loads it in one
+// coalesced access.  Workgroup b serves tile b % tiles, so with tiles dividing 8 each
+// XCD's L2 caches one tile's contributions (T floats per source) instead of all k.
+// Scores are not kept during the solve: a row's old score is read back from its previous
+// contribution (x / deg, or x itself for a row without out-edges), and the row-major N x k
+// scores are written once at the end.
 //
 // Work split per sweep (one launch + one finishing launch):
 //   row workgroups   (256/T) lane groups x 8 (T = 8: 4) consecutive rows x one k-tile; a
@@ -63,7 +65,6 @@ struct PprParams {
   uint32_t n_seed;
   const float* cin;
   float* cout;
-  float* x;
   uint32_t n, k;
   uint32_t tiles, kt_shift;  // k-tiles and log2 of their width
   float damp;
@@ -111,13 +112,22 @@ __device__ __forceinline__ uint64_t cidx(const PprParams& P, uint32_t i, uint32_
   return (((uint64_t)(j >> P.kt_shift) * P.n + i) << P.kt_shift) + (j & ((1u << P.kt_shift) - 1));
 }
 
-// update of row i, vector j from its sum z, its degree/seed word df and x_old xo
+// Scores are not stored during the solve: the contribution array holds x / deg for a row with
+// out-edges and x itself for a row without (no in-edge ever gathers it), and a row's old score
+// is read back from the contributions of the previous sweep.
+__device__ __forceinline__ float ppr_score(float c, uint32_t df) {
+  const uint32_t d = df & ~kSeedFlag;
+  return d ? c * (float)d : c;
+}
+
+// update of row i, vector j from its sum z, its degree/seed word df and x_old xo (the score
+// read back from the previous sweep's contribution, within an ulp of the score)
 __device__ __forceinline__ double ppr_finish_row(const PprParams& P, uint32_t i, uint32_t j, float z, uint32_t df,
                                                  float xo) {
   const float b = (df & kSeedFlag) ? seed_base(P, i, j) : 0.f;
   const float xn = __builtin_fmaf(P.damp, z, b);
-  P.x[(uint64_t)i * P.k + j] = xn;
-  P.cout[cidx(P, i, j)] = xn / (float)(df & ~kSeedFlag);
+  const uint32_t d = df & ~kSeedFlag;
+  P.cout[cidx(P, i, j)] = d ? xn / (float)d : xn;
   return (double)fabsf(xn - xo);
 }
 
@@ -230,13 +240,13 @@ __global__ __launch_bounds__(kThreads) void ppr_sweep_kernel(PprParams P) {
         if (light[t]) {
           const uint32_t i = g0 + t;
           df[t] = P.degflag[i];
-          xo[t] = P.x[(uint64_t)i * P.k + j];
+          xo[t] = P.cin[cidx(P, i, j)];
           zz[t] = zbuf[(slot * kR + t) * T + vl];
         }
       }
 #pragma unroll
       for (uint32_t t = 0; t < kR; ++t)
-        if (light[t]) e += ppr_finish_row(P, g0 + t, j, zz[t], df[t], xo[t]);
+        if (light[t]) e += ppr_finish_row(P, g0 + t, j, zz[t], df[t], ppr_score(xo[t], df[t]));
     }
     e = slot_reduce<T>(e, red);
     if (threadIdx.x < (uint32_t)T) P.blk_err[((uint64_t)tile * P.row_blocks + b) * T + threadIdx.x] = e;
@@ -310,7 +320,7 @@ __global__ __launch_bounds__(kThreads) void ppr_finish_kernel(PprParams P, uint3
       if (ok[u]) {
         z[u] = P.hub_part[(uint64_t)s0[u] * P.k + jv[u]];
         df[u] = P.degflag[row[u]];
-        xo[u] = P.x[(uint64_t)row[u] * P.k + jv[u]];
+        xo[u] = P.cin[cidx(P, row[u], jv[u])];
       }
     }
 #pragma unroll
@@ -326,7 +336,7 @@ __global__ __launch_bounds__(kThreads) void ppr_finish_kernel(PprParams P, uint3
         for (uint32_t r = 0; r < 32; ++r)
           if (s + r < s1[u]) z[u] += v[r];
       }
-      const double e = ppr_finish_row(P, row[u], jv[u], z[u], df[u], xo[u]);
+      const double e = ppr_finish_row(P, row[u], jv[u], z[u], df[u], ppr_score(xo[u], df[u]));
       if (e > 0.0) {
         if (jv[u] < kFinMaxK) atomicAdd(&errj[jv[u]], err_to_fixed(e));
         else atomicAdd(P.acc + (uint64_t)P.iter * P.k + jv[u], err_to_fixed(e));
@@ -365,12 +375,12 @@ __global__ __launch_bounds__(kThreads) void ppr_finish_kernel(PprParams P, uint3
 
 __global__ __launch_bounds__(kThreads) void ppr_init_kernel(uint32_t n, uint32_t k, uint32_t kt_shift,
                                                             const uint32_t* __restrict__ deg, float init,
-                                                            float* __restrict__ x, float* __restrict__ c0) {
+                                                            float* __restrict__ c0) {
   const uint64_t t = (uint64_t)blockIdx.x * kThreads + threadIdx.x;
   if (t < (uint64_t)n * k) {
     const uint32_t i = (uint32_t)(t / k), j = (uint32_t)(t % k);
-    x[t] = init;
-    c0[(((uint64_t)(j >> kt_shift) * n + i) << kt_shift) + (j & ((1u << kt_shift) - 1))] = init / (float)deg[i];
+    const uint32_t d = deg[i];
+    c0[(((uint64_t)(j >> kt_shift) * n + i) << kt_shift) + (j & ((1u << kt_shift) - 1))] = d ? init / (float)d : init;
   }
 }
 
@@ -402,16 +412,38 @@ __global__ __launch_bounds__(kThreads) void gather_u32_kernel(uint32_t m, const 
   if (q < m) out[q] = table[idx[q]];
 }
 
-// rows of the relabelled solve back to original ids: out[order[r]][:] = x[r][:]
-__global__ __launch_bounds__(kThreads) void unpermute_rows_kernel(uint32_t n, uint32_t k,
-                                                                  const uint32_t* __restrict__ order,
-                                                                  const float* __restrict__ x,
-                                                                  float* __restrict__ out) {
+// sweeps vector j executed: the first t with acc[t][j] below the threshold, plus one (or all)
+__global__ __launch_bounds__(kThreads) void ppr_its_kernel(uint32_t k, const unsigned long long* __restrict__ acc,
+                                                           uint64_t iters, double threshold,
+                                                           uint32_t* __restrict__ its) {
+  const uint32_t j = blockIdx.x * kThreads + threadIdx.x;
+  if (j >= k) return;
+  uint64_t n_it = iters;
+  for (uint64_t q = 0; q < iters; ++q)
+    if (fixed_to_err(acc[q * k + j]) < threshold) {
+      n_it = q + 1;
+      break;
+    }
+  its[j] = (uint32_t)n_it;
+}
+
+// Final scores from the contributions: vector j's last sweep t = its[j] - 1 wrote c0 when t
+// is odd, c1 when even (c0 before any sweep).  With a relabelled solve row r is original
+// vertex order[r].
+__global__ __launch_bounds__(kThreads) void ppr_scores_kernel(uint32_t n, uint32_t k, uint32_t kt_shift,
+                                                              const uint32_t* __restrict__ order,
+                                                              const uint32_t* __restrict__ degflag,
+                                                              const float* __restrict__ c0,
+                                                              const float* __restrict__ c1,
+                                                              const uint32_t* __restrict__ its,
+                                                              float* __restrict__ out) {
   const uint64_t t = (uint64_t)blockIdx.x * kThreads + threadIdx.x;
-  if (t < (uint64_t)n * k) {
-    const uint32_t r = (uint32_t)(t / k), j = (uint32_t)(t % k);
-    out[(uint64_t)order[r] * k + j] = x[t];
-  }
+  if (t >= (uint64_t)n * k) return;
+  const uint32_t r = (uint32_t)(t / k), j = (uint32_t)(t % k);
+  const uint32_t nj = its[j];
+  const float* c = nj == 0 ? c0 : ((nj - 1) & 1) ? c0 : c1;
+  const float v = c[(((uint64_t)(j >> kt_shift) * n + r) << kt_shift) + (j & ((1u << kt_shift) - 1))];
+  out[(uint64_t)(order ? order[r] : r) * k + j] = ppr_score(v, degflag[r]);
 }
 
 inline uint32_t grid1(uint64_t m) { return (uint32_t)ceil_div(m ? m : 1, kThreads); }
@@ -662,8 +694,6 @@ void ppr_run(nwg_ctx* ctx, nwg_graph* g, uint32_t k, const std::vector<uint32_t>
   const uint64_t nc = (uint64_t)tiles * n * tile;  // tile-major contributions, last tile padded
   st.c0.ensure(nc);
   st.c1.ensure(nc);
-  if (a.order) st.x.ensure(nk);
-  float* x = a.order ? st.x.p : scores_dev;
   st.hub_part.ensure((uint64_t)std::max<uint32_t>(n_segs, 1) * k);
   st.blk_err.ensure((uint64_t)row_blocks * tiles * tile);
   st.acc.ensure(std::max<uint64_t>(max_iters, 1) * k);
@@ -680,7 +710,6 @@ void ppr_run(nwg_ctx* ctx, nwg_graph* g, uint32_t k, const std::vector<uint32_t>
   P.seed_vec = st.seed_vec.p;
   P.seed_base = st.seed_base.p;
   P.n_seed = ns;
-  P.x = x;
   P.n = n;
   P.k = k;
   P.tiles = tiles;
@@ -710,7 +739,7 @@ void ppr_run(nwg_ctx* ctx, nwg_graph* g, uint32_t k, const std::vector<uint32_t>
 
   const float init = (float)(1.0 / n);
   NWG_HIP(hipEventRecord(ctx->ev0, s));
-  ppr_init_kernel<<<grid1(nk), kThreads, 0, s>>>(n, k, P.kt_shift, a.deg, init, x, c0p);
+  ppr_init_kernel<<<grid1(nk), kThreads, 0, s>>>(n, k, P.kt_shift, a.deg, init, c0p);
   NWG_LAUNCH_CHECK();
   ppr_zero_acc_kernel<<<grid1(max_iters * k), kThreads, 0, s>>>(st.acc.p, max_iters * k);
   NWG_LAUNCH_CHECK();
@@ -744,10 +773,11 @@ void ppr_run(nwg_ctx* ctx, nwg_graph* g, uint32_t k, const std::vector<uint32_t>
     NWG_LAUNCH_CHECK();
   }
   NWG_HIP(hipEventRecord(ctx->ev1, s));
-  if (a.order) {
-    unpermute_rows_kernel<<<grid1(nk), kThreads, 0, s>>>(n, k, a.order, x, scores_dev);
-    NWG_LAUNCH_CHECK();
-  }
+  st.its.ensure(k);
+  ppr_its_kernel<<<grid1(k), kThreads, 0, s>>>(k, st.acc.p, max_iters, (double)threshold, st.its.p);
+  ppr_scores_kernel<<<grid1(nk), kThreads, 0, s>>>(n, k, P.kt_shift, a.order, st.degflag.p, c0p, c1p, st.its.p,
+                                                   scores_dev);
+  NWG_LAUNCH_CHECK();
   std::vector<unsigned long long> h_acc(std::max<uint64_t>(max_iters, 1) * k);
   NWG_HIP(hipMemcpyAsync(h_acc.data(), st.acc.p, max_iters * k * 8ull, hipMemcpyDeviceToHost, s));
   NWG_HIP(hipStreamSynchronize(s));
