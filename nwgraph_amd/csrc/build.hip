@@ -87,10 +87,29 @@ __global__ __launch_bounds__(256) void fill_u32_kernel(uint32_t* __restrict__ p,
   if (i < n) p[i] = v;
 }
 
-__global__ __launch_bounds__(256) void degree_kernel(const uint32_t* __restrict__ col, uint64_t nnz,
-                                                     uint32_t* __restrict__ deg) {
-  const uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (p < nnz) atomicAdd(&deg[col[p]], 1u);
+// Out-degree histogram of the CSC's source ids.  Sources below kDegLds are counted in a
+// per-workgroup LDS histogram and flushed once per workgroup: on skewed graphs the hubs
+// take most entries, and a million global atomics on one hub's counter serialise.  (RMAT's
+// hubs have low ids; in the degree-ordered numbering every hub does.)  Each workgroup takes
+// a contiguous range of kDegPer entries.
+constexpr uint32_t kDegLds = 16384;
+constexpr uint32_t kDegThreads = 1024;
+constexpr uint32_t kDegPer = 1u << 20;
+__global__ __launch_bounds__(kDegThreads) void degree_kernel(const uint32_t* __restrict__ col, uint64_t nnz,
+                                                             uint32_t n, uint32_t* __restrict__ deg) {
+  __shared__ uint32_t h[kDegLds];
+  for (uint32_t q = threadIdx.x; q < kDegLds; q += kDegThreads) h[q] = 0u;
+  __syncthreads();
+  const uint64_t p0 = (uint64_t)blockIdx.x * kDegPer;
+  const uint64_t p1 = p0 + kDegPer < nnz ? p0 + kDegPer : nnz;
+  for (uint64_t p = p0 + threadIdx.x; p < p1; p += kDegThreads) {
+    const uint32_t u = col[p];
+    if (u < kDegLds) atomicAdd(&h[u], 1u);
+    else atomicAdd(&deg[u], 1u);
+  }
+  __syncthreads();
+  for (uint32_t q = threadIdx.x; q < kDegLds && q < n; q += kDegThreads)
+    if (h[q]) atomicAdd(&deg[q], h[q]);
 }
 
 // CSR keys from a CSC: (source << b) | target, payload = CSC position; one thread per CSC row.
@@ -129,11 +148,31 @@ void sorted_keys_to_csx(hipStream_t s, const uint64_t* keys, uint64_t m, uint32_
   NWG_LAUNCH_CHECK();
 }
 
+// flags a CSC given in device memory whose row pointers are not monotone from 0 to nnz or
+// whose entries are not below n (a plain store of 1: every flagging thread writes the same)
+__global__ __launch_bounds__(256) void csc_check_kernel(uint32_t n, uint64_t nnz, const uint32_t* __restrict__ ptr,
+                                                        const uint32_t* __restrict__ idx, uint32_t* __restrict__ bad) {
+  const uint64_t t = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+  bool b = false;
+  if (t < nnz) b = idx[t] >= n;
+  if (t < n) b = b || ptr[t] > ptr[t + 1];
+  if (t == 0) b = b || ptr[0] != 0u || ptr[n] != (uint32_t)nnz;
+  if (b) *bad = 1u;
+}
+
+void validate_device_csc(hipStream_t s, uint32_t n, uint64_t nnz, const uint32_t* ptr, const uint32_t* idx) {
+  DevBuf<uint32_t> bad(1);
+  fill_u32_kernel<<<1, 256, 0, s>>>(bad.p, 1, 0u);
+  csc_check_kernel<<<grid_for(std::max<uint64_t>(nnz, n)), 256, 0, s>>>(n, nnz, ptr, idx, bad.p);
+  NWG_LAUNCH_CHECK();
+  if (read_back_u32(s, bad.p)) throw Error(NWG_EINVAL, "device CSC: row pointers not monotone from 0 to nnz, or an entry >= n");
+}
+
 void out_degrees_device(hipStream_t s, uint32_t n, uint64_t nnz, const uint32_t* col, uint32_t* deg) {
   fill_u32_kernel<<<grid_for(n), 256, 0, s>>>(deg, n, 0u);
   NWG_LAUNCH_CHECK();
   if (nnz) {
-    degree_kernel<<<grid_for(nnz), 256, 0, s>>>(col, nnz, deg);
+    degree_kernel<<<(uint32_t)ceil_div(nnz, kDegPer), kDegThreads, 0, s>>>(col, nnz, n, deg);
     NWG_LAUNCH_CHECK();
   }
 }

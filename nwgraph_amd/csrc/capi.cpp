@@ -109,6 +109,8 @@ int nwg_graph_create_csc(nwg_ctx* ctx, uint32_t n, uint64_t nnz, const uint32_t*
     g->nnz = nnz;
     stage_in(s, g->csc_ptr, csc_ptr, (size_t)n + 1, mem_kind);
     stage_in(s, g->csc_idx, csc_idx, nnz, mem_kind);
+    // a device CSC is checked on the device before any kernel indexes by its entries
+    if (mem_kind != NWG_MEM_HOST) nwg::validate_device_csc(s, n, nnz, g->csc_ptr.p, g->csc_idx.p);
     if (outdeg) {
       stage_in(s, g->outdeg, outdeg, n, mem_kind);
     } else {

@@ -320,6 +320,8 @@ int key_bits_for(uint32_t n);  // bits of a vertex id field in packed sort keys
 // row pointers + entries from keys (row << b | entry) sorted ascending
 void sorted_keys_to_csx(hipStream_t s, const uint64_t* keys, uint64_t m, uint32_t n, int b, uint32_t* ptr,
                         uint32_t* ent);
+// throws NWG_EINVAL unless ptr is monotone from 0 to nnz and every entry of idx is below n
+void validate_device_csc(hipStream_t s, uint32_t n, uint64_t nnz, const uint32_t* ptr, const uint32_t* idx);
 void out_degrees_device(hipStream_t s, uint32_t n, uint64_t nnz, const uint32_t* col, uint32_t* deg);
 
 // ---- primitives (scan.hip / sort.hip) ----

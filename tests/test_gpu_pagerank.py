@@ -406,6 +406,38 @@ def test_delta_schedule_cycle_stops_like_the_closed_form(ctx):
     assert np.allclose(x, ref, rtol=1e-6)
 
 
+def test_device_csc_is_checked_and_degrees_match(ctx):
+    """A CSC handed over in device memory is checked on the device (an entry >= n or a
+    non-monotone row pointer is rejected before any kernel indexes by it), and the device
+    out-degree histogram equals numpy's bincount (hub ids below and above the LDS range)."""
+    import ctypes
+    import torch
+    from nwgraph_amd import _lib
+
+    n, ind, col, deg = O.rmat_csc(15, 16, seed=9)
+    dptr = torch.from_numpy(ind.astype(np.int32)).cuda()
+    didx = torch.from_numpy(col.astype(np.int32)).cuda()
+    h = ctypes.c_void_p()
+    rc = ctx.lib.nwg_graph_create_csc(ctx.handle, n, col.size, ctypes.c_void_p(dptr.data_ptr()),
+                                      ctypes.c_void_p(didx.data_ptr()), None, _lib.NWG_MEM_DEVICE, ctypes.byref(h))
+    assert rc == 0
+    g = ctx._wrap(h)
+    _, _, od = g.csc()
+    assert np.array_equal(od, np.bincount(col, minlength=n).astype(np.uint32))
+    g.close()
+    bad = didx.clone()
+    bad[12345] = n  # one entry out of range
+    h2 = ctypes.c_void_p()
+    rc = ctx.lib.nwg_graph_create_csc(ctx.handle, n, col.size, ctypes.c_void_p(dptr.data_ptr()),
+                                      ctypes.c_void_p(bad.data_ptr()), None, _lib.NWG_MEM_DEVICE, ctypes.byref(h2))
+    assert rc == _lib.NWG_EINVAL and not h2.value
+    badp = dptr.clone()
+    badp[7] = badp[9] + 1  # non-monotone row pointers
+    rc = ctx.lib.nwg_graph_create_csc(ctx.handle, n, col.size, ctypes.c_void_p(badp.data_ptr()),
+                                      ctypes.c_void_p(didx.data_ptr()), None, _lib.NWG_MEM_DEVICE, ctypes.byref(h2))
+    assert rc == _lib.NWG_EINVAL
+
+
 @pytest.mark.parametrize("kind", ["rmat", "grid"])
 def test_autotuned_plan_matches_v10(ctx, kind):
     """The autotuner's plan (pull or binned, whichever timed faster) keeps the v10 gate."""
