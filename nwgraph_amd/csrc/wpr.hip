@@ -13,7 +13,8 @@
 //   dL/dw~(u->k) = (dL/dw(u->k) - sum_j dL/dw(u->j) w(u->j)) / s_u.
 //
 // Sweep schedule: the rows are cut into work items of up to 256 consecutive rows with at
-// most kItemNnz nonzeros (values staged in LDS, one lane per row) or one longer row
+// most kItemNnz nonzeros (values staged in LDS; one lane per short row, one wave per row of
+// more than kLaneRow nonzeros) or one longer row
 // (whole-workgroup reduction); a persistent grid walks the items round-robin and adds
 // its L1 partial to acc[k] once per sweep.
 #include <hip/hip_runtime.h>
@@ -31,6 +32,8 @@ namespace {
 constexpr int kThreads = 256;
 constexpr uint32_t kItemNnz = 4096;
 constexpr uint32_t kItemRows = 256;
+constexpr uint32_t kGather = 8;    // gathers a lane issues before it uses them
+constexpr uint32_t kLaneRow = 16;  // longest row summed by one lane; longer rows take a wave
 
 struct WParams {
   const uint32_t* rp;
@@ -51,6 +54,8 @@ struct WParams {
 __global__ __launch_bounds__(kThreads) void wsweep_kernel(WParams P) {
   if (P.iter > 0 && fixed_to_err(P.acc[P.iter - 1]) < P.threshold) return;
   __shared__ float vals[kItemNnz];
+  __shared__ uint32_t lrows[kItemRows];   // an item's long rows (offsets from its first row)
+  __shared__ uint32_t wlong[kThreads / 64];
   __shared__ float red_f[kThreads / 64];
   __shared__ double red_d[kThreads / 64];
   double e = 0.0;
@@ -58,8 +63,24 @@ __global__ __launch_bounds__(kThreads) void wsweep_kernel(WParams P) {
     const uint32_t r0 = P.item_r0[it], r1 = P.item_r1[it];
     const uint32_t p0 = P.rp[r0], p1 = P.rp[r1];
     if (p1 - p0 > kItemNnz) {
+      // one long row: a lane's edges p0 + lane + t * kThreads, kGather gathers in flight,
+      // added in that order
       float z = 0.f;
-      for (uint32_t p = p0 + threadIdx.x; p < p1; p += kThreads) z = __builtin_fmaf(P.w[p], P.xin[P.col[p]], z);
+      for (uint32_t p = p0 + threadIdx.x; p < p1; p += kThreads * kGather) {
+        uint32_t c[kGather];
+        float wv[kGather], xv[kGather];
+#pragma unroll
+        for (uint32_t t = 0; t < kGather; ++t) {
+          const uint32_t q = p + t * kThreads;
+          c[t] = q < p1 ? P.col[q] : 0u;
+          wv[t] = q < p1 ? P.w[q] : 0.f;
+        }
+#pragma unroll
+        for (uint32_t t = 0; t < kGather; ++t) xv[t] = p + t * kThreads < p1 ? P.xin[c[t]] : 0.f;
+#pragma unroll
+        for (uint32_t t = 0; t < kGather; ++t)
+          if (p + t * kThreads < p1) z = __builtin_fmaf(wv[t], xv[t], z);
+      }
       z = block_sum<kThreads>(z, red_f);
       if (threadIdx.x == 0) {
         const float xn = __builtin_fmaf(P.damp, z, P.base[r0]);
@@ -67,16 +88,66 @@ __global__ __launch_bounds__(kThreads) void wsweep_kernel(WParams P) {
         P.xout[r0] = xn;
       }
     } else {
-      for (uint32_t q = threadIdx.x; q < p1 - p0; q += kThreads) vals[q] = P.w[p0 + q] * P.xin[P.col[p0 + q]];
+      // stage the item's products w * x[col], kGather gathers of a lane in flight
+      const uint32_t m = p1 - p0;
+      for (uint32_t q0 = threadIdx.x; q0 < m; q0 += kThreads * kGather) {
+        uint32_t c[kGather];
+        float wv[kGather], xv[kGather];
+#pragma unroll
+        for (uint32_t t = 0; t < kGather; ++t) {
+          const uint32_t q = q0 + t * kThreads;
+          c[t] = q < m ? P.col[p0 + q] : 0u;
+          wv[t] = q < m ? P.w[p0 + q] : 0.f;
+        }
+#pragma unroll
+        for (uint32_t t = 0; t < kGather; ++t) xv[t] = q0 + t * kThreads < m ? P.xin[c[t]] : 0.f;
+#pragma unroll
+        for (uint32_t t = 0; t < kGather; ++t)
+          if (q0 + t * kThreads < m) vals[q0 + t * kThreads] = wv[t] * xv[t];
+      }
       __syncthreads();
+      // rows of up to kLaneRow entries: one lane each, in row order; longer rows: one wave
+      // each (lane-strided partial sums, then a butterfly), listed in row order through a
+      // ballot so that every row's sum is computed the same way on every run
       const uint32_t i = r0 + threadIdx.x;
+      uint32_t a = 0, b = 0;
       if (i < r1) {
-        const uint32_t a = P.rp[i] - p0, b = P.rp[i + 1] - p0;
+        a = P.rp[i] - p0;
+        b = P.rp[i + 1] - p0;
+      }
+      const bool longrow = b - a > kLaneRow;
+      if (i < r1 && !longrow) {
         float z = 0.f;
         for (uint32_t q = a; q < b; ++q) z += vals[q];
         const float xn = __builtin_fmaf(P.damp, z, P.base[i]);
         e += (double)fabsf(xn - P.xin[i]);
         P.xout[i] = xn;
+      }
+      const uint32_t lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+      const unsigned long long lm = __ballot(longrow);
+      if (lane == 0) wlong[wv] = (uint32_t)__popcll(lm);
+      __syncthreads();
+      uint32_t off = 0, nlong = 0;
+      for (uint32_t q = 0; q < kThreads / 64; ++q) {
+        off += q < wv ? wlong[q] : 0u;
+        nlong += wlong[q];
+      }
+      if (longrow)
+        lrows[off + __builtin_amdgcn_mbcnt_hi((uint32_t)(lm >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)lm, 0u))] =
+            threadIdx.x;
+      __syncthreads();
+      for (uint32_t k = wv; k < nlong; k += kThreads / 64) {
+        const uint32_t ri = r0 + lrows[k];
+        const uint32_t la = P.rp[ri] - p0, lb = P.rp[ri + 1] - p0;
+        float z = 0.f;
+        for (uint32_t q = la + lane; q < lb; q += 64) z += vals[q];
+#pragma unroll
+        for (int o = 32; o >= 1; o >>= 1) z += __shfl_xor(z, o, 64);
+        if (lane == 0) {
+          const float xn = __builtin_fmaf(P.damp, z, P.base[ri]);
+          e += (double)fabsf(xn - P.xin[ri]);
+          P.xout[ri] = xn;
+        }
       }
       __syncthreads();
     }
@@ -128,6 +199,8 @@ __global__ __launch_bounds__(kThreads) void csr_rowsum_kernel(const uint32_t* __
                                                               const uint32_t* __restrict__ item_r1, uint32_t n_items,
                                                               float* __restrict__ out) {
   __shared__ float vals[kItemNnz];
+  __shared__ uint32_t lrows[kItemRows];   // an item's long rows (offsets from its first row)
+  __shared__ uint32_t wlong[kThreads / 64];
   __shared__ float red_f[kThreads / 64];
   for (uint32_t it = blockIdx.x; it < n_items; it += gridDim.x) {
     const uint32_t r0 = item_r0[it], r1 = item_r1[it];
@@ -150,11 +223,46 @@ __global__ __launch_bounds__(kThreads) void csr_rowsum_kernel(const uint32_t* __
         vals[q] = b ? a[e] * b[e] : a[e];
       }
       __syncthreads();
+      // rows of up to kLaneRow entries: one lane each; longer rows: one wave each (lane-strided
+      // compensated partial sums, butterfly sums of the partials and of their compensations),
+      // listed in row order through a ballot
       const uint32_t u = r0 + threadIdx.x;
+      uint32_t ra = 0, rb = 0;
       if (u < r1) {
+        ra = rp[u] - p0;
+        rb = rp[u + 1] - p0;
+      }
+      const bool longrow = rb - ra > kLaneRow;
+      if (u < r1 && !longrow) {
         Kahan k;
-        for (uint32_t q = rp[u] - p0; q < rp[u + 1] - p0; ++q) k.add(vals[q]);
+        for (uint32_t q = ra; q < rb; ++q) k.add(vals[q]);
         out[u] = k.s - k.c;
+      }
+      const uint32_t lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+      const unsigned long long lm = __ballot(longrow);
+      if (lane == 0) wlong[wv] = (uint32_t)__popcll(lm);
+      __syncthreads();
+      uint32_t off = 0, nlong = 0;
+      for (uint32_t q = 0; q < kThreads / 64; ++q) {
+        off += q < wv ? wlong[q] : 0u;
+        nlong += wlong[q];
+      }
+      if (longrow)
+        lrows[off + __builtin_amdgcn_mbcnt_hi((uint32_t)(lm >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)lm, 0u))] =
+            threadIdx.x;
+      __syncthreads();
+      for (uint32_t k2 = wv; k2 < nlong; k2 += kThreads / 64) {
+        const uint32_t ri = r0 + lrows[k2];
+        const uint32_t la = rp[ri] - p0, lb = rp[ri + 1] - p0;
+        Kahan k;
+        for (uint32_t q = la + lane; q < lb; q += 64) k.add(vals[q]);
+        float hi = k.s, lo = -k.c;
+#pragma unroll
+        for (int o = 32; o >= 1; o >>= 1) {
+          hi += __shfl_xor(hi, o, 64);
+          lo += __shfl_xor(lo, o, 64);
+        }
+        if (lane == 0) out[ri] = hi + lo;
       }
     }
     __syncthreads();
