@@ -159,6 +159,50 @@ void page_rank(const Graph& graph, const std::vector<typename Graph::vertex_id_t
                 damping_factor, threshold, max_iters);
 }
 
+// Same signature as the reference's Jacobi twin page_rank_v10
+// (experimental/algorithms/page_rank.hpp:377-432): the device path of page_rank above is
+// this schedule already.
+template <csc_layout Graph, typename Real>
+void page_rank_v10(const Graph& graph, const std::vector<typename Graph::vertex_id_type>& degrees,
+                   std::vector<Real>& page_rank, Real damping_factor, Real threshold, size_t max_iters,
+                   size_t num_threads) {
+  hip::page_rank(graph, degrees, page_rank, damping_factor, threshold, max_iters, num_threads);
+}
+
+// Same signature as the reference's delta PageRank page_rank_v13
+// (experimental/algorithms/page_rank.hpp:560-625): un-normalised scores (about N in total).
+// The graph lives for this call: it runs on the pull kernels (nwg_graph_pagerank_delta).
+template <csc_layout Graph, typename Real>
+void page_rank_v13(const Graph& graph, const std::vector<typename Graph::vertex_id_type>& degrees,
+                   std::vector<Real>& page_rank, Real damping_factor, Real threshold, size_t max_iters,
+                   size_t /*num_threads*/) {
+  static_assert(std::is_same_v<Real, float>, "the MI355X path computes float scores");
+  static_assert(sizeof(typename Graph::vertex_id_type) == 4, "vertex ids must be 32-bit");
+  static_assert(sizeof(graph.indices_[0]) == 4, "row offsets must be 32-bit");
+  const auto& src = std::get<0>(graph.to_be_indexed_);
+  const size_t n = graph.indices_.size() - 1;
+  if (page_rank.size() != n) throw std::invalid_argument("page_rank must be pre-sized to graph.size()");
+  if (!degrees.empty() && degrees.size() != n) throw std::invalid_argument("degrees must have graph.size() entries");
+  nwg_ctx* ctx = context::thread_default().get();
+  nwg_graph* g = nullptr;
+  check(ctx,
+        nwg_graph_create_csc(ctx, (uint32_t)n, src.size(), reinterpret_cast<const uint32_t*>(graph.indices_.data()),
+                             reinterpret_cast<const uint32_t*>(src.data()),
+                             degrees.empty() ? nullptr : reinterpret_cast<const uint32_t*>(degrees.data()),
+                             NWG_MEM_HOST, &g),
+        "nwg_graph_create_csc");
+  nwg_pr_info info{};
+  const int rc = nwg_graph_pagerank_delta(ctx, g, page_rank.data(), damping_factor, threshold, max_iters,
+                                          NWG_MEM_HOST, &info);
+  nwg_graph_destroy(g);
+  check(ctx, rc, "nwg_graph_pagerank_delta");
+  if (trace_enabled()) {
+    detail::trace_row("iter", "error", "time", "outgoing");
+    const double per = info.iterations ? info.ms_per_iter * 1e-3 : 0.0;
+    for (uint64_t k = 0; k < info.iterations; ++k) detail::trace_iter(k, 0.0, per);
+  }
+}
+
 }  // namespace hip
 
 #ifdef NW_GRAPH_ADJACENCY_HPP
@@ -174,6 +218,11 @@ inline void page_rank(const adjacency<1>& graph, const std::vector<adjacency<1>:
 #ifndef NW_GRAPH_PAGE_RANK_HPP
 // Without the reference's algorithm header, nw::graph::page_rank is the GPU path.
 using hip::page_rank;
+#endif
+#ifndef NW_GRAPH_PAGE_RANK_EXPERIMENTAL_HPP
+// Without the reference's experimental header, its v10 and v13 are the GPU paths too.
+using hip::page_rank_v10;
+using hip::page_rank_v13;
 #endif
 
 }  // namespace graph

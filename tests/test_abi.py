@@ -59,7 +59,8 @@ def test_ppr_seed_generator_matches_oracle():
 
 
 def test_cpp_dropin_header_compiles(tmp_path):
-    """The C++ drop-in compiles against a graph with adjacency<1>'s layout (no GPU needed)."""
+    """The C++ drop-ins (page_rank and the experimental page_rank_v10 / page_rank_v13
+    signatures) compile against a graph with adjacency<1>'s layout (no GPU needed)."""
     import shutil
     import subprocess
 
@@ -75,6 +76,8 @@ def test_cpp_dropin_header_compiles(tmp_path):
         "  std::vector<uint32_t> deg = {1, 1};\n"
         "  std::vector<float> pr(2);\n"
         "  if (false) nw::graph::page_rank(g, deg, pr, 0.85f, 1e-4f, 20, 1);\n"
+        "  if (false) nw::graph::page_rank_v10(g, deg, pr, 0.85f, 1e-4f, 20, 1);\n"
+        "  if (false) nw::graph::page_rank_v13(g, deg, pr, 0.85f, 1e-4f, 20, 1);\n"
         "  return (int)g.size() - 2;\n"
         "}\n")
     r = subprocess.run(["g++", "-std=c++20", "-fsyntax-only", "-I" + os.path.join(ROOT, "include"), str(src)],
