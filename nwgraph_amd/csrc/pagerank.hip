@@ -323,10 +323,12 @@ __global__ __launch_bounds__(kThreads) void pr_base_kernel(uint32_t n, const flo
 }
 
 // scores in original ids from the relabelled solve: out[order[i]] = x[i]
-__global__ __launch_bounds__(kThreads) void pr_unpermute_kernel(uint32_t n, const uint32_t* __restrict__ order,
+// scores back to the original numbering: out[v] = x[rank[v]] (rank: original -> new id).
+// A gather: the writes are whole lines, and the reads of x come mostly from MALL.
+__global__ __launch_bounds__(kThreads) void pr_unpermute_kernel(uint32_t n, const uint32_t* __restrict__ rank,
                                                                 const float* __restrict__ x, float* __restrict__ out) {
-  const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
-  if (i < n) out[order[i]] = x[i];
+  const uint32_t v = blockIdx.x * kThreads + threadIdx.x;
+  if (v < n) out[v] = x[rank[v]];
 }
 
 // ---------------------------------------------------------------------------
@@ -884,7 +886,7 @@ void pagerank_run(nwg_ctx* ctx, nwg_graph* g, const float* teleport_dev, float d
   if (scores_dev) {
     const uint32_t* order = g->active(g->plan.params.relabel != 0).order;
     if (order) {
-      pr_unpermute_kernel<<<blocks_for(n), kThreads, 0, s>>>(n, order, st.x.p, scores_dev);
+      pr_unpermute_kernel<<<blocks_for(n), kThreads, 0, s>>>(n, g->rl->rank.p, st.x.p, scores_dev);
       NWG_LAUNCH_CHECK();
     } else {
       NWG_HIP(hipMemcpyAsync(scores_dev, st.x.p, (size_t)n * sizeof(float), hipMemcpyDeviceToDevice, s));
