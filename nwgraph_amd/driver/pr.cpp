@@ -66,7 +66,8 @@ struct Args {
                "      -v, --verify        verify results\n"
                "      -V, --verbose       run in verbose mode\n"
                "      --autotune          choose the sweep plan on the device first (not in the reference)\n"
-               "  Versions: 10 = Jacobi schedule (reproducible), 11 = in-place schedule (the reference's default)\n";
+               "  Versions: 10 = Jacobi schedule (reproducible), 11 = in-place schedule (the reference's default),\n"
+               "            13 = delta PageRank (un-normalised scores)\n";
   std::exit(rc);
 }
 
