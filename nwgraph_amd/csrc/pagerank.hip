@@ -322,7 +322,6 @@ __global__ __launch_bounds__(kThreads) void pr_base_kernel(uint32_t n, const flo
   if (i < n) base[i] = (float)(omd * (double)v[order ? order[i] : i]);
 }
 
-// scores in original ids from the relabelled solve: out[order[i]] = x[i]
 // scores back to the original numbering: out[v] = x[rank[v]] (rank: original -> new id).
 // A gather: the writes are whole lines, and the reads of x come mostly from MALL.
 __global__ __launch_bounds__(kThreads) void pr_unpermute_kernel(uint32_t n, const uint32_t* __restrict__ rank,
