@@ -261,6 +261,55 @@ def test_edge_cases(ctx, plan):
     assert info4["iterations"] == 30 and np.max(np.abs(x4 - x64)) <= 1e-6
 
 
+@pytest.mark.parametrize("plan", [{"binned": 0}, {"binned": 0, "relabel": 1}, {"binned": 1},
+                                  {"binned": 1, "relabel": 1}])
+def test_closed_forms_and_invariances(ctx, plan):
+    """What the mathematics fixes, on every sweep kind: a regular digraph keeps x = 1/N, a
+    Jacobi sweep without dangling vertices keeps the total score 1, a vertex without
+    in-edges gets exactly base and no score falls below it, the input edge order does not
+    matter (bit-identical), and renaming the vertices renames the scores."""
+    base1 = lambda n: np.float32((1.0 - np.float64(np.float32(D))) / n)
+    # directed cycle and complete digraph: in- and out-degree equal everywhere
+    for n, s, d in [(1000, np.arange(1000), (np.arange(1000) + 1) % 1000),
+                    (64, np.repeat(np.arange(64), 64), np.tile(np.arange(64), 64))]:
+        g = with_plan(ctx.graph_from_edges(n, s, d), plan)
+        x, info = g.pagerank(D, 0.0, 20)
+        assert info["iterations"] == 20
+        assert np.max(np.abs(x.astype(np.float64) * n - 1.0)) < 1e-6, n
+    # no dangling vertices (a Hamiltonian cycle plus random edges): sum of scores stays 1
+    rng = np.random.default_rng(21)
+    n = 3001
+    s = np.concatenate([np.arange(n), rng.integers(0, n, 40000)])
+    d = np.concatenate([(np.arange(n) + 1) % n, rng.integers(0, n, 40000)])
+    g = with_plan(ctx.graph_from_edges(n, s, d), plan)
+    _, _, deg = g.csc()
+    assert np.all(deg > 0)
+    x, _ = g.pagerank(D, 0.0, 20)
+    assert abs(x.sum(dtype=np.float64) - 1.0) < 1e-5
+    assert np.all(x >= base1(n))
+    # an RMAT graph: isolated and source-only vertices, hubs, dangling vertices
+    scale = 12
+    n = 1 << scale
+    s, d = ctx.rmat_edges(scale, 0, 16 * n, seed=17)
+    g = with_plan(ctx.graph_from_edges(n, s, d), plan)
+    ptr, _, _ = g.csc()
+    x, info = g.pagerank(D, 1e-4, 20)
+    no_in = np.diff(ptr.astype(np.int64)) == 0
+    assert no_in.any() and np.all(x[no_in] == base1(n)) and np.all(x >= base1(n))
+    # the same edges in another order: the same CSC, bit-identical scores
+    p = rng.permutation(s.size)
+    gp = with_plan(ctx.graph_from_edges(n, s[p], d[p]), plan)
+    xp, infop = gp.pagerank(D, 1e-4, 20)
+    assert infop["iterations"] == info["iterations"] and np.array_equal(xp, x)
+    # vertices renamed by a permutation pi: score of pi(v) = score of v up to FP32 summation order
+    pi = rng.permutation(n).astype(np.uint32)
+    gr = with_plan(ctx.graph_from_edges(n, pi[s], pi[d]), plan)
+    xr, infor = gr.pagerank(D, 0.0, 20)
+    x20, _ = g.pagerank(D, 0.0, 20)
+    assert infor["iterations"] == 20
+    assert np.max(np.abs(xr[pi].astype(np.float64) - x20) / x20) < 1e-5
+
+
 def test_repeat_calls_bit_identical_and_timing_mode(ctx):
     g = ctx.rmat(15, 16, seed=6)
     a, ia = g.pagerank(D, 1e-4, 20)
