@@ -92,6 +92,34 @@ def test_ppr_reuses_state_across_calls_and_plans(ctx):
     check(s5, range(5))
 
 
+def test_ppr_full_size_linearity_rmat22(ctx):
+    """k=64 on RMAT-22, 20 sweeps.  A solve is affine in the teleport vector and every vector
+    starts from the same 1/N, so the vector over A u B (16 seeds, 1/16 each) must be the mean
+    of the vectors over A and over B (8 seeds each): 21 such triples, within 1e-5 in L1.  One
+    vector is also checked against the oracle on the device-built CSC."""
+    import os
+
+    g = ctx.rmat(22, 16, seed=1)
+    n = g.n
+    rng = np.random.default_rng(22)
+    ids = rng.choice(n, 21 * 16 + 1, replace=False).astype(np.uint32)
+    seeds = []
+    for t in range(21):
+        a, b = np.sort(ids[16 * t:16 * t + 8]), np.sort(ids[16 * t + 8:16 * t + 16])
+        seeds += [a, b, np.sort(np.concatenate([a, b]))]
+    seeds.append(ids[-1:])
+    X, its, _ = g.ppr(seeds, D, 0.0, 20)
+    assert X.shape == (n, 64) and np.all(its == 20)
+    for t in range(21):
+        xa, xb, xab = (X[:, 3 * t + q].astype(np.float64) for q in range(3))
+        assert np.abs(xab - 0.5 * (xa + xb)).sum() < 1e-5, t
+    ptr, idx, deg = g.csc()
+    j = 63
+    ref, it, _, _ = O.pr_v10(n, ptr, idx, deg, D, 0.0, 20, teleport=teleport(n, seeds[j]),
+                             nthreads=min(16, os.cpu_count() or 1))
+    assert it == 20 and np.max(np.abs(X[:, j].astype(np.float64) - ref)) <= 1e-6
+
+
 def test_ppr_rejects_bad_seeds(ctx):
     from nwgraph_amd import NwgError
 
