@@ -56,6 +56,23 @@ def host_threads():
     return max(1, min(ncpu, 16))
 
 
+def host_cpu():
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except AttributeError:
+        usable = os.cpu_count() or 1
+    return {"model": model, "logical_cpus": os.cpu_count() or 1, "usable_cpus": usable}
+
+
 def cpu_baseline(ptr, idx, deg, sweeps):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle as O
@@ -64,6 +81,8 @@ def cpu_baseline(ptr, idx, deg, sweeps):
     n = ptr.size - 1
     _, it, _, sec = O.pr_v11_parallel(n, ptr, idx, deg, 0.85, 0.0, sweeps, threads)
     t = float(np.median(sec))
+    # one sweep on one thread: the sequential Gauss-Seidel form of the same loop
+    _, _, _, sec1 = O.pr_v11_parallel(n, ptr, idx, deg, 0.85, 0.0, 1, 1)
     return {
         "value": round(idx.size / t / 1e9, 4),
         "unit": "GTEPS",
@@ -72,6 +91,10 @@ def cpu_baseline(ptr, idx, deg, sweeps):
         "sample": f"{it} sweeps of the v11 in-place loop (page_rank.hpp:124-139 restated in C, OpenMP static "
                   f"chunks, -O3 -march=x86-64-v3) on the same CSC copied to the host; median sweep "
                   f"{t * 1e3:.1f} ms",
+        "sweep_ms_min_avg_max": [round(float(np.min(sec)) * 1e3, 1), round(float(np.mean(sec)) * 1e3, 1),
+                                 round(float(np.max(sec)) * 1e3, 1)],
+        "one_thread_sweep_ms": round(float(sec1[0]) * 1e3, 1),
+        "host": host_cpu(),
     }
 
 
