@@ -30,10 +30,11 @@ HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
 
 # HBM bytes of one sweep measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE (each scaled by
 # what the counter reads for a stream copy of known size), for the configurations profiled in
-# profiles/: (workload, scale, binned, relabel) -> (bytes per sweep, profile file)
+# profiles/: (workload, RMAT scale or grid width, binned, relabel) -> (bytes per sweep, profile file)
 MEASURED_TRAFFIC = {
     ("rmat", 25, 1, 1): (4025675332, "profiles/pr_binned_sweep_rmat25.md"),
     ("rmat", 25, 0, 1): (23300197580, "profiles/pr_sweep_rmat25_relabel.md"),
+    ("grid", 4096, 1, 0): (779613909, "profiles/pr_binned_sweep_grid4096.md"),
 }
 METRIC = "GTEPS per PR iteration + % HBM roofline; batched PPR/adjoint vs plain"
 
@@ -207,8 +208,9 @@ def run_plain(args, ctx, grid=False):
         kernel_desc = ("one sweep (pr_tile_kernel + pr_group_kernel<4..64> + pr_sgroup_kernel<4..64> + pr_hubseg_kernel"
                        " + pr_finish_kernel, those the plan uses)")
     traffic, traffic_src = args.traffic, "--traffic" if args.traffic else None
-    if traffic is None and not grid and args.edge_factor == 16:
-        hit = MEASURED_TRAFFIC.get(("rmat", args.scale, int(binned), int(plan.get("relabel", 0))))
+    if traffic is None and (grid or args.edge_factor == 16):
+        key = ("grid", args.width) if grid else ("rmat", args.scale)
+        hit = MEASURED_TRAFFIC.get(key + (int(binned), int(plan.get("relabel", 0))))
         if hit:
             traffic, traffic_src = hit[0], f"rocprofv3 FETCH_SIZE + WRITE_SIZE, {hit[1]}"
     out = base_line(args, round(value, 3), "GTEPS", ms_per_step, "f32", {
