@@ -111,6 +111,41 @@ def test_raw_gradient_matches_central_finite_differences(ctx):
     assert np.linalg.norm(got - fd) / np.linalg.norm(fd) < 1e-3
 
 
+def test_full_size_adjoint_identities_rmat20(ctx):
+    """RMAT-20 with random raw weights, every solve run to its float fixed point (150 sweeps,
+    0.85^150 < 1e-10).  Identities that hold for any size; on a dense FP64 system each one
+    fails (relative error 3e-2 or more, against the 1e-4 bound) under the mistake named:
+    - L = c.x is linear in the teleport, so c.x = (dL/dv).v (an adjoint solved with A instead
+      of A^T);
+    - sum_p wn[p] dL/dwn[p] = d lambda.(A x) = lambda.x - (dL/dv).v, with lambda = (dL/dv) / (1-d)
+      (lambda and x swapped in the edge gradient);
+    - L does not change when one source's raw weights are all scaled, so for every source u,
+      sum over its out-edges of w~[p] dL/dw~[p] = 0 (the normaliser's mean term dropped)."""
+    g = ctx.rmat(20, 16, seed=3)
+    n = g.n
+    ptr, col, _ = g.csc()
+    rng = np.random.default_rng(20)
+    w_raw = rng.uniform(0.1, 1.0, col.size).astype(np.float32)
+    g.set_weights(w_raw, normalize=True)
+    v = np.full(n, 1.0 / n, np.float32)
+    c = O.normal_vector(n, seed=13).astype(np.float32)
+    x, fi = g.wpr_forward(v, D, 0.0, 150)
+    gw, gv, bi = g.wpr_backward(x, c, D, 0.0, 150, raw_weights=False)
+    assert fi["iterations"] == 150 and bi["iterations"] == 150
+    x64, c64, v64, gv64 = (a.astype(np.float64) for a in (x, c, v, gv))
+    loss = c64 @ x64
+    assert abs(loss - gv64 @ v64) <= 1e-4 * np.abs(c64 * x64).sum()
+    s = np.bincount(col, weights=w_raw.astype(np.float64), minlength=n)
+    wn = w_raw.astype(np.float64) / s[col]
+    lam = gv64 / (1.0 - np.float64(np.float32(D)))
+    lhs = wn @ gw.astype(np.float64)
+    assert abs(lhs - (lam @ x64 - gv64 @ v64)) <= 1e-4 * np.abs(wn * gw).sum()
+    graw, _, _ = g.wpr_backward(x, c, D, 0.0, 150, raw_weights=True)
+    t = w_raw.astype(np.float64) * graw.astype(np.float64)
+    per_source = np.bincount(col, weights=t, minlength=n)
+    assert np.abs(per_source).sum() <= 1e-4 * np.abs(t).sum()
+
+
 def test_autograd_function(ctx):
     torch = pytest.importorskip("torch")
     from nwgraph_amd.autograd import weighted_pagerank
