@@ -199,3 +199,30 @@ def test_multigraph_autotune_keeps_results(ctx):
     X1, its1, _, _ = mg.pagerank(D, 0.0, 20)
     assert np.all(its0 == 20) and np.all(its1 == 20)
     assert np.max(np.abs(X0 - X1) / np.maximum(X0, 1e-30)) < 1e-5  # summation order only
+
+
+@pytest.mark.parametrize("binned", [0, 1])
+def test_multigraph_256_graphs_match_single_solves(ctx, binned):
+    """The benchmark's union: 256 RMAT graphs of scales 10..18 (the same seeds), 20 sweeps.
+    Every graph's slice of the union equals its own single-graph solve (another kernel and
+    another summation order) within 1e-6, and the largest graph also equals the oracle."""
+    import os
+
+    rng = np.random.default_rng(1)
+    scales = rng.integers(10, 19, 256)
+    graphs = [ctx.rmat(int(s), 16, seed=1000 + q) for q, s in enumerate(scales)]
+    mg = ctx.multigraph(graphs)
+    mg.set_plan(binned)
+    X, its, _, _ = mg.pagerank(D, 0.0, 20)
+    assert np.all(its == 20)
+    for q, g in enumerate(graphs):
+        x, info = g.pagerank(D, 0.0, 20)
+        assert info["iterations"] == 20
+        assert np.max(np.abs(X[mg.offsets[q]:mg.offsets[q + 1]].astype(np.float64) - x)) <= 1e-6, q
+    q = int(np.argmax([g.nnz for g in graphs]))
+    ptr, idx, deg = graphs[q].csc()
+    ref, it, _, _ = O.pr_v10(graphs[q].n, ptr, idx, deg, D, 0.0, 20, nthreads=min(16, os.cpu_count() or 1))
+    assert it == 20 and np.max(np.abs(X[mg.offsets[q]:mg.offsets[q + 1]].astype(np.float64) - ref)) <= 1e-6
+    mg.close()
+    for g in graphs:
+        g.close()
