@@ -373,14 +373,18 @@ __global__ __launch_bounds__(kThreads) void ppr_finish_kernel(PprParams P, uint3
   }
 }
 
+// c0 = init / outdeg (init itself for a row without out-edges) for every vector j < k, in
+// the tile-major layout: blockIdx.y is the k-tile, and x runs over its n x T floats in
+// address order, so (row, lane) come from shifts and the stores are contiguous.
 __global__ __launch_bounds__(kThreads) void ppr_init_kernel(uint32_t n, uint32_t k, uint32_t kt_shift,
                                                             const uint32_t* __restrict__ deg, float init,
                                                             float* __restrict__ c0) {
-  const uint64_t t = (uint64_t)blockIdx.x * kThreads + threadIdx.x;
-  if (t < (uint64_t)n * k) {
-    const uint32_t i = (uint32_t)(t / k), j = (uint32_t)(t % k);
+  const uint64_t q = (uint64_t)blockIdx.x * kThreads + threadIdx.x;  // offset inside the tile
+  const uint32_t i = (uint32_t)(q >> kt_shift);
+  const uint32_t j = (blockIdx.y << kt_shift) + (uint32_t)(q & ((1u << kt_shift) - 1));
+  if (i < n && j < k) {
     const uint32_t d = deg[i];
-    c0[(((uint64_t)(j >> kt_shift) * n + i) << kt_shift) + (j & ((1u << kt_shift) - 1))] = d ? init / (float)d : init;
+    c0[((uint64_t)blockIdx.y * n << kt_shift) + q] = d ? init / (float)d : init;
   }
 }
 
@@ -741,7 +745,12 @@ void ppr_run(nwg_ctx* ctx, nwg_graph* g, uint32_t k, const std::vector<uint32_t>
 
   const float init = (float)(1.0 / n);
   NWG_HIP(hipEventRecord(ctx->ev0, s));
-  ppr_init_kernel<<<grid1(nk), kThreads, 0, s>>>(n, k, P.kt_shift, a.deg, init, c0p);
+  {
+    const uint64_t per_tile = (uint64_t)n << P.kt_shift;
+    NWG_REQUIRE(ceil_div(per_tile, kThreads) < (1ull << 31) && P.tiles < 65536u, "batched solve too large for one launch");
+    ppr_init_kernel<<<dim3((uint32_t)ceil_div(per_tile, kThreads), P.tiles), kThreads, 0, s>>>(n, k, P.kt_shift,
+                                                                                              a.deg, init, c0p);
+  }
   NWG_LAUNCH_CHECK();
   ppr_zero_acc_kernel<<<grid1(max_iters * k), kThreads, 0, s>>>(st.acc.p, max_iters * k);
   NWG_LAUNCH_CHECK();
