@@ -442,8 +442,17 @@ __global__ __launch_bounds__(kThreads) void ppr_scores_kernel(uint32_t n, uint32
                                                               const uint32_t* __restrict__ its,
                                                               float* __restrict__ out) {
   const uint64_t t = (uint64_t)blockIdx.x * kThreads + threadIdx.x;
-  if (t >= (uint64_t)n * k) return;
-  const uint32_t r = (uint32_t)(t / k), j = (uint32_t)(t % k);
+  const uint64_t nk = (uint64_t)n * k;
+  if (t >= nk) return;
+  uint32_t r, j;
+  if (nk <= 0xFFFFFFFFull) {  // uniform over the launch: a 32-bit division is far cheaper
+    const uint32_t t32 = (uint32_t)t;
+    r = t32 / k;
+    j = t32 - r * k;
+  } else {
+    r = (uint32_t)(t / k);
+    j = (uint32_t)(t % k);
+  }
   const uint32_t nj = its[j];
   const float* c = nj == 0 ? c0 : ((nj - 1) & 1) ? c0 : c1;
   const float v = c[(((uint64_t)(j >> kt_shift) * n + r) << kt_shift) + (j & ((1u << kt_shift) - 1))];
