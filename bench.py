@@ -419,12 +419,54 @@ def run_copy(args, ctx):
     })
 
 
+def run_ceilings(args, ctx):
+    """The device's streaming ceilings: copy, read-only and write-only passes over
+    --copy-mib MiB, with and without nontemporal loads / stores, at 2, 4 and 8 workgroups
+    per CU.  Rates count the bytes the pass must move (read + written)."""
+    import ctypes
+
+    import torch
+
+    nbytes = args.copy_mib << 20
+    a = torch.empty(nbytes // 4, dtype=torch.float32, device="cuda").fill_(1.0)
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    ms = ctypes.c_double()
+    names = {0: "copy", 1: "read", 2: "write"}
+    results = []
+    for kind in (0, 1, 2):
+        for flags in (0, 1, 2, 3):
+            if (kind == 1 and flags & 2) or (kind == 2 and flags & 1):
+                continue  # a read pass has no stores, a write pass no loads
+            for wg in (2, 4, 8):
+                times = []
+                for _ in range(args.warmup + args.steps):
+                    ctx.check(ctx.lib.nwg_stream_probe(ctx.handle, kind, flags, wg, ctypes.c_void_p(b.data_ptr()),
+                                                       ctypes.c_void_p(a.data_ptr()), nbytes, ctypes.byref(ms)))
+                    times.append(ms.value)
+                t = float(np.median(times[args.warmup:]))
+                moved = nbytes * (2 if kind == 0 else 1)
+                gbs = moved / (t * 1e-3) / 1e9
+                results.append({"pass": names[kind], "nt_load": bool(flags & 1), "nt_store": bool(flags & 2),
+                                "wg_per_cu": wg, "ms": round(t, 4), "GBps": round(gbs, 1)})
+                log(f"{names[kind]:5s} ntl={flags & 1} nts={(flags >> 1) & 1} wg/cu={wg}: {gbs:8.1f} GB/s")
+    best = {k: max(r["GBps"] for r in results if r["pass"] == k) for k in ("copy", "read", "write")}
+    out = base_line(args, best["copy"], "GB/s (best copy, read + write)", 0.0, "u8", {
+        "workload": f"streaming ceilings over {args.copy_mib} MiB: copy / read-only / write-only, nontemporal "
+                    f"variants, 2-8 workgroups per CU, four 16-byte accesses in flight per lane",
+        "best_GBps": best, "frac_of_8TBs": {k: round(v / HBM_PEAK_GBS, 4) for k, v in best.items()},
+        "probes": results,
+    })
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="plain", choices=["plain", "grid", "ppr", "multi", "wpr", "copy"])
+    ap.add_argument("--workload", default="plain",
+                    choices=["plain", "grid", "ppr", "multi", "wpr", "copy", "ceilings"])
     ap.add_argument("--scale", type=int, default=25)
     ap.add_argument("--edge-factor", type=int, default=16)
     ap.add_argument("--width", type=int, default=4096)
@@ -472,6 +514,8 @@ def main():
         out = run_multi(args, ctx)
     elif args.workload == "copy":
         out = run_copy(args, ctx)
+    elif args.workload == "ceilings":
+        out = run_ceilings(args, ctx)
     else:
         out = run_wpr(args, ctx)
     print(json.dumps(out), flush=True)

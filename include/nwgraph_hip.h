@@ -53,6 +53,12 @@ typedef struct nwg_pr_info {
   uint64_t error_trace_len; /* capacity of error_trace */
   uint32_t flags;        /* in: NWG_INFO_* request bits */
   double sweep_ms_avg;   /* out with NWG_INFO_KERNEL_TIMING: mean duration of one sweep launch */
+  double* time_trace;    /* [host, optional] with NWG_INFO_KERNEL_TIMING: each executed sweep's
+                            duration in ms (the reference traces every sweep's time, page_rank.hpp:123-142) */
+  uint64_t time_trace_len; /* capacity of time_trace */
+  double init_ms[2];     /* out with NWG_INFO_KERNEL_TIMING: device time of the score init and of the
+                            contribution init (life_timers "init page rank", "init contrib",
+                            page_rank.hpp:99-118) */
 } nwg_pr_info;
 
 /* nwg_pr_info.flags: launch the sweeps one by one (instead of as one captured HIP
@@ -103,7 +109,9 @@ int nwg_pagerank_f32(nwg_ctx* ctx, uint32_t n, uint64_t nnz, const uint32_t* csc
 
 /* ------------------------------------------------------------ graph objects */
 /* Upload a CSC (rows = targets, entries = sources; each row ascending or not) --
- * the layout of nw::graph::adjacency<1> (adjacency.hpp:86-192, compressed.hpp:64-102). */
+ * the layout of nw::graph::adjacency<1> (adjacency.hpp:86-192, compressed.hpp:64-102).
+ * A given outdeg must count at least each vertex's entries in csc_idx (build_degrees
+ * counts exactly those, common.hpp:110-121), else NWG_EINVAL. */
 int nwg_graph_create_csc(nwg_ctx* ctx, uint32_t n, uint64_t nnz, const uint32_t* csc_ptr,
                          const uint32_t* csc_idx, const uint32_t* outdeg /*nullable*/, int mem_kind,
                          nwg_graph** out);
@@ -156,7 +164,9 @@ int nwg_generate_ppr_seeds(uint32_t n, uint32_t k, uint32_t s, uint64_t seed, ui
 /* ------------------------------------------------------ plain PageRank solve */
 /* PageRank on a resident graph.  scores: n floats ([host] or [device] per mem_kind).
  * teleport: NULL for the reference's uniform base (1-d)/N, else n floats (same
- * mem_kind) giving the personalised base (1-d)*teleport[i].  Replaces page_rank
+ * mem_kind) giving the personalised base (1-d)*teleport[i]; its entries must be finite,
+ * in [0, 1] and sum to at most 1 (+1e-4 rounding slack), else NWG_EINVAL (scores then
+ * stay in [0, 1], the range the sweeps' exact fixed-point sums cover).  Replaces page_rank
  * (page_rank.hpp:89-148) with the v10 Jacobi schedule; the whole max_iters loop
  * runs as one captured HIP graph with on-device convergence (no host round trip
  * per sweep). */
@@ -258,6 +268,17 @@ int nwg_wpr_backward(nwg_ctx* ctx, nwg_graph* g, const float* x, const float* gr
  * streaming kernel; returns the kernel time in *ms.  Used to calibrate HBM bandwidth and
  * the hardware byte counters against a transfer of known size. */
 int nwg_stream_copy(nwg_ctx* ctx, void* dst, const void* src, uint64_t bytes, double* ms);
+/* Streaming ceilings: kind 0 copies src to dst, 1 only reads src, 2 only writes dst;
+ * flags bit 0 = nontemporal loads, bit 1 = nontemporal stores; wg_per_cu workgroups of
+ * 256 lanes per CU (1..32), four 16-byte accesses in flight per lane.  Returns the kernel
+ * time in *ms.  No counterpart in the reference (a measurement aid). */
+#define NWG_PROBE_COPY 0u
+#define NWG_PROBE_READ 1u
+#define NWG_PROBE_WRITE 2u
+#define NWG_PROBE_NT_LOAD 1u
+#define NWG_PROBE_NT_STORE 2u
+int nwg_stream_probe(nwg_ctx* ctx, uint32_t kind, uint32_t flags, uint32_t wg_per_cu, void* dst,
+                     const void* src, uint64_t bytes, double* ms);
 
 /* ------------------------------------------------------------------ input */
 /* Matrix Market reader (read_mm<directed>, io/mmio.hpp:171-217 with mm_fill :86-103):

@@ -10,10 +10,12 @@
 // with the same parameters, the same in-place output, the same per-sweep trace on
 // stdout (pagerank::trace, page_rank.hpp:42-52) and the same init timers
 // (life_timer, util/timer.hpp:120-133).  The work runs in libnwgraph_hip.so through the
-// C ABI of <nwgraph_hip.h>: the CSC is uploaded, the whole sweep loop runs on the GPU as
-// one captured HIP graph (Jacobi schedule of page_rank_v10), and the scores are copied
-// back.  Errors become exceptions: std::invalid_argument, std::bad_alloc or
-// nw::graph::hip::error.
+// C ABI of <nwgraph_hip.h>: the CSC is uploaded, the sweep loop runs on the GPU (Jacobi
+// schedule of page_rank_v10, convergence checked on the device), and the scores are copied
+// back.  With the trace on (trace_enabled(), the default, as the reference always prints
+// it) the inits and sweeps are launched one by one and each is timed with HIP events;
+// with it off the loop is one captured HIP graph.  Errors become exceptions:
+// std::invalid_argument, std::bad_alloc or nw::graph::hip::error.
 //
 // Accepted graphs: anything laid out like adjacency<1> -- `indices_` (N+1 row offsets)
 // and `std::get<0>(to_be_indexed_)` (source id of every in-edge), both 32-bit -- such as
@@ -27,7 +29,6 @@
 
 #include <nwgraph_hip.h>
 
-#include <chrono>
 #include <cstdint>
 #include <iomanip>
 #include <iostream>
@@ -111,26 +112,37 @@ inline void trace_iter(uint64_t iter, double error, double seconds) {
 }
 }  // namespace detail
 
+// The reference's output around a solve (page_rank.hpp:99-142): the init timers, the trace
+// header and one line per sweep with its L1 change and its time in seconds.  With
+// NWG_INFO_KERNEL_TIMING the times are the device's: each init launch and each sweep.
+inline void print_trace(const nwg_pr_info& info, const std::vector<double>& errors,
+                        const std::vector<double>& sweep_ms) {
+  std::cout << "# [ init page rank ]: " << info.init_ms[0] << " ms" << std::endl;
+  detail::trace_row("iter", "error", "time", "outgoing");
+  std::cout << "# [ init contrib ]: " << info.init_ms[1] << " ms" << std::endl;
+  for (uint64_t k = 0; k < info.iterations && k < errors.size(); ++k)
+    detail::trace_iter(k, errors[k], k < sweep_ms.size() ? sweep_ms[k] * 1e-3 : 0.0);
+}
+
 // Core entry on raw CSC arrays (host memory).  Returns the sweeps executed.
 inline uint64_t page_rank_csc(uint32_t n, uint64_t nnz, const uint32_t* indices, const uint32_t* sources,
                               const uint32_t* degrees, float* scores, float damping_factor, float threshold,
                               size_t max_iters, nwg_ctx* ctx = nullptr) {
   if (!ctx) ctx = context::thread_default().get();
-  std::vector<double> trace(max_iters ? max_iters : 1);
+  std::vector<double> trace(max_iters ? max_iters : 1), times(max_iters ? max_iters : 1);
   nwg_pr_info info{};
   info.error_trace = trace.data();
   info.error_trace_len = trace.size();
-  const auto t0 = std::chrono::steady_clock::now();
+  if (trace_enabled()) {
+    // sweeps launched one by one, each timed, as the reference times each sweep
+    info.flags = NWG_INFO_KERNEL_TIMING;
+    info.time_trace = times.data();
+    info.time_trace_len = times.size();
+  }
   check(ctx, nwg_pagerank_f32(ctx, n, nnz, indices, sources, degrees, scores, damping_factor, threshold, max_iters,
                               &info),
         "nwg_pagerank_f32");
-  if (trace_enabled()) {
-    const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
-    std::cout << "# [ init page rank ]: " << ms - info.ms_total << " ms" << std::endl;
-    detail::trace_row("iter", "error", "time", "outgoing");
-    const double per = info.iterations ? info.ms_per_iter * 1e-3 : 0.0;
-    for (uint64_t k = 0; k < info.iterations; ++k) detail::trace_iter(k, trace[k], per);
-  }
+  if (trace_enabled()) print_trace(info, trace, times);
   return info.iterations;
 }
 
@@ -191,15 +203,20 @@ void page_rank_v13(const Graph& graph, const std::vector<typename Graph::vertex_
                              degrees.empty() ? nullptr : reinterpret_cast<const uint32_t*>(degrees.data()),
                              NWG_MEM_HOST, &g),
         "nwg_graph_create_csc");
+  std::vector<double> times(max_iters ? max_iters : 1);
   nwg_pr_info info{};
+  if (trace_enabled()) {
+    info.flags = NWG_INFO_KERNEL_TIMING;
+    info.time_trace = times.data();
+    info.time_trace_len = times.size();
+  }
   const int rc = nwg_graph_pagerank_delta(ctx, g, page_rank.data(), damping_factor, threshold, max_iters,
                                           NWG_MEM_HOST, &info);
   nwg_graph_destroy(g);
   check(ctx, rc, "nwg_graph_pagerank_delta");
   if (trace_enabled()) {
     detail::trace_row("iter", "error", "time", "outgoing");
-    const double per = info.iterations ? info.ms_per_iter * 1e-3 : 0.0;
-    for (uint64_t k = 0; k < info.iterations; ++k) detail::trace_iter(k, 0.0, per);
+    for (uint64_t k = 0; k < info.iterations; ++k) detail::trace_iter(k, 0.0, times[k] * 1e-3);
   }
 }
 

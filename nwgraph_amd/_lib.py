@@ -42,6 +42,9 @@ class PrInfo(ctypes.Structure):
         ("error_trace_len", c_u64),
         ("flags", c_u32),
         ("sweep_ms_avg", c_double),
+        ("time_trace", P_double),
+        ("time_trace_len", c_u64),
+        ("init_ms", c_double * 2),
     ]
 
 
@@ -118,6 +121,7 @@ _SIGNATURES = {
     "nwg_wpr_backward": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_float, c_float, c_u64,
                                  c_int, c_int, ctypes.POINTER(PrInfo)]),
     "nwg_stream_copy": (c_int, [c_void_p, c_void_p, c_void_p, c_u64, P_double]),
+    "nwg_stream_probe": (c_int, [c_void_p, c_u32, c_u32, c_u32, c_void_p, c_void_p, c_u64, P_double]),
     "nwg_read_mm":(c_int, [ctypes.c_char_p, P_u32, P_u64, ctypes.POINTER(P_u32), ctypes.POINTER(P_u32)]),
     "nwg_read_nw": (c_int, [ctypes.c_char_p, P_u32, P_u64, ctypes.POINTER(P_u32), ctypes.POINTER(P_u32)]),
     "nwg_write_nw": (c_int, [ctypes.c_char_p, c_u32, c_u64, c_void_p, c_void_p]),

@@ -112,7 +112,10 @@ int nwg_graph_create_csc(nwg_ctx* ctx, uint32_t n, uint64_t nnz, const uint32_t*
     // a device CSC is checked on the device before any kernel indexes by its entries
     if (mem_kind != NWG_MEM_HOST) nwg::validate_device_csc(s, n, nnz, g->csc_ptr.p, g->csc_idx.p);
     if (outdeg) {
+      // a smaller out-degree than a vertex's out-edges would let its contributions sum past
+      // its score, out of the range the sweeps' exact fixed-point sums cover
       stage_in(s, g->outdeg, outdeg, n, mem_kind);
+      nwg::check_outdeg_device(s, n, nnz, g->csc_idx.p, g->outdeg.p);
     } else {
       g->outdeg.alloc(n);
       nwg::out_degrees_device(s, n, nnz, g->csc_idx.p, g->outdeg.p);

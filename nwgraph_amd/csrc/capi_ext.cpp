@@ -27,6 +27,16 @@ int nwg_stream_copy(nwg_ctx* ctx, void* dst, const void* src, uint64_t bytes, do
   });
 }
 
+int nwg_stream_probe(nwg_ctx* ctx, uint32_t kind, uint32_t flags, uint32_t wg_per_cu, void* dst, const void* src,
+                     uint64_t bytes, double* ms) {
+  if (!ctx) return NWG_EINVAL;
+  return guarded(ctx, [&] {
+    DeviceGuard dg(ctx->device);
+    const float t = nwg::stream_probe_device(ctx->stream, ctx->ev0, ctx->ev1, kind, flags, wg_per_cu, dst, src, bytes);
+    if (ms) *ms = t;
+  });
+}
+
 int nwg_graph_autotune(nwg_ctx* ctx, nwg_graph* g, nwg_plan_params* chosen) {
   if (!ctx) return NWG_EINVAL;
   return guarded(ctx, [&] {

@@ -9,6 +9,9 @@
 
 namespace nwg {
 
+// Native 4 x 32-bit vector (one 16-byte load or store); the nontemporal builtins take it.
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+
 __device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
 
 // Butterfly sum over the low `width` lanes of each width-aligned group (width <= 64).

@@ -147,6 +147,92 @@ __global__ __launch_bounds__(256) void stream_copy_kernel(const uint4* __restric
 }
 }  // namespace
 
+// Bandwidth probes: the device's streaming ceilings for a copy, a read-only pass and a
+// write-only pass, each with or without nontemporal loads / stores.  Four 16-byte accesses
+// in flight per lane, grid-stride.  The read-only pass folds what it reads into one value and
+// stores it only when it equals a value it never takes, so the loads cannot be dropped.
+template <bool kNtLoad>
+__device__ __forceinline__ u32x4 probe_ld(const u32x4* p) {
+  if constexpr (kNtLoad) return __builtin_nontemporal_load(p);
+  else return *p;
+}
+template <bool kNtStore>
+__device__ __forceinline__ void probe_st(u32x4* p, u32x4 v) {
+  if constexpr (kNtStore) __builtin_nontemporal_store(v, p);
+  else *p = v;
+}
+template <int kKind, bool kNtLoad, bool kNtStore>
+__global__ __launch_bounds__(256) void stream_probe_kernel(const u32x4* __restrict__ src, u32x4* __restrict__ dst,
+                                                           uint64_t n16) {
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  u32x4 acc = {0u, 0u, 0u, 0u};
+  const u32x4 fill = {(uint32_t)i, 1u, 2u, 3u};
+  for (; i + 3 * stride < n16; i += 4 * stride) {
+    if constexpr (kKind == 2) {  // write only
+      probe_st<kNtStore>(dst + i, fill);
+      probe_st<kNtStore>(dst + i + stride, fill);
+      probe_st<kNtStore>(dst + i + 2 * stride, fill);
+      probe_st<kNtStore>(dst + i + 3 * stride, fill);
+    } else {
+      const u32x4 a = probe_ld<kNtLoad>(src + i), b = probe_ld<kNtLoad>(src + i + stride),
+                  c = probe_ld<kNtLoad>(src + i + 2 * stride), d = probe_ld<kNtLoad>(src + i + 3 * stride);
+      if constexpr (kKind == 0) {  // copy
+        probe_st<kNtStore>(dst + i, a);
+        probe_st<kNtStore>(dst + i + stride, b);
+        probe_st<kNtStore>(dst + i + 2 * stride, c);
+        probe_st<kNtStore>(dst + i + 3 * stride, d);
+      } else {  // read only
+        acc ^= a ^ b ^ c ^ d;
+      }
+    }
+  }
+  for (; i < n16; i += stride) {
+    if constexpr (kKind == 2) probe_st<kNtStore>(dst + i, fill);
+    else if constexpr (kKind == 0) probe_st<kNtStore>(dst + i, probe_ld<kNtLoad>(src + i));
+    else acc ^= probe_ld<kNtLoad>(src + i);
+  }
+  if constexpr (kKind == 1)
+    if (acc.x == 0x9E3779B9u && acc.y == 0x7F4A7C15u && acc.z == 0x85EBCA6Bu && acc.w == 0xC2B2AE35u)
+      dst[0] = acc;
+}
+
+template <int kKind>
+static void launch_probe(hipStream_t s, uint32_t grid, uint32_t flags, const u32x4* src, u32x4* dst, uint64_t n16) {
+  switch (flags & 3u) {
+    case 0: stream_probe_kernel<kKind, false, false><<<grid, 256, 0, s>>>(src, dst, n16); break;
+    case 1: stream_probe_kernel<kKind, true, false><<<grid, 256, 0, s>>>(src, dst, n16); break;
+    case 2: stream_probe_kernel<kKind, false, true><<<grid, 256, 0, s>>>(src, dst, n16); break;
+    default: stream_probe_kernel<kKind, true, true><<<grid, 256, 0, s>>>(src, dst, n16); break;
+  }
+}
+
+float stream_probe_device(hipStream_t s, hipEvent_t e0, hipEvent_t e1, uint32_t kind, uint32_t flags,
+                          uint32_t wg_per_cu, void* dst, const void* src, uint64_t bytes) {
+  NWG_REQUIRE(kind <= 2, "probe kind must be 0 (copy), 1 (read) or 2 (write)");
+  NWG_REQUIRE(bytes % 16 == 0 && bytes > 0, "probe size must be a positive multiple of 16 bytes");
+  NWG_REQUIRE(((uintptr_t)dst | (uintptr_t)src) % 16 == 0, "probe buffers must be 16-byte aligned");
+  NWG_REQUIRE(dst != nullptr && (kind == 2 || src != nullptr), "probe buffers must not be null");
+  NWG_REQUIRE(wg_per_cu >= 1 && wg_per_cu <= 32, "workgroups per CU must be in [1, 32]");
+  const uint64_t n16 = bytes / 16;
+  int dev = 0, cus = 0;
+  NWG_HIP(hipGetDevice(&dev));
+  NWG_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+  const uint32_t grid = (uint32_t)std::min<uint64_t>(ceil_div(n16, 256), (uint64_t)cus * wg_per_cu);
+  const u32x4* sp = static_cast<const u32x4*>(src);
+  u32x4* dp = static_cast<u32x4*>(dst);
+  NWG_HIP(hipEventRecord(e0, s));
+  if (kind == 0) launch_probe<0>(s, grid, flags, sp, dp, n16);
+  else if (kind == 1) launch_probe<1>(s, grid, flags, sp, dp, n16);
+  else launch_probe<2>(s, grid, flags, sp, dp, n16);
+  NWG_LAUNCH_CHECK();
+  NWG_HIP(hipEventRecord(e1, s));
+  NWG_HIP(hipEventSynchronize(e1));
+  float ms = 0.f;
+  NWG_HIP(hipEventElapsedTime(&ms, e0, e1));
+  return ms;
+}
+
 float stream_copy_device(hipStream_t s, hipEvent_t e0, hipEvent_t e1, void* dst, const void* src, uint64_t bytes) {
   NWG_REQUIRE(bytes % 16 == 0, "copy size must be a multiple of 16 bytes");
   NWG_REQUIRE(((uintptr_t)dst | (uintptr_t)src) % 16 == 0, "copy buffers must be 16-byte aligned");

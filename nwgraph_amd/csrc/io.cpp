@@ -5,6 +5,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <fstream>
+#include <new>
 #include <sstream>
 #include <string>
 #include <vector>
@@ -12,6 +13,31 @@
 #include "../../include/nwgraph_hip.h"
 
 namespace {
+
+// Runs a reader or writer body so that nothing throws across the C boundary: allocation
+// failures become NWG_ENOMEM, any other exception (length_error from a count read out of a
+// file, stream failures) NWG_EFORMAT.
+template <class F>
+int io_guarded(F&& f) {
+  try {
+    return f();
+  } catch (const std::bad_alloc&) {
+    return NWG_ENOMEM;
+  } catch (...) {
+    return NWG_EFORMAT;
+  }
+}
+
+// Size of an open stream in bytes (0 when it cannot be told); counts read from a file
+// are checked against it before anything is allocated for them.
+uint64_t stream_bytes(std::istream& in) {
+  const std::streampos here = in.tellg();
+  if (here < 0) return 0;
+  in.seekg(0, std::ios::end);
+  const std::streampos end = in.tellg();
+  in.seekg(here);
+  return end < 0 ? 0 : (uint64_t)end;
+}
 
 struct Edges {
   std::vector<uint32_t> s, d;
@@ -45,7 +71,7 @@ extern "C" {
 // "%%MatrixMarket matrix coordinate <field> <symmetric|general>", '%' comment lines,
 // "n0 n1 nnz", then nnz lines "i j [value]" (1-based).  A symmetric file also adds
 // (j, i) for i != j.  n = max(n0, max id + 1) (edge_list.hpp:188-199).
-int nwg_read_mm(const char* path, uint32_t* n, uint64_t* ne, uint32_t** src, uint32_t** dst) {
+static int read_mm(const char* path, uint32_t* n, uint64_t* ne, uint32_t** src, uint32_t** dst) {
   if (!path || !n || !ne || !src || !dst) return NWG_EINVAL;
   std::ifstream in(path);
   if (!in) return NWG_EINVAL;
@@ -63,6 +89,9 @@ int nwg_read_mm(const char* path, uint32_t* n, uint64_t* ne, uint32_t** src, uin
     if (line.empty() || line[0] != '%') break;
   unsigned long long n0 = 0, n1 = 0, nnz = 0;
   if (std::sscanf(line.c_str(), "%llu %llu %llu", &n0, &n1, &nnz) != 3) return NWG_EFORMAT;
+  // every entry line takes at least 4 bytes ("i j\n"): a header count beyond that is malformed
+  const uint64_t bytes = stream_bytes(in);
+  if (nnz > bytes / 4) return NWG_EFORMAT;
   Edges e;
   e.s.reserve((sym ? 2 : 1) * nnz);
   e.d.reserve((sym ? 2 : 1) * nnz);
@@ -86,16 +115,21 @@ int nwg_read_mm(const char* path, uint32_t* n, uint64_t* ne, uint32_t** src, uin
   return export_edges(e, ne, src, dst);
 }
 
+int nwg_read_mm(const char* path, uint32_t* n, uint64_t* ne, uint32_t** src, uint32_t** dst) {
+  return io_guarded([&] { return read_mm(path, n, ne, src, dst); });
+}
+
 // NW binary edge list: char magic[27] "NW Graph index_edge_list", int directedness,
 // size_t N, bool is_open, then struct_of_arrays: size_t nedges, size_t narrays and per
 // array size_t count, size_t elsize, raw data (edge_list.hpp:237-280,
 // graph_base.hpp:72-87, containers/soa.hpp:332-375).
 static const char kMagic[27] = "NW Graph index_edge_list";
 
-int nwg_read_nw(const char* path, uint32_t* n, uint64_t* ne, uint32_t** src, uint32_t** dst) {
+static int read_nw(const char* path, uint32_t* n, uint64_t* ne, uint32_t** src, uint32_t** dst) {
   if (!path || !n || !ne || !src || !dst) return NWG_EINVAL;
   std::ifstream in(path, std::ios::binary);
   if (!in) return NWG_EINVAL;
+  const uint64_t bytes = stream_bytes(in);
   char magic[27];
   int32_t dirn = 0;
   uint64_t N = 0, st = 0, na = 0;
@@ -114,7 +148,7 @@ int nwg_read_nw(const char* path, uint32_t* n, uint64_t* ne, uint32_t** src, uin
     uint64_t cnt = 0, es = 0;
     in.read(reinterpret_cast<char*>(&cnt), 8);
     in.read(reinterpret_cast<char*>(&es), 8);
-    if (!in || cnt != st || es != 4) return NWG_EFORMAT;
+    if (!in || cnt != st || es != 4 || cnt > bytes / 4) return NWG_EFORMAT;
     cols[a]->resize(cnt);
     in.read(reinterpret_cast<char*>(cols[a]->data()), (std::streamsize)(cnt * 4));
     if (!in) return NWG_EFORMAT;
@@ -123,7 +157,11 @@ int nwg_read_nw(const char* path, uint32_t* n, uint64_t* ne, uint32_t** src, uin
   return export_edges(e, ne, src, dst);
 }
 
-int nwg_write_nw(const char* path, uint32_t n, uint64_t ne, const uint32_t* src, const uint32_t* dst) {
+int nwg_read_nw(const char* path, uint32_t* n, uint64_t* ne, uint32_t** src, uint32_t** dst) {
+  return io_guarded([&] { return read_nw(path, n, ne, src, dst); });
+}
+
+static int write_nw(const char* path, uint32_t n, uint64_t ne, const uint32_t* src, const uint32_t* dst) {
   if (!path || ((!src || !dst) && ne)) return NWG_EINVAL;
   std::ofstream out(path, std::ios::binary);
   if (!out) return NWG_EINVAL;
@@ -145,6 +183,10 @@ int nwg_write_nw(const char* path, uint32_t n, uint64_t ne, const uint32_t* src,
   return out ? NWG_OK : NWG_EFORMAT;
 }
 
+int nwg_write_nw(const char* path, uint32_t n, uint64_t ne, const uint32_t* src, const uint32_t* dst) {
+  return io_guarded([&] { return write_nw(path, n, ne, src, dst); });
+}
+
 // Adjacency (index_adjacency<idx>::serialize, adjacency.hpp:176-191): the graph base
 // (graph_base.hpp:72-87: size_t N, bool is_open), then the indexed struct of arrays
 // (containers/compressed.hpp:377-413): char magic[34] "NW GRAPH indexed_struct_of_arrays",
@@ -156,7 +198,7 @@ int nwg_write_nw(const char* path, uint32_t n, uint64_t ne, const uint32_t* src,
 // uint32 form (adjacency<idx>, defaults.hpp:23-24) is read and written.
 static const char kAdjMagic[34] = "NW GRAPH indexed_struct_of_arrays";
 
-int nwg_write_adjacency(const char* path, uint32_t n, uint64_t nnz, const uint32_t* ptr, const uint32_t* idx) {
+static int write_adjacency(const char* path, uint32_t n, uint64_t nnz, const uint32_t* ptr, const uint32_t* idx) {
   if (!path || !ptr || (!idx && nnz)) return NWG_EINVAL;
   if ((uint64_t)ptr[n] != nnz || ptr[0] != 0) return NWG_EINVAL;
   std::ofstream out(path, std::ios::binary);
@@ -179,10 +221,15 @@ int nwg_write_adjacency(const char* path, uint32_t n, uint64_t nnz, const uint32
   return out ? NWG_OK : NWG_EFORMAT;
 }
 
-int nwg_read_adjacency(const char* path, uint32_t* n, uint64_t* nnz, uint32_t** ptr, uint32_t** idx) {
+int nwg_write_adjacency(const char* path, uint32_t n, uint64_t nnz, const uint32_t* ptr, const uint32_t* idx) {
+  return io_guarded([&] { return write_adjacency(path, n, nnz, ptr, idx); });
+}
+
+static int read_adjacency(const char* path, uint32_t* n, uint64_t* nnz, uint32_t** ptr, uint32_t** idx) {
   if (!path || !n || !nnz || !ptr || !idx) return NWG_EINVAL;
   std::ifstream in(path, std::ios::binary);
   if (!in) return NWG_EINVAL;
+  const uint64_t bytes = stream_bytes(in);
   uint64_t N = 0, nfield = 0, st = 0, es = 0, ne = 0, narr = 0, cnt = 0, es2 = 0;
   char open_flag = 0, magic[34];
   in.read(reinterpret_cast<char*>(&N), 8);
@@ -192,14 +239,15 @@ int nwg_read_adjacency(const char* path, uint32_t* n, uint64_t* nnz, uint32_t** 
   in.read(reinterpret_cast<char*>(&nfield), 8);
   in.read(reinterpret_cast<char*>(&st), 8);
   in.read(reinterpret_cast<char*>(&es), 8);
-  if (!in || es != 4 || st != N + 1 || N > 0xFFFFFFFFull) return NWG_EFORMAT;
+  if (!in || es != 4 || N > 0xFFFFFFFFull || st != N + 1 || st > bytes / 4) return NWG_EFORMAT;
   std::vector<uint32_t> rp(st);
   in.read(reinterpret_cast<char*>(rp.data()), (std::streamsize)(st * 4));
   in.read(reinterpret_cast<char*>(&ne), 8);
   in.read(reinterpret_cast<char*>(&narr), 8);
   in.read(reinterpret_cast<char*>(&cnt), 8);
   in.read(reinterpret_cast<char*>(&es2), 8);
-  if (!in || narr != 1 || cnt != ne || es2 != 4 || rp[0] != 0 || rp[N] != ne) return NWG_EFORMAT;
+  if (!in || narr != 1 || cnt != ne || es2 != 4 || rp[0] != 0 || rp[N] != ne || ne > bytes / 4)
+    return NWG_EFORMAT;
   for (uint64_t i = 0; i < N; ++i)
     if (rp[i + 1] < rp[i]) return NWG_EFORMAT;
   std::vector<uint32_t> col(ne);
@@ -221,6 +269,10 @@ int nwg_read_adjacency(const char* path, uint32_t* n, uint64_t* nnz, uint32_t** 
   *ptr = a;
   *idx = b;
   return NWG_OK;
+}
+
+int nwg_read_adjacency(const char* path, uint32_t* n, uint64_t* nnz, uint32_t** ptr, uint32_t** idx) {
+  return io_guarded([&] { return read_adjacency(path, n, nnz, ptr, idx); });
 }
 
 void nwg_free(void* p) { std::free(p); }

@@ -325,18 +325,27 @@ void multigraph_run(nwg_ctx* ctx, nwg_multigraph* mg, float damping, float thres
     mg->cap.reset();
     hipStream_t cs = ctx->capture_stream;
     NWG_HIP(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
-    multi_init_kernel<<<grid1(std::max<uint64_t>(mg->nv, max_iters * G)), kThreads, 0, cs>>>(
-        mg->nv, mg->deg.p, mg->vinit.p, mg->x.p, mg->c0.p, mg->acc.p, max_iters * G);
-    for (uint64_t k = 0; k < max_iters; ++k) {
-      if (mg->binned) {
-        B.iter = (uint32_t)k;
-        binned_enqueue_sweep(cs, B);
-        continue;
+    try {
+      multi_init_kernel<<<grid1(std::max<uint64_t>(mg->nv, max_iters * G)), kThreads, 0, cs>>>(
+          mg->nv, mg->deg.p, mg->vinit.p, mg->x.p, mg->c0.p, mg->acc.p, max_iters * G);
+      for (uint64_t k = 0; k < max_iters; ++k) {
+        if (mg->binned) {
+          B.iter = (uint32_t)k;
+          binned_enqueue_sweep(cs, B);  // throws on a failed launch
+          continue;
+        }
+        P.iter = (uint32_t)k;
+        P.cin = (k & 1) ? mg->c1.p : mg->c0.p;
+        P.cout = (k & 1) ? mg->c0.p : mg->c1.p;
+        multi_sweep_kernel<<<grid, kThreads, 0, cs>>>(P);
       }
-      P.iter = (uint32_t)k;
-      P.cin = (k & 1) ? mg->c1.p : mg->c0.p;
-      P.cout = (k & 1) ? mg->c0.p : mg->c1.p;
-      multi_sweep_kernel<<<grid, kThreads, 0, cs>>>(P);
+    } catch (...) {
+      // leave the capture stream out of capture mode, so that later solves can record again
+      hipGraph_t tmp = nullptr;
+      (void)hipStreamEndCapture(cs, &tmp);
+      if (tmp) (void)hipGraphDestroy(tmp);
+      mg->cap.reset();
+      throw;
     }
     hipGraph_t gr = nullptr;
     const hipError_t e1 = hipGetLastError();

@@ -218,7 +218,8 @@ struct Captured {
 // Cached buffers of the weighted forward / adjoint (wpr.hip).
 struct WState {
   DevBuf<float> y0, y1, base, lambda, t;
-  DevBuf<unsigned long long> acc;
+  DevBuf<double> part, err;  // per-workgroup L1 partials; per-sweep L1 change
+  DevBuf<uint32_t> done;     // per sweep: workgroups finished
   DevBuf<uint32_t> row_of;  // CSC position -> row
 };
 
@@ -246,6 +247,7 @@ struct PrState {
   DevBuf<float> bvals;             // binned sweep: one value per (bin, source) pair
   DevBuf<unsigned long long> bpart;  // binned sweep: partial rows of split bins
   DevBuf<unsigned long long> bacc;   // binned sweep: per-sweep L1 change, kBinAccLanes lanes
+  DevBuf<double> tcheck;             // teleport check: sum, invalid-entry flag
   Captured cap;
 };
 
@@ -323,6 +325,8 @@ void sorted_keys_to_csx(hipStream_t s, const uint64_t* keys, uint64_t m, uint32_
 // throws NWG_EINVAL unless ptr is monotone from 0 to nnz and every entry of idx is below n
 void validate_device_csc(hipStream_t s, uint32_t n, uint64_t nnz, const uint32_t* ptr, const uint32_t* idx);
 void out_degrees_device(hipStream_t s, uint32_t n, uint64_t nnz, const uint32_t* col, uint32_t* deg);
+// throws NWG_EINVAL when some outdeg[u] is below the number of entries u has in col
+void check_outdeg_device(hipStream_t s, uint32_t n, uint64_t nnz, const uint32_t* col, const uint32_t* outdeg);
 
 // ---- primitives (scan.hip / sort.hip) ----
 // exclusive scan of n values; out has n+1 entries (out[n] = total).
@@ -341,6 +345,10 @@ void edge_weights_device(hipStream_t s, uint32_t n, const uint32_t* csc_ptr, con
                          uint64_t seed, float* w);
 // streaming copy for bandwidth / counter calibration; returns the kernel time in ms
 float stream_copy_device(hipStream_t s, hipEvent_t e0, hipEvent_t e1, void* dst, const void* src, uint64_t bytes);
+// bandwidth probe (kind 0 copy, 1 read only, 2 write only; flags bit 0 nontemporal loads,
+// bit 1 nontemporal stores; wg_per_cu workgroups of 256 lanes per CU); kernel time in ms
+float stream_probe_device(hipStream_t s, hipEvent_t e0, hipEvent_t e1, uint32_t kind, uint32_t flags,
+                          uint32_t wg_per_cu, void* dst, const void* src, uint64_t bytes);
 
 // ---- PageRank (pagerank.hip) ----
 // (re)build the sweep plan of g with the given parameters; g is unchanged on failure

@@ -38,6 +38,8 @@ namespace nwg {
 namespace {
 
 constexpr int kThreads = 256;
+// rounding slack allowed above 1 in the sum of a teleport vector
+constexpr double kTeleportSlack = 1e-4;
 
 struct SweepParams {
   const uint32_t* rp;
@@ -256,16 +258,25 @@ __global__ __launch_bounds__(kThreads) void pr_finish_kernel(SweepParams P) {
   if (threadIdx.x == 0 && e > 0.0) atomicAdd(P.acc + P.iter, err_to_fixed(e));
 }
 
-// x = init, contrib = init / outdeg (page_rank.hpp:100-104,114-118); acc cleared.
+// x = init, and with c0 contrib = init / outdeg (page_rank.hpp:100-104,114-118); acc cleared.
 __global__ __launch_bounds__(kThreads) void pr_init_kernel(uint32_t n, const uint32_t* __restrict__ deg, float init,
                                                            float* __restrict__ x, float* __restrict__ c0,
                                                            unsigned long long* __restrict__ acc, uint64_t iters) {
   const uint64_t i = (uint64_t)blockIdx.x * kThreads + threadIdx.x;
   if (i < n) {
     x[i] = init;
-    c0[i] = init / (float)deg[i];
+    if (c0) c0[i] = init / (float)deg[i];
   }
   if (i < iters) acc[i] = 0ull;
+}
+
+// contrib = x / outdeg (page_rank.hpp:114-118): the contribution init on its own, when the
+// two inits are timed separately
+__global__ __launch_bounds__(kThreads) void pr_contrib_init_kernel(uint32_t n, const uint32_t* __restrict__ deg,
+                                                                   const float* __restrict__ x,
+                                                                   float* __restrict__ c0) {
+  const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
+  if (i < n) c0[i] = x[i] / (float)deg[i];
 }
 
 // delta schedule (page_rank_v13, experimental/algorithms/page_rank.hpp:570-577): scores 0,
@@ -311,6 +322,26 @@ __global__ __launch_bounds__(kThreads) void pr_delta_push_kernel(SweepParams P, 
     dlt[i] = dv;
   }
   if (__syncthreads_or(pushed) && threadIdx.x == 0) flags[(uint64_t)P.iter * stride] = 1ull;
+}
+
+// Teleport check: out[0] += sum of v (double), out[1] = 1 when some entry is negative, above
+// 1 or not finite.  The sweeps assume scores stay in [0, 1] (the binned sweep's exact row
+// sums cover [0, 16), the fixed-point L1 change of a sweep [0, 32)), which holds for every
+// teleport with entries >= 0 summing to at most 1.
+__global__ __launch_bounds__(kThreads) void pr_teleport_check_kernel(uint32_t n, const float* __restrict__ v,
+                                                                     double* __restrict__ out) {
+  __shared__ double red_d[kThreads / 64];
+  const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
+  double t = 0.0;
+  bool bad = false;
+  if (i < n) {
+    const float x = v[i];
+    bad = !(x >= 0.f && x <= 1.f);  // also catches NaN
+    t = bad ? 0.0 : (double)x;
+  }
+  if (__syncthreads_or(bad) && threadIdx.x == 0) out[1] = 1.0;
+  t = block_sum<kThreads>(t, red_d);
+  if (threadIdx.x == 0 && t > 0.0) atomicAdd(out, t);
 }
 
 // personalised base (1-d) * v rounded once from double; with a relabelled CSC row i
@@ -666,8 +697,9 @@ static SweepParams make_params(nwg_graph* g, PrState& st, bool personalised, flo
   return P;
 }
 
-// Enqueue init + max_iters sweeps on stream s.  With ev (2*iters events) every sweep
-// kernel is bracketed by an event pair.
+// Enqueue init + max_iters sweeps on stream s.  With ev (2*iters + 4 events) every sweep
+// is bracketed by an event pair, and the score and contribution inits (then two launches)
+// by the last two pairs.
 // one sweep: tiles, each non-empty group class, hub segments, finish
 template <int NT>
 static void enqueue_sweep(hipStream_t s, const SweepParams& P) {
@@ -764,9 +796,15 @@ static void enqueue_solve(hipStream_t s, nwg_graph* g, PrState& st, SweepParams 
     // page_rank_v13: contrib0 holds the deltas, contrib1 the residuals, x the scores; the
     // push kernel's flags go to acc, where the pull kernels' stopping test reads them
     P.delta = 1;
+    if (ev) NWG_HIP(hipEventRecord(ev[2 * iters], s));
     pr_delta_init_kernel<<<blocks_for(std::max<uint64_t>(n, iters)), kThreads, 0, s>>>(
         n, (float)(1.0 - (double)P.damp), st.x.p, st.contrib0.p, st.contrib1.p, st.acc.p, iters);
     NWG_LAUNCH_CHECK();
+    if (ev) {
+      NWG_HIP(hipEventRecord(ev[2 * iters + 1], s));
+      NWG_HIP(hipEventRecord(ev[2 * iters + 2], s));
+      NWG_HIP(hipEventRecord(ev[2 * iters + 3], s));
+    }
     for (uint64_t k = 0; k < iters; ++k) {
       P.iter = (uint32_t)k;
       if (ev) NWG_HIP(hipEventRecord(ev[2 * k], s));
@@ -779,8 +817,18 @@ static void enqueue_solve(hipStream_t s, nwg_graph* g, PrState& st, SweepParams 
   }
   const bool inplace = sched == PrSchedule::kInPlace;
   const float init = (float)(1.0 / n);  // page_rank.hpp:93
-  pr_init_kernel<<<blocks_for(std::max<uint64_t>(n, iters)), kThreads, 0, s>>>(n, P.deg, init, st.x.p,
-                                                                               st.contrib0.p, st.acc.p, iters);
+  if (ev) {
+    NWG_HIP(hipEventRecord(ev[2 * iters], s));
+    pr_init_kernel<<<blocks_for(std::max<uint64_t>(n, iters)), kThreads, 0, s>>>(n, P.deg, init, st.x.p, nullptr,
+                                                                                 st.acc.p, iters);
+    NWG_HIP(hipEventRecord(ev[2 * iters + 1], s));
+    NWG_HIP(hipEventRecord(ev[2 * iters + 2], s));
+    pr_contrib_init_kernel<<<blocks_for(n), kThreads, 0, s>>>(n, P.deg, st.x.p, st.contrib0.p);
+    NWG_HIP(hipEventRecord(ev[2 * iters + 3], s));
+  } else {
+    pr_init_kernel<<<blocks_for(std::max<uint64_t>(n, iters)), kThreads, 0, s>>>(n, P.deg, init, st.x.p,
+                                                                                 st.contrib0.p, st.acc.p, iters);
+  }
   NWG_LAUNCH_CHECK();
   for (uint64_t k = 0; k < iters; ++k) {
     P.iter = (uint32_t)k;
@@ -819,6 +867,15 @@ void pagerank_run(nwg_ctx* ctx, nwg_graph* g, const float* teleport_dev, float d
   }
   const bool personalised = teleport_dev != nullptr;
   if (personalised) {
+    st.tcheck.ensure(2);
+    NWG_HIP(hipMemsetAsync(st.tcheck.p, 0, 2 * sizeof(double), s));
+    pr_teleport_check_kernel<<<blocks_for(n), kThreads, 0, s>>>(n, teleport_dev, st.tcheck.p);
+    NWG_LAUNCH_CHECK();
+    double chk[2] = {0.0, 0.0};
+    NWG_HIP(hipMemcpyAsync(chk, st.tcheck.p, sizeof chk, hipMemcpyDeviceToHost, s));
+    NWG_HIP(hipStreamSynchronize(s));
+    NWG_REQUIRE(chk[1] == 0.0, "teleport entries must be finite and in [0, 1]");
+    NWG_REQUIRE(chk[0] <= 1.0 + kTeleportSlack, "teleport entries must sum to at most 1");
     st.base.ensure(n);
     pr_base_kernel<<<blocks_for(n), kThreads, 0, s>>>(n, teleport_dev, g->active(g->plan.params.relabel != 0).order,
                                                       1.0 - (double)damping, st.base.p);
@@ -869,7 +926,7 @@ void pagerank_run(nwg_ctx* ctx, nwg_graph* g, const float* teleport_dev, float d
     NWG_HIP(hipGraphLaunch(st.cap.exec, s));
     NWG_HIP(hipEventRecord(ctx->ev1, s));
   } else {
-    evs.resize(2 * max_iters);
+    evs.resize(2 * max_iters + 4);
     for (auto& e : evs) NWG_HIP(hipEventCreate(&e));
     NWG_HIP(hipEventRecord(ctx->ev0, s));
     try {
@@ -912,11 +969,18 @@ void pagerank_run(nwg_ctx* ctx, nwg_graph* g, const float* teleport_dev, float d
       break;
     }
   }
+  double init_ms[2] = {0.0, 0.0};
   if (timing) {
     for (uint64_t k = 0; k < iters; ++k) {
       float t = 0.f;
       NWG_HIP(hipEventElapsedTime(&t, evs[2 * k], evs[2 * k + 1]));
       sweep_ms += t;
+      if (info->time_trace && k < info->time_trace_len) info->time_trace[k] = t;
+    }
+    for (int q = 0; q < 2; ++q) {
+      float t = 0.f;
+      NWG_HIP(hipEventElapsedTime(&t, evs[2 * max_iters + 2 * q], evs[2 * max_iters + 2 * q + 1]));
+      init_ms[q] = t;
     }
     for (auto& e : evs) (void)hipEventDestroy(e);
     sweep_ms = iters ? sweep_ms / (double)iters : 0.0;
@@ -927,6 +991,8 @@ void pagerank_run(nwg_ctx* ctx, nwg_graph* g, const float* teleport_dev, float d
     info->ms_total = ms_total;
     info->ms_per_iter = iters ? ms_total / (double)iters : 0.0;
     info->sweep_ms_avg = sweep_ms;
+    info->init_ms[0] = init_ms[0];
+    info->init_ms[1] = init_ms[1];
     if (info->error_trace)
       for (uint64_t k = 0; k < iters && k < info->error_trace_len; ++k)
         info->error_trace[k] = sched == PrSchedule::kDelta ? 0.0 : fixed_to_err(acc[k]);

@@ -15,8 +15,14 @@
 // Sweep schedule: the rows are cut into work items of up to 256 consecutive rows with at
 // most kItemNnz nonzeros (values staged in LDS; one lane per short row, one wave per row of
 // more than kLaneRow nonzeros) or one longer row
-// (whole-workgroup reduction); a persistent grid walks the items round-robin and adds
-// its L1 partial to acc[k] once per sweep.
+// (whole-workgroup reduction); a persistent grid walks the items round-robin.
+//
+// The L1 change of a sweep is summed in double with a fixed order: every workgroup stores
+// its partial, and the last workgroup to finish (a counter per sweep) adds all partials
+// in workgroup order and stores err[k].  The adjoint's changes have no a-priori bound (a
+// loss such as -sum log x gives |grad_x| ~ n), so a fixed-point accumulator of bounded
+// range could wrap; this sum cannot, and it does not depend on completion order, so the
+// sweep count is reproducible.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -46,13 +52,34 @@ struct WParams {
   const float* xin;
   float* xout;
   float damp;
-  unsigned long long* acc;
+  double* part;    // per workgroup: L1 partial of the current sweep
+  uint32_t* done;  // per sweep: workgroups that stored their partial
+  double* err;     // per sweep: L1 change, stored by the sweep's last workgroup
   uint32_t iter;
   double threshold;
 };
 
+// Workgroup-order sum of the per-workgroup partials `e` (valid in thread 0) of sweep
+// P.iter into P.err[P.iter], by whichever workgroup finishes last.
+__device__ __forceinline__ void fold_sweep_error(const WParams& P, double e, double* red_d) {
+  __shared__ uint32_t last;
+  if (threadIdx.x == 0) {
+    P.part[blockIdx.x] = e;
+    __threadfence();  // release the partial before counting it
+    last = atomicAdd(P.done + P.iter, 1u) == gridDim.x - 1 ? 1u : 0u;
+  }
+  __syncthreads();
+  if (!last) return;
+  __threadfence();  // acquire every other workgroup's partial
+  double s = 0.0;
+  for (uint32_t b = threadIdx.x; b < gridDim.x; b += kThreads)
+    s += __hip_atomic_load(P.part + b, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  s = block_sum<kThreads>(s, red_d);
+  if (threadIdx.x == 0) P.err[P.iter] = s;
+}
+
 __global__ __launch_bounds__(kThreads) void wsweep_kernel(WParams P) {
-  if (P.iter > 0 && fixed_to_err(P.acc[P.iter - 1]) < P.threshold) return;
+  if (P.iter > 0 && P.err[P.iter - 1] < P.threshold) return;
   __shared__ float vals[kItemNnz];
   __shared__ uint32_t lrows[kItemRows];   // an item's long rows (offsets from its first row)
   __shared__ uint32_t wlong[kThreads / 64];
@@ -153,22 +180,19 @@ __global__ __launch_bounds__(kThreads) void wsweep_kernel(WParams P) {
     }
   }
   e = block_sum<kThreads>(e, red_d);
-  if (threadIdx.x == 0 && e > 0.0) atomicAdd(P.acc + P.iter, err_to_fixed(e));
+  fold_sweep_error(P, e, red_d);
 }
 
-__global__ __launch_bounds__(kThreads) void wfill_kernel(uint32_t n, float v, float* __restrict__ x,
-                                                         unsigned long long* __restrict__ acc, uint64_t nacc) {
+// y = v (src null) or y = src; the per-sweep counters and errors cleared
+__global__ __launch_bounds__(kThreads) void winit_kernel(uint32_t n, const float* __restrict__ src, float v,
+                                                         float* __restrict__ dst, uint32_t* __restrict__ done,
+                                                         double* __restrict__ err, uint64_t iters) {
   const uint64_t t = (uint64_t)blockIdx.x * kThreads + threadIdx.x;
-  if (t < n) x[t] = v;
-  if (t < nacc) acc[t] = 0ull;
-}
-
-__global__ __launch_bounds__(kThreads) void wcopy_kernel(uint32_t n, const float* __restrict__ src,
-                                                         float* __restrict__ dst, unsigned long long* __restrict__ acc,
-                                                         uint64_t nacc) {
-  const uint64_t t = (uint64_t)blockIdx.x * kThreads + threadIdx.x;
-  if (t < n) dst[t] = src[t];
-  if (t < nacc) acc[t] = 0ull;
+  if (t < n) dst[t] = src ? src[t] : v;
+  if (t < iters) {
+    done[t] = 0u;
+    err[t] = 0.0;
+  }
 }
 
 // base_i = (float)((1-d) * v_i) (v null: uniform 1/n)
@@ -401,15 +425,14 @@ static uint64_t fixed_point(nwg_ctx* ctx, WState& st, const uint32_t* rp, const 
   hipStream_t s = ctx->stream;
   st.y0.ensure(n);
   st.y1.ensure(n);
-  st.acc.ensure(std::max<uint64_t>(max_iters, 1));
+  st.done.ensure(std::max<uint64_t>(max_iters, 1));
+  st.err.ensure(std::max<uint64_t>(max_iters, 1));
   float* y[2] = {st.y0.p, st.y1.p};
   const uint32_t grid = persistent_grid(n_items);
+  st.part.ensure(grid);
   NWG_HIP(hipEventRecord(ctx->ev0, s));
-  if (y0_uniform)
-    wfill_kernel<<<grid1(std::max<uint64_t>(n, max_iters)), kThreads, 0, s>>>(n, (float)(1.0 / n), y[0], st.acc.p,
-                                                                            max_iters);
-  else
-    wcopy_kernel<<<grid1(std::max<uint64_t>(n, max_iters)), kThreads, 0, s>>>(n, y0, y[0], st.acc.p, max_iters);
+  winit_kernel<<<grid1(std::max<uint64_t>(n, max_iters)), kThreads, 0, s>>>(
+      n, y0_uniform ? nullptr : y0, (float)(1.0 / n), y[0], st.done.p, st.err.p, max_iters);
   NWG_LAUNCH_CHECK();
   WParams P{};
   P.rp = rp;
@@ -420,7 +443,9 @@ static uint64_t fixed_point(nwg_ctx* ctx, WState& st, const uint32_t* rp, const 
   P.n_items = n_items;
   P.base = base;
   P.damp = damping;
-  P.acc = st.acc.p;
+  P.part = st.part.p;
+  P.done = st.done.p;
+  P.err = st.err.p;
   P.threshold = (double)threshold;
   for (uint64_t k = 0; k < max_iters; ++k) {
     P.iter = (uint32_t)k;
@@ -430,13 +455,13 @@ static uint64_t fixed_point(nwg_ctx* ctx, WState& st, const uint32_t* rp, const 
     NWG_LAUNCH_CHECK();
   }
   NWG_HIP(hipEventRecord(ctx->ev1, s));
-  std::vector<unsigned long long> h(std::max<uint64_t>(max_iters, 1));
-  NWG_HIP(hipMemcpyAsync(h.data(), st.acc.p, max_iters * 8, hipMemcpyDeviceToHost, s));
+  std::vector<double> h(std::max<uint64_t>(max_iters, 1));
+  NWG_HIP(hipMemcpyAsync(h.data(), st.err.p, max_iters * sizeof(double), hipMemcpyDeviceToHost, s));
   NWG_HIP(hipStreamSynchronize(s));
   uint64_t iters = max_iters;
   double last = 0.0;
   for (uint64_t k = 0; k < max_iters; ++k) {
-    last = fixed_to_err(h[k]);
+    last = h[k];
     if (last < (double)threshold) {
       iters = k + 1;
       break;
@@ -453,7 +478,7 @@ static uint64_t fixed_point(nwg_ctx* ctx, WState& st, const uint32_t* rp, const 
     info->ms_per_iter = iters ? ms / (double)iters : 0.0;
     info->sweep_ms_avg = 0.0;
     if (info->error_trace)
-      for (uint64_t k = 0; k < iters && k < info->error_trace_len; ++k) info->error_trace[k] = fixed_to_err(h[k]);
+      for (uint64_t k = 0; k < iters && k < info->error_trace_len; ++k) info->error_trace[k] = h[k];
   }
   return iters;
 }

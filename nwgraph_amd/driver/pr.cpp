@@ -18,16 +18,22 @@
 // ignores them.
 #include <nwgraph_hip.h>
 
+#include <unistd.h>
+
 #include <algorithm>
 #include <chrono>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <ctime>
+#include <filesystem>
 #include <fstream>
 #include <iomanip>
 #include <iostream>
 #include <map>
 #include <numeric>
+#include <random>
+#include <sstream>
 #include <string>
 #include <tuple>
 #include <vector>
@@ -175,6 +181,111 @@ nwg_graph* load(nwg_ctx* ctx, const std::string& file, bool verbose) {
   return g;
 }
 
+// Compiler facts of the Log's CXX / CXX_ID / CXX_VER / Build columns (the reference takes
+// them from its CMake configuration, config.h).
+#ifndef NWG_CXX_PATH
+#define NWG_CXX_PATH "c++"
+#endif
+#if defined(__clang__)
+constexpr const char* kCxxId = "Clang";
+#elif defined(__GNUC__)
+constexpr const char* kCxxId = "GNU";
+#else
+constexpr const char* kCxxId = "unknown";
+#endif
+constexpr const char* kCxxVersion = __VERSION__;
+constexpr const char* kBuildType = "Release";
+constexpr const char* kTbbMalloc = "OFF";  // no TBB here
+
+// The reference's result log (bench/gapbs/Log.hpp:39-179): one row per sample, with a run
+// uuid, library, git branch and revision, compiler, build, date, host, benchmark, version
+// ("v" + id), threads, the graph's file stem cut at its first '_', and the sample's time.
+// "-" writes to stdout.  Column widths and number formats follow the reference.
+class Log {
+public:
+  explicit Log(const std::string& path) : os_(path == "-" ? std::cout : out_) {
+    std::random_device rd;
+    std::mt19937 gen(rd());
+    std::uniform_int_distribution<int> dis(97, 122);
+    uuid_.resize(kUuidSize);
+    for (char& c : uuid_) c = (char)dis(gen);
+    std::memset(host_, 0, sizeof host_);
+    if (gethostname(host_, sizeof host_) != 0) std::cerr << "truncated host name\n";
+    host_[sizeof host_ - 1] = 0;
+    if (path != "-") {
+      std::filesystem::path full = path;
+      if (full.has_parent_path()) std::filesystem::create_directory(full.parent_path());
+      out_.open(path, std::ofstream::out | std::ofstream::app);
+    }
+    const std::time_t now = std::time(nullptr);
+    char day[16];
+    std::strftime(day, sizeof day, "%Y-%m-%d", std::gmtime(&now));
+    date_ = day;
+    git_branch_ = run("git rev-parse --abbrev-ref HEAD 2>/dev/null");
+    git_version_ = run("git log --pretty=format:%h -n 1 2>/dev/null");
+    if (!git_version_.empty() && std::system("git diff --quiet --exit-code 2>/dev/null") != 0) git_version_ += "+";
+    cxx_w_ = std::max(std::strlen(NWG_CXX_PATH), std::strlen("CXX")) + 2;
+    cxx_id_w_ = std::max(std::strlen(kCxxId), std::strlen("CXX_ID")) + 2;
+    cxx_ver_w_ = std::max(std::strlen(kCxxVersion), std::strlen("CXX_VER")) + 2;
+    build_w_ = std::max(std::strlen(kBuildType), std::strlen("Build")) + 2;
+    tbb_w_ = std::max(std::strlen(kTbbMalloc), std::strlen("TBBMALLOC")) + 2;
+  }
+
+  void header(const std::vector<std::string>& times) {
+    auto p = os_.precision(15);
+    auto f = os_.setf(std::ios::fixed, std::ios::floatfield);
+    os_ << std::setw(kUuidSize + 2) << std::left << "UUID" << std::setw(10) << std::left << "Library"
+        << std::setw(10) << std::left << "Branch" << std::setw(10) << std::left << "Revision" << std::setw(cxx_w_)
+        << std::left << "CXX" << std::setw(cxx_id_w_) << std::left << "CXX_ID" << std::setw(cxx_ver_w_) << std::left
+        << "CXX_VER" << std::setw(build_w_) << std::left << "Build" << std::setw(tbb_w_) << std::left << "TBBMALLOC"
+        << std::setw(15) << std::left << "Date" << std::setw(sizeof host_) << std::left << "Host" << std::setw(10)
+        << std::left << "Benchmark" << std::setw(10) << std::left << "Version" << std::setw(10) << std::left
+        << "Threads" << std::setw(32) << std::left << "Graph";
+    for (const auto& t : times) os_ << std::setw(20) << std::left << t;
+    os_ << "\n";
+    os_.precision(p);
+    os_.setf(f, std::ios::floatfield);
+  }
+
+  void row(const std::string& benchmark, long version, long threads, const std::string& graph, double seconds) {
+    auto p = os_.precision(8);
+    auto f = os_.setf(std::ios::scientific, std::ios::floatfield);
+    std::string stem = std::filesystem::path(graph).stem().string();
+    const size_t cut = stem.find('_');
+    if (cut < stem.size()) stem.resize(cut);
+    os_ << std::setw(kUuidSize + 2) << std::left << uuid_ << std::setw(10) << std::left << "NWG_HIP" << std::setw(10)
+        << std::left << git_branch_ << std::setw(10) << std::left << git_version_ << std::setw(cxx_w_) << std::left
+        << NWG_CXX_PATH << std::setw(cxx_id_w_) << std::left << kCxxId << std::setw(cxx_ver_w_) << std::left
+        << kCxxVersion << std::setw(build_w_) << std::left << kBuildType << std::setw(tbb_w_) << std::left
+        << kTbbMalloc << std::setw(15) << std::left << date_ << std::setw(sizeof host_) << std::left << host_
+        << std::setw(10) << std::left << benchmark << std::setw(10) << std::left << ("v" + std::to_string(version))
+        << std::setw(10) << std::left << threads << std::setw(32) << std::left << stem << std::setw(20) << std::left
+        << seconds << "\n";
+    os_.precision(p);
+    os_.setf(f, std::ios::floatfield);
+  }
+
+private:
+  static std::string run(const char* cmd) {
+    std::string out;
+    if (FILE* f = popen(cmd, "r")) {
+      char buf[256];
+      while (std::fgets(buf, sizeof buf, f)) out += buf;
+      pclose(f);
+    }
+    std::istringstream in(out);
+    std::string word;
+    in >> word;
+    return word;
+  }
+  static constexpr size_t kUuidSize = 24;
+  std::string uuid_, date_, git_branch_, git_version_;
+  char host_[20];
+  size_t cxx_w_ = 0, cxx_id_w_ = 0, cxx_ver_w_ = 0, build_w_ = 0, tbb_w_ = 0;
+  std::ofstream out_;
+  std::ostream& os_;
+};
+
 }  // namespace
 
 int main(int argc, char** argv) {
@@ -205,7 +316,7 @@ int main(int argc, char** argv) {
       std::cout << "\n";
     }
     std::vector<float> rankings(n);
-    std::vector<double> trace(args.max_iters);
+    std::vector<double> trace(args.max_iters), sweep_ms(args.max_iters);
     for (long thread : args.threads) {
       for (long id : args.versions) {
         for (long trial = 0; trial < args.trials; ++trial) {
@@ -216,6 +327,10 @@ int main(int argc, char** argv) {
           nwg_pr_info info{};
           info.error_trace = trace.data();
           info.error_trace_len = trace.size();
+          // every sweep timed on the device, as the reference times each sweep (page_rank.hpp:123-142)
+          info.flags = NWG_INFO_KERNEL_TIMING;
+          info.time_trace = sweep_ms.data();
+          info.time_trace_len = sweep_ms.size();
           const auto t0 = std::chrono::steady_clock::now();
           // version 11 is the reference's in-place schedule, version 10 its Jacobi twin,
           // version 13 the delta (residual) PageRank with un-normalised scores
@@ -232,9 +347,7 @@ int main(int argc, char** argv) {
                   id == 11 ? "nwg_graph_pagerank_inplace" : "nwg_graph_pagerank");
           }
           const double sec = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
-          nw::graph::hip::detail::trace_row("iter", "error", "time", "outgoing");
-          for (uint64_t k = 0; k < info.iterations; ++k)
-            nw::graph::hip::detail::trace_iter(k, trace[k], info.ms_per_iter * 1e-3);
+          nw::graph::hip::print_trace(info, trace, sweep_ms);
           times[std::make_tuple(file, id, thread)].push_back(sec);
         }
         if (args.verify) {  // print_n_ranks (pr.cpp:46-52): top 10 by score
@@ -264,15 +377,11 @@ int main(int argc, char** argv) {
               << std::fixed << *mn << std::setw(20) << std::left << std::setprecision(6) << std::fixed << avg
               << std::setw(20) << std::left << std::setprecision(6) << std::fixed << *mx << "\n";
   }
-  if (!args.log_file.empty()) {
-    std::ofstream log(args.log_file, std::ios::app);
-    if (args.log_header) log << "Benchmark Version Threads File Min Avg Max Tolerance Device\n";
-    for (auto& [cfg, s] : times) {
-      const auto [mn, mx] = std::minmax_element(s.begin(), s.end());
-      const double avg = std::accumulate(s.begin(), s.end(), 0.0) / s.size();
-      log << "pr " << std::get<1>(cfg) << " " << std::get<2>(cfg) << " " << std::get<0>(cfg) << " " << *mn << " "
-          << avg << " " << *mx << " " << args.tolerance << " gfx950\n";
-    }
+  if (!args.log_file.empty()) {  // log("pr", file, times, header, "Time(s)", "Tolerance") (pr.cpp:171-175)
+    Log log(args.log_file);
+    if (args.log_header) log.header({"Time(s)", "Tolerance"});
+    for (auto& [cfg, s] : times)
+      for (double t : s) log.row("pr", std::get<1>(cfg), std::get<2>(cfg), std::get<0>(cfg), t);
   }
   return 0;
 }

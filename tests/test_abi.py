@@ -196,3 +196,51 @@ def test_adjacency_reader_rejects_bad_files(tmp_path):
             nwgraph_amd.Adjacency([0], []).deserialize(q)
     with pytest.raises(NwgError):  # the writer checks the offsets against the ids
         nwgraph_amd.Adjacency([0, 1, 3], [0]).serialize(tmp_path / "w.adj")
+
+
+def test_readers_reject_oversized_counts_without_throwing(tmp_path):
+    """Counts taken from a file header are checked against the file size before anything is
+    allocated for them: an absurd count is a malformed file (NWG_EFORMAT), never an
+    exception escaping the C boundary (which would abort the process)."""
+    import nwgraph_amd
+    from nwgraph_amd import NwgError, _lib
+
+    mm = tmp_path / "huge.mtx"
+    mm.write_text("%%MatrixMarket matrix coordinate pattern general\n3 3 4000000000000000000\n1 2\n")
+    with pytest.raises(NwgError) as e:
+        nwgraph_amd.read_mm(mm)
+    assert e.value.code == _lib.NWG_EFORMAT
+    mm_sym = tmp_path / "huge_sym.mtx"
+    mm_sym.write_text("%%MatrixMarket matrix coordinate pattern symmetric\n3 3 18446744073709551615\n1 2\n")
+    with pytest.raises(NwgError) as e:
+        nwgraph_amd.read_mm(mm_sym)
+    assert e.value.code == _lib.NWG_EFORMAT
+
+    # NW edge list whose header claims 2^61 edges
+    hdr = b"NW Graph index_edge_list\0\0\0" + np.int32(1).tobytes() + np.uint64(4).tobytes() + b"\0"
+    big = np.uint64(1 << 61)
+    nw = tmp_path / "huge.nw"
+    nw.write_bytes(hdr + big.tobytes() + np.uint64(2).tobytes() + big.tobytes() + np.uint64(4).tobytes()
+                   + np.zeros(8, np.uint32).tobytes())
+    with pytest.raises(NwgError) as e:
+        nwgraph_amd.read_nw(nw)
+    assert e.value.code == _lib.NWG_EFORMAT
+
+    # adjacency whose offsets and ids claim far more entries than the file holds
+    good = _adjacency_bytes(4, [0, 1, 3, 3, 4], [2, 0, 3, 1])
+    huge_n = (np.uint64(1 << 31).tobytes() + b"\0" + b"NW GRAPH indexed_struct_of_arrays\0"
+              + np.uint32(5).tobytes() + np.uint32(0).tobytes() + np.uint64((1 << 31) + 1).tobytes()
+              + np.uint64(4).tobytes() + good[67:])
+    q = tmp_path / "huge_n.adj"
+    q.write_bytes(huge_n)
+    with pytest.raises(NwgError) as e:
+        nwgraph_amd.Adjacency([0], []).deserialize(q)
+    assert e.value.code == _lib.NWG_EFORMAT
+    ne_off = len(good) - (4 * 4 + 8 + 8 + 8 + 8)  # start of the struct-of-arrays header
+    huge_ne = bytearray(good)
+    huge_ne[ne_off:ne_off + 8] = np.uint64(1 << 62).tobytes()
+    huge_ne[ne_off + 16:ne_off + 24] = np.uint64(1 << 62).tobytes()
+    q.write_bytes(bytes(huge_ne))
+    with pytest.raises(NwgError) as e:
+        nwgraph_amd.Adjacency([0], []).deserialize(q)
+    assert e.value.code == _lib.NWG_EFORMAT

@@ -215,6 +215,52 @@ def test_pr_driver_karate(tmp_path):
     rows = [l for l in text.splitlines() if "karate.mtx" in l and not l.startswith("Reading")]
     assert len(rows) == 2 and {int(r.split()[1]) for r in rows} == {10, 11}
     assert all(int(r.split()[2]) == 4 for r in rows)
+    # every solve prints the reference's init timers around the trace header, then one line
+    # per sweep with its own time (page_rank.hpp:99-142)
+    lines = text.splitlines()
+    heads = [k for k, l in enumerate(lines) if l.split() == ["iter", "error", "time", "outgoing"]]
+    assert len(heads) == 4
+    for k in heads:
+        assert lines[k - 1].startswith("# [ init page rank ]: ") and lines[k - 1].endswith(" ms")
+        assert lines[k + 1].startswith("# [ init contrib ]: ") and lines[k + 1].endswith(" ms")
+        first = lines[k + 2].split()
+        assert first[0] == "0" and float(first[2]) > 0 and first[3] == "0"
+
+
+def test_pr_driver_log_matches_the_reference_layout(tmp_path):
+    """--log / --log-header write the reference's result table (bench/gapbs/Log.hpp:39-179):
+    one row per sample with uuid, library, git branch and revision, compiler, build, TBB
+    malloc, date, host, benchmark, version, threads, graph stem and the sample's time."""
+    import datetime
+    import subprocess
+
+    exe = os.path.join(os.path.dirname(GOLDEN), "..", "nwgraph_amd", "bin", "pr")
+    log = tmp_path / "logs" / "pr.log"
+    out = subprocess.run([exe, "-f", os.path.join(GOLDEN, "karate.mtx"), "-n", "3", "--version", "10",
+                          "--log", str(log), "--log-header", "2"], capture_output=True, text=True, timeout=120,
+                         cwd=tmp_path)
+    assert out.returncode == 0, out.stderr
+    lines = log.read_text().splitlines()
+    assert len(lines) == 4
+    head = lines[0].split()
+    assert head == ["UUID", "Library", "Branch", "Revision", "CXX", "CXX_ID", "CXX_VER", "Build", "TBBMALLOC",
+                    "Date", "Host", "Benchmark", "Version", "Threads", "Graph", "Time(s)", "Tolerance"]
+    # fixed widths: UUID column 26, Library/Branch/Revision 10 each
+    assert lines[0].index("Library") == 26 and lines[0].index("Branch") == 36
+    uuids = set()
+    for row in lines[1:]:
+        uuid = row[:26].strip()
+        assert len(uuid) == 24 and uuid.isalpha() and uuid.islower()
+        uuids.add(uuid)
+        assert row[26:36].strip() == "NWG_HIP"
+        f = row.split()
+        # ... Date, Host, Benchmark, Version, Threads, Graph (stem of karate.mtx), Time
+        assert f[-5:-1] == ["pr", "v10", "2", "karate"]
+        datetime.date.fromisoformat(f[-7])
+        t = f[-1]
+        assert "e" in t and float(t) > 0  # scientific, precision 8
+        assert len(t.split("e")[0].split(".")[1]) == 8
+    assert len(uuids) == 1
 
 
 def with_plan(g, plan):
@@ -579,3 +625,118 @@ def test_uncleaned_graph_with_duplicates_and_loops(ctx, plan):
     ind, col = O.csc(n, s, d)
     deg = O.out_degrees(n, ind, col)
     check_against_oracle(g, n, ind, col, deg)
+
+
+# ---------------------------------------------------------------- boundary checks
+def test_teleport_is_validated(ctx):
+    """A personalised teleport must be finite, in [0, 1] and sum to at most 1: the binned
+    sweep's exact row sums and the fixed-point L1 change assume scores in [0, 1].  Invalid
+    vectors are rejected with NWG_EINVAL on both plans; a valid sub-stochastic one (sum 0.5)
+    on a graph of over 2^18 edges (binned sweep by default) matches the oracle."""
+    from nwgraph_amd import NwgError, _lib
+
+    g = ctx.rmat(16, 16, seed=4)
+    n, ind, col, deg = O.rmat_csc(16, 16, seed=4)
+    assert g.nnz >= 1 << 18 and g.get_plan()["binned"] == 1
+    bad = {
+        "ones": np.ones(n, np.float32),
+        "negative": np.full(n, 1.0 / n, np.float32) * np.where(np.arange(n) == 7, -1, 1).astype(np.float32),
+        "nan": np.where(np.arange(n) == 3, np.nan, 1.0 / n).astype(np.float32),
+        "sum 1.01": np.full(n, 1.01 / n, np.float32),
+    }
+    for plan in ({"binned": 1}, {"binned": 0}):
+        g.set_plan(**plan)
+        for name, v in bad.items():
+            with pytest.raises(NwgError) as e:
+                g.pagerank(D, 1e-4, 20, teleport=v)
+            assert e.value.code == _lib.NWG_EINVAL, name
+    g.set_plan(binned=1)
+    rng = np.random.default_rng(1)
+    v = rng.uniform(size=n)
+    v = (0.5 * v / v.sum()).astype(np.float32)
+    check_against_oracle(g, n, ind, col, deg, teleport=v)
+
+
+def test_given_outdegrees_are_checked(ctx):
+    """nwg_graph_create_csc checks a caller's out-degrees against the CSC: one below a
+    vertex's entry count is rejected (its contributions would sum past its score); larger
+    ones are accepted and used as given, like the reference's degrees vector."""
+    from nwgraph_amd import NwgError, _lib
+
+    n, ind, col, deg = O.rmat_csc(12, 16, seed=5)
+    low = deg.copy()
+    u = int(np.argmax(deg))
+    low[u] -= 1
+    with pytest.raises(NwgError) as e:
+        ctx.graph_from_csc(ind, col, low)
+    assert e.value.code == _lib.NWG_EINVAL
+    high = deg + 1
+    g = ctx.graph_from_csc(ind, col, high)
+    x, info = g.pagerank(D, 1e-4, 20)
+    ref, it, _, _ = O.pr_v10(n, ind, col, high, D, 1e-4, 20)
+    assert info["iterations"] == it and np.max(np.abs(x - ref)) <= 1e-6
+
+
+@pytest.mark.parametrize("kind", ["rmat25", "grid4096"])
+def test_full_size_matches_oracle_v10(ctx, kind):
+    """The headline inputs at full size, row by row: the autotuned plan (what the benchmark
+    times) against the oracle's v10 run on the device-built CSC, same sweep count, every row
+    within 1e-6, and the device out-degrees against numpy's bincount of the whole CSC."""
+    import os
+
+    if kind == "rmat25":
+        g, iters = ctx.rmat(25, 16, seed=1), 20
+    else:
+        g, iters = ctx.grid(4096, seed=1), 100
+    g.autotune()
+    x, info = g.pagerank(D, 1e-4, iters)
+    ptr, idx, deg = g.csc()
+    n = g.n
+    assert np.array_equal(deg, np.bincount(idx, minlength=n).astype(np.uint32))
+    threads = min(16, os.cpu_count() or 1)
+    ref, it, tr, _ = O.pr_v10(n, ptr, idx, deg, D, 1e-4, iters, nthreads=threads)
+    assert info["iterations"] == it, (info["iterations"], it)
+    err = np.abs(x.astype(np.float64) - ref)
+    assert float(err.max()) <= 1e-6, (int(err.argmax()), float(err.max()))
+    assert np.allclose(info["error_trace"], tr, rtol=1e-3, atol=1e-12)
+
+
+def test_cpp_dropin_runs_against_the_oracle(tmp_path):
+    """The C++ drop-in (include/nwgraph_hip/page_rank.hpp) compiled into a caller that uses it
+    the way bench/gapbs/pr.cpp does, on RMAT-16: page_rank, page_rank_v10 and the overload for
+    the reference's adjacency<1> (which must win over a generic template) all match the
+    oracle's v10 within 1e-6, with the reference's trace on stdout."""
+    import re
+    import shutil
+    import subprocess
+
+    from conftest import ROOT
+
+    if not shutil.which("g++"):
+        pytest.skip("g++ not available")
+    import nwgraph_amd
+
+    n, ind, col, deg = O.rmat_csc(16, 16, seed=1)
+    adj = tmp_path / "rmat16.adj"
+    nwgraph_amd.Adjacency(ind, col).serialize(adj)
+    exe = tmp_path / "dropin_run"
+    libdir = os.path.join(ROOT, "nwgraph_amd")
+    subprocess.run(["g++", "-O2", "-std=c++20", "-I", os.path.join(ROOT, "include"),
+                    os.path.join(ROOT, "tests", "cpp", "dropin_run.cpp"), "-o", str(exe), "-L", libdir,
+                    "-lnwgraph_hip", f"-Wl,-rpath,{libdir}"], check=True)
+    out = subprocess.run([str(exe), str(adj), str(tmp_path / "scores")], capture_output=True, text=True,
+                         timeout=120)
+    assert out.returncode == 0, out.stderr
+    assert "template calls: 0" in out.stdout
+    lines = out.stdout.splitlines()
+    assert re.match(r"# \[ init page rank \]: [0-9.e+-]+ ms", lines[0])
+    assert lines[1].split() == ["iter", "error", "time", "outgoing"]
+    assert re.match(r"# \[ init contrib \]: [0-9.e+-]+ ms", lines[2])
+    ref, it, tr, _ = O.pr_v10(n, ind, col, deg, D, 1e-4, 20, nthreads=8)
+    rows = [l.split() for l in lines[3:3 + it]]
+    assert [int(r[0]) for r in rows] == list(range(it))
+    assert np.allclose([float(r[1]) for r in rows], tr, rtol=1e-3)
+    assert all(float(r[2]) > 0 for r in rows) and all(r[3] == "0" for r in rows)
+    for suffix in ("v11", "v10", "adj"):
+        x = np.fromfile(tmp_path / f"scores.{suffix}", np.float32)
+        assert x.size == n and np.max(np.abs(x.astype(np.float64) - ref)) <= 1e-6, suffix

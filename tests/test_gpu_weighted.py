@@ -108,7 +108,92 @@ def test_raw_gradient_matches_central_finite_differences(ctx):
     h = 1e-6
     fd = np.array([(loss_with(p, h) - loss_with(p, -h)) / (2 * h) for p in sample])
     got = gw_gpu[sample].astype(np.float64)
-    assert np.linalg.norm(got - fd) / np.linalg.norm(fd) < 1e-3
+    assert np.linalg.norm(got - fd) / np.linalg.norm(fd) < 1e-4
+
+
+def test_gradients_at_tolerance_1e4_match_fp64_with_the_same_sweeps(ctx):
+    """The task's setting: forward and adjoint both stopped at L1 tolerance 1e-4.  The FP64
+    side runs the same Jacobi iterations for the same numbers of sweeps (forward from 1/n,
+    adjoint from grad_x), so the comparison isolates the FP32 arithmetic: edge and teleport
+    gradients within 1e-4 relative (L2).  The distance to the converged gradient (the
+    truncation of the solves themselves) is checked to be what the tolerance implies."""
+    g, n, ind, col, deg, rows, w_raw = setup_graph(ctx, 12, seed=4, wseed=2)
+    g.set_weights(w_raw, normalize=True)
+    A, wn, s = dense_system(n, ind, col, rows, w_raw)
+    v = np.full(n, 1.0 / n, np.float32)
+    c = O.normal_vector(n, seed=17)
+    x, fi = g.wpr_forward(v, D, 1e-4, 100)
+    gw_gpu, gv_gpu, bi = g.wpr_backward(x, c.astype(np.float32), D, 1e-4, 100, raw_weights=False)
+    kf, kb = fi["iterations"], bi["iterations"]
+    assert 1 < kf < 100 and 1 < kb < 100
+    dd = float(np.float32(D))
+    x64, itf = O.fp64_jacobi(n, ind, col, deg, w=wn, v=v.astype(np.float64), damping=dd, tol=0.0, max_iters=kf)
+    lam64, itb = O.fp64_adjoint(n, ind, col, deg, c, w=wn, damping=dd, tol=0.0, max_iters=kb)
+    assert itf == kf and itb == kb
+    gw64 = dd * lam64[rows] * x64[col]
+    gv64 = (1.0 - dd) * lam64
+    rel = lambda a, b: np.linalg.norm(a.astype(np.float64) - b) / np.linalg.norm(b)
+    assert rel(gw_gpu, gw64) < 1e-4
+    assert rel(gv_gpu, gv64) < 1e-4
+    # the stopping decisions of FP64 with the same tolerance (within one sweep: an error
+    # within FP32 rounding of the threshold may fall on either side)
+    _, itf_tol = O.fp64_jacobi(n, ind, col, deg, w=wn, v=v.astype(np.float64), damping=dd, tol=1e-4, max_iters=100)
+    _, itb_tol = O.fp64_adjoint(n, ind, col, deg, c, w=wn, damping=dd, tol=1e-4, max_iters=100)
+    assert abs(itf_tol - kf) <= 1 and abs(itb_tol - kb) <= 1
+    # and against the converged gradient: the truncation error a 1e-4 L1 tolerance leaves
+    M = np.eye(n) - D * A
+    lam = np.linalg.solve(M.T, c)
+    xd = np.linalg.solve(M, (1 - D) * v.astype(np.float64))
+    assert rel(gw_gpu, D * lam[rows] * xd[col]) < 1e-2
+
+
+def test_adjoint_with_large_gradient_matches_dense_fp64(ctx):
+    """A loss whose gradient is large (|grad_x| ~ 1e3 per vertex) with a positive threshold:
+    lambda matches the dense FP64 solution and the solve stops within a sweep of an FP64 solve
+    with the same threshold (the threshold is set above the FP32 rounding floor of the L1
+    change, n * ulp(lambda))."""
+    g, n, ind, col, deg, rows, w_raw = setup_graph(ctx, 12, seed=6, wseed=3)
+    g.set_weights(w_raw, normalize=True)
+    A, wn, _ = dense_system(n, ind, col, rows, w_raw)
+    v = np.full(n, 1.0 / n, np.float32)
+    c = (1000.0 * O.normal_vector(n, seed=5)).astype(np.float32)
+    x, _ = g.wpr_forward(v, D, 1e-7, 300)
+    thr = 10.0
+    _, gv, bi = g.wpr_backward(x, c, D, thr, 400, raw_weights=False)
+    dd = float(np.float32(D))
+    lam_it, it64 = O.fp64_adjoint(n, ind, col, deg, c.astype(np.float64), w=wn, damping=dd, tol=thr, max_iters=400)
+    assert bi["iterations"] > 40
+    assert abs(bi["iterations"] - it64) <= 1, (bi["iterations"], it64)
+    assert 0.0 < bi["final_error"] < thr
+    lam_gpu = gv.astype(np.float64) / (1.0 - dd)
+    lam = np.linalg.solve(np.eye(n) - dd * A.T, c.astype(np.float64))
+    assert np.linalg.norm(lam_gpu - lam) / np.linalg.norm(lam) < 1e-5
+    assert np.linalg.norm(lam_gpu - lam_it) / np.linalg.norm(lam_it) < 1e-5
+
+
+def test_adjoint_l1_change_does_not_overflow_rmat20(ctx):
+    """RMAT-20 (thousands of workgroups per sweep) with |grad_x| ~ 1e3: the first adjoint
+    sweeps change lambda by ~1e9 in L1.  The error trace must equal the FP64 iteration's
+    (first sweep: |d A^T g|_1) and the solve must stop within a sweep of the FP64 iteration
+    with the same threshold, with lambda equal to its result; a bounded-range accumulator
+    that saturates or wraps reports a wrong error and stops at the wrong sweep."""
+    g = ctx.rmat(20, 16, seed=2)
+    n = g.n
+    g.random_weights(seed=7, normalize=True)
+    ind, col, deg = g.csc()
+    wn = g.get_weights().astype(np.float64)
+    c = (1000.0 * O.normal_vector(n, seed=9)).astype(np.float32)
+    thr = 1e5  # above the FP32 rounding floor of an L1 change over 1M entries of size ~1e3-1e5
+    _, gv, bi = g.wpr_backward(np.full(n, 1.0 / n, np.float32), c, D, thr, 200, raw_weights=False)
+    dd = float(np.float32(D))
+    lam1, _ = O.fp64_adjoint(n, ind, col, deg, c.astype(np.float64), w=wn, damping=dd, tol=0.0, max_iters=1)
+    err0 = np.abs(lam1 - c.astype(np.float64)).sum()
+    assert err0 > 1e8
+    assert abs(bi["error_trace"][0] - err0) <= 1e-5 * err0, (bi["error_trace"][0], err0)
+    lam_it, it64 = O.fp64_adjoint(n, ind, col, deg, c.astype(np.float64), w=wn, damping=dd, tol=thr, max_iters=200)
+    assert bi["iterations"] > 30 and abs(bi["iterations"] - it64) <= 1, (bi["iterations"], it64)
+    lam_gpu = gv.astype(np.float64) / (1.0 - dd)
+    assert np.linalg.norm(lam_gpu - lam_it) / np.linalg.norm(lam_it) < 1e-4
 
 
 def test_full_size_adjoint_identities_rmat20(ctx):

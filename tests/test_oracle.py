@@ -314,6 +314,55 @@ def test_v11_blocks_limits():
     assert it_gs == 6 and np.array_equal(gs, x)
 
 
+@pytest.mark.parametrize("threshold", [1e-4, 0.0])
+def test_v11_parallel_one_thread_is_sequential_gauss_seidel(threshold):
+    """nwo_pr_v11_parallel (the CPU-baseline loop, page_rank.hpp:124-139) on one thread is the
+    sequential in-place sweep: bit for bit nwo_pr_v11_blocks(nblocks=1), which the pure-Python
+    Gauss-Seidel restatement above pins.  A Jacobi write (contrib not updated in place), an
+    unfused update or a dropped term in the error would each break the equality."""
+    n, ind, col, deg = O.rmat_csc(12)
+    gs, it_gs, tr_gs = O.pr_v11(n, ind, col, deg, threshold=threshold, max_iters=20, nblocks=1)
+    par, it_p, tr_p, sec = O.pr_v11_parallel(n, ind, col, deg, threshold=threshold, max_iters=20, nthreads=1)
+    assert it_p == it_gs and np.array_equal(par, gs) and np.array_equal(tr_p, tr_gs)
+    assert sec.size == it_p and np.all(sec > 0)
+    # and it is not the Jacobi iteration
+    jac, _, _, _ = O.pr_v10(n, ind, col, deg, threshold=threshold, max_iters=20)
+    assert not np.array_equal(par, jac)
+
+
+def test_v11_parallel_threads_reach_the_v10_fixed_point():
+    """With 8 threads the in-place loop is a schedule-dependent block Gauss-Seidel; it must
+    converge to the same fixed point as v10, and its error trace must fall."""
+    n, ind, col, deg = O.rmat_csc(12)
+    par, it, tr, _ = O.pr_v11_parallel(n, ind, col, deg, threshold=1e-9, max_iters=500, nthreads=8)
+    jac, _, _, _ = O.pr_v10(n, ind, col, deg, threshold=1e-9, max_iters=500)
+    x64, _ = O.fp64_jacobi(n, ind, col, deg)
+    assert it < 500 and tr[-1] < 1e-9
+    assert np.max(np.abs(par - jac)) < 1e-7
+    assert np.max(np.abs(par - x64)) < 1e-7
+    # the error falls sweep after sweep until it reaches float rounding noise (~1e-8 here)
+    above = tr[1:][tr[1:] > 1e-6]
+    assert above.size >= 10 and np.all(np.diff(above) < 0)
+
+
+def test_normal_vector_moments():
+    """nwo_normal_vector (Box-Muller over Philox words) against the moments of N(0, 1) over
+    10^6 draws (bounds ~7 standard errors), a Kolmogorov-Smirnov test, independence of the
+    two values of each Box-Muller pair, determinism per seed, and an odd length."""
+    import scipy.stats as st
+
+    x = O.normal_vector(1_000_000, seed=3)
+    assert np.all(np.isfinite(x))
+    assert abs(x.mean()) < 7e-3
+    assert abs(x.var() - 1.0) < 1e-2
+    assert abs(st.skew(x)) < 1.7e-2
+    assert abs(st.kurtosis(x)) < 3.5e-2
+    assert st.kstest(x, "norm").pvalue > 1e-3
+    assert abs(np.corrcoef(x[0::2], x[1::2])[0, 1]) < 7e-3
+    assert np.array_equal(x[:1001], O.normal_vector(1001, seed=3))
+    assert not np.array_equal(x[:1000], O.normal_vector(1000, seed=4))
+
+
 def test_v11_and_v10_converge_to_same_fixed_point():
     n, ind, col, deg = O.rmat_csc(10)
     gs, _, _ = O.pr_v11(n, ind, col, deg, threshold=1e-9, max_iters=500)
