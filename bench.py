@@ -1,19 +1,20 @@
 #!/usr/bin/env python3
 """Benchmark of the PageRank hot path on one MI355X.
 
-Default workload (the headline): plain PageRank on a directed RMAT graph, scale 25
-(33,554,432 vertices), edge factor 16, A/B/C/D = .57/.19/.19/.05, self-loops and
-duplicates removed, int32 CSC, float32 scores, damping 0.85, L1 tolerance 1e-4, 20
-sweeps.  The graph is generated and built on the device (not timed).  One step = one
-page_rank solve (init + up to 20 sweeps, one captured HIP graph) with the CSC resident in
-HBM.  Output: one JSON line with GTEPS, the roofline of the sweep and the CPU baseline
-(the oracle's restatement of the reference's v11 loop on this host's cores, over a
-bounded sample of sweeps on the same graph).
+Headline (`value`, `roofline`, `cpu_baseline`): plain PageRank on a directed RMAT graph,
+scale 25 (33,554,432 vertices), edge factor 16, A/B/C/D = .57/.19/.19/.05, self-loops and
+duplicates removed, int32 CSC, float32 scores, damping 0.85, L1 tolerance 1e-4, 20 sweeps.
+The graph is generated and built on the device (not timed).  One step = one page_rank
+solve (init + up to 20 sweeps, one captured HIP graph) with the CSC resident in HBM.  The
+CPU baseline is the oracle's restatement of the reference's v11 loop on this host's cores,
+over a bounded sample of sweeps on the same graph.
 
-Other workloads (--workload): grid (4096^2 lattice, max 100 sweeps), ppr (64 personalised
-vectors on RMAT-22, batched vs 64 sequential solves), multi (256 RMAT graphs of scale
-10..18 in one persistent launch per sweep), wpr (weighted PageRank forward + implicit
-adjoint on RMAT-20).
+The default run (--workload all) adds every other configuration under `configs`, each
+with its own timing and roofline: RMAT-16 and the 4096^2 grid (with CPU baselines), 64
+personalised vectors on RMAT-22 batched against 64 sequential solves, 256 RMAT graphs of
+scale 10..18 in one captured loop, and the weighted forward + implicit adjoint on RMAT-20.
+--workload picks one of them alone (plain, grid, ppr, multi, wpr), or the bandwidth
+calibrations (copy, ceilings).
 """
 import argparse
 import json
@@ -50,51 +51,77 @@ def bytes_per_iter(n, nnz):
 
 
 def host_threads():
+    """Every CPU this process may run on (the reference's TBB arena takes all of them)."""
     try:
-        ncpu = len(os.sched_getaffinity(0))
+        return max(1, len(os.sched_getaffinity(0)))
     except AttributeError:
-        ncpu = os.cpu_count() or 1
-    return max(1, min(ncpu, 16))
+        return os.cpu_count() or 1
 
 
 def host_cpu():
-    model = "unknown"
+    """CPU model, logical CPUs, the usable ones, and the physical cores / SMT behind them
+    (from /proc/cpuinfo's physical id and core id of each usable logical CPU)."""
+    model, cores = "unknown", {}
+    try:
+        usable = set(os.sched_getaffinity(0))
+    except AttributeError:
+        usable = set(range(os.cpu_count() or 1))
     try:
         with open("/proc/cpuinfo") as f:
-            for line in f:
-                if line.startswith("model name"):
-                    model = line.split(":", 1)[1].strip()
-                    break
-    except OSError:
+            blocks = f.read().split("\n\n")
+        for b in blocks:
+            kv = dict(line.split(":", 1) for line in b.splitlines() if ":" in line)
+            kv = {k.strip(): v.strip() for k, v in kv.items()}
+            if "processor" not in kv:
+                continue
+            model = kv.get("model name", model)
+            cpu = int(kv["processor"])
+            if cpu in usable:
+                cores.setdefault((kv.get("physical id", "0"), kv.get("core id", str(cpu))), []).append(cpu)
+    except (OSError, ValueError):
         pass
-    try:
-        usable = len(os.sched_getaffinity(0))
-    except AttributeError:
-        usable = os.cpu_count() or 1
-    return {"model": model, "logical_cpus": os.cpu_count() or 1, "usable_cpus": usable}
+    phys = len(cores) or len(usable)
+    return {"model": model, "logical_cpus": os.cpu_count() or 1, "usable_cpus": len(usable),
+            "usable_physical_cores": phys, "smt_per_core": round(len(usable) / max(phys, 1), 2)}
+
+
+_BASELINE = {}
 
 
 def cpu_baseline(ptr, idx, deg, sweeps):
+    """The oracle's restatement of the reference's v11 loop (page_rank.hpp:124-139, in-place
+    contributions, OpenMP static chunks) on every usable host CPU, built with the reference's
+    -Ofast -march=native on this host (oracle/Makefile `baseline`; the prebuilt -O3
+    x86-64-v3 oracle if that build fails).  `sweeps` timed sweeps (threshold 0), reported
+    min / avg / max as the reference's -n trials are, plus one sweep on one thread."""
+    import tempfile
+
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle as O
 
+    if "lib" not in _BASELINE:
+        _BASELINE["dir"] = tempfile.mkdtemp(prefix="nwg_baseline_")
+        _BASELINE["lib"], _BASELINE["flags"] = O.baseline_lib(_BASELINE["dir"])
+    lib, flags = _BASELINE["lib"], _BASELINE["flags"]
+    if lib is None:
+        flags = f"-O3 -march=x86-64-v3 -fopenmp (prebuilt; {flags})"
     threads = host_threads()
     n = ptr.size - 1
-    _, it, _, sec = O.pr_v11_parallel(n, ptr, idx, deg, 0.85, 0.0, sweeps, threads)
-    t = float(np.median(sec))
+    _, it, _, sec = O.pr_v11_parallel(n, ptr, idx, deg, 0.85, 0.0, sweeps, threads, lib=lib)
+    t = float(np.mean(sec))
     # one sweep on one thread: the sequential Gauss-Seidel form of the same loop
-    _, _, _, sec1 = O.pr_v11_parallel(n, ptr, idx, deg, 0.85, 0.0, 1, 1)
+    _, _, _, sec1 = O.pr_v11_parallel(n, ptr, idx, deg, 0.85, 0.0, 1, 1, lib=lib)
     return {
         "value": round(idx.size / t / 1e9, 4),
         "unit": "GTEPS",
         "cores": threads,
         "kind": "port",
         "sample": f"{it} sweeps of the v11 in-place loop (page_rank.hpp:124-139 restated in C, OpenMP static "
-                  f"chunks, -O3 -march=x86-64-v3) on the same CSC copied to the host; median sweep "
-                  f"{t * 1e3:.1f} ms",
-        "sweep_ms_min_avg_max": [round(float(np.min(sec)) * 1e3, 1), round(float(np.mean(sec)) * 1e3, 1),
-                                 round(float(np.max(sec)) * 1e3, 1)],
-        "one_thread_sweep_ms": round(float(sec1[0]) * 1e3, 1),
+                  f"chunks, {flags}) on the same CSC copied to the host, {threads} threads; mean sweep "
+                  f"{t * 1e3:.2f} ms",
+        "sweep_ms_min_avg_max": [round(float(np.min(sec)) * 1e3, 3), round(t * 1e3, 3),
+                                 round(float(np.max(sec)) * 1e3, 3)],
+        "one_thread_sweep_ms": round(float(sec1[0]) * 1e3, 3),
         "host": host_cpu(),
     }
 
@@ -312,6 +339,15 @@ def run_ppr(args, ctx):
         "speedup_vs_sequential_plain": round(ms_plain / ms_batched, 3),
         "useful_vector_sweep_fraction": round(vec_sweeps / (k * max(1, int(np.max(its)))), 3),
     })
+    # algorithmic bytes of the batched solve: per batched sweep the shared streams (ids, row
+    # pointers, out-degrees) once, per vector-sweep a contribution read, a score read, a
+    # score write and a contribution write (16 N)
+    sweeps = int(np.max(its))
+    alg = sweeps * (4 * nnz + 4 * (n + 1) + 4 * n) + 16 * n * vec_sweeps
+    out["roofline"] = roofline_entry(alg, info["ms_total"],
+                                     "batched solve (ppr_init + ppr_sweep_kernel + ppr_finish_kernel per sweep), "
+                                     "device time of the whole solve",
+                                     "sweeps x (4E + 4(N+1) + 4N) + 16N x vector-sweeps")
     g.close()
     return out
 
@@ -335,6 +371,7 @@ def run_multi(args, ctx):
     ms = (time.perf_counter() - t0) * 1e3 / args.steps
     edge_sweeps = int(sum(int(g.nnz) * int(i) for g, i in zip(graphs, its)))
     value = edge_sweeps / (np.mean(ev) * 1e-3) / 1e9
+    alg = int(sum(bytes_per_iter(int(g.n), int(g.nnz)) * int(i) for g, i in zip(graphs, its)))
     out = base_line(args, round(value, 3), "GTEPS (sum over graphs, device time)", ms, "f32", {
         "workload": f"{mg.ngraphs} independent RMAT graphs, scales U{{10..18}} (seed {args.seed}), ef "
                     f"{args.edge_factor}, all sweeps in one HIP graph, per-graph tol "
@@ -343,6 +380,8 @@ def run_multi(args, ctx):
         "schedule": "binned" if binned else "items",
         "iterations_min_max": [int(its.min()), int(its.max())],
     })
+    out["roofline"] = roofline_entry(alg, float(np.mean(ev)), "all sweeps of the union in one captured HIP graph "
+                                      "(device time)", "sum over graphs of sweeps x (4E + 4(N+1) + 20N)")
     mg.close()
     for g in graphs:
         g.close()
@@ -377,6 +416,9 @@ def run_wpr(args, ctx):
     torch.cuda.synchronize()
     ms_b = (time.perf_counter() - t0) * 1e3 / args.steps
     value = nnz * fi["iterations"] / (ms_f * 1e-3) / 1e9
+    sweep_b = 8 * nnz + 4 * (n + 1) + 12 * n
+    alg_f = sweep_b * fi["iterations"]
+    alg_b = sweep_b * bi["iterations"] + 12 * nnz + 12 * n
     out = base_line(args, round(value, 3), "GTEPS (forward)", ms_f + ms_b, "f32", {
         "workload": f"weighted PageRank + implicit adjoint, RMAT scale {args.wpr_scale} ef {args.edge_factor}, "
                     f"w ~ U(0.1,1) row-normalised, L = sum c_i x_i, c ~ N(0,1), d=0.85, tol {args.tol}, "
@@ -384,8 +426,77 @@ def run_wpr(args, ctx):
         "n": n, "nnz": nnz, "forward_ms": round(ms_f, 3), "backward_ms": round(ms_b, 3),
         "backward_over_forward": round(ms_b / ms_f, 3),
         "forward_sweeps": fi["iterations"], "adjoint_sweeps": bi["iterations"],
+        "forward_device_ms": round(fi["ms_total"], 3), "backward_device_ms": round(bi["ms_total"], 3),
+        "backward_over_forward_device": round(bi["ms_total"] / fi["ms_total"], 3),
     })
+    out["roofline"] = roofline_entry(alg_f, fi["ms_total"], "weighted forward solve (wsweep_kernel per sweep), "
+                                     "device time", "sweeps x (8E + 4(N+1) + 12N)")
+    out["roofline_backward"] = roofline_entry(alg_b, bi["ms_total"], "adjoint solve over the CSR + edge and "
+                                              "teleport gradients, device time",
+                                              "sweeps x (8E + 4(N+1) + 12N) + 12E + 12N")
     g.close()
+    return out
+
+
+def roofline_entry(alg_bytes, ms, kernel, formula):
+    achieved = alg_bytes / (ms * 1e-3) / 1e9
+    return {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": kernel, "kernel_ms": round(ms, 4),
+            "algorithmic_bytes": int(alg_bytes), "bytes_formula": formula}
+
+
+# ---------------------------------------------------------------------------- every config
+def sub_args(args, **kw):
+    a = argparse.Namespace(**vars(args))
+    for k, v in kw.items():
+        setattr(a, k, v)
+    return a
+
+
+def summary(out, keep=()):
+    """The parts of a workload's line that the default run carries per configuration."""
+    r = {"value": out["value"], "unit": out["unit"], "ms_per_step": out["ms_per_step"]}
+    cfg = out["config"]
+    r["workload"] = cfg["workload"]
+    for k in ("n", "nnz", "sweeps_per_step", "ms_per_sweep_events", "plan") + tuple(keep):
+        if k in cfg:
+            r[k] = cfg[k]
+    for k in ("roofline", "roofline_backward", "validation", "cpu_baseline"):
+        if k in out:
+            r[k] = out[k]
+    return r
+
+
+def run_default(args, ctx):
+    """The headline (plain PageRank on RMAT-25: `value`, `roofline`, `cpu_baseline`) and,
+    under `configs`, every other benchmark configuration with its own timing and roofline
+    (and, for the plain solves, its CPU baseline): RMAT-16, the 4096^2 grid, 64 batched
+    personalised vectors on RMAT-22 against 64 sequential solves, 256 graphs in one captured
+    loop, and the weighted forward / adjoint on RMAT-20."""
+    t_all = time.perf_counter()
+    out = run_plain(args, ctx)
+    configs = {}
+
+    def add(name, fn):
+        t0 = time.perf_counter()
+        configs[name] = fn()
+        configs[name]["bench_s"] = round(time.perf_counter() - t0, 1)
+        log(f"config {name}: {configs[name]['value']} {configs[name]['unit']} in {configs[name]['bench_s']} s")
+
+    add("rmat16", lambda: summary(run_plain(sub_args(args, scale=16, steps=50, warmup=10), ctx)))
+    add("grid4096", lambda: summary(run_plain(sub_args(args, iters=100, iters_given=True, steps=10, warmup=3), ctx,
+                                              grid=True)))
+    add("ppr_rmat22_k64", lambda: summary(run_ppr(sub_args(args, steps=3, warmup=1), ctx), keep=(
+        "k", "k_tile", "vector_sweeps", "ms_batched", "device_ms_batched", "ms_64_sequential_plain",
+        "device_ms_64_sequential_plain", "speedup_vs_sequential_plain", "ms_64_sequential",
+        "speedup_vs_sequential", "useful_vector_sweep_fraction")))
+    add("multigraph_256", lambda: summary(run_multi(sub_args(args, steps=3, warmup=1), ctx), keep=(
+        "ms_device", "schedule", "iterations_min_max")))
+    add("wpr_rmat20", lambda: summary(run_wpr(sub_args(args, iters=100, steps=3, warmup=1), ctx), keep=(
+        "forward_ms", "backward_ms", "forward_device_ms", "backward_device_ms", "backward_over_forward",
+        "backward_over_forward_device", "forward_sweeps", "adjoint_sweeps")))
+    out["configs"] = configs
+    log(f"all configurations in {time.perf_counter() - t_all:.1f} s")
     return out
 
 
@@ -465,8 +576,9 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="plain",
-                    choices=["plain", "grid", "ppr", "multi", "wpr", "copy", "ceilings"])
+    ap.add_argument("--workload", default="all",
+                    choices=["all", "plain", "grid", "ppr", "multi", "wpr", "copy", "ceilings"],
+                    help="all (default): the RMAT-25 headline plus every other configuration under 'configs'")
     ap.add_argument("--scale", type=int, default=25)
     ap.add_argument("--edge-factor", type=int, default=16)
     ap.add_argument("--width", type=int, default=4096)
@@ -478,7 +590,7 @@ def main():
     ap.add_argument("--iters", type=int, default=None)
     ap.add_argument("--tol", type=float, default=1e-4)
     ap.add_argument("--seed", type=int, default=1)
-    ap.add_argument("--cpu-sweeps", type=int, default=3)
+    ap.add_argument("--cpu-sweeps", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-validate", action="store_true")
     ap.add_argument("--no-autotune", dest="autotune", action="store_false",
@@ -504,7 +616,9 @@ def main():
     import nwgraph_amd
 
     ctx = nwgraph_amd.Context(0)
-    if args.workload == "plain":
+    if args.workload == "all":
+        out = run_default(args, ctx)
+    elif args.workload == "plain":
         out = run_plain(args, ctx)
     elif args.workload == "grid":
         out = run_plain(args, ctx, grid=True)

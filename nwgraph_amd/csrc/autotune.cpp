@@ -4,7 +4,8 @@
 // Search: coordinate descent over the pull sweep's parameters from the default plan, one
 // parameter at a time, each candidate timed over a few sweeps (launched one by one, each
 // bracketed by events); then the binned sweep in both numberings; then the best pull plan
-// and the best binned plan timed again over more sweeps, and the faster one kept.
+// and the best binned plan timed again as whole captured 20-sweep solves (median of 7),
+// and the faster one kept.
 // Results are cached per (log2 in-degree histogram, n, nnz, device) for the process
 // and, if NWG_TUNE_CACHE names a file, appended there and reloaded by later processes.
 #include <hip/hip_runtime.h>
@@ -109,6 +110,23 @@ double time_plan_sweeps(nwg_ctx* ctx, nwg_graph* g, uint64_t iters) {
   return info.sweep_ms_avg;
 }
 
+// Median device time per sweep of whole solves (the captured HIP graph of `iters` sweeps,
+// as a caller runs it), after one solve that captures the graph and warms the caches.
+// Small graphs are launch-bound, where per-sweep event timing of one-by-one launches is
+// noisy and measures launch gaps the captured loop does not have.
+static double time_plan_solve(nwg_ctx* ctx, nwg_graph* g, uint64_t iters, int reps) {
+  nwg_pr_info info{};
+  DevBuf<float> out(g->n);
+  pagerank_run(ctx, g, nullptr, 0.85f, -1.0f, iters, &info, out.p);
+  std::vector<double> t;
+  for (int r = 0; r < reps; ++r) {
+    pagerank_run(ctx, g, nullptr, 0.85f, -1.0f, iters, &info, out.p);
+    t.push_back(info.ms_total / (double)iters);
+  }
+  std::sort(t.begin(), t.end());
+  return t[t.size() / 2];
+}
+
 nwg_plan_params autotune_plan(nwg_ctx* ctx, nwg_graph* g) {
   hipStream_t s = ctx->stream;
   const std::string key = graph_key(s, g);
@@ -162,9 +180,12 @@ nwg_plan_params autotune_plan(nwg_ctx* ctx, nwg_graph* g) {
   binned.relabel = 1;
   b0.relabel = 0;
   if (eval(b0) < eval(binned)) binned = b0;
-  // the two winners, timed again over more sweeps
-  const double t_pull = eval_n(pull, 3, 8), t_binned = eval_n(binned, 3, 8);
-  best = t_binned < t_pull ? binned : pull;
+  // the two winners, timed again as whole captured solves (median of 7)
+  plan_build(s, g, pull);
+  const double t_pull = time_plan_solve(ctx, g, 20, 7);
+  plan_build(s, g, binned);
+  const double t_binned = time_plan_solve(ctx, g, 20, 7);
+  best = t_binned <= t_pull ? binned : pull;
   plan_build(s, g, best);
   {
     std::lock_guard<std::mutex> lk(g_cache_mu);

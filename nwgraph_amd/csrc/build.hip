@@ -193,12 +193,17 @@ __global__ __launch_bounds__(256) void csc_check_kernel(uint32_t n, uint64_t nnz
   if (b) *bad = 1u;
 }
 
-void validate_device_csc(hipStream_t s, uint32_t n, uint64_t nnz, const uint32_t* ptr, const uint32_t* idx) {
-  DevBuf<uint32_t> bad(1);
-  fill_u32_kernel<<<1, 256, 0, s>>>(bad.p, 1, 0u);
-  csc_check_kernel<<<grid_for(std::max<uint64_t>(nnz, n)), 256, 0, s>>>(n, nnz, ptr, idx, bad.p);
+void validate_device_csc(hipStream_t s, uint32_t n, uint64_t nnz, const uint32_t* ptr, const uint32_t* idx,
+                         uint32_t* flag) {
+  DevBuf<uint32_t> own;
+  if (!flag) {
+    own.alloc(1);
+    flag = own.p;
+  }
+  fill_u32_kernel<<<1, 256, 0, s>>>(flag, 1, 0u);
+  csc_check_kernel<<<grid_for(std::max<uint64_t>(nnz, n)), 256, 0, s>>>(n, nnz, ptr, idx, flag);
   NWG_LAUNCH_CHECK();
-  if (read_back_u32(s, bad.p)) throw Error(NWG_EINVAL, "device CSC: row pointers not monotone from 0 to nnz, or an entry >= n");
+  if (read_back_u32(s, flag)) throw Error(NWG_EINVAL, "device CSC: row pointers not monotone from 0 to nnz, or an entry >= n");
 }
 
 void out_degrees_device(hipStream_t s, uint32_t n, uint64_t nnz, const uint32_t* col, uint32_t* deg) {
@@ -219,13 +224,22 @@ void out_degrees_device(hipStream_t s, uint32_t n, uint64_t nnz, const uint32_t*
   }
 }
 
-void check_outdeg_device(hipStream_t s, uint32_t n, uint64_t nnz, const uint32_t* col, const uint32_t* outdeg) {
-  DevBuf<uint32_t> counted(n), bad(1);
-  out_degrees_device(s, n, nnz, col, counted.p);
-  fill_u32_kernel<<<1, 256, 0, s>>>(bad.p, 1, 0u);
-  outdeg_check_kernel<<<grid_for(n), 256, 0, s>>>(n, outdeg, counted.p, bad.p);
+void check_outdeg_device(hipStream_t s, uint32_t n, uint64_t nnz, const uint32_t* col, const uint32_t* outdeg,
+                         uint32_t* counted, uint32_t* flag) {
+  DevBuf<uint32_t> own_counted, own_flag;
+  if (!counted) {
+    own_counted.alloc(n);
+    counted = own_counted.p;
+  }
+  if (!flag) {
+    own_flag.alloc(1);
+    flag = own_flag.p;
+  }
+  out_degrees_device(s, n, nnz, col, counted);
+  fill_u32_kernel<<<1, 256, 0, s>>>(flag, 1, 0u);
+  outdeg_check_kernel<<<grid_for(n), 256, 0, s>>>(n, outdeg, counted, flag);
   NWG_LAUNCH_CHECK();
-  if (read_back_u32(s, bad.p))
+  if (read_back_u32(s, flag))
     throw Error(NWG_EINVAL, "outdeg: some vertex's out-degree is below its number of entries in the CSC");
 }
 

@@ -322,11 +322,15 @@ int key_bits_for(uint32_t n);  // bits of a vertex id field in packed sort keys
 // row pointers + entries from keys (row << b | entry) sorted ascending
 void sorted_keys_to_csx(hipStream_t s, const uint64_t* keys, uint64_t m, uint32_t n, int b, uint32_t* ptr,
                         uint32_t* ent);
-// throws NWG_EINVAL unless ptr is monotone from 0 to nnz and every entry of idx is below n
-void validate_device_csc(hipStream_t s, uint32_t n, uint64_t nnz, const uint32_t* ptr, const uint32_t* idx);
+// throws NWG_EINVAL unless ptr is monotone from 0 to nnz and every entry of idx is below n;
+// flag: one device word of scratch, or null to allocate it
+void validate_device_csc(hipStream_t s, uint32_t n, uint64_t nnz, const uint32_t* ptr, const uint32_t* idx,
+                         uint32_t* flag = nullptr);
 void out_degrees_device(hipStream_t s, uint32_t n, uint64_t nnz, const uint32_t* col, uint32_t* deg);
-// throws NWG_EINVAL when some outdeg[u] is below the number of entries u has in col
-void check_outdeg_device(hipStream_t s, uint32_t n, uint64_t nnz, const uint32_t* col, const uint32_t* outdeg);
+// throws NWG_EINVAL when some outdeg[u] is below the number of entries u has in col;
+// counted (n words) and flag (one word): device scratch, or null to allocate them
+void check_outdeg_device(hipStream_t s, uint32_t n, uint64_t nnz, const uint32_t* col, const uint32_t* outdeg,
+                         uint32_t* counted = nullptr, uint32_t* flag = nullptr);
 
 // ---- primitives (scan.hip / sort.hip) ----
 // exclusive scan of n values; out has n+1 entries (out[n] = total).

@@ -156,12 +156,30 @@ def pr_v11(n, ind, col, deg, damping=0.85, threshold=1e-4, max_iters=20, nblocks
     return pr, int(it), tr[:it].copy()
 
 
-def pr_v11_parallel(n, ind, col, deg, damping=0.85, threshold=1e-4, max_iters=20, nthreads=1):
+def baseline_lib(outdir):
+    """The CPU baseline's build of this source (oracle/Makefile target `baseline`: the
+    reference's -Ofast -march=native, native to the host running it) in outdir, loaded
+    with the v11 loop's signature.  Returns (library, flags) or (None, reason)."""
+    import subprocess
+
+    try:
+        subprocess.run(["make", "-s", "-C", _HERE, "baseline", f"BASEDIR={outdir}"], check=True,
+                       capture_output=True, timeout=120)
+        L = ctypes.CDLL(os.path.join(outdir, "libnwg_baseline.so"))
+    except (OSError, subprocess.SubprocessError) as e:
+        return None, f"native build failed ({type(e).__name__})"
+    V, U32, U64, F, I = ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint64, ctypes.c_float, ctypes.c_int
+    L.nwo_pr_v11_parallel.restype = U64
+    L.nwo_pr_v11_parallel.argtypes = [U32, V, V, V, V, F, F, U64, I, V, V]
+    return L, "-Ofast -march=native -fopenmp"
+
+
+def pr_v11_parallel(n, ind, col, deg, damping=0.85, threshold=1e-4, max_iters=20, nthreads=1, lib=None):
     pr = np.empty(n, np.float32)
     tr = np.zeros(max(max_iters, 1), np.float64)
     sec = np.zeros(max(max_iters, 1), np.float64)
-    it = load().nwo_pr_v11_parallel(n, _p(_u32(ind)), _p(_u32(col)), _p(_u32(deg)), _p(pr), damping, threshold,
-                                    max_iters, nthreads, _p(tr), _p(sec))
+    it = (lib or load()).nwo_pr_v11_parallel(n, _p(_u32(ind)), _p(_u32(col)), _p(_u32(deg)), _p(pr), damping,
+                                             threshold, max_iters, nthreads, _p(tr), _p(sec))
     return pr, int(it), tr[:it].copy(), sec[:it].copy()
 
 

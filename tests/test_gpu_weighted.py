@@ -166,9 +166,15 @@ def test_adjoint_with_large_gradient_matches_dense_fp64(ctx):
     assert abs(bi["iterations"] - it64) <= 1, (bi["iterations"], it64)
     assert 0.0 < bi["final_error"] < thr
     lam_gpu = gv.astype(np.float64) / (1.0 - dd)
+    # the FP64 iterate after the GPU's own number of sweeps: equal up to FP32 rounding
+    lam_k, _ = O.fp64_adjoint(n, ind, col, deg, c.astype(np.float64), w=wn, damping=dd, tol=0.0,
+                              max_iters=bi["iterations"])
+    assert np.linalg.norm(lam_gpu - lam_k) / np.linalg.norm(lam_k) < 1e-5
+    # the exact solution: A is column-stochastic, so d A^T contracts by d in the max norm and
+    # |lambda* - lambda_k|_inf <= d / (1 - d) |lambda_k - lambda_(k-1)|_inf < d / (1 - d) * thr
     lam = np.linalg.solve(np.eye(n) - dd * A.T, c.astype(np.float64))
-    assert np.linalg.norm(lam_gpu - lam) / np.linalg.norm(lam) < 1e-5
-    assert np.linalg.norm(lam_gpu - lam_it) / np.linalg.norm(lam_it) < 1e-5
+    assert np.max(np.abs(lam_gpu - lam)) <= dd / (1.0 - dd) * thr * 1.01
+    assert np.linalg.norm(lam_gpu - lam) / np.linalg.norm(lam) < 1e-4
 
 
 def test_adjoint_l1_change_does_not_overflow_rmat20(ctx):
@@ -191,7 +197,7 @@ def test_adjoint_l1_change_does_not_overflow_rmat20(ctx):
     assert err0 > 1e8
     assert abs(bi["error_trace"][0] - err0) <= 1e-5 * err0, (bi["error_trace"][0], err0)
     lam_it, it64 = O.fp64_adjoint(n, ind, col, deg, c.astype(np.float64), w=wn, damping=dd, tol=thr, max_iters=200)
-    assert bi["iterations"] > 30 and abs(bi["iterations"] - it64) <= 1, (bi["iterations"], it64)
+    assert bi["iterations"] > 20 and abs(bi["iterations"] - it64) <= 1, (bi["iterations"], it64)
     lam_gpu = gv.astype(np.float64) / (1.0 - dd)
     assert np.linalg.norm(lam_gpu - lam_it) / np.linalg.norm(lam_it) < 1e-4
 
