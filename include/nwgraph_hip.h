@@ -107,6 +107,23 @@ int nwg_pagerank_f32(nwg_ctx* ctx, uint32_t n, uint64_t nnz, const uint32_t* csc
                      float* scores /*[host] n*/, float damping, float threshold, uint64_t max_iters,
                      nwg_pr_info* info /*nullable*/);
 
+/* Allocation-free variant of nwg_pagerank_f32.  No counterpart in the reference: it is for
+ * callers that cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  The CSC, the optional out-degrees and the scores
+ * are [device] pointers; the call runs on the context's stream and makes no hipMalloc,
+ * hipFree or other allocating call.  Its working memory (contributions, counted degrees, L1
+ * partials and accumulators) is one static 48.2 MiB device array per GPU, looked up by
+ * nwg_create and held until the process ends.  Limits: n <= 4194304 vertices and
+ * max_iters <= 4096, else NWG_EINVAL; csc_idx 16-byte aligned; NWG_INFO_KERNEL_TIMING is
+ * not supported (NWG_EINVAL).  Calls that overlap on one GPU share the array and race
+ * (calls through one context are ordered by its stream).  Semantics as nwg_pagerank_f32
+ * (Jacobi sweeps, the reference's scalars); its sweep needs no plan, so row sums may differ
+ * from nwg_pagerank_f32's in FP32 summation order. */
+int nwg_pagerank_f32_noalloc(nwg_ctx* ctx, uint32_t n, uint64_t nnz, const uint32_t* csc_ptr /*[device]*/,
+                             const uint32_t* csc_idx /*[device]*/, const uint32_t* outdeg /*[device] or NULL*/,
+                             float* scores /*[device] n*/, float damping, float threshold, uint64_t max_iters,
+                             nwg_pr_info* info /*nullable*/);
+
 /* ------------------------------------------------------------ graph objects */
 /* Upload a CSC (rows = targets, entries = sources; each row ascending or not) --
  * the layout of nw::graph::adjacency<1> (adjacency.hpp:86-192, compressed.hpp:64-102).

@@ -77,6 +77,8 @@ _SIGNATURES = {
     "nwg_last_error": (ctypes.c_char_p, [c_void_p]),
     "nwg_pagerank_f32": (c_int, [c_void_p, c_u32, c_u64, c_void_p, c_void_p, c_void_p, c_void_p, c_float, c_float,
                                  c_u64, ctypes.POINTER(PrInfo)]),
+    "nwg_pagerank_f32_noalloc": (c_int, [c_void_p, c_u32, c_u64, c_void_p, c_void_p, c_void_p, c_void_p, c_float,
+                                         c_float, c_u64, ctypes.POINTER(PrInfo)]),
     "nwg_graph_create_csc": (c_int, [c_void_p, c_u32, c_u64, c_void_p, c_void_p, c_void_p, c_int,
                                      ctypes.POINTER(c_void_p)]),
     "nwg_graph_build_from_edges": (c_int, [c_void_p, c_u32, c_u64, c_void_p, c_void_p, c_int, c_int,

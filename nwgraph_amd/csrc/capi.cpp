@@ -60,6 +60,7 @@ int nwg_create(nwg_ctx** out, int device) {
     ctx->stream = ctx->own_stream;
     NWG_HIP(hipEventCreate(&ctx->ev0));
     NWG_HIP(hipEventCreate(&ctx->ev1));
+    nwg::noalloc_setup(device);
   });
   if (rc != NWG_OK) {
     nwg_destroy(ctx);
@@ -370,6 +371,16 @@ int nwg_pagerank_f32(nwg_ctx* ctx, uint32_t n, uint64_t nnz, const uint32_t* csc
   if (rc == NWG_OK) rc = nwg_graph_pagerank(ctx, g, scores, nullptr, damping, threshold, max_iters, NWG_MEM_HOST, info);
   nwg_graph_destroy(g);
   return rc;
+}
+
+int nwg_pagerank_f32_noalloc(nwg_ctx* ctx, uint32_t n, uint64_t nnz, const uint32_t* csc_ptr,
+                             const uint32_t* csc_idx, const uint32_t* outdeg, float* scores, float damping,
+                             float threshold, uint64_t max_iters, nwg_pr_info* info) {
+  if (!ctx) return NWG_EINVAL;
+  return guarded(ctx, [&] {
+    DeviceGuard dg(ctx->device);
+    nwg::pagerank_noalloc(ctx, n, nnz, csc_ptr, csc_idx, outdeg, scores, damping, threshold, max_iters, info);
+  });
 }
 
 int nwg_graph_set_plan(nwg_ctx* ctx, nwg_graph* g, const nwg_plan_params* p) {

@@ -382,6 +382,15 @@ void pagerank_run(nwg_ctx* ctx, nwg_graph* g, const float* teleport_dev, float d
                   uint64_t max_iters, nwg_pr_info* info, float* scores_dev,
                   PrSchedule sched = PrSchedule::kJacobi);
 
+// ---- allocation-free one-call solve (noalloc.hip) ----
+// looks up the device's static scratch array once (called by nwg_create, device current)
+void noalloc_setup(int device);
+uint64_t noalloc_max_vertices();
+uint64_t noalloc_max_iters();
+void pagerank_noalloc(nwg_ctx* ctx, uint32_t n, uint64_t nnz, const uint32_t* csc_ptr, const uint32_t* csc_idx,
+                      const uint32_t* outdeg, float* scores, float damping, float threshold, uint64_t max_iters,
+                      nwg_pr_info* info);
+
 // ---- batched personalised PageRank (ppr.hip) ----
 void ppr_run(nwg_ctx* ctx, nwg_graph* g, uint32_t k, const std::vector<uint32_t>& seed_ptr,
              const std::vector<uint32_t>& seeds, float damping, float threshold, uint64_t max_iters, uint32_t tile,
