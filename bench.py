@@ -571,13 +571,54 @@ def run_ceilings(args, ctx):
     return out
 
 
+def run_mall(args, ctx):
+    """Whether a buffer is served from the last-level cache (MALL) rather than HBM, by size:
+    the rate of a pass that reads one buffer 16 times in one launch, and of a read that follows
+    a write of the same buffer (as a pass that consumes what the previous pass produced)."""
+    import ctypes
+
+    import torch
+
+    ms = ctypes.c_double()
+    sizes = [32, 64, 128, 192, 256, 384, 512, 1024, 4096]
+    a = torch.empty((max(sizes) << 20) // 4, dtype=torch.float32, device="cuda").fill_(1.0)
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+
+    def probe(kind, flags, dst, src, nbytes):
+        ctx.check(ctx.lib.nwg_stream_probe(ctx.handle, kind, flags, 2, ctypes.c_void_p(dst.data_ptr()),
+                                           ctypes.c_void_p(src.data_ptr()), nbytes, ctypes.byref(ms)))
+        return ms.value
+
+    rows = []
+    for mib in sizes:
+        nb = mib << 20
+        rr, wr, rd = [], [], []
+        for _ in range(args.warmup + args.steps):
+            rr.append(probe(3, 0, b, a, nb))
+            wr.append(probe(2, 0, b, a, nb))
+            rd.append(probe(1, 0, a, b, nb))
+        k = args.warmup
+        row = {"MiB": mib, "reread16_GBps": round(16 * nb / (np.median(rr[k:]) * 1e-3) / 1e9, 1),
+               "write_GBps": round(nb / (np.median(wr[k:]) * 1e-3) / 1e9, 1),
+               "read_after_write_GBps": round(nb / (np.median(rd[k:]) * 1e-3) / 1e9, 1),
+               "read_after_write_us": round(float(np.median(rd[k:])) * 1e3, 2)}
+        rows.append(row)
+        log(f"{mib:5d} MiB: reread x16 {row['reread16_GBps']:8.1f} GB/s, write {row['write_GBps']:8.1f}, read after "
+            f"write {row['read_after_write_GBps']:8.1f} GB/s ({row['read_after_write_us']} us)")
+    return base_line(args, rows[0]["reread16_GBps"], "GB/s (reread x16 at the smallest size)", 0.0, "u8", {
+        "workload": "cache residency by buffer size: one buffer read 16 times per launch; a read after a write of "
+                    "the same buffer; 2 workgroups of 256 lanes per CU, four 16-byte accesses in flight per lane",
+        "rows": rows})
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="all",
-                    choices=["all", "plain", "grid", "ppr", "multi", "wpr", "copy", "ceilings"],
+                    choices=["all", "plain", "grid", "ppr", "multi", "wpr", "copy", "ceilings", "mall"],
                     help="all (default): the RMAT-25 headline plus every other configuration under 'configs'")
     ap.add_argument("--scale", type=int, default=25)
     ap.add_argument("--edge-factor", type=int, default=16)
@@ -630,6 +671,8 @@ def main():
         out = run_copy(args, ctx)
     elif args.workload == "ceilings":
         out = run_ceilings(args, ctx)
+    elif args.workload == "mall":
+        out = run_mall(args, ctx)
     else:
         out = run_wpr(args, ctx)
     print(json.dumps(out), flush=True)

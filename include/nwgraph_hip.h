@@ -74,7 +74,9 @@ typedef struct nwg_plan_params {
   uint32_t slice_min;    /* rows with slice_min <= in-degree < hub_min split their in-edges by source
                             slice ((id / 32) mod 8) so that each XCD gathers from one slice (0 = off;
                             else above short_max); costs a second copy of the column ids */
-  uint32_t stream_hints; /* non-temporal hints: 0 none, 1 the column-id stream, 2 every streamed array */
+  uint32_t stream_hints; /* non-temporal hints.  Pull sweep: 0 none, 1 the column-id stream, 2 every
+                            streamed array.  Binned sweep: 0 none, 1 loads of the streams read once
+                            per sweep, 2 also the stores of pair values, partial rows and scores */
   uint32_t ppr_tile;     /* k-tile of the batched solve: 8, 16, 32 or 64 vectors per lane group (0 = widest <= k) */
   uint32_t relabel;      /* 1: sweep a copy of the CSC renumbered by out-degree (results returned in original ids) */
   uint32_t binned;       /* 1: binned sweep (propagation blocking over a tiled copy of the CSC, 8 bytes per
@@ -285,13 +287,15 @@ int nwg_wpr_backward(nwg_ctx* ctx, nwg_graph* g, const float* x, const float* gr
  * streaming kernel; returns the kernel time in *ms.  Used to calibrate HBM bandwidth and
  * the hardware byte counters against a transfer of known size. */
 int nwg_stream_copy(nwg_ctx* ctx, void* dst, const void* src, uint64_t bytes, double* ms);
-/* Streaming ceilings: kind 0 copies src to dst, 1 only reads src, 2 only writes dst;
+/* Streaming ceilings: kind 0 copies src to dst, 1 only reads src, 2 only writes dst, 3 reads
+ * src 16 times in one launch (the rate of a buffer that stays cached);
  * flags bit 0 = nontemporal loads, bit 1 = nontemporal stores; wg_per_cu workgroups of
  * 256 lanes per CU (1..32), four 16-byte accesses in flight per lane.  Returns the kernel
  * time in *ms.  No counterpart in the reference (a measurement aid). */
 #define NWG_PROBE_COPY 0u
 #define NWG_PROBE_READ 1u
 #define NWG_PROBE_WRITE 2u
+#define NWG_PROBE_REREAD 3u
 #define NWG_PROBE_NT_LOAD 1u
 #define NWG_PROBE_NT_STORE 2u
 int nwg_stream_probe(nwg_ctx* ctx, uint32_t kind, uint32_t flags, uint32_t wg_per_cu, void* dst,

@@ -180,6 +180,21 @@ nwg_plan_params autotune_plan(nwg_ctx* ctx, nwg_graph* g) {
   binned.relabel = 1;
   b0.relabel = 0;
   if (eval(b0) < eval(binned)) binned = b0;
+  // and its cache hints (non-temporal loads / stores of the once-read streams)
+  {
+    double bt = 1e30;
+    nwg_plan_params bh = binned;
+    for (uint32_t h : {0u, 1u, 2u}) {
+      nwg_plan_params c = binned;
+      c.stream_hints = h;
+      const double ms = eval(c);
+      if (ms < bt) {
+        bt = ms;
+        bh = c;
+      }
+    }
+    binned = bh;
+  }
   // the two winners, timed again as whole captured solves (median of 7)
   plan_build(s, g, pull);
   const double t_pull = time_plan_solve(ctx, g, 20, 7);

@@ -48,6 +48,23 @@ constexpr uint32_t kNoSlot = 0xFFFFFFFFu;     // bin-pass item of a bin that is 
 constexpr uint32_t kGroups1 = 4;              // groups of 8 entries in flight per lane: chunk pass
 constexpr uint32_t kGroups2 = 2;              // and bin pass (more would cost it its second workgroup per CU)
 
+// Cache hints (the plan's stream_hints): NT >= 1 loads the streams each lane reads once
+// (row offsets, source offsets, partial rows, staged contributions, scores, out-degrees)
+// non-temporally; NT >= 2 also stores the pair values, partial rows, scores and
+// contributions non-temporally.  Pair values and hot-tile source ids are always cached
+// loads: the eight entry slots of a lane re-read the same lines.
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+template <int NT, class T>
+__device__ __forceinline__ T bld(const T* p) {
+  if constexpr (NT >= 1) return __builtin_nontemporal_load(p);
+  else return *p;
+}
+template <int NT, class T>
+__device__ __forceinline__ void bst(T* p, T v) {
+  if constexpr (NT >= 2) __builtin_nontemporal_store(v, p);
+  else *p = v;
+}
+
 // Row sums are exact 64-bit fixed-point sums in units of 2^-60.  A float contribution
 // c in [0, 1] is converted with float operations that are all exact: s = c * 2^28,
 // h = floor(s) (h <= 2^28; (float)h is exact), r = s - h (the fraction bits of s), and
@@ -105,12 +122,13 @@ __device__ __forceinline__ bool graph_frozen(const BinParams& P, uint32_t g) {
 
 
 // the reference's row update (page_rank.hpp:133-135) -> |x_new - x_old|
+template <int NT>
 __device__ __forceinline__ double finish_row(const BinParams& P, uint32_t i, float z) {
-  const float b = P.basev ? P.basev[i] : P.base;
-  const float xo = P.x[i];
+  const float b = P.basev ? bld<NT>(P.basev + i) : P.base;
+  const float xo = bld<NT>(P.x + i);
   const float xn = __builtin_fmaf(P.damp, z, b);
-  P.x[i] = xn;
-  P.cout[i] = xn / (float)P.deg[i];
+  bst<NT>(P.x + i, xn);
+  bst<NT>(P.cout + i, xn / (float)bld<NT>(P.deg + i));
   return (double)fabsf(xn - xo);
 }
 
@@ -126,7 +144,7 @@ constexpr uint32_t kGraphSlots = 64;  // graphs whose L1 changes a workgroup fir
 // flight together.
 // MULTI: whether P.row_graph is set (the kernels are instantiated per case, so the one-graph
 // kernels carry no per-graph code).
-template <int NT, int MAXR, bool MULTI, class Z>
+template <int NT, int MAXR, bool MULTI, int HINT, class Z>
 __device__ __forceinline__ void finish_rows(const BinParams& P, uint32_t r0, uint32_t nrows, Z z, double* red,
                                             unsigned long long* gerr) {
   static_assert(MAXR % NT == 0, "whole rounds of rows");
@@ -136,7 +154,7 @@ __device__ __forceinline__ void finish_rows(const BinParams& P, uint32_t r0, uin
 #pragma unroll
     for (uint32_t k = 0; k < MAXR / NT; ++k) {
       const uint32_t v = threadIdx.x + k * NT;
-      if (v < nrows) e += finish_row(P, r0 + v, z(v));
+      if (v < nrows) e += finish_row<HINT>(P, r0 + v, z(v));
     }
     e = wave_sum(e);
     if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = e;
@@ -156,7 +174,7 @@ __device__ __forceinline__ void finish_rows(const BinParams& P, uint32_t r0, uin
       if (v >= nrows) continue;
       const uint32_t g = P.row_graph[r0 + v];
       if (graph_frozen(P, g)) continue;
-      const double e = finish_row(P, r0 + v, z(v));
+      const double e = finish_row<HINT>(P, r0 + v, z(v));
       if (e > 0.0) {
         if (g - g0 < kGraphSlots) atomicAdd(&gerr[g - g0], err_to_fixed(e));
         else atomicAdd(P.acc + (uint64_t)P.iter * P.ngraphs + g, err_to_fixed(e));
@@ -172,6 +190,7 @@ __device__ __forceinline__ void finish_rows(const BinParams& P, uint32_t r0, uin
 // chunk pass: vals[p] = contrib[src(p)] over the pairs of the tiles of one item.  Tiles are
 // padded to multiples of 8 pairs (pads have source kPadSrc and get 0), so a lane moves 8
 // pairs with one 16-byte id load and two 16-byte value stores.
+template <int NT>
 __global__ __launch_bounds__(kBT) void pr_chunk_kernel(BinParams P) {
   if (converged_before(P)) return;
   __shared__ float4 cst4[kChunkSrc / 4];
@@ -190,19 +209,20 @@ __global__ __launch_bounds__(kBT) void pr_chunk_kernel(BinParams P) {
   const uint32_t u0 = it.x << kChunkBits;
   if ((uint64_t)u0 + kChunkSrc <= P.n) {
     constexpr uint32_t kV = kChunkSrc / 4 / kBT;
-    const float4* src = reinterpret_cast<const float4*>(P.cin + u0);
-    float4 v[kV];
+    const f32x4* src = reinterpret_cast<const f32x4*>(P.cin + u0);
+    f32x4* dst = reinterpret_cast<f32x4*>(cst4);
+    f32x4 v[kV];
 #pragma unroll
-    for (uint32_t k = 0; k < kV; ++k) v[k] = src[threadIdx.x + k * kBT];
+    for (uint32_t k = 0; k < kV; ++k) v[k] = bld<NT>(src + threadIdx.x + k * kBT);
 #pragma unroll
-    for (uint32_t k = 0; k < kV; ++k) cst4[threadIdx.x + k * kBT] = v[k];
+    for (uint32_t k = 0; k < kV; ++k) dst[threadIdx.x + k * kBT] = v[k];
   } else {
     for (uint32_t q = threadIdx.x; q < kChunkSrc; q += kBT) cst[q] = (u0 + q < P.n) ? P.cin[u0 + q] : 0.f;
   }
   __syncthreads();
   for (uint32_t g0 = 0; g0 < m; g0 += kGroups1 * kBT) {
     uint32_t p[kGroups1];
-    uint4 uu[kGroups1];
+    u32x4 uu[kGroups1];
 #pragma unroll
     for (uint32_t h = 0; h < kGroups1; ++h) {
       const uint32_t g = g0 + h * kBT + threadIdx.x;
@@ -214,7 +234,7 @@ __global__ __launch_bounds__(kBT) void pr_chunk_kernel(BinParams P) {
           else hi = mid - 1;
         }
         p[h] = sstart[lo] + 8u * (g - sflat[lo]);
-        uu[h] = *reinterpret_cast<const uint4*>(P.bsrc + p[h]);
+        uu[h] = bld<NT>(reinterpret_cast<const u32x4*>(P.bsrc + p[h]));
       }
     }
 #pragma unroll
@@ -227,8 +247,8 @@ __global__ __launch_bounds__(kBT) void pr_chunk_kernel(BinParams P) {
         const uint32_t u = (w[t >> 1] >> (16 * (t & 1))) & 0xFFFFu;
         o[t] = u == kPadSrc ? 0.f : cst[u];
       }
-      *reinterpret_cast<float4*>(P.vals + p[h]) = make_float4(o[0], o[1], o[2], o[3]);
-      *reinterpret_cast<float4*>(P.vals + p[h] + 4) = make_float4(o[4], o[5], o[6], o[7]);
+      bst<NT>(reinterpret_cast<f32x4*>(P.vals + p[h]), f32x4{o[0], o[1], o[2], o[3]});
+      bst<NT>(reinterpret_cast<f32x4*>(P.vals + p[h] + 4), f32x4{o[4], o[5], o[6], o[7]});
     }
   }
 }
@@ -241,7 +261,7 @@ __global__ __launch_bounds__(kBT) void pr_chunk_kernel(BinParams P) {
 // pair's value is hot[source] (pads, source kPadSrc, add 0); else it is P.vals[pair].
 // Entries are ordered by source, so one row rarely repeats inside a lane or an instruction:
 // each entry is one LDS add, with no run tracking.
-template <bool HOT>
+template <bool HOT, int NT>
 __device__ __forceinline__ void bin_range(const BinParams& P, uint32_t e0, uint32_t e1, unsigned long long* lacc,
                                           const float* hot) {
   // the wave index is uniform over the wave: in a scalar register, the block indices and the
@@ -253,7 +273,7 @@ __device__ __forceinline__ void bin_range(const BinParams& P, uint32_t e0, uint3
   const char* sbase = reinterpret_cast<const char*>(P.bsrc);
   const uint32_t ke = e1 / kBinBlock;
   for (uint32_t k0 = e0 / kBinBlock + w; k0 < ke; k0 += kGroups2 * kWaves) {
-    uint4 rw[kGroups2];
+    u32x4 rw[kGroups2];
     uint32_t pb[kGroups2];
 #pragma unroll
     for (uint32_t h = 0; h < kGroups2; ++h) {
@@ -261,7 +281,7 @@ __device__ __forceinline__ void bin_range(const BinParams& P, uint32_t e0, uint3
       // all blocks' loads are issued before any is waited on
       const uint32_t k = k0 + h * kWaves;
       const uint32_t kc = k < ke ? k : k0;
-      rw[h] = *reinterpret_cast<const uint4*>(P.brow + kc * kBinBlock + 8u * lane);
+      rw[h] = bld<NT>(reinterpret_cast<const u32x4*>(P.brow + kc * kBinBlock + 8u * lane));
       pb[h] = P.bpid[kc];
     }
     uint32_t rows[kGroups2][8];
@@ -295,6 +315,8 @@ __device__ __forceinline__ void bin_range(const BinParams& P, uint32_t e0, uint3
         }
       }
     }
+    // (Measured: issuing both blocks' value loads before the first add, or prefetching the
+    // next round's row offsets, made this pass slower at RMAT-25: 0.65 against 0.60 ms.)
 #pragma unroll
     for (uint32_t h = 0; h < kGroups2; ++h) {
       if (k0 + h * kWaves >= ke) continue;  // wave-uniform
@@ -308,7 +330,7 @@ __device__ __forceinline__ void bin_range(const BinParams& P, uint32_t e0, uint3
 // its rows here, an item of a split bin stores its partial rows in its slot of P.part.
 // The range's part inside the bin's hot tile (a prefix of the bin) reads its values from
 // the hot contributions staged in LDS.
-template <bool MULTI>
+template <bool MULTI, int NT>
 __global__ __launch_bounds__(kBT) void pr_bin_kernel(BinParams P) {
   if (converged_before(P)) return;
   __shared__ unsigned long long lacc[kBinRows];
@@ -328,7 +350,7 @@ __global__ __launch_bounds__(kBT) void pr_bin_kernel(BinParams P) {
 #pragma unroll
         for (uint32_t k = 0; k < kR; ++k) {
           const uint32_t q = threadIdx.x + k * kBT;
-          v[k] = q < P.n_hot ? P.cin[q] : 0.f;
+          v[k] = q < P.n_hot ? bld<NT>(P.cin + q) : 0.f;
         }
 #pragma unroll
         for (uint32_t k = 0; k < kR; ++k) {
@@ -340,18 +362,18 @@ __global__ __launch_bounds__(kBT) void pr_bin_kernel(BinParams P) {
   }
   __syncthreads();
   if constexpr (!MULTI) {
-    if (he > it.y) bin_range<true>(P, it.y, he, lacc, hot);
+    if (he > it.y) bin_range<true, NT>(P, it.y, he, lacc, hot);
   }
-  bin_range<false>(P, he, it.z, lacc, hot);
+  bin_range<false, NT>(P, he, it.z, lacc, hot);
   __syncthreads();
   const uint32_t r0 = b << kBinBits;
   if (it.w != kNoSlot) {
     unsigned long long* dst = P.part + (uint64_t)it.w * kBinRows;
-    for (uint32_t v = threadIdx.x; v < kBinRows; v += kBT) dst[v] = lacc[v];
+    for (uint32_t v = threadIdx.x; v < kBinRows; v += kBT) bst<NT>(dst + v, lacc[v]);
     return;
   }
   const uint32_t nrows = r0 < P.n ? min(kBinRows, P.n - r0) : 0u;
-  finish_rows<kBT, kBinRows, MULTI>(P, r0, nrows, [&](uint32_t v) { return dequantize(lacc[v]); }, red, gerr);
+  finish_rows<kBT, kBinRows, MULTI, NT>(P, r0, nrows, [&](uint32_t v) { return dequantize(lacc[v]); }, red, gerr);
 }
 
 // rows of the bins split over several bin-pass items: add the items' partial rows (exact,
@@ -359,7 +381,7 @@ __global__ __launch_bounds__(kBT) void pr_bin_kernel(BinParams P) {
 // of one split bin, each row's slots shared by L lanes (slot k on lane k mod L); the host
 // picks L per bin from its slot count, so a bin with hundreds of slots is read by many
 // workgroups at once and a bin with a few by few.  One workgroup per unit.
-template <bool MULTI>
+template <bool MULTI, int NT>
 __global__ __launch_bounds__(256) void pr_bin_combine_kernel(BinParams P) {
   if (converged_before(P)) return;
   __shared__ double red[4];
@@ -376,16 +398,16 @@ __global__ __launch_bounds__(256) void pr_bin_combine_kernel(BinParams P) {
     unsigned long long s0 = 0ull, s1 = 0ull, s2 = 0ull, s3 = 0ull;
     uint32_t k = lane;
     for (; k + 3 * lanes < it.z; k += 4 * lanes) {
-      s0 += src[(uint64_t)k * kBinRows];
-      s1 += src[(uint64_t)(k + lanes) * kBinRows];
-      s2 += src[(uint64_t)(k + 2 * lanes) * kBinRows];
-      s3 += src[(uint64_t)(k + 3 * lanes) * kBinRows];
+      s0 += bld<NT>(src + (uint64_t)k * kBinRows);
+      s1 += bld<NT>(src + (uint64_t)(k + lanes) * kBinRows);
+      s2 += bld<NT>(src + (uint64_t)(k + 2 * lanes) * kBinRows);
+      s3 += bld<NT>(src + (uint64_t)(k + 3 * lanes) * kBinRows);
     }
-    for (; k < it.z; k += lanes) s0 += src[(uint64_t)k * kBinRows];
+    for (; k < it.z; k += lanes) s0 += bld<NT>(src + (uint64_t)k * kBinRows);
     psum[threadIdx.x] = (s0 + s1) + (s2 + s3);
   }
   __syncthreads();
-  finish_rows<256, 256, MULTI>(
+  finish_rows<256, 256, MULTI, NT>(
       P, r0, nrows,
       [&](uint32_t v) {
         unsigned long long t = 0ull;
@@ -710,16 +732,23 @@ void binned_build(hipStream_t s, const ActiveCsc& a, uint32_t n, uint64_t nnz, B
   L.built = true;
 }
 
-void binned_enqueue_sweep(hipStream_t s, const BinParams& P) {
-  if (P.n_items1) pr_chunk_kernel<<<P.n_items1, kBT, 0, s>>>(P);
+template <int NT>
+static void enqueue_sweep_hinted(hipStream_t s, const BinParams& P) {
+  if (P.n_items1) pr_chunk_kernel<NT><<<P.n_items1, kBT, 0, s>>>(P);
   if (P.row_graph) {
-    pr_bin_kernel<true><<<P.n_items2, kBT, 0, s>>>(P);
-    if (P.n_items3) pr_bin_combine_kernel<true><<<P.n_items3, 256, 0, s>>>(P);
+    pr_bin_kernel<true, NT><<<P.n_items2, kBT, 0, s>>>(P);
+    if (P.n_items3) pr_bin_combine_kernel<true, NT><<<P.n_items3, 256, 0, s>>>(P);
   } else {
-    pr_bin_kernel<false><<<P.n_items2, kBT, 0, s>>>(P);
-    if (P.n_items3) pr_bin_combine_kernel<false><<<P.n_items3, 256, 0, s>>>(P);
+    pr_bin_kernel<false, NT><<<P.n_items2, kBT, 0, s>>>(P);
+    if (P.n_items3) pr_bin_combine_kernel<false, NT><<<P.n_items3, 256, 0, s>>>(P);
   }
   NWG_LAUNCH_CHECK();
+}
+
+void binned_enqueue_sweep(hipStream_t s, const BinParams& P) {
+  if (P.hints >= 2) enqueue_sweep_hinted<2>(s, P);
+  else if (P.hints == 1) enqueue_sweep_hinted<1>(s, P);
+  else enqueue_sweep_hinted<0>(s, P);
 }
 
 void binned_clear_residuals(hipStream_t s, unsigned long long* lanes, uint64_t iters) {

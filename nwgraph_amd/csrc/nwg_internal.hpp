@@ -181,6 +181,7 @@ struct BinParams {
   double threshold;
   const uint32_t* row_graph;  // several graphs in one union: graph of every row (null: one graph)
   uint32_t ngraphs;
+  uint32_t hints;  // non-temporal hints (the plan's stream_hints): 0 none, 1 loads, 2 loads and stores
 };
 
 // Identity of a captured sweep loop: re-capture when any of it changes.

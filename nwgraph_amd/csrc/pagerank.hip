@@ -759,6 +759,7 @@ static BinParams make_bin_params(nwg_graph* g, PrState& st, const SweepParams& P
   B.n_items1 = L.n_items1;
   B.acc = st.bacc.p;
   B.threshold = P.threshold;
+  B.hints = g->plan.params.stream_hints;
   return B;
 }
 
