@@ -401,33 +401,46 @@ def run_wpr(args, ctx):
     gw = torch.empty(nnz, dtype=torch.float32, device="cuda")
     gv = torch.empty(n, dtype=torch.float32, device="cuda")
     torch.cuda.synchronize()
+    # both solves stop on the relative L1 tolerance (NWG_INFO_RELATIVE_THRESHOLD): the forward at
+    # tol * ||v||_1 (= tol: the same stop as an absolute tolerance), the adjoint at
+    # tol * ||c||_1 / (1 - d), the same relative accuracy for a right-hand side of any scale
     for _ in range(args.warmup):
-        g.wpr_forward_device(v.data_ptr(), x.data_ptr(), 0.85, args.tol, args.iters)
-        g.wpr_backward_device(x.data_ptr(), c.data_ptr(), gw.data_ptr(), gv.data_ptr(), 0.85, args.tol, args.iters)
+        g.wpr_forward_device(v.data_ptr(), x.data_ptr(), 0.85, args.tol, args.iters, relative=True)
+        g.wpr_backward_device(x.data_ptr(), c.data_ptr(), gw.data_ptr(), gv.data_ptr(), 0.85, args.tol, args.iters,
+                              relative=True)
     t0 = time.perf_counter()
     for _ in range(args.steps):
-        fi = g.wpr_forward_device(v.data_ptr(), x.data_ptr(), 0.85, args.tol, args.iters)
+        fi = g.wpr_forward_device(v.data_ptr(), x.data_ptr(), 0.85, args.tol, args.iters, relative=True)
     torch.cuda.synchronize()
     ms_f = (time.perf_counter() - t0) * 1e3 / args.steps
     t0 = time.perf_counter()
     for _ in range(args.steps):
         bi = g.wpr_backward_device(x.data_ptr(), c.data_ptr(), gw.data_ptr(), gv.data_ptr(), 0.85, args.tol,
-                                   args.iters)
+                                   args.iters, relative=True)
     torch.cuda.synchronize()
     ms_b = (time.perf_counter() - t0) * 1e3 / args.steps
+    # the adjoint with an absolute tolerance: its L1 changes start near d ||A^T c||_1 ~ 1e5-1e6,
+    # about ten orders of magnitude above 1e-4 (some 140 sweeps at d = 0.85), where the forward's
+    # start near 1 is four
+    ba = g.wpr_backward_device(x.data_ptr(), c.data_ptr(), gw.data_ptr(), gv.data_ptr(), 0.85, args.tol, args.iters)
+    torch.cuda.synchronize()
     value = nnz * fi["iterations"] / (ms_f * 1e-3) / 1e9
     sweep_b = 8 * nnz + 4 * (n + 1) + 12 * n
     alg_f = sweep_b * fi["iterations"]
     alg_b = sweep_b * bi["iterations"] + 12 * nnz + 12 * n
     out = base_line(args, round(value, 3), "GTEPS (forward)", ms_f + ms_b, "f32", {
         "workload": f"weighted PageRank + implicit adjoint, RMAT scale {args.wpr_scale} ef {args.edge_factor}, "
-                    f"w ~ U(0.1,1) row-normalised, L = sum c_i x_i, c ~ N(0,1), d=0.85, tol {args.tol}, "
+                    f"w ~ U(0.1,1) row-normalised, L = sum c_i x_i, c ~ N(0,1), d=0.85, relative L1 tol "
+                    f"{args.tol} (forward: x {args.tol} * ||v||_1; adjoint: x {args.tol} * ||c||_1 / (1-d)), "
                     f"max {args.iters}",
         "n": n, "nnz": nnz, "forward_ms": round(ms_f, 3), "backward_ms": round(ms_b, 3),
         "backward_over_forward": round(ms_b / ms_f, 3),
         "forward_sweeps": fi["iterations"], "adjoint_sweeps": bi["iterations"],
         "forward_device_ms": round(fi["ms_total"], 3), "backward_device_ms": round(bi["ms_total"], 3),
         "backward_over_forward_device": round(bi["ms_total"] / fi["ms_total"], 3),
+        "forward_threshold": fi["threshold_used"], "adjoint_threshold": bi["threshold_used"],
+        "adjoint_absolute_tolerance": {"threshold": float(np.float32(args.tol)), "sweeps": ba["iterations"],
+                                       "final_error": ba["final_error"], "device_ms": round(ba["ms_total"], 3)},
     })
     out["roofline"] = roofline_entry(alg_f, fi["ms_total"], "weighted forward solve (wsweep_kernel per sweep), "
                                      "device time", "sweeps x (8E + 4(N+1) + 12N)")
@@ -494,7 +507,8 @@ def run_default(args, ctx):
         "ms_device", "schedule", "iterations_min_max")))
     add("wpr_rmat20", lambda: summary(run_wpr(sub_args(args, iters=100, steps=3, warmup=1), ctx), keep=(
         "forward_ms", "backward_ms", "forward_device_ms", "backward_device_ms", "backward_over_forward",
-        "backward_over_forward_device", "forward_sweeps", "adjoint_sweeps")))
+        "backward_over_forward_device", "forward_sweeps", "adjoint_sweeps", "forward_threshold", "adjoint_threshold",
+        "adjoint_absolute_tolerance")))
     out["configs"] = configs
     log(f"all configurations in {time.perf_counter() - t_all:.1f} s")
     return out

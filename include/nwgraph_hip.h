@@ -59,11 +59,20 @@ typedef struct nwg_pr_info {
   double init_ms[2];     /* out with NWG_INFO_KERNEL_TIMING: device time of the score init and of the
                             contribution init (life_timers "init page rank", "init contrib",
                             page_rank.hpp:99-118) */
+  double threshold_used; /* out (weighted solves): the absolute L1 change the stop compared with */
 } nwg_pr_info;
 
 /* nwg_pr_info.flags: launch the sweeps one by one (instead of as one captured HIP
  * graph) and time each sweep kernel with HIP events on the context's stream. */
 #define NWG_INFO_KERNEL_TIMING 1u
+/* nwg_pr_info.flags, weighted forward / adjoint only: the threshold is relative.  A solve
+ * stops when its L1 change falls below threshold * ||b||_1 / (1 - d), b the constant term of
+ * its fixed point: (1 - d) v for the forward (so the scale is ||v||_1, and 1 for the uniform
+ * teleport: a unit teleport stops exactly as with an absolute threshold), grad_x for the
+ * adjoint (whose L1 changes scale with ||grad_x||_1: an absolute threshold meant for a unit
+ * teleport asks a large gradient for many more orders of magnitude).  A zero b keeps the
+ * absolute threshold. */
+#define NWG_INFO_RELATIVE_THRESHOLD 2u
 
 /* Tunable scheduling plan of the pull sweep (see nwg_graph_autotune). */
 typedef struct nwg_plan_params {

@@ -19,6 +19,7 @@ NWG_EFORMAT = -74
 NWG_MEM_HOST = 0
 NWG_MEM_DEVICE = 1
 NWG_INFO_KERNEL_TIMING = 1
+NWG_INFO_RELATIVE_THRESHOLD = 2
 
 c_u32 = ctypes.c_uint32
 c_u64 = ctypes.c_uint64
@@ -45,6 +46,7 @@ class PrInfo(ctypes.Structure):
         ("time_trace", P_double),
         ("time_trace_len", c_u64),
         ("init_ms", c_double * 2),
+        ("threshold_used", c_double),
     ]
 
 

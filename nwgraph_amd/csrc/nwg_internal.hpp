@@ -220,6 +220,8 @@ struct Captured {
 struct WState {
   DevBuf<float> y0, y1, base, lambda, t;
   DevBuf<double> part, err;  // per-workgroup L1 partials; per-sweep L1 change
+  DevBuf<double> scale;      // relative threshold: ||b||_1 / (1 - d)
+  DevBuf<uint32_t> scale_done;
   DevBuf<uint32_t> done;     // per sweep: workgroups finished
   DevBuf<uint32_t> row_of;  // CSC position -> row
 };

@@ -57,7 +57,12 @@ struct WParams {
   double* err;     // per sweep: L1 change, stored by the sweep's last workgroup
   uint32_t iter;
   double threshold;
+  const double* thr_scale;  // relative threshold: the stop compares with threshold * *thr_scale
 };
+
+__device__ __forceinline__ double stop_threshold(const WParams& P) {
+  return P.thr_scale ? P.threshold * *P.thr_scale : P.threshold;
+}
 
 // Workgroup-order sum of the per-workgroup partials `e` (valid in thread 0) of sweep
 // P.iter into P.err[P.iter], by whichever workgroup finishes last.
@@ -79,7 +84,7 @@ __device__ __forceinline__ void fold_sweep_error(const WParams& P, double e, dou
 }
 
 __global__ __launch_bounds__(kThreads) void wsweep_kernel(WParams P) {
-  if (P.iter > 0 && P.err[P.iter - 1] < P.threshold) return;
+  if (P.iter > 0 && P.err[P.iter - 1] < stop_threshold(P)) return;
   __shared__ float vals[kItemNnz];
   __shared__ uint32_t lrows[kItemRows];   // an item's long rows (offsets from its first row)
   __shared__ uint32_t wlong[kThreads / 64];
@@ -293,6 +298,39 @@ __global__ __launch_bounds__(kThreads) void csr_rowsum_kernel(const uint32_t* __
   }
 }
 
+// Scale of a relative threshold, ||b||_1 / (1 - d) for the fixed point's constant term b
+// (1 when b is null: the uniform teleport (1 - d) / n summing to 1 - d), or 1 when b is 0.
+// Contiguous ranges per workgroup, partials added in workgroup order by the last workgroup
+// to finish: deterministic.
+constexpr uint32_t kNormBlocks = 256;
+__global__ __launch_bounds__(kThreads) void l1_scale_kernel(uint32_t n, const float* __restrict__ b, double inv_omd,
+                                                            double* __restrict__ part, uint32_t* __restrict__ done,
+                                                            double* __restrict__ scale) {
+  __shared__ double red_d[kThreads / 64];
+  __shared__ uint32_t last;
+  double e = 0.0;
+  if (b) {
+    const uint64_t per = ((uint64_t)n + gridDim.x - 1) / gridDim.x;
+    const uint64_t i0 = (uint64_t)blockIdx.x * per, i1 = i0 + per < n ? i0 + per : n;
+    for (uint64_t i = i0 + threadIdx.x; i < i1; i += kThreads) e += (double)fabsf(b[i]);
+  }
+  e = block_sum<kThreads>(e, red_d);
+  if (threadIdx.x == 0) {
+    part[blockIdx.x] = e;
+    __threadfence();
+    last = atomicAdd(done, 1u) == gridDim.x - 1 ? 1u : 0u;
+  }
+  __syncthreads();
+  if (!last) return;
+  __threadfence();
+  if (threadIdx.x == 0) {
+    double s = 0.0;
+    for (uint32_t q = 0; q < gridDim.x; ++q) s += __hip_atomic_load(part + q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    *scale = !b ? 1.0 : (s > 0.0 ? s * inv_omd : 1.0);
+    *done = 0u;
+  }
+}
+
 // row of every CSC position (upper bound in the row pointers)
 __global__ __launch_bounds__(kThreads) void row_of_kernel(uint32_t n, uint64_t nnz, const uint32_t* __restrict__ rp,
                                                           uint32_t* __restrict__ row) {
@@ -418,10 +456,12 @@ void weights_set(hipStream_t s, nwg_graph* g, const float* w_dev, bool normalize
 
 // Runs the fixed point y <- fma(d, sum w y, base) from y0 over (rp, col, w) in the cached
 // buffers of `st`; returns the sweeps executed and leaves the result in `out`.
+// rhs: the constant term b whose L1 norm scales a relative threshold (null: the uniform
+// teleport).
 static uint64_t fixed_point(nwg_ctx* ctx, WState& st, const uint32_t* rp, const uint32_t* col, const float* w,
                             const DevBuf<uint32_t>& r0, const DevBuf<uint32_t>& r1, uint32_t n_items, uint32_t n,
                             const float* base, const float* y0, bool y0_uniform, float damping, float threshold,
-                            uint64_t max_iters, float* out, nwg_pr_info* info) {
+                            uint64_t max_iters, float* out, nwg_pr_info* info, const float* rhs) {
   hipStream_t s = ctx->stream;
   st.y0.ensure(n);
   st.y1.ensure(n);
@@ -429,7 +469,16 @@ static uint64_t fixed_point(nwg_ctx* ctx, WState& st, const uint32_t* rp, const 
   st.err.ensure(std::max<uint64_t>(max_iters, 1));
   float* y[2] = {st.y0.p, st.y1.p};
   const uint32_t grid = persistent_grid(n_items);
-  st.part.ensure(grid);
+  st.part.ensure(std::max<uint32_t>(grid, kNormBlocks));
+  const bool relative = info && (info->flags & NWG_INFO_RELATIVE_THRESHOLD);
+  if (relative) {
+    st.scale.ensure(1);
+    st.scale_done.ensure(1);
+    NWG_HIP(hipMemsetAsync(st.scale_done.p, 0, sizeof(uint32_t), s));
+    l1_scale_kernel<<<kNormBlocks, kThreads, 0, s>>>(n, rhs, 1.0 / (1.0 - (double)damping), st.part.p,
+                                                    st.scale_done.p, st.scale.p);
+    NWG_LAUNCH_CHECK();
+  }
   NWG_HIP(hipEventRecord(ctx->ev0, s));
   winit_kernel<<<grid1(std::max<uint64_t>(n, max_iters)), kThreads, 0, s>>>(
       n, y0_uniform ? nullptr : y0, (float)(1.0 / n), y[0], st.done.p, st.err.p, max_iters);
@@ -447,6 +496,7 @@ static uint64_t fixed_point(nwg_ctx* ctx, WState& st, const uint32_t* rp, const 
   P.done = st.done.p;
   P.err = st.err.p;
   P.threshold = (double)threshold;
+  P.thr_scale = relative ? st.scale.p : nullptr;
   for (uint64_t k = 0; k < max_iters; ++k) {
     P.iter = (uint32_t)k;
     P.xin = y[k & 1];
@@ -457,12 +507,15 @@ static uint64_t fixed_point(nwg_ctx* ctx, WState& st, const uint32_t* rp, const 
   NWG_HIP(hipEventRecord(ctx->ev1, s));
   std::vector<double> h(std::max<uint64_t>(max_iters, 1));
   NWG_HIP(hipMemcpyAsync(h.data(), st.err.p, max_iters * sizeof(double), hipMemcpyDeviceToHost, s));
+  double scale = 1.0;
+  if (relative) NWG_HIP(hipMemcpyAsync(&scale, st.scale.p, sizeof(double), hipMemcpyDeviceToHost, s));
   NWG_HIP(hipStreamSynchronize(s));
+  const double stop = relative ? (double)threshold * scale : (double)threshold;
   uint64_t iters = max_iters;
   double last = 0.0;
   for (uint64_t k = 0; k < max_iters; ++k) {
     last = h[k];
-    if (last < (double)threshold) {
+    if (last < stop) {
       iters = k + 1;
       break;
     }
@@ -477,6 +530,7 @@ static uint64_t fixed_point(nwg_ctx* ctx, WState& st, const uint32_t* rp, const 
     info->ms_total = ms;
     info->ms_per_iter = iters ? ms / (double)iters : 0.0;
     info->sweep_ms_avg = 0.0;
+    info->threshold_used = stop;
     if (info->error_trace)
       for (uint64_t k = 0; k < iters && k < info->error_trace_len; ++k) info->error_trace[k] = h[k];
   }
@@ -500,7 +554,8 @@ void wpr_forward(nwg_ctx* ctx, nwg_graph* g, const float* teleport_dev, float* x
   wbase_kernel<<<grid1(g->n), kThreads, 0, s>>>(g->n, teleport_dev, 1.0 - (double)damping, st.base.p);
   NWG_LAUNCH_CHECK();
   fixed_point(ctx, st, g->csc_ptr.p, g->csc_idx.p, g->w_csc.p, g->w_items_csc_r0, g->w_items_csc_r1,
-              g->w_items_csc_n, g->n, st.base.p, nullptr, true, damping, threshold, max_iters, x_dev, info);
+              g->w_items_csc_n, g->n, st.base.p, nullptr, true, damping, threshold, max_iters, x_dev, info,
+              teleport_dev ? st.base.p : nullptr);
 }
 
 void wpr_backward(nwg_ctx* ctx, nwg_graph* g, const float* x_dev, const float* gx_dev, float* gw_dev, float* gv_dev,
@@ -516,7 +571,7 @@ void wpr_backward(nwg_ctx* ctx, nwg_graph* g, const float* x_dev, const float* g
   st.lambda.ensure(n);
   // adjoint fixed point: lambda = g + d A^T lambda, started from g
   fixed_point(ctx, st, g->csr_ptr.p, g->csr_idx.p, g->w_csr.p, g->w_items_csr_r0, g->w_items_csr_r1,
-              g->w_items_csr_n, n, gx_dev, gx_dev, false, damping, threshold, max_iters, st.lambda.p, info);
+              g->w_items_csr_n, n, gx_dev, gx_dev, false, damping, threshold, max_iters, st.lambda.p, info, gx_dev);
   if (gw_dev && nnz) {
     if (!st.row_of.n) {
       st.row_of.alloc(nnz);

@@ -8,7 +8,8 @@ import ctypes
 import numpy as np
 
 from . import _lib
-from ._lib import NWG_INFO_KERNEL_TIMING, NWG_MEM_DEVICE, NWG_MEM_HOST, NwgError, PlanParams, PrInfo
+from ._lib import (NWG_INFO_KERNEL_TIMING, NWG_INFO_RELATIVE_THRESHOLD, NWG_MEM_DEVICE, NWG_MEM_HOST, NwgError,
+                   PlanParams, PrInfo)
 
 RMAT_ABC = (5700, 1900, 1900)  # A/B/C = 0.57/0.19/0.19 in units of 1/10000 (D = 0.05)
 
@@ -171,12 +172,12 @@ class DeviceGraph:
 
     # ---- PageRank -----------------------------------------------------------
     @staticmethod
-    def _info(max_iters, kernel_timing=False):
+    def _info(max_iters, kernel_timing=False, relative=False):
         info = PrInfo()
         trace = np.zeros(max(int(max_iters), 1), np.float64)
         info.error_trace = trace.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
         info.error_trace_len = trace.size
-        info.flags = NWG_INFO_KERNEL_TIMING if kernel_timing else 0
+        info.flags = (NWG_INFO_KERNEL_TIMING if kernel_timing else 0) | (NWG_INFO_RELATIVE_THRESHOLD if relative else 0)
         return info, trace
 
     @staticmethod
@@ -188,6 +189,7 @@ class DeviceGraph:
             "ms_total": float(info.ms_total),
             "ms_per_iter": float(info.ms_per_iter),
             "sweep_ms_avg": float(info.sweep_ms_avg),
+            "threshold_used": float(info.threshold_used),
             "error_trace": trace[:it].copy(),
         }
 
@@ -280,28 +282,30 @@ class DeviceGraph:
         self.ctx.check(self.ctx.lib.nwg_graph_get_weights(self.ctx.handle, self.handle, _ptr(w)))
         return w
 
-    def wpr_forward(self, teleport=None, damping=0.85, threshold=1e-4, max_iters=100):
+    def wpr_forward(self, teleport=None, damping=0.85, threshold=1e-4, max_iters=100, relative=False):
+        """relative=True: stop when the L1 change falls below threshold * ||teleport||_1."""
         x = np.empty(self.n, np.float32)
         tele = None if teleport is None else np.ascontiguousarray(teleport, np.float32)
-        info, trace = self._info(max_iters)
+        info, trace = self._info(max_iters, relative=relative)
         self.ctx.check(self.ctx.lib.nwg_wpr_forward(self.ctx.handle, self.handle, _ptr(tele), _ptr(x), float(damping),
                                                     float(threshold), int(max_iters), NWG_MEM_HOST,
                                                     ctypes.byref(info)))
         return x, self._info_dict(info, trace)
 
-    def wpr_backward(self, x, grad_x, damping=0.85, threshold=1e-4, max_iters=100, raw_weights=True):
+    def wpr_backward(self, x, grad_x, damping=0.85, threshold=1e-4, max_iters=100, raw_weights=True, relative=False):
+        """relative=True: stop when the L1 change falls below threshold * ||grad_x||_1 / (1 - d)."""
         xa = np.ascontiguousarray(x, np.float32)
         ga = np.ascontiguousarray(grad_x, np.float32)
         gw = np.empty(self.nnz, np.float32)
         gv = np.empty(self.n, np.float32)
-        info, trace = self._info(max_iters)
+        info, trace = self._info(max_iters, relative=relative)
         self.ctx.check(self.ctx.lib.nwg_wpr_backward(self.ctx.handle, self.handle, _ptr(xa), _ptr(ga), _ptr(gw),
                                                      _ptr(gv), float(damping), float(threshold), int(max_iters),
                                                      1 if raw_weights else 0, NWG_MEM_HOST, ctypes.byref(info)))
         return gw, gv, self._info_dict(info, trace)
 
-    def wpr_forward_device(self, teleport_ptr, x_ptr, damping=0.85, threshold=1e-4, max_iters=100):
-        info, trace = self._info(max_iters)
+    def wpr_forward_device(self, teleport_ptr, x_ptr, damping=0.85, threshold=1e-4, max_iters=100, relative=False):
+        info, trace = self._info(max_iters, relative=relative)
         self.ctx.check(self.ctx.lib.nwg_wpr_forward(self.ctx.handle, self.handle,
                                                     ctypes.c_void_p(teleport_ptr) if teleport_ptr else None,
                                                     ctypes.c_void_p(x_ptr), float(damping), float(threshold),
@@ -309,8 +313,8 @@ class DeviceGraph:
         return self._info_dict(info, trace)
 
     def wpr_backward_device(self, x_ptr, gx_ptr, gw_ptr, gv_ptr, damping=0.85, threshold=1e-4, max_iters=100,
-                            raw_weights=True):
-        info, trace = self._info(max_iters)
+                            raw_weights=True, relative=False):
+        info, trace = self._info(max_iters, relative=relative)
         self.ctx.check(self.ctx.lib.nwg_wpr_backward(self.ctx.handle, self.handle, ctypes.c_void_p(x_ptr),
                                                      ctypes.c_void_p(gx_ptr),
                                                      ctypes.c_void_p(gw_ptr) if gw_ptr else None,

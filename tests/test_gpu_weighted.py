@@ -202,6 +202,61 @@ def test_adjoint_l1_change_does_not_overflow_rmat20(ctx):
     assert np.linalg.norm(lam_gpu - lam_it) / np.linalg.norm(lam_it) < 1e-4
 
 
+def test_relative_threshold_scales_with_the_gradient_norm(ctx):
+    """NWG_INFO_RELATIVE_THRESHOLD: the adjoint stops at threshold * ||grad_x||_1 / (1 - d), the
+    FP64 iteration with that absolute threshold stops within a sweep of it, scaling grad_x
+    scales lambda and keeps the sweep count, and for the uniform teleport the forward is the
+    same solve as with the absolute threshold."""
+    g, n, ind, col, deg, rows, w_raw = setup_graph(ctx, 12, seed=7, wseed=4)
+    g.set_weights(w_raw, normalize=True)
+    _, wn, _ = dense_system(n, ind, col, rows, w_raw)
+    dd = float(np.float32(D))
+    v = np.full(n, 1.0 / n, np.float32)
+    x_abs, fa = g.wpr_forward(None, D, 1e-4, 100)
+    x_rel, fr = g.wpr_forward(None, D, 1e-4, 100, relative=True)
+    assert fr["iterations"] == fa["iterations"] and np.array_equal(x_rel, x_abs)
+    assert fr["threshold_used"] == float(np.float32(1e-4))  # the scale of the uniform teleport is 1
+    c = O.normal_vector(n, seed=21).astype(np.float32)
+    scale = np.abs(c.astype(np.float64)).sum() / (1.0 - dd)
+    _, gv1, b1 = g.wpr_backward(x_abs, c, D, 1e-4, 200, raw_weights=False, relative=True)
+    assert b1["threshold_used"] == pytest.approx(float(np.float32(1e-4)) * scale, rel=1e-9)
+    assert 5 < b1["iterations"] < 200
+    _, it64 = O.fp64_adjoint(n, ind, col, deg, c.astype(np.float64), w=wn, damping=dd,
+                             tol=b1["threshold_used"], max_iters=200)
+    assert abs(b1["iterations"] - it64) <= 1, (b1["iterations"], it64)
+    _, gv1000, b1000 = g.wpr_backward(x_abs, 1000.0 * c, D, 1e-4, 200, raw_weights=False, relative=True)
+    assert b1000["iterations"] == b1["iterations"]
+    assert np.linalg.norm(gv1000 / 1000.0 - gv1) / np.linalg.norm(gv1) < 1e-5
+
+
+def test_relative_tolerance_gradients_match_fp64_rmat16(ctx):
+    """Config-5 form (L = sum c_i x_i, c ~ N(0,1)) at relative L1 tolerance 1e-4 for both solves
+    on RMAT-16: edge and teleport gradients within 1e-4 relative of FP64 run for the same
+    numbers of sweeps, and both sweep counts within one of the FP64 iterations' own stops."""
+    g = ctx.rmat(16, 16, seed=5)
+    n = g.n
+    g.random_weights(seed=11, normalize=True)
+    ind, col, deg = g.csc()
+    wn = g.get_weights().astype(np.float64)
+    rows = np.repeat(np.arange(n), np.diff(ind.astype(np.int64)))
+    c = O.normal_vector(n, seed=31).astype(np.float32)
+    x, fi = g.wpr_forward(None, D, 1e-4, 100, relative=True)
+    gw, gv, bi = g.wpr_backward(x, c, D, 1e-4, 100, raw_weights=False, relative=True)
+    kf, kb = fi["iterations"], bi["iterations"]
+    assert 1 < kf < 100 and 1 < kb < 100
+    dd = float(np.float32(D))
+    x64, _ = O.fp64_jacobi(n, ind, col, deg, w=wn, v=np.full(n, 1.0 / n), damping=dd, tol=0.0, max_iters=kf)
+    lam64, _ = O.fp64_adjoint(n, ind, col, deg, c.astype(np.float64), w=wn, damping=dd, tol=0.0, max_iters=kb)
+    rel = lambda a, b: np.linalg.norm(a.astype(np.float64) - b) / np.linalg.norm(b)
+    assert rel(gw, dd * lam64[rows] * x64[col]) < 1e-4
+    assert rel(gv, (1.0 - dd) * lam64) < 1e-4
+    _, itf = O.fp64_jacobi(n, ind, col, deg, w=wn, v=np.full(n, 1.0 / n), damping=dd, tol=fi["threshold_used"],
+                           max_iters=100)
+    _, itb = O.fp64_adjoint(n, ind, col, deg, c.astype(np.float64), w=wn, damping=dd, tol=bi["threshold_used"],
+                            max_iters=100)
+    assert abs(itf - kf) <= 1 and abs(itb - kb) <= 1, (kf, itf, kb, itb)
+
+
 def test_full_size_adjoint_identities_rmat20(ctx):
     """RMAT-20 with random raw weights, every solve run to its float fixed point (150 sweeps,
     0.85^150 < 1e-10).  Identities that hold for any size; on a dense FP64 system each one
