@@ -137,6 +137,58 @@ __global__ __launch_bounds__(kDegThreads) void degree_kernel(const uint32_t* __r
     if (h[q]) atomicAdd(&deg[q], h[q]);
 }
 
+// Out-degrees of a large CSC without global atomics: the ids are grouped by id range
+// (2^kDegRangeBits ids per range) with a radix sort of a copy over the range bits only,
+// then every range is counted in an LDS histogram by the workgroups of its work items
+// (kDegItem entries each); a range held by one item stores its counts, the items of a range
+// split over several add theirs.  At RMAT-25 (5.3e8 ids) the global-atomic histogram is
+// bound by the device's atomic rate (~1.3e10 per second); this path streams.
+constexpr int kDegRangeBits = 14;
+constexpr uint32_t kDegRange = 1u << kDegRangeBits;
+constexpr uint64_t kDegSortMin = 1ull << 24;  // entries from which the range path is taken
+constexpr uint32_t kDegItem = 1u << 20;
+constexpr uint32_t kDegCountThreads = 512;
+
+// start[r] = first position whose id's range is >= r, r in [0, nr]; every r written once
+__global__ __launch_bounds__(256) void range_start_kernel(const uint32_t* __restrict__ ids, uint64_t nnz, uint32_t nr,
+                                                          uint32_t* __restrict__ start) {
+  const uint64_t p = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+  if (p > nnz) return;
+  const uint32_t a = p == 0 ? 0u : (ids[p - 1] >> kDegRangeBits) + 1u;
+  uint32_t b = p == nnz ? nr : (ids[p] >> kDegRangeBits);
+  if (b > nr) b = nr;  // ids are below n (checked by the callers); never write past the table
+  for (uint32_t r = a; r <= b; ++r) start[r] = (uint32_t)p;
+}
+
+// one work item {range, first, end, split}: LDS counts of the range's ids in [first, end)
+__global__ __launch_bounds__(kDegCountThreads) void range_count_kernel(const uint32_t* __restrict__ ids,
+                                                                       const uint4* __restrict__ items, uint32_t n,
+                                                                       uint32_t* __restrict__ deg) {
+  __shared__ uint32_t h[kDegRange];
+  const uint4 it = items[blockIdx.x];
+  for (uint32_t q = threadIdx.x; q < kDegRange; q += kDegCountThreads) h[q] = 0u;
+  __syncthreads();
+  constexpr uint32_t kU = 4;
+  for (uint32_t p = it.y + threadIdx.x; p < it.z; p += kDegCountThreads * kU) {
+    uint32_t u[kU];
+#pragma unroll
+    for (uint32_t t = 0; t < kU; ++t) {
+      const uint32_t q = p + t * kDegCountThreads;
+      u[t] = q < it.z ? __builtin_nontemporal_load(ids + q) : 0u;
+    }
+#pragma unroll
+    for (uint32_t t = 0; t < kU; ++t)
+      if (p + t * kDegCountThreads < it.z) atomicAdd(&h[u[t] & (kDegRange - 1)], 1u);
+  }
+  __syncthreads();
+  const uint32_t v0 = it.x << kDegRangeBits;
+  const uint32_t cnt = n - v0 < kDegRange ? n - v0 : kDegRange;
+  for (uint32_t q = threadIdx.x; q < cnt; q += kDegCountThreads) {
+    if (!it.w) deg[v0 + q] = h[q];
+    else if (h[q]) atomicAdd(&deg[v0 + q], h[q]);
+  }
+}
+
 // *bad = 1 when some given out-degree is below the number of entries of its vertex
 __global__ __launch_bounds__(256) void outdeg_check_kernel(uint32_t n, const uint32_t* __restrict__ given,
                                                            const uint32_t* __restrict__ counted,
@@ -206,10 +258,42 @@ void validate_device_csc(hipStream_t s, uint32_t n, uint64_t nnz, const uint32_t
   if (read_back_u32(s, flag)) throw Error(NWG_EINVAL, "device CSC: row pointers not monotone from 0 to nnz, or an entry >= n");
 }
 
-void out_degrees_device(hipStream_t s, uint32_t n, uint64_t nnz, const uint32_t* col, uint32_t* deg) {
+// The range path for a large CSC (ids below n < 2^32 assumed checked): a sorted copy, range
+// starts read back, work items built on the host.
+static void out_degrees_by_range(hipStream_t s, uint32_t n, uint64_t nnz, const uint32_t* col, uint32_t* deg) {
+  const uint32_t nr = (uint32_t)ceil_div(n, kDegRange);
+  DevBuf<uint32_t> ids(nnz), tmp(nnz), start((uint64_t)nr + 1);
+  NWG_HIP(hipMemcpyAsync(ids.p, col, nnz * sizeof(uint32_t), hipMemcpyDeviceToDevice, s));
+  radix_sort_keys_u32(s, ids.p, tmp.p, nnz, kDegRangeBits, std::max(kDegRangeBits, bits_for(n)));
+  range_start_kernel<<<grid_for(nnz + 1), 256, 0, s>>>(ids.p, nnz, nr, start.p);
+  NWG_LAUNCH_CHECK();
+  std::vector<uint32_t> hs((size_t)nr + 1);
+  NWG_HIP(hipMemcpyAsync(hs.data(), start.p, hs.size() * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+  NWG_HIP(hipStreamSynchronize(s));
+  std::vector<uint4> items;
+  for (uint32_t r = 0; r < nr; ++r) {
+    const uint32_t a = hs[r], b = hs[r + 1];
+    const uint32_t split = b - a > kDegItem ? 1u : 0u;
+    for (uint64_t q = a; q < b; q += kDegItem)
+      items.push_back(make_uint4(r, (uint32_t)q, (uint32_t)std::min<uint64_t>(b, q + kDegItem), split));
+  }
+  if (items.empty()) return;
+  DevBuf<uint4> ditems(items.size());
+  NWG_HIP(hipMemcpyAsync(ditems.p, items.data(), items.size() * sizeof(uint4), hipMemcpyHostToDevice, s));
+  range_count_kernel<<<(uint32_t)items.size(), kDegCountThreads, 0, s>>>(ids.p, ditems.p, n, deg);
+  NWG_LAUNCH_CHECK();
+  NWG_HIP(hipStreamSynchronize(s));  // the temporaries are freed on return
+}
+
+void out_degrees_device(hipStream_t s, uint32_t n, uint64_t nnz, const uint32_t* col, uint32_t* deg,
+                        bool may_allocate) {
   NWG_REQUIRE(reinterpret_cast<uintptr_t>(col) % 16 == 0, "column ids must be 16-byte aligned");
   fill_u32_kernel<<<grid_for(n), 256, 0, s>>>(deg, n, 0u);
   NWG_LAUNCH_CHECK();
+  if (may_allocate && nnz >= kDegSortMin && nnz < (1ull << 32)) {
+    out_degrees_by_range(s, n, nnz, col, deg);
+    return;
+  }
   if (nnz) {
     int dev = 0, cus = 0;
     NWG_HIP(hipGetDevice(&dev));
@@ -235,7 +319,7 @@ void check_outdeg_device(hipStream_t s, uint32_t n, uint64_t nnz, const uint32_t
     own_flag.alloc(1);
     flag = own_flag.p;
   }
-  out_degrees_device(s, n, nnz, col, counted);
+  out_degrees_device(s, n, nnz, col, counted, own_counted.p != nullptr);
   fill_u32_kernel<<<1, 256, 0, s>>>(flag, 1, 0u);
   outdeg_check_kernel<<<grid_for(n), 256, 0, s>>>(n, outdeg, counted, flag);
   NWG_LAUNCH_CHECK();

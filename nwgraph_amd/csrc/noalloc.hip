@@ -214,7 +214,7 @@ void pagerank_noalloc(nwg_ctx* ctx, uint32_t n, uint64_t nnz, const uint32_t* cs
   if (outdeg) {
     check_outdeg_device(s, n, nnz, csc_idx, outdeg, cdeg, flag);
   } else {
-    out_degrees_device(s, n, nnz, csc_idx, cdeg);  // build_degrees, common.hpp:110-121
+    out_degrees_device(s, n, nnz, csc_idx, cdeg, false);  // build_degrees, common.hpp:110-121
     deg = cdeg;
   }
 

@@ -329,7 +329,10 @@ void sorted_keys_to_csx(hipStream_t s, const uint64_t* keys, uint64_t m, uint32_
 // flag: one device word of scratch, or null to allocate it
 void validate_device_csc(hipStream_t s, uint32_t n, uint64_t nnz, const uint32_t* ptr, const uint32_t* idx,
                          uint32_t* flag = nullptr);
-void out_degrees_device(hipStream_t s, uint32_t n, uint64_t nnz, const uint32_t* col, uint32_t* deg);
+// out-degree histogram of col (ids < n); may_allocate = false keeps to the atomic histogram,
+// which needs no device memory beyond deg
+void out_degrees_device(hipStream_t s, uint32_t n, uint64_t nnz, const uint32_t* col, uint32_t* deg,
+                        bool may_allocate = true);
 // throws NWG_EINVAL when some outdeg[u] is below the number of entries u has in col;
 // counted (n words) and flag (one word): device scratch, or null to allocate them
 void check_outdeg_device(hipStream_t s, uint32_t n, uint64_t nnz, const uint32_t* col, const uint32_t* outdeg,
@@ -342,6 +345,9 @@ void exclusive_scan_u32(hipStream_t s, const uint32_t* in, uint32_t* out, uint64
 // Sorted result ends in keys/vals (tmp buffers must have n entries).
 void radix_sort_pairs(hipStream_t s, uint64_t* keys, uint64_t* keys_tmp, uint32_t* vals,
                       uint32_t* vals_tmp, uint64_t n, int key_bits);
+// stable LSD radix sort of 32-bit keys by their bits [first_bit, end_bit) (the others are
+// carried along unsorted); result in keys (keys_tmp: n entries of scratch).
+void radix_sort_keys_u32(hipStream_t s, uint32_t* keys, uint32_t* keys_tmp, uint64_t n, int first_bit, int end_bit);
 
 // ---- generators (gen.hip) ----
 void rmat_edges_device(hipStream_t s, uint32_t scale, uint32_t a, uint32_t b, uint32_t c, uint64_t seed,

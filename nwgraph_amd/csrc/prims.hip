@@ -87,7 +87,8 @@ constexpr int kRsItems = 16;
 constexpr int kRsTile = kRsThreads * kRsItems;
 constexpr int kRsWaves = kRsThreads / 64;
 
-__global__ __launch_bounds__(kRsThreads) void rs_upsweep_kernel(const uint64_t* __restrict__ keys, uint64_t n,
+template <class K>
+__global__ __launch_bounds__(kRsThreads) void rs_upsweep_kernel(const K* __restrict__ keys, uint64_t n,
                                                                 int shift, uint32_t nb,
                                                                 uint32_t* __restrict__ tile_counts,
                                                                 uint32_t* __restrict__ hist) {
@@ -106,15 +107,15 @@ __global__ __launch_bounds__(kRsThreads) void rs_upsweep_kernel(const uint64_t* 
   hist[(uint64_t)threadIdx.x * nb + blockIdx.x] = c;
 }
 
-template <bool HasVals>
-__global__ __launch_bounds__(kRsThreads) void rs_scatter_kernel(const uint64_t* __restrict__ keys_in,
+template <class K, bool HasVals>
+__global__ __launch_bounds__(kRsThreads) void rs_scatter_kernel(const K* __restrict__ keys_in,
                                                                 const uint32_t* __restrict__ vals_in,
-                                                                uint64_t* __restrict__ keys_out,
+                                                                K* __restrict__ keys_out,
                                                                 uint32_t* __restrict__ vals_out, uint64_t n,
                                                                 int shift, uint32_t nb,
                                                                 const uint32_t* __restrict__ tile_counts,
                                                                 const uint32_t* __restrict__ hist_scan) {
-  __shared__ uint64_t sk[kRsTile];
+  __shared__ K sk[kRsTile];
   __shared__ uint32_t sv[HasVals ? kRsTile : 1];
   __shared__ uint32_t wave_cnt[kRsWaves][256];
   __shared__ uint32_t lofs[256];
@@ -144,7 +145,7 @@ __global__ __launch_bounds__(kRsThreads) void rs_scatter_kernel(const uint64_t* 
   for (int j = 0; j < kRsItems; ++j) {
     const uint32_t li = (uint32_t)j * kRsThreads + tid;
     const bool valid = li < tile_n;
-    uint64_t key = 0;
+    K key = 0;
     uint32_t val = 0;
     if (valid) {
       key = keys_in[base + li];
@@ -185,7 +186,7 @@ __global__ __launch_bounds__(kRsThreads) void rs_scatter_kernel(const uint64_t* 
   for (int j = 0; j < kRsItems; ++j) {
     const uint32_t li = (uint32_t)j * kRsThreads + tid;
     if (li < tile_n) {
-      const uint64_t key = sk[li];
+      const K key = sk[li];
       const uint32_t dg = (uint32_t)(key >> shift) & 0xFFu;
       const uint64_t o = (uint64_t)gofs[dg] + (li - lofs[dg]);
       keys_out[o] = key;
@@ -207,14 +208,15 @@ void radix_sort_pairs(hipStream_t s, uint64_t* keys, uint64_t* keys_tmp, uint32_
   uint32_t *vi = vals, *vo = vals_tmp;
   int passes = 0;
   for (int shift = 0; shift < key_bits; shift += 8) {
-    rs_upsweep_kernel<<<nb, kRsThreads, 0, s>>>(ki, n, shift, nb, tile_counts.p, hist.p);
+    rs_upsweep_kernel<uint64_t><<<nb, kRsThreads, 0, s>>>(ki, n, shift, nb, tile_counts.p, hist.p);
     NWG_LAUNCH_CHECK();
     exclusive_scan_u32(s, hist.p, hist_scan.p, nb64 * 256);
     if (vals)
-      rs_scatter_kernel<true><<<nb, kRsThreads, 0, s>>>(ki, vi, ko, vo, n, shift, nb, tile_counts.p, hist_scan.p);
+      rs_scatter_kernel<uint64_t, true><<<nb, kRsThreads, 0, s>>>(ki, vi, ko, vo, n, shift, nb, tile_counts.p,
+                                                                  hist_scan.p);
     else
-      rs_scatter_kernel<false><<<nb, kRsThreads, 0, s>>>(ki, vi, ko, vo, n, shift, nb, tile_counts.p,
-                                                         hist_scan.p);
+      rs_scatter_kernel<uint64_t, false><<<nb, kRsThreads, 0, s>>>(ki, vi, ko, vo, n, shift, nb, tile_counts.p,
+                                                                   hist_scan.p);
     NWG_LAUNCH_CHECK();
     std::swap(ki, ko);
     std::swap(vi, vo);
@@ -224,6 +226,29 @@ void radix_sort_pairs(hipStream_t s, uint64_t* keys, uint64_t* keys_tmp, uint32_
     NWG_HIP(hipMemcpyAsync(keys, ki, n * sizeof(uint64_t), hipMemcpyDeviceToDevice, s));
     if (vals) NWG_HIP(hipMemcpyAsync(vals, vi, n * sizeof(uint32_t), hipMemcpyDeviceToDevice, s));
   }
+  NWG_HIP(hipStreamSynchronize(s));
+}
+
+void radix_sort_keys_u32(hipStream_t s, uint32_t* keys, uint32_t* keys_tmp, uint64_t n, int first_bit, int end_bit) {
+  if (n < 2 || end_bit <= first_bit) return;
+  NWG_REQUIRE(n < (1ull << 32), "radix sort supports fewer than 2^32 keys");
+  NWG_REQUIRE(first_bit >= 0 && end_bit <= 32, "key bits must lie in [0, 32)");
+  const uint64_t nb64 = ceil_div(n, kRsTile);
+  const uint32_t nb = (uint32_t)nb64;
+  DevBuf<uint32_t> tile_counts(nb64 * 256), hist(nb64 * 256), hist_scan(nb64 * 256 + 1);
+  uint32_t *ki = keys, *ko = keys_tmp;
+  int passes = 0;
+  for (int shift = first_bit; shift < end_bit; shift += 8) {
+    rs_upsweep_kernel<uint32_t><<<nb, kRsThreads, 0, s>>>(ki, n, shift, nb, tile_counts.p, hist.p);
+    NWG_LAUNCH_CHECK();
+    exclusive_scan_u32(s, hist.p, hist_scan.p, nb64 * 256);
+    rs_scatter_kernel<uint32_t, false><<<nb, kRsThreads, 0, s>>>(ki, nullptr, ko, nullptr, n, shift, nb,
+                                                                 tile_counts.p, hist_scan.p);
+    NWG_LAUNCH_CHECK();
+    std::swap(ki, ko);
+    ++passes;
+  }
+  if (passes & 1) NWG_HIP(hipMemcpyAsync(keys, ki, n * sizeof(uint32_t), hipMemcpyDeviceToDevice, s));
   NWG_HIP(hipStreamSynchronize(s));
 }
 

@@ -501,6 +501,25 @@ def test_delta_schedule_cycle_stops_like_the_closed_form(ctx):
     assert np.allclose(x, ref, rtol=1e-6)
 
 
+def test_out_degrees_by_id_range_match_bincount(ctx):
+    """Out-degrees of a CSC with at least 2^24 entries are counted per 16384-id range after a
+    radix sort over the range bits (no global atomics).  RMAT-22 (65M entries; its low ids
+    take more than 2^20 entries, so that range is split over several workgroups): the built
+    graph's degrees and those computed for an uploaded CSC without degrees equal bincount."""
+    g = ctx.rmat(22, 16, seed=4)
+    ptr, col, deg = g.csc()
+    n = ptr.size - 1
+    assert col.size >= (1 << 24)
+    want = np.bincount(col, minlength=n).astype(np.uint32)
+    assert np.count_nonzero(col < 16384) > (1 << 20)
+    assert np.array_equal(deg, want)
+    g2 = ctx.graph_from_csc(ptr, col)
+    _, _, deg2 = g2.csc()
+    assert np.array_equal(deg2, want)
+    g2.close()
+    g.close()
+
+
 def test_device_csc_is_checked_and_degrees_match(ctx):
     """A CSC handed over in device memory is checked on the device (an entry >= n or a
     non-monotone row pointer is rejected before any kernel indexes by it), and the device
