@@ -315,7 +315,8 @@ int nwg_stream_probe(nwg_ctx* ctx, uint32_t kind, uint32_t flags, uint32_t wg_pe
  * 1-based -> 0-based, symmetric files mirror off-diagonal entries, n = max(header
  * rows, max id + 1).  *src and *dst are malloc'ed; free with nwg_free. */
 int nwg_read_mm(const char* path, uint32_t* n, uint64_t* ne, uint32_t** src, uint32_t** dst);
-/* NW binary edge list (edge_list::serialize/deserialize, edge_list.hpp:237-280). */
+/* NW binary edge list (edge_list::serialize/deserialize, edge_list.hpp:237-280).  A file with
+ * an id at or above its own vertex count is malformed (NWG_EFORMAT). */
 int nwg_read_nw(const char* path, uint32_t* n, uint64_t* ne, uint32_t** src, uint32_t** dst);
 int nwg_write_nw(const char* path, uint32_t n, uint64_t ne, const uint32_t* src, const uint32_t* dst);
 /* Binary adjacency (index_adjacency<idx>::serialize / deserialize, adjacency.hpp:176-191,

@@ -152,6 +152,9 @@ static int read_nw(const char* path, uint32_t* n, uint64_t* ne, uint32_t** src, 
     cols[a]->resize(cnt);
     in.read(reinterpret_cast<char*>(cols[a]->data()), (std::streamsize)(cnt * 4));
     if (!in) return NWG_EFORMAT;
+    // a list the reference writes has N >= max id + 1 (edge_list.hpp:188-199)
+    for (uint32_t v : *cols[a])
+      if (v >= N) return NWG_EFORMAT;
   }
   *n = (uint32_t)N;
   return export_edges(e, ne, src, dst);

@@ -9,7 +9,8 @@ import os
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "libnwg_oracle.so")
+# NWG_ORACLE_LIB names another build of the same source (the sanitizer build of the tests)
+LIB_PATH = os.environ.get("NWG_ORACLE_LIB") or os.path.join(_HERE, "libnwg_oracle.so")
 
 u32p = ctypes.c_void_p
 _lib = None
