@@ -97,13 +97,31 @@ __device__ __forceinline__ float dequantize(unsigned long long v) {
   return __builtin_ldexpf((float)mant, ex);  // exact: mant < 2^24, result a normal float
 }
 
+// Weighted sweep: an entry's value w * v may be negative (the adjoint) and of any magnitude.
+// The solve picks a power of two 2^t that maps every product and every partial row sum into
+// [-1/2, 1/2] (a bound it proves from the weights and its right-hand side), and the scaled
+// value is summed in two's complement fixed point in units of 2^-60: exact integer adds,
+// independent of order and split, no overflow (|sum| < 2^59 units).
+__device__ __forceinline__ unsigned long long quantize_signed(float v, int t) {
+  const float sc = __builtin_ldexpf(v, t);  // a power-of-two scale: exact (tiny values may flush)
+  const unsigned long long q = quantize(fabsf(sc));
+  return sc < 0.f ? 0ull - q : q;
+}
+__device__ __forceinline__ float dequantize_signed(unsigned long long v, int t) {
+  const bool neg = (long long)v < 0;
+  const float f = __builtin_ldexpf(dequantize(neg ? 0ull - v : v), -t);
+  return neg ? -f : f;
+}
+
 // One graph: every workgroup of sweep k exits when sweep k-1's change was below the
 // threshold.  Several graphs (row_graph set): the passes always run and each graph's rows
 // are frozen once that graph has converged (graph_frozen).
 // One graph's change of sweep k is the sum of its kBinAccLanes lane counters; every wave
 // loads and sums them (exactly, in fixed point), so all waves of a workgroup agree.
+template <bool W = false>
 __device__ __forceinline__ bool converged_before(const BinParams& P) {
   static_assert(kBinAccLanes == 64, "one lane counter per wavefront lane");
+  if constexpr (W) return P.iter > 0 && P.werr[P.iter - 1] < P.threshold * (P.thr_scale ? *P.thr_scale : 1.0);
   if (P.row_graph || P.iter == 0) return false;
   unsigned long long v =
       P.acc[(uint64_t)(P.iter - 1) * kBinAccStride + (uint64_t)(threadIdx.x & 63u) * kBinAccPad];
@@ -122,8 +140,14 @@ __device__ __forceinline__ bool graph_frozen(const BinParams& P, uint32_t g) {
 
 
 // the reference's row update (page_rank.hpp:133-135) -> |x_new - x_old|
-template <int NT>
+template <int NT, bool W = false>
 __device__ __forceinline__ double finish_row(const BinParams& P, uint32_t i, float z) {
+  if constexpr (W) {  // x' = fma(d, z, base_i) from the previous iterate cin into x
+    const float xo = bld<NT>(P.cin + i);
+    const float xn = __builtin_fmaf(P.damp, z, bld<NT>(P.basev + i));
+    bst<NT>(P.x + i, xn);
+    return (double)fabsf(xn - xo);
+  }
   const float b = P.basev ? bld<NT>(P.basev + i) : P.base;
   const float xo = bld<NT>(P.x + i);
   const float xn = __builtin_fmaf(P.damp, z, b);
@@ -144,17 +168,19 @@ constexpr uint32_t kGraphSlots = 64;  // graphs whose L1 changes a workgroup fir
 // flight together.
 // MULTI: whether P.row_graph is set (the kernels are instantiated per case, so the one-graph
 // kernels carry no per-graph code).
-template <int NT, int MAXR, bool MULTI, int HINT, class Z>
+// W (weighted): the workgroup's L1 change is stored in P.wpart[slot] instead of a lane counter.
+template <int NT, int MAXR, bool MULTI, int HINT, bool W = false, class Z>
 __device__ __forceinline__ void finish_rows(const BinParams& P, uint32_t r0, uint32_t nrows, Z z, double* red,
-                                            unsigned long long* gerr) {
+                                            unsigned long long* gerr, uint32_t slot = 0) {
   static_assert(MAXR % NT == 0, "whole rounds of rows");
+  static_assert(!(MULTI && W), "the weighted sweep solves one graph");
   if constexpr (!MULTI) {
     (void)gerr;
     double e = 0.0;
 #pragma unroll
     for (uint32_t k = 0; k < MAXR / NT; ++k) {
       const uint32_t v = threadIdx.x + k * NT;
-      if (v < nrows) e += finish_row<HINT>(P, r0 + v, z(v));
+      if (v < nrows) e += finish_row<HINT, W>(P, r0 + v, z(v));
     }
     e = wave_sum(e);
     if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = e;
@@ -162,7 +188,8 @@ __device__ __forceinline__ void finish_rows(const BinParams& P, uint32_t r0, uin
     if (threadIdx.x == 0) {
       double t = 0.0;
       for (uint32_t q = 0; q < NT / 64; ++q) t += red[q];
-      if (t > 0.0) atomicAdd(residual_lane(P), err_to_fixed(t));
+      if constexpr (W) P.wpart[slot] = t;
+      else if (t > 0.0) atomicAdd(residual_lane(P), err_to_fixed(t));
     }
   } else {
     for (uint32_t q = threadIdx.x; q < kGraphSlots; q += NT) gerr[q] = 0ull;
@@ -190,9 +217,9 @@ __device__ __forceinline__ void finish_rows(const BinParams& P, uint32_t r0, uin
 // chunk pass: vals[p] = contrib[src(p)] over the pairs of the tiles of one item.  Tiles are
 // padded to multiples of 8 pairs (pads have source kPadSrc and get 0), so a lane moves 8
 // pairs with one 16-byte id load and two 16-byte value stores.
-template <int NT>
+template <int NT, bool W = false>
 __global__ __launch_bounds__(kBT) void pr_chunk_kernel(BinParams P) {
-  if (converged_before(P)) return;
+  if (converged_before<W>(P)) return;
   __shared__ float4 cst4[kChunkSrc / 4];
   float* cst = reinterpret_cast<float*>(cst4);
   __shared__ uint32_t sstart[kBinMaxSegs];
@@ -261,7 +288,7 @@ __global__ __launch_bounds__(kBT) void pr_chunk_kernel(BinParams P) {
 // pair's value is hot[source] (pads, source kPadSrc, add 0); else it is P.vals[pair].
 // Entries are ordered by source, so one row rarely repeats inside a lane or an instruction:
 // each entry is one LDS add, with no run tracking.
-template <bool HOT, int NT>
+template <bool HOT, int NT, bool W = false>
 __device__ __forceinline__ void bin_range(const BinParams& P, uint32_t e0, uint32_t e1, unsigned long long* lacc,
                                           const float* hot) {
   // the wave index is uniform over the wave: in a scalar register, the block indices and the
@@ -275,6 +302,7 @@ __device__ __forceinline__ void bin_range(const BinParams& P, uint32_t e0, uint3
   for (uint32_t k0 = e0 / kBinBlock + w; k0 < ke; k0 += kGroups2 * kWaves) {
     u32x4 rw[kGroups2];
     uint32_t pb[kGroups2];
+    f32x4 wv[W ? kGroups2 : 1][2];
 #pragma unroll
     for (uint32_t h = 0; h < kGroups2; ++h) {
       // a block past the range re-reads block k0 (in bounds) and is skipped below, so that
@@ -283,6 +311,11 @@ __device__ __forceinline__ void bin_range(const BinParams& P, uint32_t e0, uint3
       const uint32_t kc = k < ke ? k : k0;
       rw[h] = bld<NT>(reinterpret_cast<const u32x4*>(P.brow + kc * kBinBlock + 8u * lane));
       pb[h] = P.bpid[kc];
+      if constexpr (W) {  // the lane's 8 weights: 32 bytes, 32-byte aligned
+        const f32x4* wp = reinterpret_cast<const f32x4*>(P.bw + kc * kBinBlock + 8u * lane);
+        wv[h][0] = bld<NT>(wp);
+        wv[h][1] = bld<NT>(wp + 1);
+      }
     }
     uint32_t rows[kGroups2][8];
     float vs[kGroups2][8];
@@ -320,8 +353,14 @@ __device__ __forceinline__ void bin_range(const BinParams& P, uint32_t e0, uint3
 #pragma unroll
     for (uint32_t h = 0; h < kGroups2; ++h) {
       if (k0 + h * kWaves >= ke) continue;  // wave-uniform
+      if constexpr (W) {
 #pragma unroll
-      for (uint32_t t = 0; t < 8; ++t) atomicAdd(&lacc[rows[h][t]], quantize(vs[h][t]));
+        for (uint32_t t = 0; t < 8; ++t)
+          atomicAdd(&lacc[rows[h][t]], quantize_signed(wv[h][t >> 2][t & 3] * vs[h][t], P.qshift));
+      } else {
+#pragma unroll
+        for (uint32_t t = 0; t < 8; ++t) atomicAdd(&lacc[rows[h][t]], quantize(vs[h][t]));
+      }
     }
   }
 }
@@ -330,9 +369,9 @@ __device__ __forceinline__ void bin_range(const BinParams& P, uint32_t e0, uint3
 // its rows here, an item of a split bin stores its partial rows in its slot of P.part.
 // The range's part inside the bin's hot tile (a prefix of the bin) reads its values from
 // the hot contributions staged in LDS.
-template <bool MULTI, int NT>
+template <bool MULTI, int NT, bool W = false>
 __global__ __launch_bounds__(kBT) void pr_bin_kernel(BinParams P) {
-  if (converged_before(P)) return;
+  if (converged_before<W>(P)) return;
   __shared__ unsigned long long lacc[kBinRows];
   __shared__ double red[kBT / 64];
   __shared__ unsigned long long gerr[MULTI ? kGraphSlots : 1];
@@ -362,18 +401,24 @@ __global__ __launch_bounds__(kBT) void pr_bin_kernel(BinParams P) {
   }
   __syncthreads();
   if constexpr (!MULTI) {
-    if (he > it.y) bin_range<true, NT>(P, it.y, he, lacc, hot);
+    if (he > it.y) bin_range<true, NT, W>(P, it.y, he, lacc, hot);
   }
-  bin_range<false, NT>(P, he, it.z, lacc, hot);
+  bin_range<false, NT, W>(P, he, it.z, lacc, hot);
   __syncthreads();
   const uint32_t r0 = b << kBinBits;
   if (it.w != kNoSlot) {
     unsigned long long* dst = P.part + (uint64_t)it.w * kBinRows;
     for (uint32_t v = threadIdx.x; v < kBinRows; v += kBT) bst<NT>(dst + v, lacc[v]);
+    if constexpr (W)
+      if (threadIdx.x == 0) P.wpart[blockIdx.x] = 0.0;  // this item finishes no rows
     return;
   }
   const uint32_t nrows = r0 < P.n ? min(kBinRows, P.n - r0) : 0u;
-  finish_rows<kBT, kBinRows, MULTI, NT>(P, r0, nrows, [&](uint32_t v) { return dequantize(lacc[v]); }, red, gerr);
+  if constexpr (W)
+    finish_rows<kBT, kBinRows, MULTI, NT, true>(
+        P, r0, nrows, [&](uint32_t v) { return dequantize_signed(lacc[v], P.qshift); }, red, gerr, blockIdx.x);
+  else
+    finish_rows<kBT, kBinRows, MULTI, NT>(P, r0, nrows, [&](uint32_t v) { return dequantize(lacc[v]); }, red, gerr);
 }
 
 // rows of the bins split over several bin-pass items: add the items' partial rows (exact,
@@ -381,9 +426,9 @@ __global__ __launch_bounds__(kBT) void pr_bin_kernel(BinParams P) {
 // of one split bin, each row's slots shared by L lanes (slot k on lane k mod L); the host
 // picks L per bin from its slot count, so a bin with hundreds of slots is read by many
 // workgroups at once and a bin with a few by few.  One workgroup per unit.
-template <bool MULTI, int NT>
+template <bool MULTI, int NT, bool W = false>
 __global__ __launch_bounds__(256) void pr_bin_combine_kernel(BinParams P) {
-  if (converged_before(P)) return;
+  if (converged_before<W>(P)) return;
   __shared__ double red[4];
   __shared__ unsigned long long gerr[MULTI ? kGraphSlots : 1];
   __shared__ unsigned long long psum[256];
@@ -407,21 +452,35 @@ __global__ __launch_bounds__(256) void pr_bin_combine_kernel(BinParams P) {
     psum[threadIdx.x] = (s0 + s1) + (s2 + s3);
   }
   __syncthreads();
-  finish_rows<256, 256, MULTI, NT>(
+  finish_rows<256, 256, MULTI, NT, W>(
       P, r0, nrows,
       [&](uint32_t v) {
         unsigned long long t = 0ull;
         for (uint32_t q = 0; q < lanes; ++q) t += psum[q * rows + v];
-        return dequantize(t);
+        if constexpr (W) return dequantize_signed(t, P.qshift);
+        else return dequantize(t);
       },
-      red, gerr);
+      red, gerr, P.n_items2 + blockIdx.x);
+}
+
+// weighted sweep: the sweep's L1 change, the bin items' and combine units' partials added in
+// a fixed order (strided per lane, then a fixed tree)
+__global__ __launch_bounds__(256) void pr_wfold_kernel(BinParams P) {
+  if (converged_before<true>(P)) return;
+  __shared__ double red[4];
+  const uint32_t m = P.n_items2 + P.n_items3;
+  double e = 0.0;
+  for (uint32_t q = threadIdx.x; q < m; q += 256) e += P.wpart[q];
+  e = block_sum<256>(e, red);
+  if (threadIdx.x == 0) P.werr[P.iter] = e;
 }
 
 // ---- layout construction ----
 // sort key of CSC position p (row v, source u): (bin of v, u), and v's offset in its bin.
 // A stable sort by this key orders each bin by source, then row (CSC order).
+// With pos, low[p] is p itself (a weighted layout looks up each entry's weight and row).
 __global__ __launch_bounds__(256) void bin_key_kernel(uint32_t n, uint64_t nnz, const uint32_t* __restrict__ rp,
-                                                      const uint32_t* __restrict__ col, int ub,
+                                                      const uint32_t* __restrict__ col, int ub, bool pos,
                                                       uint64_t* __restrict__ keys, uint32_t* __restrict__ low) {
   const uint64_t p = (uint64_t)blockIdx.x * 256 + threadIdx.x;
   if (p >= nnz) return;
@@ -432,7 +491,7 @@ __global__ __launch_bounds__(256) void bin_key_kernel(uint32_t n, uint64_t nnz, 
     else hi = mid;
   }
   keys[p] = ((uint64_t)(lo >> kBinBits) << ub) | col[p];
-  low[p] = lo & kRowMask;
+  low[p] = pos ? (uint32_t)p : (lo & kRowMask);
 }
 
 // tile of a sort key (tile column: 0 for a source below n_hot, else 1 + chunk; without hot
@@ -480,7 +539,9 @@ __global__ __launch_bounds__(256) void bin_place_kernel(uint64_t nnz, const uint
                                                         const uint32_t* __restrict__ etp,
                                                         const uint32_t* __restrict__ ptp,
                                                         uint16_t* __restrict__ brow, uint16_t* __restrict__ bsrc,
-                                                        uint32_t* __restrict__ bpid) {
+                                                        uint32_t* __restrict__ bpid, const uint32_t* __restrict__ rp,
+                                                        uint32_t n, const float* __restrict__ w,
+                                                        float* __restrict__ bw) {
   const uint64_t q = (uint64_t)blockIdx.x * 256 + threadIdx.x;
   if (q >= nnz) return;
   const uint64_t key = keys[q];
@@ -488,7 +549,19 @@ __global__ __launch_bounds__(256) void bin_place_kernel(uint64_t nnz, const uint
   const uint32_t f = flag[q];
   const uint32_t ne = etp[t] + (uint32_t)(q - tp[t]);
   const uint32_t np = ptp[t] + (pex[q] + f - 1u - tpp[t]);
-  brow[ne] = (uint16_t)(low[q] | (f ? kPairStart : 0u));
+  uint32_t lw = low[q];
+  if (w) {  // lw is the entry's CSC position: its weight, and its row's offset in the bin
+    const uint32_t p = lw;
+    uint32_t lo = 0, hi = n;
+    while (lo < hi) {
+      const uint32_t mid = (lo + hi) >> 1;
+      if (rp[mid + 1] <= p) lo = mid + 1;
+      else hi = mid;
+    }
+    lw = lo & kRowMask;
+    bw[ne] = w[p];
+  }
+  brow[ne] = (uint16_t)(lw | (f ? kPairStart : 0u));
   if (f) {
     const uint32_t u = (uint32_t)(key & ((1ull << ub) - 1));
     bsrc[np] = (uint16_t)(u < n_hot ? u : (u & (kChunkSrc - 1)));
@@ -536,7 +609,8 @@ __global__ __launch_bounds__(256) void residual_fold_kernel(const unsigned long 
 
 }  // namespace
 
-void binned_build(hipStream_t s, const ActiveCsc& a, uint32_t n, uint64_t nnz, BinnedLayout& L, bool hot) {
+void binned_build(hipStream_t s, const ActiveCsc& a, uint32_t n, uint64_t nnz, BinnedLayout& L, bool hot,
+                  const float* w) {
   NWG_REQUIRE(nnz < (1ull << 31), "binned layout needs fewer than 2^31 entries");
   L = BinnedLayout();
   L.n_bins = (uint32_t)ceil_div(n, kBinRows);
@@ -554,7 +628,7 @@ void binned_build(hipStream_t s, const ActiveCsc& a, uint32_t n, uint64_t nnz, B
     while (ub < 32 && (1ull << ub) < n) ++ub;
     DevBuf<uint64_t> keys(nnz), keys_tmp(nnz);
     DevBuf<uint32_t> low(nnz), low_tmp(nnz);
-    bin_key_kernel<<<grid256(nnz), 256, 0, s>>>(n, nnz, a.ptr, a.idx, ub, keys.p, low.p);
+    bin_key_kernel<<<grid256(nnz), 256, 0, s>>>(n, nnz, a.ptr, a.idx, ub, w != nullptr, keys.p, low.p);
     NWG_LAUNCH_CHECK();
     int kb = ub;
     while (kb < 64 && (1ull << (kb - ub)) < L.n_bins) ++kb;
@@ -604,10 +678,15 @@ void binned_build(hipStream_t s, const ActiveCsc& a, uint32_t n, uint64_t nnz, B
     NWG_HIP(hipMemsetAsync(L.brow.p, 0, L.padded * 2, s));
     NWG_HIP(hipMemsetAsync(L.bsrc.p, 0xFF, L.pairs * 2, s));
     NWG_HIP(hipMemsetAsync(L.bpid.p, 0, L.padded / kBinBlock * 4, s));
+    if (w) {
+      L.bw.alloc(L.padded);
+      NWG_HIP(hipMemsetAsync(L.bw.p, 0, L.padded * sizeof(float), s));
+    }
     NWG_HIP(hipMemcpyAsync(d_etp.p, etp.data(), (n_tiles + 1) * 4, hipMemcpyHostToDevice, s));
     NWG_HIP(hipMemcpyAsync(d_ptp.p, ptp.data(), (n_tiles + 1) * 4, hipMemcpyHostToDevice, s));
     bin_place_kernel<<<grid256(nnz), 256, 0, s>>>(nnz, keys.p, low.p, flag, pex, ub, TC, L.n_hot, d_tp.p, d_tpp.p,
-                                                  d_etp.p, d_ptp.p, L.brow.p, L.bsrc.p, L.bpid.p);
+                                                  d_etp.p, d_ptp.p, L.brow.p, L.bsrc.p, L.bpid.p, a.ptr, n, w,
+                                                  w ? L.bw.p : nullptr);
     NWG_LAUNCH_CHECK();
     bin_pad_kernel<<<grid256(n_tiles), 256, 0, s>>>(n_tiles, d_tp.p, d_tpp.p, d_etp.p, d_ptp.p, L.brow.p, L.bsrc.p,
                                                     L.bpid.p);
@@ -622,6 +701,10 @@ void binned_build(hipStream_t s, const ActiveCsc& a, uint32_t n, uint64_t nnz, B
     NWG_HIP(hipMemsetAsync(L.brow.p, 0, L.padded * 2, s));
     NWG_HIP(hipMemsetAsync(L.bsrc.p, 0xFF, L.pairs * 2, s));
     NWG_HIP(hipMemsetAsync(L.bpid.p, 0, 4, s));
+    if (w) {
+      L.bw.alloc(L.padded);
+      NWG_HIP(hipMemsetAsync(L.bw.p, 0, L.padded * sizeof(float), s));
+    }
   }
   const uint32_t C = L.n_chunks;
   // hot tile ends (the hot tile is the first of its bin)
@@ -749,6 +832,15 @@ void binned_enqueue_sweep(hipStream_t s, const BinParams& P) {
   if (P.hints >= 2) enqueue_sweep_hinted<2>(s, P);
   else if (P.hints == 1) enqueue_sweep_hinted<1>(s, P);
   else enqueue_sweep_hinted<0>(s, P);
+}
+
+void binned_enqueue_weighted_sweep(hipStream_t s, const BinParams& P) {
+  NWG_REQUIRE(P.bw && P.wpart && P.werr && P.basev, "weighted sweep needs weights, partials and a base");
+  if (P.n_items1) pr_chunk_kernel<0, true><<<P.n_items1, kBT, 0, s>>>(P);
+  pr_bin_kernel<false, 0, true><<<P.n_items2, kBT, 0, s>>>(P);
+  if (P.n_items3) pr_bin_combine_kernel<false, 0, true><<<P.n_items3, 256, 0, s>>>(P);
+  pr_wfold_kernel<<<1, 256, 0, s>>>(P);
+  NWG_LAUNCH_CHECK();
 }
 
 void binned_clear_residuals(hipStream_t s, unsigned long long* lanes, uint64_t iters) {

@@ -127,6 +127,7 @@ struct BinnedLayout {
   DevBuf<uint4> items1;          // chunk pass: {chunk, first tile, end tile, groups of 8 pairs}
   DevBuf<uint32_t> seg_start;    // non-empty tiles of each chunk: first pair
   DevBuf<uint32_t> seg_flat;     // and offset inside their chunk-pass item
+  DevBuf<float> bw;              // weighted layouts: per entry its edge weight (pads 0)
 };
 
 struct Plan {
@@ -182,6 +183,16 @@ struct BinParams {
   const uint32_t* row_graph;  // several graphs in one union: graph of every row (null: one graph)
   uint32_t ngraphs;
   uint32_t hints;  // non-temporal hints (the plan's stream_hints): 0 none, 1 loads, 2 loads and stores
+  // Weighted sweep (bw set): entry value = weight * pair value, scaled by 2^qshift into
+  // [-1/2, 1/2] and summed in signed fixed point; cin holds the previous iterate and x the
+  // new one (double-buffered), basev the per-row constant term; the L1 change of a workgroup
+  // goes to wpart[slot] (bin items, then combine units) and a fold stores werr[iter]; a sweep
+  // exits when werr[iter - 1] < threshold * (*thr_scale, or 1).
+  const float* bw;
+  int qshift;
+  double* wpart;
+  double* werr;
+  const double* thr_scale;
 };
 
 // Identity of a captured sweep loop: re-capture when any of it changes.
@@ -224,6 +235,12 @@ struct WState {
   DevBuf<uint32_t> scale_done;
   DevBuf<uint32_t> done;     // per sweep: workgroups finished
   DevBuf<uint32_t> row_of;  // CSC position -> row
+  // binned sweep: weighted layouts of the CSC (forward) and the CSR (adjoint), built on first
+  // use and dropped when the weights change; pair values, split-bin partial rows, L1 partials
+  std::unique_ptr<BinnedLayout> bl_csc, bl_csr;
+  DevBuf<float> bvals;
+  DevBuf<unsigned long long> bpart;
+  DevBuf<double> wpart, stats;
 };
 
 // Degree-ordered copy of a CSC (relabel.hip): new id r is original vertex order[r].
@@ -297,6 +314,7 @@ struct nwg_graph {
   // optional weights (CSC order, normalised) and raw weights
   bool has_weights = false;
   bool w_normalized = false;
+  bool w_nonneg = false;  // every raw weight finite and >= 0
   nwg::DevBuf<float> w_csc, w_raw, w_csr, w_rowsum;
   // work items of the weighted sweeps over the CSC (forward) and the CSR (adjoint)
   nwg::DevBuf<uint32_t> w_items_csc_r0, w_items_csc_r1, w_items_csr_r0, w_items_csr_r1;
@@ -375,8 +393,12 @@ nwg_plan_params default_plan_params();
 // the plan a PageRank solve of g builds on first use (binned sweep and degree ordering by size)
 nwg_plan_params default_plan_for(const nwg_graph* g);
 // binned sweep (binned.hip)
-void binned_build(hipStream_t s, const ActiveCsc& a, uint32_t n, uint64_t nnz, BinnedLayout& L, bool hot);
+// w (device, CSC order of a, nullable): also lay out the entries' weights (L.bw)
+void binned_build(hipStream_t s, const ActiveCsc& a, uint32_t n, uint64_t nnz, BinnedLayout& L, bool hot,
+                  const float* w = nullptr);
 void binned_enqueue_sweep(hipStream_t s, const BinParams& P);
+// one weighted sweep (P.bw set): chunk, bin and combine passes, then the fold of the L1 partials
+void binned_enqueue_weighted_sweep(hipStream_t s, const BinParams& P);
 // lane counters of iters sweeps cleared / summed into acc[0 .. iters)
 void binned_clear_residuals(hipStream_t s, unsigned long long* lanes, uint64_t iters);
 void binned_fold_residuals(hipStream_t s, const unsigned long long* lanes, unsigned long long* acc, uint64_t iters);

@@ -26,6 +26,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <vector>
 
 #include "device_utils.hpp"
@@ -331,6 +332,41 @@ __global__ __launch_bounds__(kThreads) void l1_scale_kernel(uint32_t n, const fl
   }
 }
 
+// Per workgroup over a contiguous range: sum and max of |b| (the bound a binned weighted sweep
+// scales its fixed point by); NaN or infinity shows in the sum.
+__global__ __launch_bounds__(kThreads) void abs_stats_kernel(uint32_t n, const float* __restrict__ b,
+                                                             double* __restrict__ out) {
+  __shared__ double red_d[kThreads / 64];
+  const uint64_t per = ((uint64_t)n + gridDim.x - 1) / gridDim.x;
+  const uint64_t i0 = (uint64_t)blockIdx.x * per, i1 = i0 + per < n ? i0 + per : n;
+  double sum = 0.0, mx = 0.0;
+  for (uint64_t i = i0 + threadIdx.x; i < i1; i += kThreads) {
+    const double a = (double)fabsf(b[i]);
+    sum += a;
+    mx = a > mx ? a : mx;
+  }
+  sum = block_sum<kThreads>(sum, red_d);
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) {
+    const double y = __shfl_xor(mx, o, 64);
+    mx = y > mx ? y : mx;
+  }
+  if ((threadIdx.x & 63) == 0) red_d[threadIdx.x >> 6] = mx;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (uint32_t q = 0; q < kThreads / 64; ++q) mx = red_d[q] > mx ? red_d[q] : mx;
+    out[2 * blockIdx.x] = sum;
+    out[2 * blockIdx.x + 1] = mx;
+  }
+}
+
+// *bad = 1 when some weight is negative or not finite
+__global__ __launch_bounds__(kThreads) void weight_check_kernel(uint64_t nnz, const float* __restrict__ w,
+                                                                uint32_t* __restrict__ bad) {
+  const uint64_t p = (uint64_t)blockIdx.x * kThreads + threadIdx.x;
+  if (p < nnz && !(w[p] >= 0.f && w[p] <= 3.402823466e38f)) *bad = 1u;
+}
+
 // row of every CSC position (upper bound in the row pointers)
 __global__ __launch_bounds__(kThreads) void row_of_kernel(uint32_t n, uint64_t nnz, const uint32_t* __restrict__ rp,
                                                           uint32_t* __restrict__ row) {
@@ -449,19 +485,35 @@ void weights_set(hipStream_t s, nwg_graph* g, const float* w_dev, bool normalize
   }
   gather_kernel<<<grid1(nnz), kThreads, 0, s>>>(nnz, g->csr_to_csc.p, g->w_csc.p, g->w_csr.p);
   NWG_LAUNCH_CHECK();
+  // raw weights all finite and >= 0: the binned weighted sweep's fixed-point bound holds
+  DevBuf<uint32_t> bad(1);
+  NWG_HIP(hipMemsetAsync(bad.p, 0, sizeof(uint32_t), s));
+  if (nnz) {
+    weight_check_kernel<<<grid1(nnz), kThreads, 0, s>>>(nnz, g->w_raw.p, bad.p);
+    NWG_LAUNCH_CHECK();
+  }
+  uint32_t hb = 0;
+  NWG_HIP(hipMemcpyAsync(&hb, bad.p, sizeof hb, hipMemcpyDeviceToHost, s));
   NWG_HIP(hipStreamSynchronize(s));
   g->has_weights = true;
   g->w_normalized = normalize;
+  g->w_nonneg = hb == 0;
+  if (g->wst) {  // the weighted layouts carry the old weights
+    g->wst->bl_csc.reset();
+    g->wst->bl_csr.reset();
+  }
 }
 
 // Runs the fixed point y <- fma(d, sum w y, base) from y0 over (rp, col, w) in the cached
 // buffers of `st`; returns the sweeps executed and leaves the result in `out`.
 // rhs: the constant term b whose L1 norm scales a relative threshold (null: the uniform
-// teleport).
+// teleport).  L (nullable): run the binned weighted sweep over this layout, its products
+// scaled by 2^qshift.
 static uint64_t fixed_point(nwg_ctx* ctx, WState& st, const uint32_t* rp, const uint32_t* col, const float* w,
                             const DevBuf<uint32_t>& r0, const DevBuf<uint32_t>& r1, uint32_t n_items, uint32_t n,
                             const float* base, const float* y0, bool y0_uniform, float damping, float threshold,
-                            uint64_t max_iters, float* out, nwg_pr_info* info, const float* rhs) {
+                            uint64_t max_iters, float* out, nwg_pr_info* info, const float* rhs,
+                            const BinnedLayout* L = nullptr, int qshift = 0) {
   hipStream_t s = ctx->stream;
   st.y0.ensure(n);
   st.y1.ensure(n);
@@ -497,12 +549,50 @@ static uint64_t fixed_point(nwg_ctx* ctx, WState& st, const uint32_t* rp, const 
   P.err = st.err.p;
   P.threshold = (double)threshold;
   P.thr_scale = relative ? st.scale.p : nullptr;
-  for (uint64_t k = 0; k < max_iters; ++k) {
-    P.iter = (uint32_t)k;
-    P.xin = y[k & 1];
-    P.xout = y[(k + 1) & 1];
-    wsweep_kernel<<<grid, kThreads, 0, s>>>(P);
-    NWG_LAUNCH_CHECK();
+  if (L) {
+    // binned sweep: x double-buffered as y[k & 1] -> y[(k + 1) & 1]; its L1 change goes to st.err
+    st.bvals.ensure(L->pairs);
+    st.bpart.ensure(std::max<uint64_t>((uint64_t)L->n_slots << kBinBits, 1));
+    st.wpart.ensure(std::max<uint64_t>((uint64_t)L->n_items2 + L->n_items3, 1));
+    BinParams B{};
+    B.brow = L->brow.p;
+    B.bsrc = L->bsrc.p;
+    B.bpid = L->bpid.p;
+    B.vals = st.bvals.p;
+    B.items2 = L->items2.p;
+    B.items1 = L->items1.p;
+    B.items3 = L->items3.p;
+    B.seg_start = L->seg_start.p;
+    B.seg_flat = L->seg_flat.p;
+    B.part = st.bpart.p;
+    B.hot_end = L->hot_end.p;
+    B.n_hot = L->n_hot;
+    B.basev = base;
+    B.damp = damping;
+    B.n = n;
+    B.n_items1 = L->n_items1;
+    B.n_items2 = L->n_items2;
+    B.n_items3 = L->n_items3;
+    B.threshold = (double)threshold;
+    B.thr_scale = P.thr_scale;
+    B.bw = L->bw.p;
+    B.qshift = qshift;
+    B.wpart = st.wpart.p;
+    B.werr = st.err.p;
+    for (uint64_t k = 0; k < max_iters; ++k) {
+      B.iter = (uint32_t)k;
+      B.cin = y[k & 1];
+      B.x = y[(k + 1) & 1];
+      binned_enqueue_weighted_sweep(s, B);
+    }
+  } else {
+    for (uint64_t k = 0; k < max_iters; ++k) {
+      P.iter = (uint32_t)k;
+      P.xin = y[k & 1];
+      P.xout = y[(k + 1) & 1];
+      wsweep_kernel<<<grid, kThreads, 0, s>>>(P);
+      NWG_LAUNCH_CHECK();
+    }
   }
   NWG_HIP(hipEventRecord(ctx->ev1, s));
   std::vector<double> h(std::max<uint64_t>(max_iters, 1));
@@ -542,6 +632,60 @@ static WState& wstate(nwg_graph* g) {
   return *g->wst;
 }
 
+// The binned weighted sweep runs when the graph's plan (or, before any plan, the size-based
+// default) asks for the binned sweep and the weights are normalised per source from
+// non-negative raw weights: then every weight is in [0, 1] and each source's out-weights sum
+// to 1, which the fixed-point bound below relies on.
+static bool weighted_binned(const nwg_graph* g) {
+  const nwg_plan_params pp = g->plan.built ? g->plan.params : default_plan_for(g);
+  return pp.binned != 0 && g->w_normalized && g->w_nonneg && g->nnz > 0;
+}
+
+// Scale of the binned weighted sweep's fixed point: 2^t with 2^t * 2M in [1/2, 1), M a bound
+// of every product and partial row sum, from b (the constant term) in exact arithmetic:
+//   forward  x' = b + d A x, x0 = 1/n: |sum_u w(u->i) x_u| <= ||x||_1 <= max(1, ||b||_1 / (1-d))
+//            (A is column-stochastic: ||A x||_1 <= ||x||_1);
+//   adjoint  l' = b + d A^T l, l0 = b: |sum_i w(u->i) l_i| <= ||l||_inf <= ||b||_inf / (1-d)
+//            (each row of A^T sums to 1).
+// The factor 2 covers the FP32 rounding of the iterates.  False when b is not finite (the
+// pull sweep then runs and shows the NaN / infinity as the reference would).
+static bool fixed_point_scale(hipStream_t s, WState& st, uint32_t n, const float* b, bool forward, float damping,
+                              int* t) {
+  st.stats.ensure(2 * kNormBlocks);
+  abs_stats_kernel<<<kNormBlocks, kThreads, 0, s>>>(n, b, st.stats.p);
+  NWG_LAUNCH_CHECK();
+  std::vector<double> h(2 * kNormBlocks);
+  NWG_HIP(hipMemcpyAsync(h.data(), st.stats.p, h.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+  NWG_HIP(hipStreamSynchronize(s));
+  double sum = 0.0, mx = 0.0;
+  for (uint32_t q = 0; q < kNormBlocks; ++q) {
+    sum += h[2 * q];
+    mx = std::max(mx, h[2 * q + 1]);
+  }
+  if (!std::isfinite(sum)) return false;
+  const double omd = 1.0 - (double)damping;
+  const double M = forward ? std::max(1.0, sum / omd) : mx / omd;
+  *t = 0;
+  if (M > 0.0) std::frexp(2.0 * M, t), *t = -*t;
+  return true;
+}
+
+static const BinnedLayout* weighted_layout(hipStream_t s, nwg_graph* g, WState& st, bool csr) {
+  std::unique_ptr<BinnedLayout>& L = csr ? st.bl_csr : st.bl_csc;
+  if (!L) {
+    L.reset(new BinnedLayout());
+    const ActiveCsc a = csr ? ActiveCsc{g->csr_ptr.p, g->csr_idx.p, nullptr, nullptr}
+                            : ActiveCsc{g->csc_ptr.p, g->csc_idx.p, g->outdeg.p, nullptr};
+    try {
+      binned_build(s, a, g->n, g->nnz, *L, true, csr ? g->w_csr.p : g->w_csc.p);
+    } catch (...) {
+      L.reset();
+      throw;
+    }
+  }
+  return L.get();
+}
+
 void wpr_forward(nwg_ctx* ctx, nwg_graph* g, const float* teleport_dev, float* x_dev, float damping, float threshold,
                  uint64_t max_iters, nwg_pr_info* info) {
   hipStream_t s = ctx->stream;
@@ -553,9 +697,13 @@ void wpr_forward(nwg_ctx* ctx, nwg_graph* g, const float* teleport_dev, float* x
   st.base.ensure(g->n);
   wbase_kernel<<<grid1(g->n), kThreads, 0, s>>>(g->n, teleport_dev, 1.0 - (double)damping, st.base.p);
   NWG_LAUNCH_CHECK();
+  const BinnedLayout* L = nullptr;
+  int t = 0;
+  if (weighted_binned(g) && fixed_point_scale(s, st, g->n, st.base.p, true, damping, &t))
+    L = weighted_layout(s, g, st, false);
   fixed_point(ctx, st, g->csc_ptr.p, g->csc_idx.p, g->w_csc.p, g->w_items_csc_r0, g->w_items_csc_r1,
               g->w_items_csc_n, g->n, st.base.p, nullptr, true, damping, threshold, max_iters, x_dev, info,
-              teleport_dev ? st.base.p : nullptr);
+              teleport_dev ? st.base.p : nullptr, L, t);
 }
 
 void wpr_backward(nwg_ctx* ctx, nwg_graph* g, const float* x_dev, const float* gx_dev, float* gw_dev, float* gv_dev,
@@ -570,8 +718,12 @@ void wpr_backward(nwg_ctx* ctx, nwg_graph* g, const float* x_dev, const float* g
   WState& st = wstate(g);
   st.lambda.ensure(n);
   // adjoint fixed point: lambda = g + d A^T lambda, started from g
+  const BinnedLayout* L = nullptr;
+  int t = 0;
+  if (weighted_binned(g) && fixed_point_scale(s, st, n, gx_dev, false, damping, &t)) L = weighted_layout(s, g, st, true);
   fixed_point(ctx, st, g->csr_ptr.p, g->csr_idx.p, g->w_csr.p, g->w_items_csr_r0, g->w_items_csr_r1,
-              g->w_items_csr_n, n, gx_dev, gx_dev, false, damping, threshold, max_iters, st.lambda.p, info, gx_dev);
+              g->w_items_csr_n, n, gx_dev, gx_dev, false, damping, threshold, max_iters, st.lambda.p, info, gx_dev,
+              L, t);
   if (gw_dev && nnz) {
     if (!st.row_of.n) {
       st.row_of.alloc(nnz);

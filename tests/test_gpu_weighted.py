@@ -257,6 +257,54 @@ def test_relative_tolerance_gradients_match_fp64_rmat16(ctx):
     assert abs(itf - kf) <= 1 and abs(itb - kb) <= 1, (kf, itf, kb, itb)
 
 
+@pytest.mark.parametrize("scale", [12, 15])
+def test_binned_weighted_sweep_matches_pull_and_fp64(ctx, scale):
+    """The binned weighted sweep (weights carried per entry in the binned layout, signed fixed
+    point scaled by a power of two from the solve's own bound) against the pull sweep on the
+    same graph and against FP64 iterations run for the same sweeps: the forward, the adjoint
+    of a loss with gradients of both signs at two scales (|grad_x| ~ 1 and ~ 1e3, which moves
+    the scale by 2^10), and the edge and teleport gradients.  RMAT-15 has more entries per
+    bin than one bin-pass work item takes, so split bins and their combine pass run too."""
+    g, n, ind, col, deg, rows, w_raw = setup_graph(ctx, scale, seed=8, wseed=6)
+    g.set_weights(w_raw, normalize=True)
+    _, wn, _ = dense_system(n, ind, col, rows, w_raw) if scale <= 12 else (None, None, None)
+    if wn is None:
+        s = np.bincount(col, weights=w_raw.astype(np.float64), minlength=n)
+        wn = w_raw.astype(np.float64) / s[col]
+    dd = float(np.float32(D))
+    rel = lambda a, b: np.linalg.norm(a.astype(np.float64) - b) / np.linalg.norm(b)
+    rng = np.random.default_rng(scale)
+    v = rng.uniform(size=n)
+    v = (v / v.sum()).astype(np.float32)
+    out = {}
+    for binned in (0, 1):
+        g.set_plan(binned=binned)
+        x, fi = g.wpr_forward(v, D, 0.0, 60)  # a fixed number of sweeps on both paths
+        res = [x, fi]
+        for amp in (1.0, 1000.0):
+            c = (amp * O.normal_vector(n, seed=40 + scale)).astype(np.float32)
+            gw, gv, bi = g.wpr_backward(x, c, D, 1e-4, 200, raw_weights=False, relative=True)
+            res += [gw, gv, bi]
+        out[binned] = res
+    xp, fp = out[0][0], out[0][1]
+    xb, fb = out[1][0], out[1][1]
+    assert fb["iterations"] == fp["iterations"] == 60
+    assert np.max(np.abs(xb.astype(np.float64) - xp)) <= 1e-6 * np.abs(xp).max()
+    x64, _ = O.fp64_jacobi(n, ind, col, deg, w=wn, v=v.astype(np.float64), damping=dd, tol=0.0,
+                           max_iters=fb["iterations"])
+    assert rel(xb, x64) < 1e-5
+    for j, amp in enumerate((1.0, 1000.0)):
+        gwp, gvp, bip = out[0][2 + 3 * j: 5 + 3 * j]
+        gwb, gvb, bib = out[1][2 + 3 * j: 5 + 3 * j]
+        assert abs(bib["iterations"] - bip["iterations"]) <= 1, (amp, bib["iterations"], bip["iterations"])
+        c = (amp * O.normal_vector(n, seed=40 + scale)).astype(np.float64)
+        lam64, _ = O.fp64_adjoint(n, ind, col, deg, c, w=wn, damping=dd, tol=0.0, max_iters=bib["iterations"])
+        x64b, _ = O.fp64_jacobi(n, ind, col, deg, w=wn, v=v.astype(np.float64), damping=dd, tol=0.0,
+                                max_iters=fb["iterations"])
+        assert rel(gvb, (1.0 - dd) * lam64) < 1e-4, amp
+        assert rel(gwb, dd * lam64[rows] * x64b[col]) < 1e-4, amp
+
+
 def test_full_size_adjoint_identities_rmat20(ctx):
     """RMAT-20 with random raw weights, every solve run to its float fixed point (150 sweeps,
     0.85^150 < 1e-10).  Identities that hold for any size; on a dense FP64 system each one
