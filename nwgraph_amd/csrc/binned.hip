@@ -299,6 +299,8 @@ __device__ __forceinline__ void bin_range(const BinParams& P, uint32_t e0, uint3
   const char* vbase = reinterpret_cast<const char*>(P.vals);
   const char* sbase = reinterpret_cast<const char*>(P.bsrc);
   const uint32_t ke = e1 / kBinBlock;
+  int qs = 0;  // the weighted sweep's scale exponent (uniform: one scalar load)
+  if constexpr (W) qs = __builtin_amdgcn_readfirstlane(*P.qshift);
   for (uint32_t k0 = e0 / kBinBlock + w; k0 < ke; k0 += kGroups2 * kWaves) {
     u32x4 rw[kGroups2];
     uint32_t pb[kGroups2];
@@ -356,7 +358,7 @@ __device__ __forceinline__ void bin_range(const BinParams& P, uint32_t e0, uint3
       if constexpr (W) {
 #pragma unroll
         for (uint32_t t = 0; t < 8; ++t)
-          atomicAdd(&lacc[rows[h][t]], quantize_signed(wv[h][t >> 2][t & 3] * vs[h][t], P.qshift));
+          atomicAdd(&lacc[rows[h][t]], quantize_signed(wv[h][t >> 2][t & 3] * vs[h][t], qs));
       } else {
 #pragma unroll
         for (uint32_t t = 0; t < 8; ++t) atomicAdd(&lacc[rows[h][t]], quantize(vs[h][t]));
@@ -416,7 +418,8 @@ __global__ __launch_bounds__(kBT) void pr_bin_kernel(BinParams P) {
   const uint32_t nrows = r0 < P.n ? min(kBinRows, P.n - r0) : 0u;
   if constexpr (W)
     finish_rows<kBT, kBinRows, MULTI, NT, true>(
-        P, r0, nrows, [&](uint32_t v) { return dequantize_signed(lacc[v], P.qshift); }, red, gerr, blockIdx.x);
+        P, r0, nrows, [&, qs = *P.qshift](uint32_t v) { return dequantize_signed(lacc[v], qs); }, red, gerr,
+        blockIdx.x);
   else
     finish_rows<kBT, kBinRows, MULTI, NT>(P, r0, nrows, [&](uint32_t v) { return dequantize(lacc[v]); }, red, gerr);
 }
@@ -457,7 +460,7 @@ __global__ __launch_bounds__(256) void pr_bin_combine_kernel(BinParams P) {
       [&](uint32_t v) {
         unsigned long long t = 0ull;
         for (uint32_t q = 0; q < lanes; ++q) t += psum[q * rows + v];
-        if constexpr (W) return dequantize_signed(t, P.qshift);
+        if constexpr (W) return dequantize_signed(t, *P.qshift);
         else return dequantize(t);
       },
       red, gerr, P.n_items2 + blockIdx.x);

@@ -183,21 +183,32 @@ struct BinParams {
   const uint32_t* row_graph;  // several graphs in one union: graph of every row (null: one graph)
   uint32_t ngraphs;
   uint32_t hints;  // non-temporal hints (the plan's stream_hints): 0 none, 1 loads, 2 loads and stores
-  // Weighted sweep (bw set): entry value = weight * pair value, scaled by 2^qshift into
+  // Weighted sweep (bw set): entry value = weight * pair value, scaled by 2^(*qshift) into
   // [-1/2, 1/2] and summed in signed fixed point; cin holds the previous iterate and x the
   // new one (double-buffered), basev the per-row constant term; the L1 change of a workgroup
   // goes to wpart[slot] (bin items, then combine units) and a fold stores werr[iter]; a sweep
   // exits when werr[iter - 1] < threshold * (*thr_scale, or 1).
   const float* bw;
-  int qshift;
+  const int* qshift;  // device word: set per solve, read by the captured sweeps
   double* wpart;
   double* werr;
   const double* thr_scale;
 };
 
+// A solve's sweeps run as captured HIP graphs of kSegFirst sweeps, then kSegNext at a time,
+// with the host checking the last sweep's L1 change between segments: sweeps after
+// convergence still launch (and exit at once), and at a few microseconds per launch, 80 of
+// them cost more than the check.
+constexpr uint64_t kSegFirst = 20;
+constexpr uint64_t kSegNext = 8;
+inline uint64_t segment_end(uint64_t start, uint64_t max_iters) {
+  const uint64_t len = start == 0 ? kSegFirst : kSegNext;
+  return start + len < max_iters ? start + len : max_iters;
+}
+
 // Identity of a captured sweep loop: re-capture when any of it changes.
 struct CaptureKey {
-  static constexpr int kPtrs = 12;
+  static constexpr int kPtrs = 16;
   const void* ptrs[kPtrs] = {};
   uint64_t plan_version = 0;
   uint64_t iters = 0;
@@ -241,6 +252,13 @@ struct WState {
   DevBuf<float> bvals;
   DevBuf<unsigned long long> bpart;
   DevBuf<double> wpart, stats;
+  // captured solve loops (init + max_iters sweeps) of the forward and the adjoint; a call's
+  // varying inputs go to fixed buffers first: the adjoint's grad_x to gin, the binned sweep's
+  // scale exponent to qs
+  std::vector<std::unique_ptr<Captured>> cap_fwd, cap_adj;  // one per segment of sweeps
+  DevBuf<float> gin;
+  DevBuf<int> qs;
+  int qs_host = 0;
 };
 
 // Degree-ordered copy of a CSC (relabel.hip): new id r is original vertex order[r].
@@ -268,7 +286,7 @@ struct PrState {
   DevBuf<unsigned long long> bpart;  // binned sweep: partial rows of split bins
   DevBuf<unsigned long long> bacc;   // binned sweep: per-sweep L1 change, kBinAccLanes lanes
   DevBuf<double> tcheck;             // teleport check: sum, invalid-entry flag
-  Captured cap;
+  std::vector<std::unique_ptr<Captured>> caps;  // the solve's segments of sweeps
 };
 
 // Reusable state of the batched personalised solve: the heavy-row segments of the active

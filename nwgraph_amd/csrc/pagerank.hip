@@ -763,8 +763,10 @@ static BinParams make_bin_params(nwg_graph* g, PrState& st, const SweepParams& P
   return B;
 }
 
+// Sweeps [k0, k1) of an iters-sweep solve: the inits when k0 = 0, and for the binned sweep
+// the fold of these sweeps' residual lanes into acc.
 static void enqueue_solve(hipStream_t s, nwg_graph* g, PrState& st, SweepParams P, uint64_t iters,
-                          hipEvent_t* ev, PrSchedule sched) {
+                          hipEvent_t* ev, PrSchedule sched, uint64_t k0, uint64_t k1) {
   const uint32_t n = g->n;
   const uint32_t nt = g->plan.params.stream_hints;
   // In place, the binned sweep reads and writes one contribution array: its chunk pass stages
@@ -777,7 +779,7 @@ static void enqueue_solve(hipStream_t s, nwg_graph* g, PrState& st, SweepParams 
   BinParams B{};
   if (binned) {
     B = make_bin_params(g, st, P);
-    binned_clear_residuals(s, st.bacc.p, iters);
+    if (k0 == 0) binned_clear_residuals(s, st.bacc.p, iters);
   }
   auto sweep = [&](const float* cin, float* cout) {
     if (binned) {
@@ -797,16 +799,18 @@ static void enqueue_solve(hipStream_t s, nwg_graph* g, PrState& st, SweepParams 
     // page_rank_v13: contrib0 holds the deltas, contrib1 the residuals, x the scores; the
     // push kernel's flags go to acc, where the pull kernels' stopping test reads them
     P.delta = 1;
-    if (ev) NWG_HIP(hipEventRecord(ev[2 * iters], s));
-    pr_delta_init_kernel<<<blocks_for(std::max<uint64_t>(n, iters)), kThreads, 0, s>>>(
-        n, (float)(1.0 - (double)P.damp), st.x.p, st.contrib0.p, st.contrib1.p, st.acc.p, iters);
-    NWG_LAUNCH_CHECK();
-    if (ev) {
-      NWG_HIP(hipEventRecord(ev[2 * iters + 1], s));
-      NWG_HIP(hipEventRecord(ev[2 * iters + 2], s));
-      NWG_HIP(hipEventRecord(ev[2 * iters + 3], s));
+    if (k0 == 0) {
+      if (ev) NWG_HIP(hipEventRecord(ev[2 * iters], s));
+      pr_delta_init_kernel<<<blocks_for(std::max<uint64_t>(n, iters)), kThreads, 0, s>>>(
+          n, (float)(1.0 - (double)P.damp), st.x.p, st.contrib0.p, st.contrib1.p, st.acc.p, iters);
+      NWG_LAUNCH_CHECK();
+      if (ev) {
+        NWG_HIP(hipEventRecord(ev[2 * iters + 1], s));
+        NWG_HIP(hipEventRecord(ev[2 * iters + 2], s));
+        NWG_HIP(hipEventRecord(ev[2 * iters + 3], s));
+      }
     }
-    for (uint64_t k = 0; k < iters; ++k) {
+    for (uint64_t k = k0; k < k1; ++k) {
       P.iter = (uint32_t)k;
       if (ev) NWG_HIP(hipEventRecord(ev[2 * k], s));
       pr_delta_push_kernel<<<blocks_for(n), kThreads, 0, s>>>(P, st.contrib1.p, st.contrib0.p, st.acc.p, 1u);
@@ -818,20 +822,22 @@ static void enqueue_solve(hipStream_t s, nwg_graph* g, PrState& st, SweepParams 
   }
   const bool inplace = sched == PrSchedule::kInPlace;
   const float init = (float)(1.0 / n);  // page_rank.hpp:93
-  if (ev) {
-    NWG_HIP(hipEventRecord(ev[2 * iters], s));
-    pr_init_kernel<<<blocks_for(std::max<uint64_t>(n, iters)), kThreads, 0, s>>>(n, P.deg, init, st.x.p, nullptr,
-                                                                                 st.acc.p, iters);
-    NWG_HIP(hipEventRecord(ev[2 * iters + 1], s));
-    NWG_HIP(hipEventRecord(ev[2 * iters + 2], s));
-    pr_contrib_init_kernel<<<blocks_for(n), kThreads, 0, s>>>(n, P.deg, st.x.p, st.contrib0.p);
-    NWG_HIP(hipEventRecord(ev[2 * iters + 3], s));
-  } else {
-    pr_init_kernel<<<blocks_for(std::max<uint64_t>(n, iters)), kThreads, 0, s>>>(n, P.deg, init, st.x.p,
-                                                                                 st.contrib0.p, st.acc.p, iters);
+  if (k0 == 0) {
+    if (ev) {
+      NWG_HIP(hipEventRecord(ev[2 * iters], s));
+      pr_init_kernel<<<blocks_for(std::max<uint64_t>(n, iters)), kThreads, 0, s>>>(n, P.deg, init, st.x.p, nullptr,
+                                                                                   st.acc.p, iters);
+      NWG_HIP(hipEventRecord(ev[2 * iters + 1], s));
+      NWG_HIP(hipEventRecord(ev[2 * iters + 2], s));
+      pr_contrib_init_kernel<<<blocks_for(n), kThreads, 0, s>>>(n, P.deg, st.x.p, st.contrib0.p);
+      NWG_HIP(hipEventRecord(ev[2 * iters + 3], s));
+    } else {
+      pr_init_kernel<<<blocks_for(std::max<uint64_t>(n, iters)), kThreads, 0, s>>>(n, P.deg, init, st.x.p,
+                                                                                   st.contrib0.p, st.acc.p, iters);
+    }
+    NWG_LAUNCH_CHECK();
   }
-  NWG_LAUNCH_CHECK();
-  for (uint64_t k = 0; k < iters; ++k) {
+  for (uint64_t k = k0; k < k1; ++k) {
     P.iter = (uint32_t)k;
     if (ev) NWG_HIP(hipEventRecord(ev[2 * k], s));
     // in place (page_rank.hpp:135): one contribution array, each row's new contribution is
@@ -841,7 +847,8 @@ static void enqueue_solve(hipStream_t s, nwg_graph* g, PrState& st, SweepParams 
     else sweep(st.contrib0.p, st.contrib1.p);
     if (ev) NWG_HIP(hipEventRecord(ev[2 * k + 1], s));
   }
-  if (binned) binned_fold_residuals(s, st.bacc.p, st.acc.p, iters);
+  if (binned && k1 > k0)
+    binned_fold_residuals(s, st.bacc.p + k0 * kBinAccStride, st.acc.p + k0, k1 - k0);
 }
 
 void pagerank_run(nwg_ctx* ctx, nwg_graph* g, const float* teleport_dev, float damping, float threshold,
@@ -906,32 +913,54 @@ void pagerank_run(nwg_ctx* ctx, nwg_graph* g, const float* teleport_dev, float d
     key.damping = damping;
     key.threshold = threshold;
     key.extra = (uint32_t)sched;
-    if (!st.cap.valid || !(st.cap.key == key)) {
-      st.cap.reset();
-      hipStream_t cs = ctx->capture_stream;
-      NWG_HIP(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
-      try {
-        enqueue_solve(cs, g, st, P, max_iters, nullptr, sched);
-      } catch (...) {
-        hipGraph_t tmp = nullptr;
-        (void)hipStreamEndCapture(cs, &tmp);
-        if (tmp) (void)hipGraphDestroy(tmp);
-        throw;
+    // the solve's segments (kSegFirst sweeps, then kSegNext at a time), each one captured
+    // graph; between segments the host reads the last sweep's L1 change and stops once the
+    // solve has converged, so sweeps after convergence launch only up to a segment's end
+    uint64_t nseg = 0;
+    for (uint64_t k0 = 0;; k0 = segment_end(k0, max_iters)) {
+      const uint64_t k1 = segment_end(k0, max_iters);
+      if (st.caps.size() <= nseg) st.caps.emplace_back(new Captured());
+      Captured& cap = *st.caps[nseg++];
+      if (!cap.valid || !(cap.key == key)) {
+        cap.reset();
+        hipStream_t cs = ctx->capture_stream;
+        NWG_HIP(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
+        try {
+          enqueue_solve(cs, g, st, P, max_iters, nullptr, sched, k0, k1);
+        } catch (...) {
+          hipGraph_t tmp = nullptr;
+          (void)hipStreamEndCapture(cs, &tmp);
+          if (tmp) (void)hipGraphDestroy(tmp);
+          throw;
+        }
+        NWG_HIP(hipStreamEndCapture(cs, &cap.graph));
+        NWG_HIP(hipGraphInstantiate(&cap.exec, cap.graph, nullptr, nullptr, 0));
+        cap.key = key;
+        cap.valid = true;
       }
-      NWG_HIP(hipStreamEndCapture(cs, &st.cap.graph));
-      NWG_HIP(hipGraphInstantiate(&st.cap.exec, st.cap.graph, nullptr, nullptr, 0));
-      st.cap.key = key;
-      st.cap.valid = true;
+      if (k1 >= max_iters) break;
     }
     NWG_HIP(hipEventRecord(ctx->ev0, s));
-    NWG_HIP(hipGraphLaunch(st.cap.exec, s));
+    uint64_t seg = 0;
+    for (uint64_t k0 = 0;; k0 = segment_end(k0, max_iters)) {
+      const uint64_t k1 = segment_end(k0, max_iters);
+      NWG_HIP(hipGraphLaunch(st.caps[seg++]->exec, s));
+      if (k1 >= max_iters) break;
+      unsigned long long last = 0ull;
+      NWG_HIP(hipMemcpyAsync(&last, st.acc.p + (k1 - 1), sizeof last, hipMemcpyDeviceToHost, s));
+      NWG_HIP(hipStreamSynchronize(s));
+      // stopped: delta schedule, nothing pushed (page_rank.hpp:616-620); else the L1 change
+      // below the threshold (page_rank.hpp:144) -- or the solve stopped earlier and this
+      // sweep's accumulator kept the 0 it was cleared to
+      if (sched == PrSchedule::kDelta ? last == 0ull : fixed_to_err(last) < (double)threshold) break;
+    }
     NWG_HIP(hipEventRecord(ctx->ev1, s));
   } else {
     evs.resize(2 * max_iters + 4);
     for (auto& e : evs) NWG_HIP(hipEventCreate(&e));
     NWG_HIP(hipEventRecord(ctx->ev0, s));
     try {
-      enqueue_solve(s, g, st, P, max_iters, evs.data(), sched);
+      enqueue_solve(s, g, st, P, max_iters, evs.data(), sched, 0, max_iters);
     } catch (...) {
       for (auto& e : evs) (void)hipEventDestroy(e);
       throw;

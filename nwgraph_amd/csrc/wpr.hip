@@ -508,11 +508,15 @@ void weights_set(hipStream_t s, nwg_graph* g, const float* w_dev, bool normalize
 // buffers of `st`; returns the sweeps executed and leaves the result in `out`.
 // rhs: the constant term b whose L1 norm scales a relative threshold (null: the uniform
 // teleport).  L (nullable): run the binned weighted sweep over this layout, its products
-// scaled by 2^qshift.
-static uint64_t fixed_point(nwg_ctx* ctx, WState& st, const uint32_t* rp, const uint32_t* col, const float* w,
-                            const DevBuf<uint32_t>& r0, const DevBuf<uint32_t>& r1, uint32_t n_items, uint32_t n,
-                            const float* base, const float* y0, bool y0_uniform, float damping, float threshold,
-                            uint64_t max_iters, float* out, nwg_pr_info* info, const float* rhs,
+// scaled by 2^qshift.  The init and the max_iters sweeps run as one HIP graph captured into
+// cap (re-captured when a buffer, the layout or a parameter changes), so the host launches
+// one graph per solve instead of up to four kernels per sweep; base, y0 and rhs must be
+// buffers that stay the same from call to call.
+static uint64_t fixed_point(nwg_ctx* ctx, WState& st, std::vector<std::unique_ptr<Captured>>& caps,
+                            const uint32_t* rp, const uint32_t* col,
+                            const float* w, const DevBuf<uint32_t>& r0, const DevBuf<uint32_t>& r1, uint32_t n_items,
+                            uint32_t n, const float* base, const float* y0, bool y0_uniform, float damping,
+                            float threshold, uint64_t max_iters, float* out, nwg_pr_info* info, const float* rhs,
                             const BinnedLayout* L = nullptr, int qshift = 0) {
   hipStream_t s = ctx->stream;
   st.y0.ensure(n);
@@ -531,10 +535,14 @@ static uint64_t fixed_point(nwg_ctx* ctx, WState& st, const uint32_t* rp, const 
                                                     st.scale_done.p, st.scale.p);
     NWG_LAUNCH_CHECK();
   }
-  NWG_HIP(hipEventRecord(ctx->ev0, s));
-  winit_kernel<<<grid1(std::max<uint64_t>(n, max_iters)), kThreads, 0, s>>>(
-      n, y0_uniform ? nullptr : y0, (float)(1.0 / n), y[0], st.done.p, st.err.p, max_iters);
-  NWG_LAUNCH_CHECK();
+  if (L) {
+    st.qs.ensure(1);
+    st.qs_host = qshift;
+    NWG_HIP(hipMemcpyAsync(st.qs.p, &st.qs_host, sizeof(int), hipMemcpyHostToDevice, s));
+    st.bvals.ensure(L->pairs);
+    st.bpart.ensure(std::max<uint64_t>((uint64_t)L->n_slots << kBinBits, 1));
+    st.wpart.ensure(std::max<uint64_t>((uint64_t)L->n_items2 + L->n_items3, 1));
+  }
   WParams P{};
   P.rp = rp;
   P.col = col;
@@ -549,50 +557,109 @@ static uint64_t fixed_point(nwg_ctx* ctx, WState& st, const uint32_t* rp, const 
   P.err = st.err.p;
   P.threshold = (double)threshold;
   P.thr_scale = relative ? st.scale.p : nullptr;
-  if (L) {
-    // binned sweep: x double-buffered as y[k & 1] -> y[(k + 1) & 1]; its L1 change goes to st.err
-    st.bvals.ensure(L->pairs);
-    st.bpart.ensure(std::max<uint64_t>((uint64_t)L->n_slots << kBinBits, 1));
-    st.wpart.ensure(std::max<uint64_t>((uint64_t)L->n_items2 + L->n_items3, 1));
-    BinParams B{};
-    B.brow = L->brow.p;
-    B.bsrc = L->bsrc.p;
-    B.bpid = L->bpid.p;
-    B.vals = st.bvals.p;
-    B.items2 = L->items2.p;
-    B.items1 = L->items1.p;
-    B.items3 = L->items3.p;
-    B.seg_start = L->seg_start.p;
-    B.seg_flat = L->seg_flat.p;
-    B.part = st.bpart.p;
-    B.hot_end = L->hot_end.p;
-    B.n_hot = L->n_hot;
-    B.basev = base;
-    B.damp = damping;
-    B.n = n;
-    B.n_items1 = L->n_items1;
-    B.n_items2 = L->n_items2;
-    B.n_items3 = L->n_items3;
-    B.threshold = (double)threshold;
-    B.thr_scale = P.thr_scale;
-    B.bw = L->bw.p;
-    B.qshift = qshift;
-    B.wpart = st.wpart.p;
-    B.werr = st.err.p;
-    for (uint64_t k = 0; k < max_iters; ++k) {
-      B.iter = (uint32_t)k;
-      B.cin = y[k & 1];
-      B.x = y[(k + 1) & 1];
-      binned_enqueue_weighted_sweep(s, B);
-    }
-  } else {
-    for (uint64_t k = 0; k < max_iters; ++k) {
-      P.iter = (uint32_t)k;
-      P.xin = y[k & 1];
-      P.xout = y[(k + 1) & 1];
-      wsweep_kernel<<<grid, kThreads, 0, s>>>(P);
+  // sweeps [k0, k1) (and the init with k0 = 0) on stream q
+  auto enqueue = [&](hipStream_t q, uint64_t k0, uint64_t k1) {
+    if (k0 == 0) {
+      winit_kernel<<<grid1(std::max<uint64_t>(n, max_iters)), kThreads, 0, q>>>(
+          n, y0_uniform ? nullptr : y0, (float)(1.0 / n), y[0], st.done.p, st.err.p, max_iters);
       NWG_LAUNCH_CHECK();
     }
+    if (L) {
+      // binned sweep: x double-buffered as y[k & 1] -> y[(k + 1) & 1]; its L1 change goes to st.err
+      BinParams B{};
+      B.brow = L->brow.p;
+      B.bsrc = L->bsrc.p;
+      B.bpid = L->bpid.p;
+      B.vals = st.bvals.p;
+      B.items2 = L->items2.p;
+      B.items1 = L->items1.p;
+      B.items3 = L->items3.p;
+      B.seg_start = L->seg_start.p;
+      B.seg_flat = L->seg_flat.p;
+      B.part = st.bpart.p;
+      B.hot_end = L->hot_end.p;
+      B.n_hot = L->n_hot;
+      B.basev = base;
+      B.damp = damping;
+      B.n = n;
+      B.n_items1 = L->n_items1;
+      B.n_items2 = L->n_items2;
+      B.n_items3 = L->n_items3;
+      B.threshold = (double)threshold;
+      B.thr_scale = P.thr_scale;
+      B.bw = L->bw.p;
+      B.qshift = st.qs.p;
+      B.wpart = st.wpart.p;
+      B.werr = st.err.p;
+      for (uint64_t k = k0; k < k1; ++k) {
+        B.iter = (uint32_t)k;
+        B.cin = y[k & 1];
+        B.x = y[(k + 1) & 1];
+        binned_enqueue_weighted_sweep(q, B);
+      }
+    } else {
+      for (uint64_t k = k0; k < k1; ++k) {
+        P.iter = (uint32_t)k;
+        P.xin = y[k & 1];
+        P.xout = y[(k + 1) & 1];
+        wsweep_kernel<<<grid, kThreads, 0, q>>>(P);
+        NWG_LAUNCH_CHECK();
+      }
+    }
+  };
+  CaptureKey key;
+  const void* ptrs[CaptureKey::kPtrs] = {st.y0.p, st.y1.p, base, y0, st.err.p, st.done.p, st.part.p, rp, col, w,
+                                         L, L ? st.bvals.p : nullptr, L ? st.bpart.p : nullptr,
+                                         L ? st.wpart.p : nullptr, L ? st.qs.p : nullptr,
+                                         relative ? st.scale.p : nullptr};
+  for (int q = 0; q < CaptureKey::kPtrs; ++q) key.ptrs[q] = ptrs[q];
+  key.iters = max_iters;
+  key.damping = damping;
+  key.threshold = threshold;
+  key.plan_version = n_items;
+  key.extra = (relative ? 1u : 0u) | (y0_uniform ? 2u : 0u);
+  // the segments' graphs, captured when missing or stale
+  uint64_t nseg = 0;
+  for (uint64_t k0 = 0; k0 < max_iters || (k0 == 0 && nseg == 0); k0 = segment_end(k0, max_iters)) {
+    const uint64_t k1 = segment_end(k0, max_iters);
+    if (caps.size() <= nseg) caps.emplace_back(new Captured());
+    Captured& cap = *caps[nseg];
+    ++nseg;
+    if (!cap.valid || !(cap.key == key)) {
+      cap.reset();
+      hipStream_t cs = ctx->capture_stream;
+      NWG_HIP(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
+      try {
+        enqueue(cs, k0, k1);
+      } catch (...) {
+        hipGraph_t tmp = nullptr;
+        (void)hipStreamEndCapture(cs, &tmp);
+        if (tmp) (void)hipGraphDestroy(tmp);
+        throw;
+      }
+      NWG_HIP(hipStreamEndCapture(cs, &cap.graph));
+      NWG_HIP(hipGraphInstantiate(&cap.exec, cap.graph, nullptr, nullptr, 0));
+      cap.key = key;
+      cap.valid = true;
+    }
+    if (k1 >= max_iters) break;
+  }
+  double scale_now = 1.0;
+  if (relative) {
+    NWG_HIP(hipMemcpyAsync(&scale_now, st.scale.p, sizeof(double), hipMemcpyDeviceToHost, s));
+    NWG_HIP(hipStreamSynchronize(s));
+  }
+  const double stop_now = relative ? (double)threshold * scale_now : (double)threshold;
+  NWG_HIP(hipEventRecord(ctx->ev0, s));
+  uint64_t seg = 0;
+  for (uint64_t k0 = 0;; k0 = segment_end(k0, max_iters), ++seg) {
+    const uint64_t k1 = segment_end(k0, max_iters);
+    NWG_HIP(hipGraphLaunch(caps[seg]->exec, s));
+    if (k1 >= max_iters) break;
+    double last_err = 0.0;  // the segment's last sweep: stop when it converged
+    NWG_HIP(hipMemcpyAsync(&last_err, st.err.p + (k1 - 1), sizeof(double), hipMemcpyDeviceToHost, s));
+    NWG_HIP(hipStreamSynchronize(s));
+    if (last_err < stop_now) break;
   }
   NWG_HIP(hipEventRecord(ctx->ev1, s));
   std::vector<double> h(std::max<uint64_t>(max_iters, 1));
@@ -701,7 +768,7 @@ void wpr_forward(nwg_ctx* ctx, nwg_graph* g, const float* teleport_dev, float* x
   int t = 0;
   if (weighted_binned(g) && fixed_point_scale(s, st, g->n, st.base.p, true, damping, &t))
     L = weighted_layout(s, g, st, false);
-  fixed_point(ctx, st, g->csc_ptr.p, g->csc_idx.p, g->w_csc.p, g->w_items_csc_r0, g->w_items_csc_r1,
+  fixed_point(ctx, st, st.cap_fwd, g->csc_ptr.p, g->csc_idx.p, g->w_csc.p, g->w_items_csc_r0, g->w_items_csc_r1,
               g->w_items_csc_n, g->n, st.base.p, nullptr, true, damping, threshold, max_iters, x_dev, info,
               teleport_dev ? st.base.p : nullptr, L, t);
 }
@@ -718,12 +785,16 @@ void wpr_backward(nwg_ctx* ctx, nwg_graph* g, const float* x_dev, const float* g
   WState& st = wstate(g);
   st.lambda.ensure(n);
   // adjoint fixed point: lambda = g + d A^T lambda, started from g
+  // grad_x into a fixed buffer, so that the captured loop serves every call
+  st.gin.ensure(n);
+  NWG_HIP(hipMemcpyAsync(st.gin.p, gx_dev, (size_t)n * sizeof(float), hipMemcpyDeviceToDevice, s));
   const BinnedLayout* L = nullptr;
   int t = 0;
-  if (weighted_binned(g) && fixed_point_scale(s, st, n, gx_dev, false, damping, &t)) L = weighted_layout(s, g, st, true);
-  fixed_point(ctx, st, g->csr_ptr.p, g->csr_idx.p, g->w_csr.p, g->w_items_csr_r0, g->w_items_csr_r1,
-              g->w_items_csr_n, n, gx_dev, gx_dev, false, damping, threshold, max_iters, st.lambda.p, info, gx_dev,
-              L, t);
+  if (weighted_binned(g) && fixed_point_scale(s, st, n, st.gin.p, false, damping, &t))
+    L = weighted_layout(s, g, st, true);
+  fixed_point(ctx, st, st.cap_adj, g->csr_ptr.p, g->csr_idx.p, g->w_csr.p, g->w_items_csr_r0, g->w_items_csr_r1,
+              g->w_items_csr_n, n, st.gin.p, st.gin.p, false, damping, threshold, max_iters, st.lambda.p, info,
+              st.gin.p, L, t);
   if (gw_dev && nnz) {
     if (!st.row_of.n) {
       st.row_of.alloc(nnz);
