@@ -28,6 +28,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <type_traits>
 #include <vector>
 
 #include "device_utils.hpp"
@@ -45,8 +46,16 @@ constexpr uint32_t kItem1Pairs = 1u << 18;    // pairs per chunk-pass item
 constexpr uint32_t kPadSrc = 0xFFFFu;           // source offset of a pad pair (value 0)
 constexpr uint32_t kRowMask = kBinRows - 1;
 constexpr uint32_t kNoSlot = 0xFFFFFFFFu;     // bin-pass item of a bin that is one item
-constexpr uint32_t kGroups1 = 4;              // groups of 8 entries in flight per lane: chunk pass
-constexpr uint32_t kGroups2 = 2;              // and bin pass (more would cost it its second workgroup per CU)
+constexpr uint32_t kGroups1 = 4;              // groups of 8 pairs in flight per lane in the chunk pass
+
+// Inside a kBinBlock block the entries are stored slot-major: entry r of the block (in
+// layout order) sits at lane r % 64, slot r / 64 of the block's 64 x 8 array, so that a lane's
+// 16-byte load brings its 8 slots and slot t of the 64 lanes holds 64 consecutive entries.
+// Their pairs are consecutive too, so each slot's pair loads touch one or two cache lines
+// (lane-major, every slot's loads spanned the block's whole pair range, ~7 lines each).
+__host__ __device__ __forceinline__ uint32_t bin_slot(uint32_t e) {
+  return (e & ~(kBinBlock - 1u)) | ((e & 63u) << 3) | ((e >> 6) & 7u);
+}
 
 // Cache hints (the plan's stream_hints): NT >= 1 loads the streams each lane reads once
 // (row offsets, source offsets, partial rows, staged contributions, scores, out-degrees)
@@ -280,99 +289,211 @@ __global__ __launch_bounds__(kBT) void pr_chunk_kernel(BinParams P) {
   }
 }
 
-// Adds the entries [e0, e1) of one bin into lacc; e0 and e1 are block boundaries (the layout
-// aligns bins and hot tiles).  The range is walked in blocks of kBinBlock entries (8 per
-// lane), one block per wave, kGroups2 blocks in flight.  The pair of an entry is the block's
-// first pair (bpid) plus the pair starts after the block's first entry up to the entry: the
-// lanes count them with one ballot per entry slot.  HOT: the entries lie in a hot tile and a
-// pair's value is hot[source] (pads, source kPadSrc, add 0); else it is P.vals[pair].
-// Entries are ordered by source, so one row rarely repeats inside a lane or an instruction:
-// each entry is one LDS add, with no run tracking.
-template <bool HOT, int NT, bool W = false>
-__device__ __forceinline__ void bin_range(const BinParams& P, uint32_t e0, uint32_t e1, unsigned long long* lacc,
-                                          const float* hot) {
+// One wave's block of kBinBlock entries: the lane's 8 row offsets (one 16-byte load) and the
+// block's first pair.
+struct BinBlock {
+  u32x4 e;
+  uint32_t pb;
+};
+
+template <int NT>
+__device__ __forceinline__ BinBlock bin_block_load(const BinParams& P, uint32_t k, uint32_t lane) {
+  BinBlock b;
+  b.e = bld<NT>(reinterpret_cast<const u32x4*>(P.brow + k * kBinBlock + 8u * lane));
+  // the block's first pair as a vector load (every lane the same address, one request): a
+  // scalar load would have to be waited on with the LDS adds (scalar loads return out of
+  // order), which serialises the pipeline below
+  uint32_t kv = k;
+  asm volatile("" : "+v"(kv));
+  b.pb = P.bpid[kv];
+  return b;
+}
+
+// the weighted sweep's 8 weights of the lane in block k: 32 bytes, 32-byte aligned
+struct BinWeights {
+  f32x4 w[2];
+};
+template <int NT>
+__device__ __forceinline__ void bin_weights_load(const BinParams& P, uint32_t k, uint32_t lane, BinWeights& wt) {
+  const f32x4* wp = reinterpret_cast<const f32x4*>(P.bw + k * kBinBlock + 8u * lane);
+  wt.w[0] = bld<NT>(wp);
+  wt.w[1] = bld<NT>(wp + 1);
+}
+
+// Issues the loads of a block's pair data for the lane's 8 entries (not waited on): the
+// pair's value (P.vals), or in a hot tile the pair's source id.  The pair of an entry is the
+// block's first pair plus the pair starts after the block's first entry up to the entry.
+// Slot t of lane l is entry 64 t + l (bin_slot): its pair is the block's first pair, plus
+// the starts of slots 0 .. t-1 (a wave-uniform popcount of their ballots), plus the starts
+// of slot t in lanes 0 .. l (mbcnt of its ballot, and its own flag).
+template <bool HOT>
+__device__ __forceinline__ void bin_block_pairs(const BinBlock& b, uint32_t lane, const char* vbase,
+                                                const char* sbase, uint32_t (&raw)[8]) {
+  const uint32_t wd[4] = {b.e.x, b.e.y, b.e.z, b.e.w};
+  uint32_t base = b.pb;
+#pragma unroll
+  for (uint32_t t = 0; t < 8; ++t) {
+    uint32_t f = (wd[t >> 1] >> (16 * (t & 1) + 15)) & 1u;
+    if (t == 0 && lane == 0) f = 0;  // the block's first entry is in pair pb by definition
+    const unsigned long long m = __ballot(f);
+    const uint32_t pid =
+        __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, base)) + f;
+    base += (uint32_t)__popcll(m);
+    if constexpr (HOT) raw[t] = *reinterpret_cast<const uint16_t*>(sbase + (pid << 1));
+    else raw[t] = *reinterpret_cast<const uint32_t*>(vbase + (pid << 2));
+  }
+}
+
+// Adds a block's 8 entries of this lane into the LDS row accumulators (W: each times its
+// weight).
+template <bool HOT, bool W>
+__device__ __forceinline__ void bin_block_add(const BinBlock& b, const uint32_t (&raw)[8], const BinWeights& wt,
+                                              unsigned long long* lacc, const float* hot, int qs) {
+  const uint32_t wd[4] = {b.e.x, b.e.y, b.e.z, b.e.w};
+#pragma unroll
+  for (uint32_t t = 0; t < 8; ++t) {
+    const uint32_t row = (wd[t >> 1] >> (16 * (t & 1))) & kRowMask;
+    float v;
+    if constexpr (HOT) {  // pads (source kPadSrc) add 0
+      const uint32_t u = raw[t];
+      const float h = hot[min(u, kHotSrc - 1u)];
+      v = u < kHotSrc ? h : 0.f;
+    } else {
+      v = __uint_as_float(raw[t]);
+    }
+    if constexpr (W) atomicAdd(&lacc[row], quantize_signed(wt.w[t >> 2][t & 3] * v, qs));
+    else atomicAdd(&lacc[row], quantize(v));
+  }
+}
+
+// Adds the entries [e0, e1) of one bin into lacc; e0, he and e1 are block boundaries (the
+// layout aligns bins and hot tiles).  The blocks of [e0, he) lie in the bin's hot tile, where
+// a pair's value is hot[source] (pads, source kPadSrc, add 0); those of [he, e1) read it from
+// P.vals[pair].  Wave w of the workgroup takes blocks w, w + 16, ... of the whole range in one
+// software pipeline, so that the loads of three blocks overlap: while block j's entries are
+// added, block j+1's pair loads and block j+2's row offsets are in flight.  (A wave that
+// waited on each block's row offsets, then on its pair values, had one memory latency in
+// flight at a time, and a separate loop per tile drained the pipeline at the tile's end.)
+// Blocks past the wave's last are clamped to the range's last block: their loads stay in
+// bounds and they are never added.  Entries are ordered by source, so one row rarely repeats
+// inside a lane or an instruction: each entry is one LDS add, with no run tracking.
+template <int NT, bool W = false>
+__device__ __forceinline__ void bin_range(const BinParams& P, uint32_t e0, uint32_t he, uint32_t e1,
+                                          unsigned long long* lacc, const float* hot) {
   // the wave index is uniform over the wave: in a scalar register, the block indices and the
   // blocks' first pairs stay scalar too
   const uint32_t w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
   constexpr uint32_t kWaves = kBT / 64;
+  const uint32_t kb = e0 / kBinBlock + w, kh = he / kBinBlock, ke = e1 / kBinBlock;
+  if (kb >= ke) return;
+  const uint32_t nb = (ke - kb + kWaves - 1) / kWaves;
   // values are addressed by 32-bit byte offsets (the layout keeps pairs below 2^30)
   const char* vbase = reinterpret_cast<const char*>(P.vals);
   const char* sbase = reinterpret_cast<const char*>(P.bsrc);
-  const uint32_t ke = e1 / kBinBlock;
   int qs = 0;  // the weighted sweep's scale exponent (uniform: one scalar load)
   if constexpr (W) qs = __builtin_amdgcn_readfirstlane(*P.qshift);
-  for (uint32_t k0 = e0 / kBinBlock + w; k0 < ke; k0 += kGroups2 * kWaves) {
-    u32x4 rw[kGroups2];
-    uint32_t pb[kGroups2];
-    f32x4 wv[W ? kGroups2 : 1][2];
+  auto blk = [&](uint32_t j) {
+    const uint32_t k = kb + j * kWaves;
+    return k < ke ? k : ke - 1u;
+  };
+  // Block k's row offsets live in eb[k % 4], its pair loads in rb[k % 2] and (weighted) its
+  // weights in wb[k % 2]; the loop is unrolled by four so that every buffer keeps fixed
+  // registers (a register copy of a load still in flight would wait for it).  Step k issues
+  // block k+2's row offsets and block k+1's pair loads (and weights), then adds block k;
+  // the scheduling barriers keep the compiler from hoisting the adds, which wait for block
+  // k's loads, above the newer loads.  Whether a block is hot is uniform over the wave.
+  BinBlock eb[4];
+  uint32_t rb[2][8];
+  BinWeights wb[2];
+  auto pairs = [&](uint32_t k, const BinBlock& e, uint32_t(&r)[8]) {
+    if (k < kh) bin_block_pairs<true>(e, lane, vbase, sbase, r);
+    else bin_block_pairs<false>(e, lane, vbase, sbase, r);
+  };
+  auto add = [&](uint32_t k, const BinBlock& e, const uint32_t(&r)[8], const BinWeights& wt) {
+    if (k < kh) bin_block_add<true, W>(e, r, wt, lacc, hot, qs);
+    else bin_block_add<false, W>(e, r, wt, lacc, hot, qs);
+  };
+  auto step = [&](auto kc, uint32_t j) {
+    constexpr uint32_t K = decltype(kc)::value;
+    eb[(K + 2) % 4] = bin_block_load<NT>(P, blk(j + K + 2), lane);
+    if constexpr (W) bin_weights_load<NT>(P, blk(j + K + 1), lane, wb[(K + 1) % 2]);
+    pairs(blk(j + K + 1), eb[(K + 1) % 4], rb[(K + 1) % 2]);
+    __builtin_amdgcn_sched_barrier(0);
+    add(blk(j + K), eb[K % 4], rb[K % 2], wb[K % 2]);
+    __builtin_amdgcn_sched_barrier(0);
+  };
+  eb[0] = bin_block_load<NT>(P, blk(0), lane);
+  eb[1] = bin_block_load<NT>(P, blk(1), lane);
+  if constexpr (W) bin_weights_load<NT>(P, blk(0), lane, wb[0]);
+  pairs(blk(0), eb[0], rb[0]);
+  uint32_t j = 0;
+  for (; j + 4 <= nb; j += 4) {
+    step(std::integral_constant<uint32_t, 0>{}, j);
+    step(std::integral_constant<uint32_t, 1>{}, j);
+    step(std::integral_constant<uint32_t, 2>{}, j);
+    step(std::integral_constant<uint32_t, 3>{}, j);
+  }
+  const uint32_t rest = nb - j;  // 0..3 blocks
+  if (rest > 0) step(std::integral_constant<uint32_t, 0>{}, j);
+  if (rest > 1) step(std::integral_constant<uint32_t, 1>{}, j);
+  if (rest > 2) step(std::integral_constant<uint32_t, 2>{}, j);
+}
+
+// The finishing loads of one graph's rows r0 + threadIdx.x + k * kBT (k < kBinRows / kBT):
+// issued before the bin pass's last barrier, so that they are in flight while the other
+// waves finish their adds.
+struct FinishIn {
+  float xo[kBinRows / kBT];
+  float bv[kBinRows / kBT];
+  uint32_t dg[kBinRows / kBT];
+};
+template <int NT>
+__device__ __forceinline__ void finish_load(const BinParams& P, uint32_t r0, uint32_t nrows, FinishIn& in) {
 #pragma unroll
-    for (uint32_t h = 0; h < kGroups2; ++h) {
-      // a block past the range re-reads block k0 (in bounds) and is skipped below, so that
-      // all blocks' loads are issued before any is waited on
-      const uint32_t k = k0 + h * kWaves;
-      const uint32_t kc = k < ke ? k : k0;
-      rw[h] = bld<NT>(reinterpret_cast<const u32x4*>(P.brow + kc * kBinBlock + 8u * lane));
-      pb[h] = P.bpid[kc];
-      if constexpr (W) {  // the lane's 8 weights: 32 bytes, 32-byte aligned
-        const f32x4* wp = reinterpret_cast<const f32x4*>(P.bw + kc * kBinBlock + 8u * lane);
-        wv[h][0] = bld<NT>(wp);
-        wv[h][1] = bld<NT>(wp + 1);
-      }
+  for (uint32_t k = 0; k < kBinRows / kBT; ++k) {
+    const uint32_t v = threadIdx.x + k * kBT;
+    if (v < nrows) {
+      in.xo[k] = bld<NT>(P.x + r0 + v);
+      in.dg[k] = bld<NT>(P.deg + r0 + v);
+      in.bv[k] = P.basev ? bld<NT>(P.basev + r0 + v) : P.base;
     }
-    uint32_t rows[kGroups2][8];
-    float vs[kGroups2][8];
+  }
+}
+// the reference's row update (page_rank.hpp:133-135) on the prefetched rows, and the
+// workgroup's L1 change into a lane counter of the sweep (as finish_rows)
+template <int NT>
+__device__ __forceinline__ void finish_apply(const BinParams& P, uint32_t r0, uint32_t nrows, const FinishIn& in,
+                                             const unsigned long long* lacc, double* red) {
+  double e = 0.0;
 #pragma unroll
-    for (uint32_t h = 0; h < kGroups2; ++h) {
-      const uint32_t wd[4] = {rw[h].x, rw[h].y, rw[h].z, rw[h].w};
-      uint32_t fb[8];
-#pragma unroll
-      for (uint32_t t = 0; t < 8; ++t) {
-        rows[h][t] = (wd[t >> 1] >> (16 * (t & 1))) & kRowMask;
-        fb[t] = (wd[t >> 1] >> (16 * (t & 1) + 15)) & 1u;
-      }
-      if (lane == 0) fb[0] = 0;  // the block's first entry is in pair bpid by definition
-      // pair starts in the lanes below this one, all slots
-      uint32_t pid = pb[h];
-#pragma unroll
-      for (uint32_t t = 0; t < 8; ++t) {
-        const unsigned long long m = __ballot(fb[t]);
-        pid = __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, pid));
-      }
-#pragma unroll
-      for (uint32_t t = 0; t < 8; ++t) {
-        pid += fb[t];
-        if constexpr (HOT) {
-          const uint32_t u = *reinterpret_cast<const uint16_t*>(sbase + (pid << 1));
-          const float v = hot[min(u, kHotSrc - 1u)];
-          vs[h][t] = u < kHotSrc ? v : 0.f;
-        } else {
-          vs[h][t] = *reinterpret_cast<const float*>(vbase + (pid << 2));
-        }
-      }
+  for (uint32_t k = 0; k < kBinRows / kBT; ++k) {
+    const uint32_t v = threadIdx.x + k * kBT;
+    if (v < nrows) {
+      const float xn = __builtin_fmaf(P.damp, dequantize(lacc[v]), in.bv[k]);
+      bst<NT>(P.x + r0 + v, xn);
+      bst<NT>(P.cout + r0 + v, xn / (float)in.dg[k]);
+      e += (double)fabsf(xn - in.xo[k]);
     }
-    // (Measured: issuing both blocks' value loads before the first add, or prefetching the
-    // next round's row offsets, made this pass slower at RMAT-25: 0.65 against 0.60 ms.)
-#pragma unroll
-    for (uint32_t h = 0; h < kGroups2; ++h) {
-      if (k0 + h * kWaves >= ke) continue;  // wave-uniform
-      if constexpr (W) {
-#pragma unroll
-        for (uint32_t t = 0; t < 8; ++t)
-          atomicAdd(&lacc[rows[h][t]], quantize_signed(wv[h][t >> 2][t & 3] * vs[h][t], qs));
-      } else {
-#pragma unroll
-        for (uint32_t t = 0; t < 8; ++t) atomicAdd(&lacc[rows[h][t]], quantize(vs[h][t]));
-      }
-    }
+  }
+  e = wave_sum(e);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = e;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = 0.0;
+    for (uint32_t q = 0; q < kBT / 64; ++q) t += red[q];
+    if (t > 0.0) atomicAdd(residual_lane(P), err_to_fixed(t));
   }
 }
 
 // bin pass: row sums of one entry range of one bin in LDS; a bin that is one item finishes
 // its rows here, an item of a split bin stores its partial rows in its slot of P.part.
 // The range's part inside the bin's hot tile (a prefix of the bin) reads its values from
-// the hot contributions staged in LDS.
+// the hot contributions staged in LDS.  An item's hot end (P.item_he) is precomputed, so that
+// no load waits on another before the hot contributions are staged.
+// (8 waves per SIMD: the 64 KiB of row sums allow two workgroups per CU, which 64 VGPRs keep;
+// the weighted pipeline needs more registers and runs one workgroup per CU)
 template <bool MULTI, int NT, bool W = false>
-__global__ __launch_bounds__(kBT) void pr_bin_kernel(BinParams P) {
+__global__ __launch_bounds__(kBT, W ? 4 : 8) void pr_bin_kernel(BinParams P) {
   if (converged_before<W>(P)) return;
   __shared__ unsigned long long lacc[kBinRows];
   __shared__ double red[kBT / 64];
@@ -380,34 +501,45 @@ __global__ __launch_bounds__(kBT) void pr_bin_kernel(BinParams P) {
   __shared__ float hot[MULTI ? 1 : kHotSrc];
   const uint4 it = P.items2[blockIdx.x];  // {bin, first entry, end entry, partial slot or kNoSlot}
   const uint32_t b = it.x;
-  for (uint32_t q = threadIdx.x; q < kBinRows; q += kBT) lacc[q] = 0ull;
-  uint32_t he = it.y;  // end of the hot part of [it.y, it.z)
+  // end of the hot part of [it.y, it.z); the hot contributions' loads are issued first and
+  // written to LDS once the row sums are cleared
+  uint32_t he = it.y;
+  constexpr uint32_t kR = (kHotSrc + kBT - 1) / kBT;
+  float hv[kR];
   if constexpr (!MULTI) {
-    if (P.n_hot) {
-      he = min(max(P.hot_end[b], it.y), it.z);
-      if (he > it.y) {
-        constexpr uint32_t kR = (kHotSrc + kBT - 1) / kBT;
-        float v[kR];
+    he = P.item_he[blockIdx.x];
+    if (he > it.y) {
 #pragma unroll
-        for (uint32_t k = 0; k < kR; ++k) {
-          const uint32_t q = threadIdx.x + k * kBT;
-          v[k] = q < P.n_hot ? bld<NT>(P.cin + q) : 0.f;
-        }
+      for (uint32_t k = 0; k < kR; ++k) {
+        const uint32_t q = threadIdx.x + k * kBT;
+        hv[k] = q < P.n_hot ? bld<NT>(P.cin + q) : 0.f;
+      }
+    }
+  }
+  for (uint32_t q = threadIdx.x; q < kBinRows; q += kBT) lacc[q] = 0ull;
+  if constexpr (!MULTI) {
+    if (he > it.y) {
 #pragma unroll
-        for (uint32_t k = 0; k < kR; ++k) {
-          const uint32_t q = threadIdx.x + k * kBT;
-          if (q < kHotSrc) hot[q] = v[k];
-        }
+      for (uint32_t k = 0; k < kR; ++k) {
+        const uint32_t q = threadIdx.x + k * kBT;
+        if (q < kHotSrc) hot[q] = hv[k];
       }
     }
   }
   __syncthreads();
-  if constexpr (!MULTI) {
-    if (he > it.y) bin_range<true, NT, W>(P, it.y, he, lacc, hot);
-  }
-  bin_range<false, NT, W>(P, he, it.z, lacc, hot);
-  __syncthreads();
+  bin_range<NT, W>(P, it.y, he, it.z, lacc, hot);
   const uint32_t r0 = b << kBinBits;
+  const uint32_t nrows = r0 < P.n ? min(kBinRows, P.n - r0) : 0u;
+  if constexpr (!MULTI && !W) {
+    if (it.w == kNoSlot) {  // one graph, a bin that is one item: prefetch the rows, then finish
+      FinishIn in;
+      finish_load<NT>(P, r0, nrows, in);
+      __syncthreads();
+      finish_apply<NT>(P, r0, nrows, in, lacc, red);
+      return;
+    }
+  }
+  __syncthreads();
   if (it.w != kNoSlot) {
     unsigned long long* dst = P.part + (uint64_t)it.w * kBinRows;
     for (uint32_t v = threadIdx.x; v < kBinRows; v += kBT) bst<NT>(dst + v, lacc[v]);
@@ -415,7 +547,6 @@ __global__ __launch_bounds__(kBT) void pr_bin_kernel(BinParams P) {
       if (threadIdx.x == 0) P.wpart[blockIdx.x] = 0.0;  // this item finishes no rows
     return;
   }
-  const uint32_t nrows = r0 < P.n ? min(kBinRows, P.n - r0) : 0u;
   if constexpr (W)
     finish_rows<kBT, kBinRows, MULTI, NT, true>(
         P, r0, nrows, [&, qs = *P.qshift](uint32_t v) { return dequantize_signed(lacc[v], qs); }, red, gerr,
@@ -531,8 +662,9 @@ __global__ __launch_bounds__(256) void tile_ptr_kernel(uint64_t nnz, const uint6
   tpp[t] = pex[lo];
 }
 
-// sorted entry q of tile t goes to entry etp[t] + (q - tp[t]); its pair, numbered inside the
-// tile from tpp[t], to pair ptp[t] + ...; the first entry of a kBinBlock block records its pair
+// sorted entry q of tile t goes to entry etp[t] + (q - tp[t]) (stored at its bin_slot); its pair,
+// numbered inside the tile from tpp[t], to pair ptp[t] + ...; the first entry of a kBinBlock
+// block records its pair
 __global__ __launch_bounds__(256) void bin_place_kernel(uint64_t nnz, const uint64_t* __restrict__ keys,
                                                         const uint32_t* __restrict__ low,
                                                         const uint32_t* __restrict__ flag,
@@ -562,9 +694,9 @@ __global__ __launch_bounds__(256) void bin_place_kernel(uint64_t nnz, const uint
       else hi = mid;
     }
     lw = lo & kRowMask;
-    bw[ne] = w[p];
+    bw[bin_slot(ne)] = w[p];
   }
-  brow[ne] = (uint16_t)(lw | (f ? kPairStart : 0u));
+  brow[bin_slot(ne)] = (uint16_t)(lw | (f ? kPairStart : 0u));
   if (f) {
     const uint32_t u = (uint32_t)(key & ((1ull << ub) - 1));
     bsrc[np] = (uint16_t)(u < n_hot ? u : (u & (kChunkSrc - 1)));
@@ -587,7 +719,7 @@ __global__ __launch_bounds__(256) void bin_pad_kernel(uint64_t n_tiles, const ui
   const uint32_t pads = (ptp[t + 1] - ptp[t]) - np;
   for (uint32_t j = 0; j < pads; ++j) {
     const uint32_t e = etp[t] + ne + j, p = ptp[t] + np + j;
-    brow[e] = (uint16_t)kPairStart;
+    brow[bin_slot(e)] = (uint16_t)kPairStart;
     bsrc[p] = (uint16_t)kPadSrc;
     if (e % kBinBlock == 0) bpid[e / kBinBlock] = p;
   }
@@ -713,8 +845,6 @@ void binned_build(hipStream_t s, const ActiveCsc& a, uint32_t n, uint64_t nnz, B
   // hot tile ends (the hot tile is the first of its bin)
   std::vector<uint32_t> he(L.n_bins);
   for (uint32_t b = 0; b < L.n_bins; ++b) he[b] = etp[(uint64_t)b * TC + hc];
-  L.hot_end.alloc(std::max<size_t>(he.size(), 1));
-  if (!he.empty()) NWG_HIP(hipMemcpyAsync(L.hot_end.p, he.data(), he.size() * 4, hipMemcpyHostToDevice, s));
   // bin pass items: every bin, cut into ranges of at most cap2 entries (inner cuts on
   // block boundaries, so no block is walked by two items of one bin); the items of a split bin
   // get consecutive partial slots, and the bin combine units
@@ -736,6 +866,7 @@ void binned_build(hipStream_t s, const ActiveCsc& a, uint32_t n, uint64_t nnz, B
     cap2 = 1u << lg;
   }
   std::vector<uint4> it2, it3;
+  std::vector<uint32_t> ihe;
   uint32_t slots = 0;
   for (uint32_t b = 0; b < L.n_bins; ++b) {
     const uint32_t p0 = etp[(uint64_t)b * TC], p1 = etp[(uint64_t)(b + 1) * TC];
@@ -751,7 +882,33 @@ void binned_build(hipStream_t s, const ActiveCsc& a, uint32_t n, uint64_t nnz, B
       const uint32_t c = p0 + (uint32_t)((uint64_t)(p1 - p0) * q / cnt);
       return std::max(p0, c / kBinBlock * kBinBlock);
     };
-    for (uint32_t q = 0; q < cnt; ++q) it2.push_back(make_uint4(b, cut(q), cut(q + 1), cnt > 1 ? slots++ : kNoSlot));
+    for (uint32_t q = 0; q < cnt; ++q) {
+      const uint32_t c0 = cut(q), c1 = cut(q + 1);
+      it2.push_back(make_uint4(b, c0, c1, cnt > 1 ? slots++ : kNoSlot));
+      ihe.push_back(std::min(std::max(he[b], c0), c1));  // the item's part of the hot tile ends here
+    }
+  }
+  // Items of bins without entries go last.  Their rows have no in-edges, so one sweep takes
+  // each of them to its fixed point x = base (fma(d, 0, base)), and the next writes the same
+  // contribution base / deg into the other contribution buffer; from the third sweep on
+  // nothing in them changes (their L1 change is 0), and the sweeps launch only the items
+  // before them.  Degree ordering puts every vertex without in-edges after the others, so
+  // that at RMAT-25 half of the bins are empty and the rest hold no tiny bins.
+  {
+    std::vector<uint4> o2;
+    std::vector<uint32_t> ohe;
+    o2.reserve(it2.size());
+    ohe.reserve(it2.size());
+    for (int pass = 0; pass < 2; ++pass)
+      for (uint32_t q = 0; q < (uint32_t)it2.size(); ++q)
+        if ((it2[q].z > it2[q].y) == (pass == 0)) {
+          o2.push_back(it2[q]);
+          ohe.push_back(ihe[q]);
+        }
+    it2.swap(o2);
+    ihe.swap(ohe);
+    L.n_items2_live = 0;
+    while (L.n_items2_live < it2.size() && it2[L.n_items2_live].z > it2[L.n_items2_live].y) ++L.n_items2_live;
   }
   // chunk pass items: the non-empty tiles of each chunk (long tiles cut into pieces), cut
   // at cap1 pairs or kBinMaxSegs pieces; offsets and sizes in groups of 8 pairs (every
@@ -809,6 +966,8 @@ void binned_build(hipStream_t s, const ActiveCsc& a, uint32_t n, uint64_t nnz, B
   L.seg_start.alloc(std::max<size_t>(sst.size(), 1));
   L.seg_flat.alloc(std::max<size_t>(sfl.size(), 1));
   if (!it2.empty()) NWG_HIP(hipMemcpyAsync(L.items2.p, it2.data(), it2.size() * sizeof(uint4), hipMemcpyHostToDevice, s));
+  L.item_he.alloc(std::max<size_t>(ihe.size(), 1));
+  if (!ihe.empty()) NWG_HIP(hipMemcpyAsync(L.item_he.p, ihe.data(), ihe.size() * 4, hipMemcpyHostToDevice, s));
   if (!it1.empty()) NWG_HIP(hipMemcpyAsync(L.items1.p, it1.data(), it1.size() * sizeof(uint4), hipMemcpyHostToDevice, s));
   if (!sst.empty()) {
     NWG_HIP(hipMemcpyAsync(L.seg_start.p, sst.data(), sst.size() * 4, hipMemcpyHostToDevice, s));
@@ -825,7 +984,9 @@ static void enqueue_sweep_hinted(hipStream_t s, const BinParams& P) {
     pr_bin_kernel<true, NT><<<P.n_items2, kBT, 0, s>>>(P);
     if (P.n_items3) pr_bin_combine_kernel<true, NT><<<P.n_items3, 256, 0, s>>>(P);
   } else {
-    pr_bin_kernel<false, NT><<<P.n_items2, kBT, 0, s>>>(P);
+    // from the third sweep on, the items of bins without entries have nothing left to change
+    const uint32_t items = P.iter >= 2 ? P.n_items2_live : P.n_items2;
+    if (items) pr_bin_kernel<false, NT><<<items, kBT, 0, s>>>(P);
     if (P.n_items3) pr_bin_combine_kernel<false, NT><<<P.n_items3, 256, 0, s>>>(P);
   }
   NWG_LAUNCH_CHECK();

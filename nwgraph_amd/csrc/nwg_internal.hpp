@@ -115,13 +115,14 @@ constexpr uint32_t kBinAccStride = kBinAccLanes * kBinAccPad;  // words per swee
 struct BinnedLayout {
   bool built = false;
   uint32_t n_bins = 0, n_chunks = 0, n_items2 = 0, n_items1 = 0, n_items3 = 0, n_segs = 0, n_slots = 0;
+  uint32_t n_items2_live = 0;  // bin-pass items of bins with entries (the others come after them)
   uint32_t n_hot = 0;            // sources [0, n_hot) form each bin's hot tile (0: no hot tiles)
   uint64_t nnz = 0, padded = 0;  // entries, and the length of the per-entry array (pads included)
   uint64_t pairs = 0;            // length of the per-pair arrays (tiles padded to 8 pairs)
   DevBuf<uint16_t> brow;         // per entry: row & (2^kBinBits - 1), | kPairStart on a pair's first entry
   DevBuf<uint16_t> bsrc;         // per pair: source & (2^kChunkBits - 1), or the source in a hot tile
   DevBuf<uint32_t> bpid;         // per kBinBlock entries: pair of the block's first entry
-  DevBuf<uint32_t> hot_end;      // per bin: end of its hot tile (its first tile)
+  DevBuf<uint32_t> item_he;      // per bin-pass item: end of its part of the bin's hot tile (its first tile)
   DevBuf<uint4> items2;          // bin pass: {bin, first entry, end entry, partial slot (or none)}
   DevBuf<uint4> items3;          // combine pass units: {split bin, first slot, slots, first row | log2 lanes << 16}
   DevBuf<uint4> items1;          // chunk pass: {chunk, first tile, end tile, groups of 8 pairs}
@@ -170,13 +171,14 @@ struct BinParams {
   float* x;
   const float* cin;   // contributions of the previous sweep (read by the chunk pass and the hot tiles)
   float* cout;        // contributions written by the row updates (may equal cin without hot tiles)
-  const uint32_t* hot_end;
+  const uint32_t* item_he;  // per bin-pass item: end of its hot entries
   uint32_t n_hot;
   const float* basev;
   float base;
   float damp;
   uint32_t n;
   uint32_t n_items2, n_items1, n_items3;
+  uint32_t n_items2_live;  // one graph, sweeps from the third: only these items run
   unsigned long long* acc;    // one graph: iters x kBinAccStride (lanes); several graphs: iters x ngraphs
   uint32_t iter;
   double threshold;

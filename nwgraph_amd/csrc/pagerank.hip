@@ -749,13 +749,14 @@ static BinParams make_bin_params(nwg_graph* g, PrState& st, const SweepParams& P
   B.n_items3 = L.n_items3;
   B.deg = P.deg;
   B.x = P.x;
-  B.hot_end = L.hot_end.p;
+  B.item_he = L.item_he.p;
   B.n_hot = L.n_hot;
   B.basev = P.basev;
   B.base = P.base;
   B.damp = P.damp;
   B.n = g->n;
   B.n_items2 = L.n_items2;
+  B.n_items2_live = L.n_items2_live;
   B.n_items1 = L.n_items1;
   B.acc = st.bacc.p;
   B.threshold = P.threshold;

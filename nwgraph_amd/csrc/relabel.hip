@@ -2,7 +2,7 @@
 //
 // The pull sweep gathers contrib[u] once per out-edge of u, so a vertex's gather
 // frequency is its out-degree.  Renumbering vertices by out-degree (descending, ties by
-// original id) packs the most gathered contributions into the fewest cache lines: on an
+// original id; vertices without in-edges after all others) packs the most gathered contributions into the fewest cache lines: on an
 // RMAT graph the hottest million vertices then occupy one contiguous 4 MB range that the
 // L2 of each XCD keeps, instead of being spread over the whole 134 MB array at scale 25.
 // Rows are renumbered with the same permutation, so scores and contributions stay
@@ -23,11 +23,18 @@ constexpr int kThreads = 256;
 
 inline uint32_t grid1(uint64_t m) { return (uint32_t)ceil_div(m ? m : 1, kThreads); }
 
-// key = ((2^32-1 - outdeg) << b) | id : ascending order = out-degree descending, id ascending
-__global__ __launch_bounds__(kThreads) void degree_key_kernel(uint32_t n, const uint32_t* __restrict__ deg, int b,
+// key = (no in-edges << (32 + b)) | ((2^32-1 - outdeg) << b) | id : ascending order = the
+// vertices with in-edges first, each group by out-degree descending, id ascending.  (A vertex
+// without in-edges reaches its fixed point in one sweep; grouping them last leaves the
+// binned sweep whole bins with nothing to add, which it stops visiting.)
+__global__ __launch_bounds__(kThreads) void degree_key_kernel(uint32_t n, const uint32_t* __restrict__ deg,
+                                                              const uint32_t* __restrict__ ptr, int b,
                                                               uint64_t* __restrict__ keys) {
   const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
-  if (i < n) keys[i] = ((uint64_t)(0xFFFFFFFFu - deg[i]) << b) | i;
+  if (i < n) {
+    const uint64_t no_in = ptr[i + 1] == ptr[i] ? 1ull : 0ull;
+    keys[i] = (no_in << (32 + b)) | ((uint64_t)(0xFFFFFFFFu - deg[i]) << b) | i;
+  }
 }
 
 __global__ __launch_bounds__(kThreads) void order_kernel(uint32_t n, const uint64_t* __restrict__ keys, int b,
@@ -66,15 +73,16 @@ void relabel_build(hipStream_t s, nwg_graph* g) {
   const uint32_t n = g->n;
   const uint64_t nnz = g->nnz;
   const int b = key_bits_for(n);
+  NWG_REQUIRE(b <= 31, "degree ordering needs at most 2^31 vertices");
   std::unique_ptr<Relabeled> rl(new Relabeled());
   rl->order.alloc(n);
   rl->rank.alloc(n);
   rl->outdeg.alloc(n);
   {
     DevBuf<uint64_t> keys(n), tmp(n);
-    degree_key_kernel<<<grid1(n), kThreads, 0, s>>>(n, g->outdeg.p, b, keys.p);
+    degree_key_kernel<<<grid1(n), kThreads, 0, s>>>(n, g->outdeg.p, g->csc_ptr.p, b, keys.p);
     NWG_LAUNCH_CHECK();
-    radix_sort_pairs(s, keys.p, tmp.p, nullptr, nullptr, n, 32 + b);
+    radix_sort_pairs(s, keys.p, tmp.p, nullptr, nullptr, n, 33 + b);
     order_kernel<<<grid1(n), kThreads, 0, s>>>(n, keys.p, b, g->outdeg.p, rl->order.p, rl->rank.p, rl->outdeg.p);
     NWG_LAUNCH_CHECK();
     NWG_HIP(hipStreamSynchronize(s));

@@ -577,7 +577,7 @@ static uint64_t fixed_point(nwg_ctx* ctx, WState& st, std::vector<std::unique_pt
       B.seg_start = L->seg_start.p;
       B.seg_flat = L->seg_flat.p;
       B.part = st.bpart.p;
-      B.hot_end = L->hot_end.p;
+      B.item_he = L->item_he.p;
       B.n_hot = L->n_hot;
       B.basev = base;
       B.damp = damping;
