@@ -46,7 +46,7 @@ constexpr uint32_t kItem1Pairs = 1u << 18;    // pairs per chunk-pass item
 constexpr uint32_t kPadSrc = 0xFFFFu;           // source offset of a pad pair (value 0)
 constexpr uint32_t kRowMask = kBinRows - 1;
 constexpr uint32_t kNoSlot = 0xFFFFFFFFu;     // bin-pass item of a bin that is one item
-constexpr uint32_t kGroups1 = 4;              // groups of 8 pairs in flight per lane in the chunk pass
+constexpr uint32_t kGroups1 = 4;              // groups of 8 pairs in flight per lane in the chunk pass (as 8 halves)
 
 // Inside a kBinBlock block the entries are stored slot-major: entry r of the block (in
 // layout order) sits at lane r % 64, slot r / 64 of the block's 64 x 8 array, so that a lane's
@@ -63,6 +63,7 @@ __host__ __device__ __forceinline__ uint32_t bin_slot(uint32_t e) {
 // contributions non-temporally.  Pair values and hot-tile source ids are always cached
 // loads: the eight entry slots of a lane re-read the same lines.
 typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
 template <int NT, class T>
 __device__ __forceinline__ T bld(const T* p) {
   if constexpr (NT >= 1) return __builtin_nontemporal_load(p);
@@ -224,8 +225,11 @@ __device__ __forceinline__ void finish_rows(const BinParams& P, uint32_t r0, uin
 
 
 // chunk pass: vals[p] = contrib[src(p)] over the pairs of the tiles of one item.  Tiles are
-// padded to multiples of 8 pairs (pads have source kPadSrc and get 0), so a lane moves 8
-// pairs with one 16-byte id load and two 16-byte value stores.
+// padded to multiples of 8 pairs (pads have source kPadSrc and get 0), whole groups of 8 pairs
+// never cross a tile, and a lane moves half a group (4 pairs) with one 8-byte id load and one
+// 16-byte value store: consecutive lanes then store consecutive 16 bytes, whole lines, which
+// the nontemporal stores of the plan's stream_hints 2 need (a lane storing a whole group
+// with two 16-byte stores wrote every line in two halves, 343 against 315 us at RMAT-25).
 template <int NT, bool W = false>
 __global__ __launch_bounds__(kBT) void pr_chunk_kernel(BinParams P) {
   if (converged_before<W>(P)) return;
@@ -256,35 +260,38 @@ __global__ __launch_bounds__(kBT) void pr_chunk_kernel(BinParams P) {
     for (uint32_t q = threadIdx.x; q < kChunkSrc; q += kBT) cst[q] = (u0 + q < P.n) ? P.cin[u0 + q] : 0.f;
   }
   __syncthreads();
-  for (uint32_t g0 = 0; g0 < m; g0 += kGroups1 * kBT) {
-    uint32_t p[kGroups1];
-    u32x4 uu[kGroups1];
+  // half-group hg: 4 pairs of group hg >> 1
+  constexpr uint32_t kH = 2 * kGroups1;
+  const uint32_t mh = 2 * m;
+  for (uint32_t h0 = 0; h0 < mh; h0 += kH * kBT) {
+    uint32_t p[kH];
+    u32x2 uu[kH];
 #pragma unroll
-    for (uint32_t h = 0; h < kGroups1; ++h) {
-      const uint32_t g = g0 + h * kBT + threadIdx.x;
-      if (g < m) {
+    for (uint32_t h = 0; h < kH; ++h) {
+      const uint32_t hg = h0 + h * kBT + threadIdx.x;
+      if (hg < mh) {
+        const uint32_t g = hg >> 1;
         uint32_t lo = 0, hi = ns - 1;  // last tile j with sflat[j] <= g
         while (lo < hi) {
           const uint32_t mid = (lo + hi + 1) >> 1;
           if (sflat[mid] <= g) lo = mid;
           else hi = mid - 1;
         }
-        p[h] = sstart[lo] + 8u * (g - sflat[lo]);
-        uu[h] = bld<NT>(reinterpret_cast<const u32x4*>(P.bsrc + p[h]));
+        p[h] = sstart[lo] + 8u * (g - sflat[lo]) + 4u * (hg & 1u);
+        uu[h] = bld<NT>(reinterpret_cast<const u32x2*>(P.bsrc + p[h]));
       }
     }
 #pragma unroll
-    for (uint32_t h = 0; h < kGroups1; ++h) {
-      if (g0 + h * kBT + threadIdx.x >= m) continue;
-      const uint32_t w[4] = {uu[h].x, uu[h].y, uu[h].z, uu[h].w};
-      float o[8];
+    for (uint32_t h = 0; h < kH; ++h) {
+      if (h0 + h * kBT + threadIdx.x >= mh) continue;
+      const uint32_t w[2] = {uu[h].x, uu[h].y};
+      float o[4];
 #pragma unroll
-      for (uint32_t t = 0; t < 8; ++t) {
+      for (uint32_t t = 0; t < 4; ++t) {
         const uint32_t u = (w[t >> 1] >> (16 * (t & 1))) & 0xFFFFu;
         o[t] = u == kPadSrc ? 0.f : cst[u];
       }
       bst<NT>(reinterpret_cast<f32x4*>(P.vals + p[h]), f32x4{o[0], o[1], o[2], o[3]});
-      bst<NT>(reinterpret_cast<f32x4*>(P.vals + p[h] + 4), f32x4{o[4], o[5], o[6], o[7]});
     }
   }
 }

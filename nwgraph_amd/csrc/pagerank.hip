@@ -542,6 +542,9 @@ nwg_plan_params default_plan_for(const nwg_graph* g) {
   nwg_plan_params p = default_plan_params();
   if (g->nnz >= (1ull << 18) && g->nnz < (1ull << 30) - (1ull << 26)) p.binned = 1;
   if (g->nnz >= (1ull << 23)) p.relabel = 1;
+  // nontemporal streams: the pair values the chunk pass stores are read once by the bin pass
+  // (as cached stores, their write-back landed in the bin pass: RMAT-25 0.81 -> 0.72 ms)
+  if (p.binned && g->nnz >= (1ull << 23)) p.stream_hints = 2;
   return p;
 }
 
