@@ -90,6 +90,7 @@ typedef struct nwg_plan_params {
   uint32_t relabel;      /* 1: sweep a copy of the CSC renumbered by out-degree (results returned in original ids) */
   uint32_t binned;       /* 1: binned sweep (propagation blocking over a tiled copy of the CSC, 8 bytes per
                             entry; row sums exact in fixed point) instead of the degree-binned pull sweep */
+  uint32_t bin_bits;     /* binned sweep: rows per bin, log2: 13 or 14 (0 = by size: 14 from 2^28 edges) */
 } nwg_plan_params;
 
 /* ---------------------------------------------------------------- context */

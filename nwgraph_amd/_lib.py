@@ -61,10 +61,11 @@ class PlanParams(ctypes.Structure):
         ("ppr_tile", c_u32),
         ("relabel", c_u32),
         ("binned", c_u32),
+        ("bin_bits", c_u32),
     ]
 
     FIELDS = ("short_max", "group_max", "hub_min", "hub_segment", "slice_min", "stream_hints", "ppr_tile",
-              "relabel", "binned")
+              "relabel", "binned", "bin_bits")
 
     def as_dict(self):
         return {k: int(getattr(self, k)) for k in self.FIELDS}

@@ -3,9 +3,9 @@
 //
 // Search: coordinate descent over the pull sweep's parameters from the default plan, one
 // parameter at a time, each candidate timed over a few sweeps (launched one by one, each
-// bracketed by events); then the binned sweep in both numberings; then the best pull plan
-// and the best binned plan timed again as whole captured 20-sweep solves (median of 7),
-// and the faster one kept.
+// bracketed by events); then the binned sweep in both numberings, with its cache hints and
+// bin size; then the best pull plan and the best binned plan timed again as whole captured
+// 20-sweep solves (median of 7), and the faster one kept.
 // Results are cached per (log2 in-degree histogram, n, nnz, device) for the process
 // and, if NWG_TUNE_CACHE names a file, appended there and reloaded by later processes.
 #include <hip/hip_runtime.h>
@@ -33,7 +33,7 @@ bool g_file_loaded = false;
 std::string params_to_string(const nwg_plan_params& p) {
   std::ostringstream o;
   o << p.short_max << ' ' << p.group_max << ' ' << p.hub_min << ' ' << p.hub_segment << ' ' << p.slice_min
-    << ' ' << p.stream_hints << ' ' << p.ppr_tile << ' ' << p.relabel << ' ' << p.binned;
+    << ' ' << p.stream_hints << ' ' << p.ppr_tile << ' ' << p.relabel << ' ' << p.binned << ' ' << p.bin_bits;
   return o.str();
 }
 
@@ -42,6 +42,7 @@ bool params_from_stream(std::istream& in, nwg_plan_params* p) {
   if (!(in >> q.short_max >> q.group_max >> q.hub_min >> q.hub_segment >> q.slice_min >> q.stream_hints >>
         q.ppr_tile >> q.relabel >> q.binned))
     return false;
+  if (!(in >> q.bin_bits)) q.bin_bits = 0;  // entries written before the field existed
   *p = q;
   return true;
 }
@@ -194,6 +195,15 @@ nwg_plan_params autotune_plan(nwg_ctx* ctx, nwg_graph* g) {
       }
     }
     binned = bh;
+  }
+  // and its bin size (2^14-row bins: fewer pairs, one bin-pass workgroup per CU)
+  {
+    nwg_plan_params c = binned;
+    c.bin_bits = 13;
+    double b13 = eval(c);
+    c.bin_bits = 14;
+    if (eval(c) < b13) binned = c;
+    else binned.bin_bits = 13;
   }
   // the two winners, timed again as whole captured solves (median of 7)
   plan_build(s, g, pull);

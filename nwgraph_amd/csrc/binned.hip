@@ -29,6 +29,7 @@
 
 #include <algorithm>
 #include <type_traits>
+#include <utility>
 #include <vector>
 
 #include "device_utils.hpp"
@@ -39,12 +40,14 @@ namespace nwg {
 namespace {
 
 constexpr int kBT = 1024;  // threads of a bin / chunk pass workgroup (16 waves, one workgroup per CU)
-constexpr uint32_t kBinRows = 1u << kBinBits;
+// Bins of 2^BB rows, BB = 13 or 14 (the layout's choice, BinnedLayout::bin_bits): the bin-pass
+// kernels are instantiated for both.
+template <int BB>
+constexpr uint32_t bin_rows() { return 1u << BB; }
 constexpr uint32_t kChunkSrc = 1u << kChunkBits;
-constexpr uint32_t kItem2Log = 19;  // log2 of the most entries of a bin-pass item
+constexpr uint32_t kItem2Log = 20;  // log2 of the most entries of a bin-pass item
 constexpr uint32_t kItem1Pairs = 1u << 18;    // pairs per chunk-pass item
 constexpr uint32_t kPadSrc = 0xFFFFu;           // source offset of a pad pair (value 0)
-constexpr uint32_t kRowMask = kBinRows - 1;
 constexpr uint32_t kNoSlot = 0xFFFFFFFFu;     // bin-pass item of a bin that is one item
 constexpr uint32_t kGroups1 = 4;              // groups of 8 pairs in flight per lane in the chunk pass (as 8 halves)
 
@@ -353,13 +356,13 @@ __device__ __forceinline__ void bin_block_pairs(const BinBlock& b, uint32_t lane
 
 // Adds a block's 8 entries of this lane into the LDS row accumulators (W: each times its
 // weight).
-template <bool HOT, bool W>
+template <bool HOT, bool W, int BB>
 __device__ __forceinline__ void bin_block_add(const BinBlock& b, const uint32_t (&raw)[8], const BinWeights& wt,
                                               unsigned long long* lacc, const float* hot, int qs) {
   const uint32_t wd[4] = {b.e.x, b.e.y, b.e.z, b.e.w};
 #pragma unroll
   for (uint32_t t = 0; t < 8; ++t) {
-    const uint32_t row = (wd[t >> 1] >> (16 * (t & 1))) & kRowMask;
+    const uint32_t row = (wd[t >> 1] >> (16 * (t & 1))) & (bin_rows<BB>() - 1u);
     float v;
     if constexpr (HOT) {  // pads (source kPadSrc) add 0
       const uint32_t u = raw[t];
@@ -373,6 +376,27 @@ __device__ __forceinline__ void bin_block_add(const BinBlock& b, const uint32_t 
   }
 }
 
+// Pipeline depths of the bin pass: row offsets kDE blocks ahead, pair loads kDV blocks ahead,
+// in rings of kRE and kRV buffers.  Bins of 2^13 rows keep two workgroups per CU (64 VGPRs);
+// bins of 2^14 rows leave one, whose waves can hold twice the registers and go deeper.
+// The weighted sweep's blocks carry weights too: it goes less deep.
+template <bool W, int BB>
+struct BinDepth {
+  static constexpr bool kBig = BB >= 14;
+  static constexpr uint32_t DE = !kBig ? 2 : W ? 5 : 9, DV = !kBig ? 1 : W ? 2 : 4;
+  static constexpr uint32_t RE = !kBig ? 4 : W ? 6 : 10, RV = !kBig ? 2 : W ? 3 : 5;
+  static_assert(RE > DE && RV > DV && DE > DV, "pipeline rings must hold the blocks in flight");
+};
+constexpr uint32_t gcd_u32(uint32_t a, uint32_t b) { return b ? gcd_u32(b, a % b) : a; }
+template <class F, uint32_t... Ks>
+__device__ __forceinline__ void unroll_steps(F& f, uint32_t j, std::integer_sequence<uint32_t, Ks...>) {
+  (f(std::integral_constant<uint32_t, Ks>{}, j), ...);
+}
+template <class F, uint32_t... Ks>
+__device__ __forceinline__ void rest_steps(F& f, uint32_t j, uint32_t rest, std::integer_sequence<uint32_t, Ks...>) {
+  ((Ks < rest ? (f(std::integral_constant<uint32_t, Ks>{}, j), 0) : 0), ...);
+}
+
 // Adds the entries [e0, e1) of one bin into lacc; e0, he and e1 are block boundaries (the
 // layout aligns bins and hot tiles).  The blocks of [e0, he) lie in the bin's hot tile, where
 // a pair's value is hot[source] (pads, source kPadSrc, add 0); those of [he, e1) read it from
@@ -384,7 +408,7 @@ __device__ __forceinline__ void bin_block_add(const BinBlock& b, const uint32_t 
 // Blocks past the wave's last are clamped to the range's last block: their loads stay in
 // bounds and they are never added.  Entries are ordered by source, so one row rarely repeats
 // inside a lane or an instruction: each entry is one LDS add, with no run tracking.
-template <int NT, bool W = false>
+template <int NT, bool W, int BB>
 __device__ __forceinline__ void bin_range(const BinParams& P, uint32_t e0, uint32_t he, uint32_t e1,
                                           unsigned long long* lacc, const float* hot) {
   // the wave index is uniform over the wave: in a scalar register, the block indices and the
@@ -403,61 +427,61 @@ __device__ __forceinline__ void bin_range(const BinParams& P, uint32_t e0, uint3
     const uint32_t k = kb + j * kWaves;
     return k < ke ? k : ke - 1u;
   };
-  // Block k's row offsets live in eb[k % 4], its pair loads in rb[k % 2] and (weighted) its
-  // weights in wb[k % 2]; the loop is unrolled by four so that every buffer keeps fixed
-  // registers (a register copy of a load still in flight would wait for it).  Step k issues
-  // block k+2's row offsets and block k+1's pair loads (and weights), then adds block k;
-  // the scheduling barriers keep the compiler from hoisting the adds, which wait for block
-  // k's loads, above the newer loads.  Whether a block is hot is uniform over the wave.
-  BinBlock eb[4];
-  uint32_t rb[2][8];
-  BinWeights wb[2];
+  // Block k's row offsets live in eb[k % kRE], its pair loads in rb[k % kRV] and (weighted)
+  // its weights in wb[k % kRV]; the loop is unrolled by lcm(kRE, kRV) so that every buffer
+  // keeps fixed registers (a register copy of a load still in flight would wait for it).
+  // Step k issues block k+kDE's row offsets and block k+kDV's pair loads (and weights), then
+  // adds block k; the scheduling barriers keep the compiler from hoisting the adds, which
+  // wait for block k's loads, above the newer loads.  Whether a block is hot is uniform over
+  // the wave.
+  using D = BinDepth<W, BB>;
+  constexpr uint32_t kDE = D::DE, kDV = D::DV, kRE = D::RE, kRV = D::RV;
+  BinBlock eb[kRE];
+  uint32_t rb[kRV][8];
+  BinWeights wb[kRV];
   auto pairs = [&](uint32_t k, const BinBlock& e, uint32_t(&r)[8]) {
     if (k < kh) bin_block_pairs<true>(e, lane, vbase, sbase, r);
     else bin_block_pairs<false>(e, lane, vbase, sbase, r);
   };
   auto add = [&](uint32_t k, const BinBlock& e, const uint32_t(&r)[8], const BinWeights& wt) {
-    if (k < kh) bin_block_add<true, W>(e, r, wt, lacc, hot, qs);
-    else bin_block_add<false, W>(e, r, wt, lacc, hot, qs);
+    if (k < kh) bin_block_add<true, W, BB>(e, r, wt, lacc, hot, qs);
+    else bin_block_add<false, W, BB>(e, r, wt, lacc, hot, qs);
   };
   auto step = [&](auto kc, uint32_t j) {
     constexpr uint32_t K = decltype(kc)::value;
-    eb[(K + 2) % 4] = bin_block_load<NT>(P, blk(j + K + 2), lane);
-    if constexpr (W) bin_weights_load<NT>(P, blk(j + K + 1), lane, wb[(K + 1) % 2]);
-    pairs(blk(j + K + 1), eb[(K + 1) % 4], rb[(K + 1) % 2]);
+    eb[(K + kDE) % kRE] = bin_block_load<NT>(P, blk(j + K + kDE), lane);
+    if constexpr (W) bin_weights_load<NT>(P, blk(j + K + kDV), lane, wb[(K + kDV) % kRV]);
+    pairs(blk(j + K + kDV), eb[(K + kDV) % kRE], rb[(K + kDV) % kRV]);
     __builtin_amdgcn_sched_barrier(0);
-    add(blk(j + K), eb[K % 4], rb[K % 2], wb[K % 2]);
+    add(blk(j + K), eb[K % kRE], rb[K % kRV], wb[K % kRV]);
     __builtin_amdgcn_sched_barrier(0);
   };
-  eb[0] = bin_block_load<NT>(P, blk(0), lane);
-  eb[1] = bin_block_load<NT>(P, blk(1), lane);
-  if constexpr (W) bin_weights_load<NT>(P, blk(0), lane, wb[0]);
-  pairs(blk(0), eb[0], rb[0]);
-  uint32_t j = 0;
-  for (; j + 4 <= nb; j += 4) {
-    step(std::integral_constant<uint32_t, 0>{}, j);
-    step(std::integral_constant<uint32_t, 1>{}, j);
-    step(std::integral_constant<uint32_t, 2>{}, j);
-    step(std::integral_constant<uint32_t, 3>{}, j);
+#pragma unroll
+  for (uint32_t d = 0; d < kDE; ++d) eb[d] = bin_block_load<NT>(P, blk(d), lane);
+#pragma unroll
+  for (uint32_t d = 0; d < kDV; ++d) {
+    if constexpr (W) bin_weights_load<NT>(P, blk(d), lane, wb[d]);
+    pairs(blk(d), eb[d], rb[d]);
   }
-  const uint32_t rest = nb - j;  // 0..3 blocks
-  if (rest > 0) step(std::integral_constant<uint32_t, 0>{}, j);
-  if (rest > 1) step(std::integral_constant<uint32_t, 1>{}, j);
-  if (rest > 2) step(std::integral_constant<uint32_t, 2>{}, j);
+  constexpr uint32_t kU = kRE * kRV / gcd_u32(kRE, kRV);
+  uint32_t j = 0;
+  for (; j + kU <= nb; j += kU) unroll_steps(step, j, std::make_integer_sequence<uint32_t, kU>{});
+  rest_steps(step, j, nb - j, std::make_integer_sequence<uint32_t, kU - 1>{});  // 0 .. kU-1 blocks
 }
 
-// The finishing loads of one graph's rows r0 + threadIdx.x + k * kBT (k < kBinRows / kBT):
+// The finishing loads of one graph's rows r0 + threadIdx.x + k * kBT (k < 2^BB / kBT):
 // issued before the bin pass's last barrier, so that they are in flight while the other
 // waves finish their adds.
+template <int BB>
 struct FinishIn {
-  float xo[kBinRows / kBT];
-  float bv[kBinRows / kBT];
-  uint32_t dg[kBinRows / kBT];
+  float xo[bin_rows<BB>() / kBT];
+  float bv[bin_rows<BB>() / kBT];
+  uint32_t dg[bin_rows<BB>() / kBT];
 };
-template <int NT>
-__device__ __forceinline__ void finish_load(const BinParams& P, uint32_t r0, uint32_t nrows, FinishIn& in) {
+template <int NT, int BB>
+__device__ __forceinline__ void finish_load(const BinParams& P, uint32_t r0, uint32_t nrows, FinishIn<BB>& in) {
 #pragma unroll
-  for (uint32_t k = 0; k < kBinRows / kBT; ++k) {
+  for (uint32_t k = 0; k < bin_rows<BB>() / kBT; ++k) {
     const uint32_t v = threadIdx.x + k * kBT;
     if (v < nrows) {
       in.xo[k] = bld<NT>(P.x + r0 + v);
@@ -468,12 +492,12 @@ __device__ __forceinline__ void finish_load(const BinParams& P, uint32_t r0, uin
 }
 // the reference's row update (page_rank.hpp:133-135) on the prefetched rows, and the
 // workgroup's L1 change into a lane counter of the sweep (as finish_rows)
-template <int NT>
-__device__ __forceinline__ void finish_apply(const BinParams& P, uint32_t r0, uint32_t nrows, const FinishIn& in,
+template <int NT, int BB>
+__device__ __forceinline__ void finish_apply(const BinParams& P, uint32_t r0, uint32_t nrows, const FinishIn<BB>& in,
                                              const unsigned long long* lacc, double* red) {
   double e = 0.0;
 #pragma unroll
-  for (uint32_t k = 0; k < kBinRows / kBT; ++k) {
+  for (uint32_t k = 0; k < bin_rows<BB>() / kBT; ++k) {
     const uint32_t v = threadIdx.x + k * kBT;
     if (v < nrows) {
       const float xn = __builtin_fmaf(P.damp, dequantize(lacc[v]), in.bv[k]);
@@ -497,10 +521,11 @@ __device__ __forceinline__ void finish_apply(const BinParams& P, uint32_t r0, ui
 // The range's part inside the bin's hot tile (a prefix of the bin) reads its values from
 // the hot contributions staged in LDS.  An item's hot end (P.item_he) is precomputed, so that
 // no load waits on another before the hot contributions are staged.
-// (8 waves per SIMD: the 64 KiB of row sums allow two workgroups per CU, which 64 VGPRs keep;
-// the weighted pipeline needs more registers and runs one workgroup per CU)
-template <bool MULTI, int NT, bool W = false>
-__global__ __launch_bounds__(kBT, W ? 4 : 8) void pr_bin_kernel(BinParams P) {
+// (8 waves per SIMD: 64 KiB of row sums allow two workgroups per CU, which 64 VGPRs keep; the
+// weighted pipeline needs more registers, and bins of 2^14 rows more LDS: one workgroup per CU)
+template <bool MULTI, int NT, bool W, int BB>
+__global__ __launch_bounds__(kBT, (W || BB >= 14) ? 4 : 8) void pr_bin_kernel(BinParams P) {
+  constexpr uint32_t kBinRows = bin_rows<BB>();
   if (converged_before<W>(P)) return;
   __shared__ unsigned long long lacc[kBinRows];
   __shared__ double red[kBT / 64];
@@ -534,15 +559,15 @@ __global__ __launch_bounds__(kBT, W ? 4 : 8) void pr_bin_kernel(BinParams P) {
     }
   }
   __syncthreads();
-  bin_range<NT, W>(P, it.y, he, it.z, lacc, hot);
-  const uint32_t r0 = b << kBinBits;
+  bin_range<NT, W, BB>(P, it.y, he, it.z, lacc, hot);
+  const uint32_t r0 = b << BB;
   const uint32_t nrows = r0 < P.n ? min(kBinRows, P.n - r0) : 0u;
   if constexpr (!MULTI && !W) {
     if (it.w == kNoSlot) {  // one graph, a bin that is one item: prefetch the rows, then finish
-      FinishIn in;
-      finish_load<NT>(P, r0, nrows, in);
+      FinishIn<BB> in;
+      finish_load<NT, BB>(P, r0, nrows, in);
       __syncthreads();
-      finish_apply<NT>(P, r0, nrows, in, lacc, red);
+      finish_apply<NT, BB>(P, r0, nrows, in, lacc, red);
       return;
     }
   }
@@ -567,8 +592,9 @@ __global__ __launch_bounds__(kBT, W ? 4 : 8) void pr_bin_kernel(BinParams P) {
 // of one split bin, each row's slots shared by L lanes (slot k on lane k mod L); the host
 // picks L per bin from its slot count, so a bin with hundreds of slots is read by many
 // workgroups at once and a bin with a few by few.  One workgroup per unit.
-template <bool MULTI, int NT, bool W = false>
+template <bool MULTI, int NT, bool W, int BB>
 __global__ __launch_bounds__(256) void pr_bin_combine_kernel(BinParams P) {
+  constexpr uint32_t kBinRows = bin_rows<BB>();
   if (converged_before<W>(P)) return;
   __shared__ double red[4];
   __shared__ unsigned long long gerr[MULTI ? kGraphSlots : 1];
@@ -576,7 +602,7 @@ __global__ __launch_bounds__(256) void pr_bin_combine_kernel(BinParams P) {
   const uint4 it = P.items3[blockIdx.x];  // {bin, first slot, slots, first row | log2 L << 16}
   const uint32_t lanes = 1u << (it.w >> 16), rows = 256u >> (it.w >> 16);
   const uint32_t v0 = it.w & 0xFFFFu;
-  const uint32_t r0 = (it.x << kBinBits) + v0;
+  const uint32_t r0 = (it.x << BB) + v0;
   const uint32_t nrows = r0 < P.n ? min(rows, P.n - r0) : 0u;
   {
     const uint32_t row = threadIdx.x % rows, lane = threadIdx.x / rows;
@@ -621,7 +647,7 @@ __global__ __launch_bounds__(256) void pr_wfold_kernel(BinParams P) {
 // A stable sort by this key orders each bin by source, then row (CSC order).
 // With pos, low[p] is p itself (a weighted layout looks up each entry's weight and row).
 __global__ __launch_bounds__(256) void bin_key_kernel(uint32_t n, uint64_t nnz, const uint32_t* __restrict__ rp,
-                                                      const uint32_t* __restrict__ col, int ub, bool pos,
+                                                      const uint32_t* __restrict__ col, int ub, int bb, bool pos,
                                                       uint64_t* __restrict__ keys, uint32_t* __restrict__ low) {
   const uint64_t p = (uint64_t)blockIdx.x * 256 + threadIdx.x;
   if (p >= nnz) return;
@@ -631,8 +657,8 @@ __global__ __launch_bounds__(256) void bin_key_kernel(uint32_t n, uint64_t nnz, 
     if (rp[mid + 1] <= p) lo = mid + 1;
     else hi = mid;
   }
-  keys[p] = ((uint64_t)(lo >> kBinBits) << ub) | col[p];
-  low[p] = pos ? (uint32_t)p : (lo & kRowMask);
+  keys[p] = ((uint64_t)(lo >> bb) << ub) | col[p];
+  low[p] = pos ? (uint32_t)p : (lo & ((1u << bb) - 1u));
 }
 
 // tile of a sort key (tile column: 0 for a source below n_hot, else 1 + chunk; without hot
@@ -682,7 +708,7 @@ __global__ __launch_bounds__(256) void bin_place_kernel(uint64_t nnz, const uint
                                                         const uint32_t* __restrict__ ptp,
                                                         uint16_t* __restrict__ brow, uint16_t* __restrict__ bsrc,
                                                         uint32_t* __restrict__ bpid, const uint32_t* __restrict__ rp,
-                                                        uint32_t n, const float* __restrict__ w,
+                                                        uint32_t n, int bb, const float* __restrict__ w,
                                                         float* __restrict__ bw) {
   const uint64_t q = (uint64_t)blockIdx.x * 256 + threadIdx.x;
   if (q >= nnz) return;
@@ -700,7 +726,7 @@ __global__ __launch_bounds__(256) void bin_place_kernel(uint64_t nnz, const uint
       if (rp[mid + 1] <= p) lo = mid + 1;
       else hi = mid;
     }
-    lw = lo & kRowMask;
+    lw = lo & ((1u << bb) - 1u);
     bw[bin_slot(ne)] = w[p];
   }
   brow[bin_slot(ne)] = (uint16_t)(lw | (f ? kPairStart : 0u));
@@ -752,10 +778,18 @@ __global__ __launch_bounds__(256) void residual_fold_kernel(const unsigned long 
 }  // namespace
 
 void binned_build(hipStream_t s, const ActiveCsc& a, uint32_t n, uint64_t nnz, BinnedLayout& L, bool hot,
-                  const float* w) {
+                  const float* w, int bin_bits) {
+  NWG_REQUIRE(bin_bits == 0 || bin_bits == 13 || bin_bits == 14, "binned layout: bins of 2^13 or 2^14 rows");
   NWG_REQUIRE(nnz < (1ull << 31), "binned layout needs fewer than 2^31 entries");
   L = BinnedLayout();
-  L.n_bins = (uint32_t)ceil_div(n, kBinRows);
+  // Bins of 2^14 rows have fewer pairs per entry (0.35 against 0.43 at RMAT-25, so 17 % fewer
+  // pair values to store and read), but their 128 KiB of row sums leave one bin-pass
+  // workgroup per CU.  That pays for the largest graphs only: measured on MI355X, RMAT-25
+  // (529 M entries) 0.67 against 0.72 ms per sweep, while the 256-graph union (213 M) and
+  // smaller graphs were slower with them.
+  L.bin_bits = bin_bits ? bin_bits : nnz >= (1ull << 28) ? 14 : 13;
+  const uint32_t bin_rows_ = 1u << L.bin_bits;
+  L.n_bins = (uint32_t)ceil_div(n, bin_rows_);
   L.n_chunks = (uint32_t)ceil_div(n, kChunkSrc);
   L.n_hot = hot ? std::min<uint32_t>(kHotSrc, n) : 0u;
   L.nnz = nnz;
@@ -770,7 +804,7 @@ void binned_build(hipStream_t s, const ActiveCsc& a, uint32_t n, uint64_t nnz, B
     while (ub < 32 && (1ull << ub) < n) ++ub;
     DevBuf<uint64_t> keys(nnz), keys_tmp(nnz);
     DevBuf<uint32_t> low(nnz), low_tmp(nnz);
-    bin_key_kernel<<<grid256(nnz), 256, 0, s>>>(n, nnz, a.ptr, a.idx, ub, w != nullptr, keys.p, low.p);
+    bin_key_kernel<<<grid256(nnz), 256, 0, s>>>(n, nnz, a.ptr, a.idx, ub, L.bin_bits, w != nullptr, keys.p, low.p);
     NWG_LAUNCH_CHECK();
     int kb = ub;
     while (kb < 64 && (1ull << (kb - ub)) < L.n_bins) ++kb;
@@ -827,8 +861,8 @@ void binned_build(hipStream_t s, const ActiveCsc& a, uint32_t n, uint64_t nnz, B
     NWG_HIP(hipMemcpyAsync(d_etp.p, etp.data(), (n_tiles + 1) * 4, hipMemcpyHostToDevice, s));
     NWG_HIP(hipMemcpyAsync(d_ptp.p, ptp.data(), (n_tiles + 1) * 4, hipMemcpyHostToDevice, s));
     bin_place_kernel<<<grid256(nnz), 256, 0, s>>>(nnz, keys.p, low.p, flag, pex, ub, TC, L.n_hot, d_tp.p, d_tpp.p,
-                                                  d_etp.p, d_ptp.p, L.brow.p, L.bsrc.p, L.bpid.p, a.ptr, n, w,
-                                                  w ? L.bw.p : nullptr);
+                                                  d_etp.p, d_ptp.p, L.brow.p, L.bsrc.p, L.bpid.p, a.ptr, n, L.bin_bits,
+                                                  w, w ? L.bw.p : nullptr);
     NWG_LAUNCH_CHECK();
     bin_pad_kernel<<<grid256(n_tiles), 256, 0, s>>>(n_tiles, d_tp.p, d_tpp.p, d_etp.p, d_ptp.p, L.brow.p, L.bsrc.p,
                                                     L.bpid.p);
@@ -855,17 +889,19 @@ void binned_build(hipStream_t s, const ActiveCsc& a, uint32_t n, uint64_t nnz, B
   // bin pass items: every bin, cut into ranges of at most cap2 entries (inner cuts on
   // block boundaries, so no block is walked by two items of one bin); the items of a split bin
   // get consecutive partial slots, and the bin combine units
-  // Item size: the power of two nearest to entries / (4 x CUs), between 2^15 and
-  // 2^kItem2Log.  Big graphs take big items (fewer split bins, so fewer partial rows to
-  // store and combine); small ones take small items, so that their heavy bins spread over
-  // all CUs.  (Measured on MI355X: 2^19 best at RMAT-25, 2^18 for the 256-graph union,
-  // 2^17 at RMAT-23, 2^16 at RMAT-22 and below.)
+  // Item size: the power of two nearest to entries / (2 x CUs) (4 x CUs with 2^13-row bins),
+  // between 2^15 and 2^kItem2Log.  Big graphs take big items (fewer split bins, so fewer
+  // partial rows to store and combine); small ones take small items, so that their heavy
+  // bins spread over all CUs.  (Measured on MI355X with 2^13-row bins: 2^19 best at RMAT-25,
+  // 2^18 for the 256-graph union, 2^17 at RMAT-23, 2^16 at RMAT-22 and below; 2^14-row bins
+  // store twice the partial rows per split item, and RMAT-25 takes 2^20: 0.675 against
+  // 0.69 ms per sweep at 2^19 and 2^21.)
   int dev = 0, cus = 0;
   NWG_HIP(hipGetDevice(&dev));
   NWG_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
   uint32_t cap2 = 1u << 15;
   {
-    const uint64_t q = etotal / (4ull * (uint64_t)std::max(cus, 1));
+    const uint64_t q = etotal / ((L.bin_bits >= 14 ? 2ull : 4ull) * (uint64_t)std::max(cus, 1));
     uint32_t lg = 0;
     while (lg < 63 && (2ull << lg) <= q) ++lg;  // floor(log2 q)
     if (q && (double)q >= 1.4142 * (double)(1ull << lg)) ++lg;
@@ -881,7 +917,7 @@ void binned_build(hipStream_t s, const ActiveCsc& a, uint32_t n, uint64_t nnz, B
     if (cnt > 1) {
       // combine units: 256 / L rows each, L lanes per row sharing its cnt slots
       const uint32_t lg = cnt > 32 ? 3u : cnt > 8 ? 2u : 0u;
-      for (uint32_t v0 = 0; v0 < kBinRows; v0 += 256u >> lg) it3.push_back(make_uint4(b, slots, cnt, v0 | (lg << 16)));
+      for (uint32_t v0 = 0; v0 < bin_rows_; v0 += 256u >> lg) it3.push_back(make_uint4(b, slots, cnt, v0 | (lg << 16)));
     }
     auto cut = [&](uint32_t q) -> uint32_t {
       if (q == 0) return p0;
@@ -984,32 +1020,46 @@ void binned_build(hipStream_t s, const ActiveCsc& a, uint32_t n, uint64_t nnz, B
   L.built = true;
 }
 
-template <int NT>
+template <int NT, int BB>
 static void enqueue_sweep_hinted(hipStream_t s, const BinParams& P) {
   if (P.n_items1) pr_chunk_kernel<NT><<<P.n_items1, kBT, 0, s>>>(P);
   if (P.row_graph) {
-    pr_bin_kernel<true, NT><<<P.n_items2, kBT, 0, s>>>(P);
-    if (P.n_items3) pr_bin_combine_kernel<true, NT><<<P.n_items3, 256, 0, s>>>(P);
+    pr_bin_kernel<true, NT, false, BB><<<P.n_items2, kBT, 0, s>>>(P);
+    if (P.n_items3) pr_bin_combine_kernel<true, NT, false, BB><<<P.n_items3, 256, 0, s>>>(P);
   } else {
     // from the third sweep on, the items of bins without entries have nothing left to change
     const uint32_t items = P.iter >= 2 ? P.n_items2_live : P.n_items2;
-    if (items) pr_bin_kernel<false, NT><<<items, kBT, 0, s>>>(P);
-    if (P.n_items3) pr_bin_combine_kernel<false, NT><<<P.n_items3, 256, 0, s>>>(P);
+    if (items) pr_bin_kernel<false, NT, false, BB><<<items, kBT, 0, s>>>(P);
+    if (P.n_items3) pr_bin_combine_kernel<false, NT, false, BB><<<P.n_items3, 256, 0, s>>>(P);
   }
   NWG_LAUNCH_CHECK();
 }
 
+template <int BB>
+static void enqueue_sweep_bits(hipStream_t s, const BinParams& P) {
+  if (P.hints >= 2) enqueue_sweep_hinted<2, BB>(s, P);
+  else if (P.hints == 1) enqueue_sweep_hinted<1, BB>(s, P);
+  else enqueue_sweep_hinted<0, BB>(s, P);
+}
+
 void binned_enqueue_sweep(hipStream_t s, const BinParams& P) {
-  if (P.hints >= 2) enqueue_sweep_hinted<2>(s, P);
-  else if (P.hints == 1) enqueue_sweep_hinted<1>(s, P);
-  else enqueue_sweep_hinted<0>(s, P);
+  NWG_REQUIRE(P.bin_bits == 13 || P.bin_bits == 14, "binned sweep: bins of 2^13 or 2^14 rows");
+  if (P.bin_bits == 14) enqueue_sweep_bits<14>(s, P);
+  else enqueue_sweep_bits<13>(s, P);
+}
+
+template <int BB>
+static void enqueue_weighted_bits(hipStream_t s, const BinParams& P) {
+  pr_bin_kernel<false, 0, true, BB><<<P.n_items2, kBT, 0, s>>>(P);
+  if (P.n_items3) pr_bin_combine_kernel<false, 0, true, BB><<<P.n_items3, 256, 0, s>>>(P);
 }
 
 void binned_enqueue_weighted_sweep(hipStream_t s, const BinParams& P) {
   NWG_REQUIRE(P.bw && P.wpart && P.werr && P.basev, "weighted sweep needs weights, partials and a base");
+  NWG_REQUIRE(P.bin_bits == 13 || P.bin_bits == 14, "binned sweep: bins of 2^13 or 2^14 rows");
   if (P.n_items1) pr_chunk_kernel<0, true><<<P.n_items1, kBT, 0, s>>>(P);
-  pr_bin_kernel<false, 0, true><<<P.n_items2, kBT, 0, s>>>(P);
-  if (P.n_items3) pr_bin_combine_kernel<false, 0, true><<<P.n_items3, 256, 0, s>>>(P);
+  if (P.bin_bits == 14) enqueue_weighted_bits<14>(s, P);
+  else enqueue_weighted_bits<13>(s, P);
   pr_wfold_kernel<<<1, 256, 0, s>>>(P);
   NWG_LAUNCH_CHECK();
 }

@@ -244,7 +244,7 @@ void multigraph_set_binned(nwg_ctx* ctx, nwg_multigraph* mg, bool binned) {
     binned_build(s, ActiveCsc{mg->rp.p, mg->col.p, mg->deg.p, nullptr}, (uint32_t)mg->nv, mg->ne, mg->bl, false);
     mg->row_base.alloc(mg->nv ? mg->nv : 1);
     mg->bvals.alloc(mg->bl.pairs);
-    mg->bpart.alloc(std::max<uint64_t>((uint64_t)mg->bl.n_slots << kBinBits, 1));
+    mg->bpart.alloc(std::max<uint64_t>((uint64_t)mg->bl.n_slots << mg->bl.bin_bits, 1));
     NWG_HIP(hipStreamSynchronize(s));
   }
   if (mg->binned != binned) mg->cap.reset();
@@ -306,6 +306,7 @@ void multigraph_run(nwg_ctx* ctx, nwg_multigraph* mg, float damping, float thres
     B.damp = damping;
     B.n = (uint32_t)mg->nv;
     B.n_items2 = L.n_items2;
+    B.bin_bits = L.bin_bits;
     B.n_items1 = L.n_items1;
     B.n_items3 = L.n_items3;
     B.acc = mg->acc.p;

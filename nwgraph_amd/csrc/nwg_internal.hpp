@@ -87,8 +87,8 @@ constexpr uint32_t kSlices = 8;   // source slices of the sliced gather: one per
 constexpr int kSliceShift = 5;
 __host__ __device__ inline uint32_t slice_of(uint32_t u) { return (u >> kSliceShift) & (kSlices - 1); }
 // Binned sweep layout (propagation blocking) of the active CSC.  Rows form bins of
-// 2^kBinBits, sources form chunks of 2^kChunkBits; every CSC entry (row v <- source u)
-// belongs to tile (bin v >> kBinBits, chunk u >> kChunkBits).  Entries are stored tile
+// 2^bin_bits, sources form chunks of 2^kChunkBits; every CSC entry (row v <- source u)
+// belongs to tile (bin v >> bin_bits, chunk u >> kChunkBits).  Entries are stored tile
 // by tile, bin-major, each tile ordered by source, then row.  The entries of one bin that
 // share a source form a pair (bin, source): the value of a pair is written and read once,
 // however many rows of the bin the source feeds.  Per entry the layout keeps the row's
@@ -96,15 +96,14 @@ __host__ __device__ inline uint32_t slice_of(uint32_t u) { return (u >> kSliceSh
 // bits.  The chunk pass writes contrib[u] into every pair of a chunk (from the chunk
 // staged in LDS), the bin pass streams each bin's entries, finds each entry's pair by
 // counting pair starts, and adds the pair's value into LDS row accumulators.
-constexpr int kBinBits = 13;    // rows per bin: 8192 int64 accumulators = 64 KiB of LDS (2 workgroups per CU)
 constexpr int kChunkBits = 15;  // sources per chunk: 32768 floats = 128 KiB of LDS
 constexpr uint32_t kBinMaxSegs = 2048;  // tiles of one chunk-pass work item
 constexpr uint32_t kBinBlock = 512;     // entries of one bin-pass wave block (8 per lane); pair index kept per block
 constexpr uint32_t kPairStart = 0x8000u;  // entry flag: first entry of its pair
 // Hot tiles: a bin's entries whose source is below kHotSrc (the highest out-degrees in a
 // degree-ordered numbering) skip the value array: the bin pass reads them from the
-// contributions staged in its LDS next to the row accumulators (kHotSrc floats keep two
-// workgroups per CU).
+// contributions staged in its LDS next to the row accumulators (15.5 KiB; more hot sources
+// measured slower at RMAT-25: staging them costs more than their pairs).
 constexpr uint32_t kHotSrc = 3968;
 // One graph's per-sweep L1 change is added by thousands of workgroups; they spread their
 // atomics over kBinAccLanes counters kBinAccPad words apart (one address would serialise
@@ -116,10 +115,11 @@ struct BinnedLayout {
   bool built = false;
   uint32_t n_bins = 0, n_chunks = 0, n_items2 = 0, n_items1 = 0, n_items3 = 0, n_segs = 0, n_slots = 0;
   uint32_t n_items2_live = 0;  // bin-pass items of bins with entries (the others come after them)
+  int bin_bits = 13;           // rows per bin: 2^13 (64 KiB of row sums, two bin-pass workgroups per CU) or 2^14
   uint32_t n_hot = 0;            // sources [0, n_hot) form each bin's hot tile (0: no hot tiles)
   uint64_t nnz = 0, padded = 0;  // entries, and the length of the per-entry array (pads included)
   uint64_t pairs = 0;            // length of the per-pair arrays (tiles padded to 8 pairs)
-  DevBuf<uint16_t> brow;         // per entry: row & (2^kBinBits - 1), | kPairStart on a pair's first entry
+  DevBuf<uint16_t> brow;         // per entry: row & (2^bin_bits - 1), | kPairStart on a pair's first entry
   DevBuf<uint16_t> bsrc;         // per pair: source & (2^kChunkBits - 1), or the source in a hot tile
   DevBuf<uint32_t> bpid;         // per kBinBlock entries: pair of the block's first entry
   DevBuf<uint32_t> item_he;      // per bin-pass item: end of its part of the bin's hot tile (its first tile)
@@ -166,7 +166,7 @@ struct BinParams {
   const uint4* items3;
   const uint32_t* seg_start;
   const uint32_t* seg_flat;
-  unsigned long long* part;  // n_slots << kBinBits partial row sums (fixed point) of the split bins' items
+  unsigned long long* part;  // n_slots << bin_bits partial row sums (fixed point) of the split bins' items
   const uint32_t* deg;
   float* x;
   const float* cin;   // contributions of the previous sweep (read by the chunk pass and the hot tiles)
@@ -179,6 +179,7 @@ struct BinParams {
   uint32_t n;
   uint32_t n_items2, n_items1, n_items3;
   uint32_t n_items2_live;  // one graph, sweeps from the third: only these items run
+  int bin_bits;            // the layout's rows per bin (log2)
   unsigned long long* acc;    // one graph: iters x kBinAccStride (lanes); several graphs: iters x ngraphs
   uint32_t iter;
   double threshold;
@@ -414,8 +415,9 @@ nwg_plan_params default_plan_params();
 nwg_plan_params default_plan_for(const nwg_graph* g);
 // binned sweep (binned.hip)
 // w (device, CSC order of a, nullable): also lay out the entries' weights (L.bw)
+// bin_bits: rows per bin, log2 (13 or 14; 0 = by size)
 void binned_build(hipStream_t s, const ActiveCsc& a, uint32_t n, uint64_t nnz, BinnedLayout& L, bool hot,
-                  const float* w = nullptr);
+                  const float* w = nullptr, int bin_bits = 0);
 void binned_enqueue_sweep(hipStream_t s, const BinParams& P);
 // one weighted sweep (P.bw set): chunk, bin and combine passes, then the fold of the L1 partials
 void binned_enqueue_weighted_sweep(hipStream_t s, const BinParams& P);

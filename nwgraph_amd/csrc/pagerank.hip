@@ -557,6 +557,7 @@ static void validate_params(const nwg_plan_params& p) {
               "plan: ppr_tile must be 0, 8, 16, 32 or 64");
   NWG_REQUIRE(p.relabel <= 1, "plan: relabel must be 0 or 1");
   NWG_REQUIRE(p.binned <= 1, "plan: binned must be 0 or 1");
+  NWG_REQUIRE(p.bin_bits == 0 || p.bin_bits == 13 || p.bin_bits == 14, "plan: bin_bits must be 0, 13 or 14");
   NWG_REQUIRE(p.slice_min == 0 || (p.slice_min >= 2 && p.slice_min > p.short_max),
               "plan: slice_min must be 0 or above both 1 and short_max");
 }
@@ -637,7 +638,7 @@ void plan_build(hipStream_t s, nwg_graph* g, const nwg_plan_params& pp) {
     pl.scol.release();
   }
   pl.n_tiles = (uint32_t)ceil_div(n, kThreads);
-  if (pp.binned) binned_build(s, g->active(pp.relabel != 0), n, g->nnz, pl.bl, true);
+  if (pp.binned) binned_build(s, g->active(pp.relabel != 0), n, g->nnz, pl.bl, true, nullptr, (int)pp.bin_bits);
   NWG_HIP(hipStreamSynchronize(s));
   pl.built = true;
   g->plan = std::move(pl);
@@ -760,6 +761,7 @@ static BinParams make_bin_params(nwg_graph* g, PrState& st, const SweepParams& P
   B.n = g->n;
   B.n_items2 = L.n_items2;
   B.n_items2_live = L.n_items2_live;
+  B.bin_bits = L.bin_bits;
   B.n_items1 = L.n_items1;
   B.acc = st.bacc.p;
   B.threshold = P.threshold;
@@ -874,7 +876,7 @@ void pagerank_run(nwg_ctx* ctx, nwg_graph* g, const float* teleport_dev, float d
   if (g->plan.params.binned && sched != PrSchedule::kDelta) {
     const BinnedLayout& L = g->plan.bl;
     st.bvals.ensure(L.pairs);
-    st.bpart.ensure(std::max<uint64_t>((uint64_t)L.n_slots << kBinBits, 1));
+    st.bpart.ensure(std::max<uint64_t>((uint64_t)L.n_slots << L.bin_bits, 1));
     st.bacc.ensure(std::max<uint64_t>(max_iters, 1) * kBinAccStride);
   }
   const bool personalised = teleport_dev != nullptr;

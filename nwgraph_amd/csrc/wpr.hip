@@ -540,7 +540,7 @@ static uint64_t fixed_point(nwg_ctx* ctx, WState& st, std::vector<std::unique_pt
     st.qs_host = qshift;
     NWG_HIP(hipMemcpyAsync(st.qs.p, &st.qs_host, sizeof(int), hipMemcpyHostToDevice, s));
     st.bvals.ensure(L->pairs);
-    st.bpart.ensure(std::max<uint64_t>((uint64_t)L->n_slots << kBinBits, 1));
+    st.bpart.ensure(std::max<uint64_t>((uint64_t)L->n_slots << L->bin_bits, 1));
     st.wpart.ensure(std::max<uint64_t>((uint64_t)L->n_items2 + L->n_items3, 1));
   }
   WParams P{};
@@ -584,6 +584,7 @@ static uint64_t fixed_point(nwg_ctx* ctx, WState& st, std::vector<std::unique_pt
       B.n = n;
       B.n_items1 = L->n_items1;
       B.n_items2 = L->n_items2;
+      B.bin_bits = L->bin_bits;
       B.n_items3 = L->n_items3;
       B.threshold = (double)threshold;
       B.thr_scale = P.thr_scale;
@@ -739,12 +740,14 @@ static bool fixed_point_scale(hipStream_t s, WState& st, uint32_t n, const float
 
 static const BinnedLayout* weighted_layout(hipStream_t s, nwg_graph* g, WState& st, bool csr) {
   std::unique_ptr<BinnedLayout>& L = csr ? st.bl_csr : st.bl_csc;
+  const int bits = (int)(g->plan.built ? g->plan.params : default_plan_for(g)).bin_bits;
+  if (L && bits && L->bin_bits != bits) L.reset();  // the plan asks for other bins
   if (!L) {
     L.reset(new BinnedLayout());
     const ActiveCsc a = csr ? ActiveCsc{g->csr_ptr.p, g->csr_idx.p, nullptr, nullptr}
                             : ActiveCsc{g->csc_ptr.p, g->csc_idx.p, g->outdeg.p, nullptr};
     try {
-      binned_build(s, a, g->n, g->nnz, *L, true, csr ? g->w_csr.p : g->w_csc.p);
+      binned_build(s, a, g->n, g->nnz, *L, true, csr ? g->w_csr.p : g->w_csc.p, bits);
     } catch (...) {
       L.reset();
       throw;

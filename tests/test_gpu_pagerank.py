@@ -144,9 +144,11 @@ def test_pagerank_rmat_matches_v10(ctx, scale):
         dict(short_max=8, group_max=64, hub_min=100000, hub_segment=4096, slice_min=65, relabel=1, stream_hints=2),
         # nearly every row sliced: most slices empty or a single edge
         dict(short_max=0, group_max=16, hub_min=1 << 20, hub_segment=256, slice_min=2),
-        # binned sweep: one bin and one chunk at this size
+        # binned sweep: one bin and one chunk at this size, bins of 2^13 and 2^14 rows
         dict(binned=1),
         dict(binned=1, relabel=1),
+        dict(binned=1, bin_bits=14),
+        dict(binned=1, relabel=1, bin_bits=13),
     ],
 )
 def test_pagerank_plans_agree(ctx, plan):
@@ -570,12 +572,13 @@ def test_autotuned_plan_matches_v10(ctx, kind):
 
 
 @pytest.mark.parametrize("relabel", [0, 1])
-def test_binned_sweep_many_bins_matches_v10(ctx, relabel):
-    """Binned sweep at RMAT-18: 16 bins of 2^14 rows, 8 chunks of 2^15 sources, the hot
-    bins split over several items (combined through the global accumulators)."""
+@pytest.mark.parametrize("bin_bits", [13, 14])
+def test_binned_sweep_many_bins_matches_v10(ctx, relabel, bin_bits):
+    """Binned sweep at RMAT-18: 32 bins of 2^13 rows or 16 of 2^14, 8 chunks of 2^15 sources,
+    the hot bins split over several items (combined through the global accumulators)."""
     g = ctx.rmat(18, 16, seed=3)
     p = g.get_plan()
-    p.update(binned=1, relabel=relabel)
+    p.update(binned=1, relabel=relabel, bin_bits=bin_bits)
     g.set_plan(**p)
     n, ind, col, deg = O.rmat_csc(18, 16, seed=3)
     x, info, ref = check_against_oracle(g, n, ind, col, deg)
@@ -585,10 +588,11 @@ def test_binned_sweep_many_bins_matches_v10(ctx, relabel):
     assert np.array_equal(x, x2) and info2["iterations"] == info["iterations"]
 
 
-def test_binned_sweep_grid_and_personalised(ctx):
+@pytest.mark.parametrize("bin_bits", [13, 14])
+def test_binned_sweep_grid_and_personalised(ctx, bin_bits):
     g = ctx.grid(256, seed=5)
     p = g.get_plan()
-    p.update(binned=1)
+    p.update(binned=1, bin_bits=bin_bits)
     g.set_plan(**p)
     n, ind, col, deg = O.grid_csc(256, seed=5)
     x, info, ref = check_against_oracle(g, n, ind, col, deg, max_iters=100)

@@ -257,14 +257,15 @@ def test_relative_tolerance_gradients_match_fp64_rmat16(ctx):
     assert abs(itf - kf) <= 1 and abs(itb - kb) <= 1, (kf, itf, kb, itb)
 
 
-@pytest.mark.parametrize("scale", [12, 15])
-def test_binned_weighted_sweep_matches_pull_and_fp64(ctx, scale):
+@pytest.mark.parametrize("scale,bin_bits", [(12, 0), (15, 13), (15, 14)])
+def test_binned_weighted_sweep_matches_pull_and_fp64(ctx, scale, bin_bits):
     """The binned weighted sweep (weights carried per entry in the binned layout, signed fixed
     point scaled by a power of two from the solve's own bound) against the pull sweep on the
     same graph and against FP64 iterations run for the same sweeps: the forward, the adjoint
     of a loss with gradients of both signs at two scales (|grad_x| ~ 1 and ~ 1e3, which moves
     the scale by 2^10), and the edge and teleport gradients.  RMAT-15 has more entries per
-    bin than one bin-pass work item takes, so split bins and their combine pass run too."""
+    bin than one bin-pass work item takes, so split bins and their combine pass run too; it
+    runs with bins of 2^13 and of 2^14 rows."""
     g, n, ind, col, deg, rows, w_raw = setup_graph(ctx, scale, seed=8, wseed=6)
     g.set_weights(w_raw, normalize=True)
     _, wn, _ = dense_system(n, ind, col, rows, w_raw) if scale <= 12 else (None, None, None)
@@ -278,7 +279,7 @@ def test_binned_weighted_sweep_matches_pull_and_fp64(ctx, scale):
     v = (v / v.sum()).astype(np.float32)
     out = {}
     for binned in (0, 1):
-        g.set_plan(binned=binned)
+        g.set_plan(binned=binned, bin_bits=bin_bits)
         x, fi = g.wpr_forward(v, D, 0.0, 60)  # a fixed number of sweeps on both paths
         res = [x, fi]
         for amp in (1.0, 1000.0):
