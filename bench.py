@@ -382,6 +382,9 @@ def run_multi(args, ctx):
     })
     out["roofline"] = roofline_entry(alg, float(np.mean(ev)), "all sweeps of the union in one captured HIP graph "
                                       "(device time)", "sum over graphs of sweeps x (4E + 4(N+1) + 20N)")
+    if binned and args.ngraphs == 256:
+        out["traffic_per_sweep"] = {"bytes": 1388519424, "source": "rocprofv3 FETCH_SIZE + WRITE_SIZE, "
+                                    "profiles/pr_binned_multigraph_256.md"}
     mg.close()
     for g in graphs:
         g.close()
@@ -442,8 +445,11 @@ def run_wpr(args, ctx):
         "adjoint_absolute_tolerance": {"threshold": float(np.float32(args.tol)), "sweeps": ba["iterations"],
                                        "final_error": ba["final_error"], "device_ms": round(ba["ms_total"], 3)},
     })
-    out["roofline"] = roofline_entry(alg_f, fi["ms_total"], "weighted forward solve (wsweep_kernel per sweep), "
-                                     "device time", "sweeps x (8E + 4(N+1) + 12N)")
+    out["roofline"] = roofline_entry(alg_f, fi["ms_total"], "weighted forward solve (binned weighted sweep: "
+                                     "pr_chunk_kernel + pr_bin_kernel + pr_bin_combine_kernel + pr_wfold_kernel "
+                                     "per sweep), device time", "sweeps x (8E + 4(N+1) + 12N)")
+    out["traffic_per_sweep"] = {"bytes": 263619584, "source": "rocprofv3 FETCH_SIZE + WRITE_SIZE, "
+                                "profiles/wpr_adjoint_rmat20.md"} if args.wpr_scale == 20 else None
     out["roofline_backward"] = roofline_entry(alg_b, bi["ms_total"], "adjoint solve over the CSR + edge and "
                                               "teleport gradients, device time",
                                               "sweeps x (8E + 4(N+1) + 12N) + 12E + 12N")
