@@ -383,7 +383,7 @@ def run_multi(args, ctx):
     out["roofline"] = roofline_entry(alg, float(np.mean(ev)), "all sweeps of the union in one captured HIP graph "
                                       "(device time)", "sum over graphs of sweeps x (4E + 4(N+1) + 20N)")
     if binned and args.ngraphs == 256:
-        out["traffic_per_sweep"] = {"bytes": 1388519424, "source": "rocprofv3 FETCH_SIZE + WRITE_SIZE, "
+        out["traffic_per_sweep"] = {"bytes": 1355790336, "source": "rocprofv3 FETCH_SIZE + WRITE_SIZE, "
                                     "profiles/pr_binned_multigraph_256.md"}
     mg.close()
     for g in graphs:
