@@ -298,6 +298,7 @@ struct PprState {
   uint64_t seg_version = 0;  // plan_version the segments were built for (0 = none)
   uint32_t seg_tile = 0, seg_slices = 0;
   uint32_t n_heavy = 0, n_segs = 0;
+  uint32_t row_end = 0;  // one past the last row with in-edges (active numbering)
   DevBuf<uint32_t> heavy_rows, heavy_seg_ptr, heavy_seg_row;
   // heavy rows' in-edges regrouped by (row, source slice); segments [start, end) of it,
   // numbered row by row; the segment ids of each slice, slice after slice

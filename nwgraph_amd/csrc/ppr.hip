@@ -22,7 +22,8 @@
 //   finish           hub rows (partials added in segment order) and per-vector L1
 //                    partials folded into acc[iter][j] (fixed point, order independent)
 // A vector whose previous sweep converged is skipped (its lanes idle); a workgroup
-// whose k-tile is entirely converged exits at once.
+// whose k-tile is entirely converged exits at once.  From the third sweep on, the rows
+// after the last row with in-edges (their values fixed after one sweep) are not launched.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -80,7 +81,8 @@ struct PprParams {
   const uint32_t* slice_segs;  // segment ids of slice x: slice_segs[slice_off[x] .. slice_off[x + 1])
   uint32_t slice_off[9];
   uint32_t n_slices;           // source slices per k-tile; tiles * n_slices <= 8 workgroup classes
-  uint32_t light_blocks_pad;   // row workgroups of all k-tiles, rounded up to a multiple of 8
+  uint32_t heavy_blocks;       // heavy-segment workgroups (8 per slice segment slot), launched first
+  uint32_t row_blocks_run;     // row workgroups per k-tile this sweep runs (and whose L1 partials are folded)
   float* hub_part;           // n_hub_segs x k
   double* blk_err;           // (row_blocks * tiles) x T
   unsigned long long* acc;   // iters x k
@@ -153,19 +155,20 @@ __global__ __launch_bounds__(kThreads) void ppr_sweep_kernel(PprParams P) {
   __shared__ float hub_red[kThreads];
   __shared__ float zbuf[kThreads * PprRows<T>::value];
   __shared__ int any_live;
-  // Row workgroup b serves k-tile b % tiles: with tiles dividing 8, every XCD (workgroup b
-  // runs on XCD b % 8) serves one k-tile, and its L2 caches that tile's contributions only.
-  // Heavy-segment workgroups start at a multiple of 8; workgroup class c = (b - start) % 8
-  // (its XCD) serves k-tile c % tiles and source slice c / tiles, so that each of the XCDs
-  // of one k-tile caches only its slice of the sources.
-  const bool heavy = blockIdx.x >= P.light_blocks_pad;
+  // Heavy-segment workgroups come first (a multiple of 8 of them); workgroup class c = b % 8
+  // (its XCD: workgroup b runs on XCD b % 8) serves k-tile c % tiles and source slice
+  // c / tiles, so that each of the XCDs of one k-tile caches only its slice of the sources.
+  // Row workgroup lb = b - heavy_blocks serves k-tile lb % tiles: with tiles dividing 8,
+  // every XCD serves one k-tile, and its L2 caches that tile's contributions only.
+  const bool heavy = blockIdx.x < P.heavy_blocks;
   uint32_t tile, b, seg = 0;
   if (!heavy) {
-    tile = blockIdx.x % P.tiles;
-    b = blockIdx.x / P.tiles;
-    if (b >= P.row_blocks) return;  // padding up to the heavy workgroups
+    const uint32_t lb = blockIdx.x - P.heavy_blocks;
+    tile = lb % P.tiles;
+    b = lb / P.tiles;
+    if (b >= P.row_blocks_run) return;
   } else {
-    const uint32_t hb = blockIdx.x - P.light_blocks_pad, cls = hb % 8u;
+    const uint32_t hb = blockIdx.x, cls = hb % 8u;
     if (cls >= P.tiles * P.n_slices) return;
     tile = cls % P.tiles;
     const uint32_t x = cls / P.tiles, q = hb / 8u;
@@ -348,7 +351,7 @@ __global__ __launch_bounds__(kThreads) void ppr_finish_kernel(PprParams P, uint3
     if (errj[q]) atomicAdd(P.acc + (uint64_t)P.iter * P.k + q, errj[q]);
   // part 2: fold blk_err rows; this workgroup folds kFoldRows workgroup rows of every tile
   // (few workgroups, so few atomics land on each vector's accumulator)
-  const uint64_t rows_total = (uint64_t)P.row_blocks;
+  const uint64_t rows_total = (uint64_t)P.row_blocks_run;
   const uint64_t r0 = (uint64_t)blockIdx.x * kFoldRows;
   for (uint32_t tl = 0; tl < tiles; ++tl) {
     const uint32_t vl = threadIdx.x % T;
@@ -517,6 +520,13 @@ __global__ __launch_bounds__(kThreads) void light_len_kernel(uint32_t n, const u
   }
 }
 
+// *end = one past the last row with in-edges
+__global__ __launch_bounds__(kThreads) void row_end_kernel(uint32_t n, const uint32_t* __restrict__ rp,
+                                                           uint32_t* __restrict__ end) {
+  const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
+  if (i < n && rp[i + 1] > rp[i]) atomicMax(end, i + 1);
+}
+
 __global__ __launch_bounds__(kThreads) void light_copy_kernel(uint32_t n, const uint32_t* __restrict__ rp,
                                                               const uint32_t* __restrict__ col,
                                                               const uint32_t* __restrict__ lrp,
@@ -650,6 +660,11 @@ void ppr_run(nwg_ctx* ctx, nwg_graph* g, uint32_t k, const std::vector<uint32_t>
     st.lcol.alloc(std::max<uint32_t>(n_light, 1));
     light_copy_kernel<<<grid1(n), kThreads, 0, s>>>(n, a.ptr, a.idx, st.lrp.p, st.lcol.p);
     NWG_LAUNCH_CHECK();
+    DevBuf<uint32_t> end(1);
+    NWG_HIP(hipMemsetAsync(end.p, 0, 4, s));
+    row_end_kernel<<<grid1(n), kThreads, 0, s>>>(n, a.ptr, end.p);
+    NWG_LAUNCH_CHECK();
+    NWG_HIP(hipMemcpyAsync(&st.row_end, end.p, 4, hipMemcpyDeviceToHost, s));
     NWG_HIP(hipStreamSynchronize(s));
     st.seg_version = g->plan_version;
     st.seg_tile = tile;
@@ -750,7 +765,13 @@ void ppr_run(nwg_ctx* ctx, nwg_graph* g, uint32_t k, const std::vector<uint32_t>
   uint32_t max_slice = 0;
   for (uint32_t x = 0; x <= nsl; ++x) P.slice_off[x] = st.slice_off[x];
   for (uint32_t x = 0; x < nsl; ++x) max_slice = std::max(max_slice, st.slice_off[x + 1] - st.slice_off[x]);
-  P.light_blocks_pad = (uint32_t)(ceil_div((uint64_t)row_blocks * tiles, 8) * 8);
+  P.heavy_blocks = 8u * max_slice;
+  // Rows without in-edges reach their fixed point (1-d) v_j in one sweep and hold it, and the
+  // next sweep writes the same contribution into the other buffer: from the third sweep on
+  // only the row workgroups up to the last row with in-edges run (degree ordering puts
+  // every row without in-edges last, half of RMAT-22's rows).
+  const uint32_t rows_per_block = (kThreads / tile) * rows_per_group;
+  const uint32_t live_row_blocks = (uint32_t)std::min<uint64_t>(ceil_div(st.row_end, rows_per_block), row_blocks);
 
   const float init = (float)(1.0 / n);
   NWG_HIP(hipEventRecord(ctx->ev0, s));
@@ -763,30 +784,31 @@ void ppr_run(nwg_ctx* ctx, nwg_graph* g, uint32_t k, const std::vector<uint32_t>
   NWG_LAUNCH_CHECK();
   ppr_zero_acc_kernel<<<grid1(max_iters * k), kThreads, 0, s>>>(st.acc.p, max_iters * k);
   NWG_LAUNCH_CHECK();
-  const uint64_t grid64 = (uint64_t)P.light_blocks_pad + 8ull * max_slice;
+  const uint64_t grid64 = (uint64_t)P.heavy_blocks + (uint64_t)row_blocks * tiles;
   NWG_REQUIRE(grid64 < (1ull << 31), "batched solve too large for one launch");
-  const uint32_t grid = (uint32_t)grid64;
   const uint64_t fin_a = ceil_div((uint64_t)n_heavy * k, kFinItems), fin_b = ceil_div(row_blocks, kFoldRows);
   const uint32_t fin = (uint32_t)std::max<uint64_t>(std::max(fin_a, fin_b), 1);
   for (uint64_t it = 0; it < max_iters; ++it) {
     P.iter = (uint32_t)it;
     P.cin = (it & 1) ? c1p : c0p;
     P.cout = (it & 1) ? c0p : c1p;
+    P.row_blocks_run = it >= 2 ? live_row_blocks : row_blocks;
+    const uint32_t grid = P.heavy_blocks + P.row_blocks_run * tiles;  // 0 only for a graph without edges
     switch (tile) {
       case 8:
-        ppr_sweep_kernel<8><<<grid, kThreads, 0, s>>>(P);
+        if (grid) ppr_sweep_kernel<8><<<grid, kThreads, 0, s>>>(P);
         ppr_finish_kernel<8><<<fin, kThreads, 0, s>>>(P, tiles);
         break;
       case 16:
-        ppr_sweep_kernel<16><<<grid, kThreads, 0, s>>>(P);
+        if (grid) ppr_sweep_kernel<16><<<grid, kThreads, 0, s>>>(P);
         ppr_finish_kernel<16><<<fin, kThreads, 0, s>>>(P, tiles);
         break;
       case 32:
-        ppr_sweep_kernel<32><<<grid, kThreads, 0, s>>>(P);
+        if (grid) ppr_sweep_kernel<32><<<grid, kThreads, 0, s>>>(P);
         ppr_finish_kernel<32><<<fin, kThreads, 0, s>>>(P, tiles);
         break;
       default:
-        ppr_sweep_kernel<64><<<grid, kThreads, 0, s>>>(P);
+        if (grid) ppr_sweep_kernel<64><<<grid, kThreads, 0, s>>>(P);
         ppr_finish_kernel<64><<<fin, kThreads, 0, s>>>(P, tiles);
         break;
     }
