@@ -34,6 +34,29 @@ def test_ppr_matches_oracle_per_vector(ctx, k):
         assert np.max(np.abs(X[:, j].astype(np.float64) - ref)) <= 1e-6
 
 
+@pytest.mark.parametrize("tile", [8, 32])
+def test_ppr_degree_ordered_stops_launching_rows_without_in_edges(ctx, tile):
+    """On the degree-ordered copy every row without in-edges comes last, and from the third
+    sweep on the batched solve launches only the row workgroups before them (their values
+    are fixed after one sweep).  Every vector must still stop at the oracle's sweep with the
+    oracle's scores, including vectors seeded on rows without in-edges."""
+    g = ctx.rmat(12, 16, seed=8)
+    plan = g.get_plan()
+    plan.update(relabel=1, ppr_tile=tile)
+    g.set_plan(**plan)
+    n, ind, col, deg = O.rmat_csc(12, 16, seed=8)
+    seeds = O.ppr_sources(n, 16, 8, seed=4)
+    no_in = np.flatnonzero(np.diff(ind.astype(np.int64)) == 0)
+    assert no_in.size > n // 4
+    seeds[3] = np.sort(no_in[:8]).astype(seeds[3].dtype)  # a vector seeded only on rows without in-edges
+    X, its, _ = g.ppr(seeds, D, 1e-4, 30)
+    for j in range(16):
+        ref, it, tr, _ = O.pr_v10(n, ind, col, deg, D, 1e-4, 30, teleport=teleport(n, seeds[j]))
+        assert np.min(np.abs(tr - 1e-4)) > 1.2e-7, (j, tr)
+        assert its[j] == it, (j, its[j], it)
+        assert np.max(np.abs(X[:, j].astype(np.float64) - ref)) <= 1e-6
+
+
 @pytest.mark.parametrize("tile", [8, 16, 32, 64])
 def test_ppr_tiles_agree(ctx, tile):
     g = ctx.rmat(11, 16, seed=3)
