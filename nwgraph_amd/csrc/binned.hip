@@ -90,7 +90,7 @@ __device__ __forceinline__ void bst(T* p, T v) {
 __device__ __forceinline__ unsigned long long quantize(float c) {
   const float sc = c * 268435456.f;  // c * 2^28
   const uint32_t h = (uint32_t)sc;
-  const float r = sc - (float)h;
+  const float r = __builtin_amdgcn_fractf(sc);  // sc - floor(sc) = sc - h, exact
   const uint32_t l = (uint32_t)(r * 4294967296.f);  // floor(r * 2^32) < 2^32
   return ((unsigned long long)h << 32) | l;
 }
@@ -330,27 +330,44 @@ __device__ __forceinline__ void bin_weights_load(const BinParams& P, uint32_t k,
   wt.w[1] = bld<NT>(wp + 1);
 }
 
+// The ballot of a 32-bit word's bit 15 over the wave (one compare of its low half as a
+// signed 16-bit integer; the compiler tests the bit with two instructions).  Every lane is
+// active where the bin pass calls it.
+__device__ __forceinline__ unsigned long long ballot_bit15(uint32_t w) {
+  unsigned long long m;
+  asm("v_cmp_gt_i16_e64 %0, 0, %1" : "=s"(m) : "v"(w));
+  return m;
+}
+
 // Issues the loads of a block's pair data for the lane's 8 entries (not waited on): the
-// pair's value (P.vals), or in a hot tile the pair's source id.  The pair of an entry is the
-// block's first pair plus the pair starts after the block's first entry up to the entry.
-// Slot t of lane l is entry 64 t + l (bin_slot): its pair is the block's first pair, plus
-// the starts of slots 0 .. t-1 (a wave-uniform popcount of their ballots), plus the starts
-// of slot t in lanes 0 .. l (mbcnt of its ballot, and its own flag).
-template <bool HOT>
-__device__ __forceinline__ void bin_block_pairs(const BinBlock& b, uint32_t lane, const char* vbase,
-                                                const char* sbase, uint32_t (&raw)[8]) {
+// pair's value (P.vals), or in a hot tile (hot) the pair's source id.  The pair of an entry
+// is the block's first pair plus the pair starts after the block's first entry up to the
+// entry.  Slot t of lane l is entry 64 t + l (bin_slot): its pair is the block's first pair,
+// plus the starts of slots 0 .. t-1 (a popcount of their ballots), plus the starts of slot t
+// in lanes 0 .. l.  With m the slot's ballot, the last term is bit 0 of m plus mbcnt(m >> 1)
+// (the set bits 1 .. l of m).  The bin pass is bound by vector instructions: per entry this
+// is the flag test, the two mbcnt steps and the address; the rest is scalar.
+__device__ __forceinline__ void bin_block_pairs(const BinBlock& b, bool hot, const char* vbase, const char* sbase,
+                                                uint32_t (&raw)[8]) {
   const uint32_t wd[4] = {b.e.x, b.e.y, b.e.z, b.e.w};
-  uint32_t base = b.pb;
+  uint32_t base = __builtin_amdgcn_readfirstlane(b.pb);
+  uint32_t c[8], b1[8];
 #pragma unroll
   for (uint32_t t = 0; t < 8; ++t) {
-    uint32_t f = (wd[t >> 1] >> (16 * (t & 1) + 15)) & 1u;
-    if (t == 0 && lane == 0) f = 0;  // the block's first entry is in pair pb by definition
-    const unsigned long long m = __ballot(f);
-    const uint32_t pid =
-        __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, base)) + f;
+    // the pair-start flag: bit 15 of the entry's 16-bit half word
+    unsigned long long m = (t & 1) ? __ballot((int32_t)wd[t >> 1] < 0) : ballot_bit15(wd[t >> 1]);
+    if (t == 0) m &= ~1ull;  // the block's first entry is in pair pb by definition
+    const unsigned long long m1 = m >> 1;
+    c[t] = __builtin_amdgcn_mbcnt_hi((uint32_t)(m1 >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m1, 0u));
+    b1[t] = base + (uint32_t)(m & 1ull);
     base += (uint32_t)__popcll(m);
-    if constexpr (HOT) raw[t] = *reinterpret_cast<const uint16_t*>(sbase + (pid << 1));
-    else raw[t] = *reinterpret_cast<const uint32_t*>(vbase + (pid << 2));
+  }
+  if (hot) {
+#pragma unroll
+    for (uint32_t t = 0; t < 8; ++t) raw[t] = *reinterpret_cast<const uint16_t*>(sbase + ((c[t] + b1[t]) << 1));
+  } else {
+#pragma unroll
+    for (uint32_t t = 0; t < 8; ++t) raw[t] = *reinterpret_cast<const uint32_t*>(vbase + ((c[t] + b1[t]) << 2));
   }
 }
 
@@ -389,84 +406,106 @@ struct BinDepth {
 };
 constexpr uint32_t gcd_u32(uint32_t a, uint32_t b) { return b ? gcd_u32(b, a % b) : a; }
 template <class F, uint32_t... Ks>
-__device__ __forceinline__ void unroll_steps(F& f, uint32_t j, std::integer_sequence<uint32_t, Ks...>) {
-  (f(std::integral_constant<uint32_t, Ks>{}, j), ...);
+__device__ __forceinline__ void unroll_steps(F& f, std::integer_sequence<uint32_t, Ks...>) {
+  (f(std::integral_constant<uint32_t, Ks>{}), ...);
 }
-template <class F, uint32_t... Ks>
-__device__ __forceinline__ void rest_steps(F& f, uint32_t j, uint32_t rest, std::integer_sequence<uint32_t, Ks...>) {
-  ((Ks < rest ? (f(std::integral_constant<uint32_t, Ks>{}, j), 0) : 0), ...);
+// blocks of a bin-pass range handed out before its block counter: kDE + 1 per wave
+template <bool W, int BB>
+constexpr uint32_t bin_static_blocks() {
+  return (BinDepth<W, BB>::DE + 1) * (kBT / 64);
 }
 
 // Adds the entries [e0, e1) of one bin into lacc; e0, he and e1 are block boundaries (the
 // layout aligns bins and hot tiles).  The blocks of [e0, he) lie in the bin's hot tile, where
 // a pair's value is hot[source] (pads, source kPadSrc, add 0); those of [he, e1) read it from
-// P.vals[pair].  Wave w of the workgroup takes blocks w, w + 16, ... of the whole range in one
-// software pipeline, so that the loads of three blocks overlap: while block j's entries are
-// added, block j+1's pair loads and block j+2's row offsets are in flight.  (A wave that
-// waited on each block's row offsets, then on its pair values, had one memory latency in
-// flight at a time, and a separate loop per tile drained the pipeline at the tile's end.)
-// Blocks past the wave's last are clamped to the range's last block: their loads stay in
-// bounds and they are never added.  Entries are ordered by source, so one row rarely repeats
-// inside a lane or an instruction: each entry is one LDS add, with no run tracking.
+// P.vals[pair].  Each wave runs its blocks through one software pipeline, so that the loads of
+// several blocks overlap: while block j's entries are added, the pair loads of the next kDV
+// blocks and the row offsets of the next kDE are in flight.  Wave w starts with blocks
+// w, w + 16, ..., w + 16 kDE of the range; after that, with bins of 2^14 rows, each wave takes
+// the next block from the workgroup's counter *next (set to the range's first block +
+// bin_static_blocks() before the workgroup's barrier), one block per step and one step before
+// it is loaded.  (With a fixed share of blocks per wave, the waves the SIMDs' issue order
+// favours finished their shares up to a sixth of the item earlier than the others, and the
+// CU's one workgroup waited for the last.)  A wave stops at its first block past the range;
+// blocks past the range are loaded as the range's last block, so that the loads stay in
+// bounds, and never added.  Entries are ordered by source, so one row rarely repeats inside a
+// lane or an instruction: each entry is one LDS add, with no run tracking.  The sums are
+// exact, so which wave adds a block does not matter.
 template <int NT, bool W, int BB>
 __device__ __forceinline__ void bin_range(const BinParams& P, uint32_t e0, uint32_t he, uint32_t e1,
-                                          unsigned long long* lacc, const float* hot) {
+                                          unsigned long long* lacc, const float* hot, uint32_t* next) {
   // the wave index is uniform over the wave: in a scalar register, the block indices and the
   // blocks' first pairs stay scalar too
   const uint32_t w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
   constexpr uint32_t kWaves = kBT / 64;
   const uint32_t kb = e0 / kBinBlock + w, kh = he / kBinBlock, ke = e1 / kBinBlock;
   if (kb >= ke) return;
-  const uint32_t nb = (ke - kb + kWaves - 1) / kWaves;
   // values are addressed by 32-bit byte offsets (the layout keeps pairs below 2^30)
   const char* vbase = reinterpret_cast<const char*>(P.vals);
   const char* sbase = reinterpret_cast<const char*>(P.bsrc);
   int qs = 0;  // the weighted sweep's scale exponent (uniform: one scalar load)
   if constexpr (W) qs = __builtin_amdgcn_readfirstlane(*P.qshift);
-  auto blk = [&](uint32_t j) {
-    const uint32_t k = kb + j * kWaves;
-    return k < ke ? k : ke - 1u;
-  };
-  // Block k's row offsets live in eb[k % kRE], its pair loads in rb[k % kRV] and (weighted)
-  // its weights in wb[k % kRV]; the loop is unrolled by lcm(kRE, kRV) so that every buffer
-  // keeps fixed registers (a register copy of a load still in flight would wait for it).
-  // Step k issues block k+kDE's row offsets and block k+kDV's pair loads (and weights), then
-  // adds block k; the scheduling barriers keep the compiler from hoisting the adds, which
-  // wait for block k's loads, above the newer loads.  Whether a block is hot is uniform over
-  // the wave.
+  auto clamp = [&](uint32_t k) { return k < ke ? k : ke - 1u; };
+  // The block of pipeline slot t is kq[t % kRE], its row offsets eb[t % kRE], its pair loads
+  // rb[t % kRV] and (weighted) its weights wb[t % kRV]; the loop is unrolled by
+  // lcm(kRE, kRV) so that every buffer keeps fixed registers (a register copy of a load still
+  // in flight would wait for it).  Step t takes slot t+kDE's block (fetched from the counter
+  // one step earlier, into nk), fetches the next, issues slot t+kDE's row offsets and slot
+  // t+kDV's pair loads (and weights), then adds slot t; the scheduling barriers keep the
+  // compiler from hoisting the adds, which wait for slot t's loads, above the newer loads.
+  // Whether a block is hot is uniform over the wave.
   using D = BinDepth<W, BB>;
   constexpr uint32_t kDE = D::DE, kDV = D::DV, kRE = D::RE, kRV = D::RV;
+  uint32_t kq[kRE];
   BinBlock eb[kRE];
   uint32_t rb[kRV][8];
   BinWeights wb[kRV];
   auto pairs = [&](uint32_t k, const BinBlock& e, uint32_t(&r)[8]) {
-    if (k < kh) bin_block_pairs<true>(e, lane, vbase, sbase, r);
-    else bin_block_pairs<false>(e, lane, vbase, sbase, r);
+    bin_block_pairs(e, k < kh, vbase, sbase, r);
   };
   auto add = [&](uint32_t k, const BinBlock& e, const uint32_t(&r)[8], const BinWeights& wt) {
     if (k < kh) bin_block_add<true, W, BB>(e, r, wt, lacc, hot, qs);
     else bin_block_add<false, W, BB>(e, r, wt, lacc, hot, qs);
   };
-  auto step = [&](auto kc, uint32_t j) {
+  // Bins of 2^13 rows keep two workgroups per CU, and the other fills the CU while one
+  // drains: their waves keep the fixed shares w, w + 16, ... (measured: the counter costs the
+  // 4096^2 grid 2 %).
+  constexpr bool kDyn = BB >= 14;
+  uint32_t nk = kb + kDE * kWaves;  // slot kDE's block: the wave's last static one
+  // a zero the compiler cannot see as uniform: lane 0's counter fetch stays one LDS atomic
+  // whose result is read a step later (an atomic at a uniform address is rewritten to one
+  // whose result is broadcast at once, which waits for every LDS add in flight)
+  uint32_t zv = 0u;
+  asm volatile("" : "+v"(zv));
+  auto step = [&](auto kc) {
     constexpr uint32_t K = decltype(kc)::value;
-    eb[(K + kDE) % kRE] = bin_block_load<NT>(P, blk(j + K + kDE), lane);
-    if constexpr (W) bin_weights_load<NT>(P, blk(j + K + kDV), lane, wb[(K + kDV) % kRV]);
-    pairs(blk(j + K + kDV), eb[(K + kDV) % kRE], rb[(K + kDV) % kRV]);
+    constexpr uint32_t tE = (K + kDE) % kRE, tV = (K + kDV) % kRV, tVE = (K + kDV) % kRE;
+    kq[tE] = __builtin_amdgcn_readfirstlane(nk);
+    if constexpr (kDyn) {
+      if (lane == 0) nk = atomicAdd(next + zv, 1u);
+    } else {
+      nk += kWaves;
+    }
+    eb[tE] = bin_block_load<NT>(P, clamp(kq[tE]), lane);
+    if constexpr (W) bin_weights_load<NT>(P, clamp(kq[tVE]), lane, wb[tV]);
+    pairs(clamp(kq[tVE]), eb[tVE], rb[tV]);
     __builtin_amdgcn_sched_barrier(0);
-    add(blk(j + K), eb[K % kRE], rb[K % kRV], wb[K % kRV]);
+    if (kq[K % kRE] < ke) add(kq[K % kRE], eb[K % kRE], rb[K % kRV], wb[K % kRV]);
     __builtin_amdgcn_sched_barrier(0);
   };
 #pragma unroll
-  for (uint32_t d = 0; d < kDE; ++d) eb[d] = bin_block_load<NT>(P, blk(d), lane);
+  for (uint32_t d = 0; d < kDE; ++d) {
+    kq[d] = kb + d * kWaves;
+    eb[d] = bin_block_load<NT>(P, clamp(kq[d]), lane);
+  }
 #pragma unroll
   for (uint32_t d = 0; d < kDV; ++d) {
-    if constexpr (W) bin_weights_load<NT>(P, blk(d), lane, wb[d]);
-    pairs(blk(d), eb[d], rb[d]);
+    if constexpr (W) bin_weights_load<NT>(P, clamp(kq[d]), lane, wb[d]);
+    pairs(clamp(kq[d]), eb[d], rb[d]);
   }
   constexpr uint32_t kU = kRE * kRV / gcd_u32(kRE, kRV);
-  uint32_t j = 0;
-  for (; j + kU <= nb; j += kU) unroll_steps(step, j, std::make_integer_sequence<uint32_t, kU>{});
-  rest_steps(step, j, nb - j, std::make_integer_sequence<uint32_t, kU - 1>{});  // 0 .. kU-1 blocks
+  do unroll_steps(step, std::make_integer_sequence<uint32_t, kU>{});
+  while (kq[0] < ke);
 }
 
 // The finishing loads of one graph's rows r0 + threadIdx.x + k * kBT (k < 2^BB / kBT):
@@ -531,6 +570,7 @@ __global__ __launch_bounds__(kBT, (W || BB >= 14) ? 4 : 8) void pr_bin_kernel(Bi
   __shared__ double red[kBT / 64];
   __shared__ unsigned long long gerr[MULTI ? kGraphSlots : 1];
   __shared__ float hot[MULTI ? 1 : kHotSrc];
+  __shared__ uint32_t next;  // the range's next block to hand out
   const uint4 it = P.items2[blockIdx.x];  // {bin, first entry, end entry, partial slot or kNoSlot}
   const uint32_t b = it.x;
   // end of the hot part of [it.y, it.z); the hot contributions' loads are issued first and
@@ -549,6 +589,7 @@ __global__ __launch_bounds__(kBT, (W || BB >= 14) ? 4 : 8) void pr_bin_kernel(Bi
     }
   }
   for (uint32_t q = threadIdx.x; q < kBinRows; q += kBT) lacc[q] = 0ull;
+  if (threadIdx.x == 0) next = it.y / kBinBlock + bin_static_blocks<W, BB>();
   if constexpr (!MULTI) {
     if (he > it.y) {
 #pragma unroll
@@ -559,7 +600,7 @@ __global__ __launch_bounds__(kBT, (W || BB >= 14) ? 4 : 8) void pr_bin_kernel(Bi
     }
   }
   __syncthreads();
-  bin_range<NT, W, BB>(P, it.y, he, it.z, lacc, hot);
+  bin_range<NT, W, BB>(P, it.y, he, it.z, lacc, hot, &next);
   const uint32_t r0 = b << BB;
   const uint32_t nrows = r0 < P.n ? min(kBinRows, P.n - r0) : 0u;
   if constexpr (!MULTI && !W) {
