@@ -339,35 +339,31 @@ __device__ __forceinline__ unsigned long long ballot_bit15(uint32_t w) {
   return m;
 }
 
-// Issues the loads of a block's pair data for the lane's 8 entries (not waited on): the
-// pair's value (P.vals), or in a hot tile (hot) the pair's source id.  The pair of an entry
-// is the block's first pair plus the pair starts after the block's first entry up to the
-// entry.  Slot t of lane l is entry 64 t + l (bin_slot): its pair is the block's first pair,
-// plus the starts of slots 0 .. t-1 (a popcount of their ballots), plus the starts of slot t
-// in lanes 0 .. l.  With m the slot's ballot, the last term is bit 0 of m plus mbcnt(m >> 1)
-// (the set bits 1 .. l of m).  The bin pass is bound by vector instructions: per entry this
-// is the flag test, the two mbcnt steps and the address; the rest is scalar.
-__device__ __forceinline__ void bin_block_pairs(const BinBlock& b, bool hot, const char* vbase, const char* sbase,
+// Issues the loads of a block's pair data for the lane's 8 entries (not waited on): a 32-bit
+// word per entry from base[pair - sub], the pair's value (P.vals, sub 0) or in a hot tile the
+// pair's source id (P.hsrc, sub the item's hot offset).  One load form for both, so that a
+// pipeline slot's registers are written by one instruction whichever the block is (a branch
+// between two load forms into the same registers made the compiler drain every load in
+// flight before the hot one).  The pair of an entry is the block's first pair plus the pair
+// starts after the block's first entry up to the entry.  Slot t of lane l is entry 64 t + l
+// (bin_slot): its pair is the block's first pair, plus the starts of slots 0 .. t-1 (a
+// popcount of their ballots), plus the starts of slot t in lanes 0 .. l.  With m the slot's
+// ballot, the last term is bit 0 of m plus mbcnt(m >> 1) (the set bits 1 .. l of m).  The bin
+// pass is bound by vector instructions: per entry this is the flag test, the two mbcnt steps
+// and the address; the rest is scalar.
+__device__ __forceinline__ void bin_block_pairs(const BinBlock& b, const char* base, uint32_t sub,
                                                 uint32_t (&raw)[8]) {
   const uint32_t wd[4] = {b.e.x, b.e.y, b.e.z, b.e.w};
-  uint32_t base = __builtin_amdgcn_readfirstlane(b.pb);
-  uint32_t c[8], b1[8];
+  uint32_t first = __builtin_amdgcn_readfirstlane(b.pb) - sub;
 #pragma unroll
   for (uint32_t t = 0; t < 8; ++t) {
     // the pair-start flag: bit 15 of the entry's 16-bit half word
     unsigned long long m = (t & 1) ? __ballot((int32_t)wd[t >> 1] < 0) : ballot_bit15(wd[t >> 1]);
     if (t == 0) m &= ~1ull;  // the block's first entry is in pair pb by definition
     const unsigned long long m1 = m >> 1;
-    c[t] = __builtin_amdgcn_mbcnt_hi((uint32_t)(m1 >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m1, 0u));
-    b1[t] = base + (uint32_t)(m & 1ull);
-    base += (uint32_t)__popcll(m);
-  }
-  if (hot) {
-#pragma unroll
-    for (uint32_t t = 0; t < 8; ++t) raw[t] = *reinterpret_cast<const uint16_t*>(sbase + ((c[t] + b1[t]) << 1));
-  } else {
-#pragma unroll
-    for (uint32_t t = 0; t < 8; ++t) raw[t] = *reinterpret_cast<const uint32_t*>(vbase + ((c[t] + b1[t]) << 2));
+    const uint32_t c = __builtin_amdgcn_mbcnt_hi((uint32_t)(m1 >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m1, 0u));
+    raw[t] = *reinterpret_cast<const uint32_t*>(base + ((c + first + (uint32_t)(m & 1ull)) << 2));
+    first += (uint32_t)__popcll(m);
   }
 }
 
@@ -377,17 +373,17 @@ template <bool HOT, bool W, int BB>
 __device__ __forceinline__ void bin_block_add(const BinBlock& b, const uint32_t (&raw)[8], const BinWeights& wt,
                                               unsigned long long* lacc, const float* hot, int qs) {
   const uint32_t wd[4] = {b.e.x, b.e.y, b.e.z, b.e.w};
+  float hv[8];
+  if constexpr (HOT) {  // the 8 hot contributions first: one wait for the LDS reads
+#pragma unroll
+    for (uint32_t t = 0; t < 8; ++t) hv[t] = hot[min(raw[t], kHotSrc - 1u)];
+  }
 #pragma unroll
   for (uint32_t t = 0; t < 8; ++t) {
     const uint32_t row = (wd[t >> 1] >> (16 * (t & 1))) & (bin_rows<BB>() - 1u);
     float v;
-    if constexpr (HOT) {  // pads (source kPadSrc) add 0
-      const uint32_t u = raw[t];
-      const float h = hot[min(u, kHotSrc - 1u)];
-      v = u < kHotSrc ? h : 0.f;
-    } else {
-      v = __uint_as_float(raw[t]);
-    }
+    if constexpr (HOT) v = raw[t] < kHotSrc ? hv[t] : 0.f;  // pads (source kPadSrc) add 0
+    else v = __uint_as_float(raw[t]);
     if constexpr (W) atomicAdd(&lacc[row], quantize_signed(wt.w[t >> 2][t & 3] * v, qs));
     else atomicAdd(&lacc[row], quantize(v));
   }
@@ -430,19 +426,26 @@ constexpr uint32_t bin_static_blocks() {
 // blocks past the range are loaded as the range's last block, so that the loads stay in
 // bounds, and never added.  Entries are ordered by source, so one row rarely repeats inside a
 // lane or an instruction: each entry is one LDS add, with no run tracking.  The sums are
-// exact, so which wave adds a block does not matter.
-template <int NT, bool W, int BB>
-__device__ __forceinline__ void bin_range(const BinParams& P, uint32_t e0, uint32_t he, uint32_t e1,
-                                          unsigned long long* lacc, const float* hot, uint32_t* next) {
+// exact, so which wave adds a block does not matter.  sync() is the workgroup's set-up
+// (clearing lacc, staging hot) and barrier: every wave calls it once, with 2^13-row bins after
+// issuing its first blocks' row offsets, which need neither.  (Their pair loads wait on the
+// row offsets and follow the barrier: waiting before it held every wave there until the
+// slowest had its row offsets.)
+template <int NT, bool W, int BB, class Sync>
+__device__ __forceinline__ void bin_range(const BinParams& P, uint32_t e0, uint32_t he, uint32_t e1, uint32_t hd,
+                                          unsigned long long* lacc, const float* hot, uint32_t* next, Sync&& sync) {
   // the wave index is uniform over the wave: in a scalar register, the block indices and the
   // blocks' first pairs stay scalar too
   const uint32_t w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
   constexpr uint32_t kWaves = kBT / 64;
   const uint32_t kb = e0 / kBinBlock + w, kh = he / kBinBlock, ke = e1 / kBinBlock;
-  if (kb >= ke) return;
+  if (e0 >= e1) {  // an empty range (a bin without entries): nothing to load
+    sync();
+    return;
+  }
   // values are addressed by 32-bit byte offsets (the layout keeps pairs below 2^30)
   const char* vbase = reinterpret_cast<const char*>(P.vals);
-  const char* sbase = reinterpret_cast<const char*>(P.bsrc);
+  const char* hbase = reinterpret_cast<const char*>(P.hsrc);
   int qs = 0;  // the weighted sweep's scale exponent (uniform: one scalar load)
   if constexpr (W) qs = __builtin_amdgcn_readfirstlane(*P.qshift);
   auto clamp = [&](uint32_t k) { return k < ke ? k : ke - 1u; };
@@ -461,7 +464,8 @@ __device__ __forceinline__ void bin_range(const BinParams& P, uint32_t e0, uint3
   uint32_t rb[kRV][8];
   BinWeights wb[kRV];
   auto pairs = [&](uint32_t k, const BinBlock& e, uint32_t(&r)[8]) {
-    bin_block_pairs(e, k < kh, vbase, sbase, r);
+    const bool h = k < kh;  // uniform: the selects are scalar
+    bin_block_pairs(e, h ? hbase : vbase, h ? hd : 0u, r);
   };
   auto add = [&](uint32_t k, const BinBlock& e, const uint32_t(&r)[8], const BinWeights& wt) {
     if (k < kh) bin_block_add<true, W, BB>(e, r, wt, lacc, hot, qs);
@@ -493,10 +497,23 @@ __device__ __forceinline__ void bin_range(const BinParams& P, uint32_t e0, uint3
     if (kq[K % kRE] < ke) add(kq[K % kRE], eb[K % kRE], rb[K % kRV], wb[K % kRV]);
     __builtin_amdgcn_sched_barrier(0);
   };
+  // bins of 2^13 rows: the first blocks' row offsets are issued before the workgroup's set-up
+  // and barrier (sync) and in flight over it; bins of 2^14 rows, whose one workgroup per CU
+  // waits for its slowest wave at the barrier: after it (measured: 2-5 % faster on the 4096^2
+  // grid and the weighted sweep, 1 % slower at RMAT-25)
+  constexpr bool kEarly = BB < 14;
+  if constexpr (!kEarly) {
+    sync();
+    if (kb >= ke) return;  // a wave past the range's last block
+  }
 #pragma unroll
   for (uint32_t d = 0; d < kDE; ++d) {
     kq[d] = kb + d * kWaves;
     eb[d] = bin_block_load<NT>(P, clamp(kq[d]), lane);
+  }
+  if constexpr (kEarly) {
+    sync();
+    if (kb >= ke) return;  // a wave past the range's last block (its loads were the last block's)
   }
 #pragma unroll
   for (uint32_t d = 0; d < kDV; ++d) {
@@ -573,13 +590,15 @@ __global__ __launch_bounds__(kBT, (W || BB >= 14) ? 4 : 8) void pr_bin_kernel(Bi
   __shared__ uint32_t next;  // the range's next block to hand out
   const uint4 it = P.items2[blockIdx.x];  // {bin, first entry, end entry, partial slot or kNoSlot}
   const uint32_t b = it.x;
-  // end of the hot part of [it.y, it.z); the hot contributions' loads are issued first and
-  // written to LDS once the row sums are cleared
-  uint32_t he = it.y;
+  // end of the hot part of [it.y, it.z); the hot contributions' loads are issued first, then
+  // the first blocks' loads, and the contributions are written to LDS once the row sums are
+  // cleared (bin_range's sync)
+  uint32_t he = it.y, hd = 0;
   constexpr uint32_t kR = (kHotSrc + kBT - 1) / kBT;
   float hv[kR];
   if constexpr (!MULTI) {
     he = P.item_he[blockIdx.x];
+    hd = P.item_hd[blockIdx.x];
     if (he > it.y) {
 #pragma unroll
       for (uint32_t k = 0; k < kR; ++k) {
@@ -588,19 +607,20 @@ __global__ __launch_bounds__(kBT, (W || BB >= 14) ? 4 : 8) void pr_bin_kernel(Bi
       }
     }
   }
-  for (uint32_t q = threadIdx.x; q < kBinRows; q += kBT) lacc[q] = 0ull;
-  if (threadIdx.x == 0) next = it.y / kBinBlock + bin_static_blocks<W, BB>();
-  if constexpr (!MULTI) {
-    if (he > it.y) {
+  bin_range<NT, W, BB>(P, it.y, he, it.z, hd, lacc, hot, &next, [&]() {
+    for (uint32_t q = threadIdx.x; q < kBinRows; q += kBT) lacc[q] = 0ull;
+    if (threadIdx.x == 0) next = it.y / kBinBlock + bin_static_blocks<W, BB>();
+    if constexpr (!MULTI) {
+      if (he > it.y) {
 #pragma unroll
-      for (uint32_t k = 0; k < kR; ++k) {
-        const uint32_t q = threadIdx.x + k * kBT;
-        if (q < kHotSrc) hot[q] = hv[k];
+        for (uint32_t k = 0; k < kR; ++k) {
+          const uint32_t q = threadIdx.x + k * kBT;
+          if (q < kHotSrc) hot[q] = hv[k];
+        }
       }
     }
-  }
-  __syncthreads();
-  bin_range<NT, W, BB>(P, it.y, he, it.z, lacc, hot, &next);
+    __syncthreads();
+  });
   const uint32_t r0 = b << BB;
   const uint32_t nrows = r0 < P.n ? min(kBinRows, P.n - r0) : 0u;
   if constexpr (!MULTI && !W) {
@@ -799,6 +819,16 @@ __global__ __launch_bounds__(256) void bin_pad_kernel(uint64_t n_tiles, const ui
   }
 }
 
+// the hot tile of bin b (pairs ptp[b TC] ..) as 32-bit words at hsrc[hbase[b] ..]; one
+// workgroup per bin
+__global__ __launch_bounds__(256) void hot_src_kernel(const uint32_t* __restrict__ ptp, uint32_t TC,
+                                                      const uint32_t* __restrict__ hbase,
+                                                      const uint16_t* __restrict__ bsrc, uint32_t* __restrict__ hsrc) {
+  const uint32_t b = blockIdx.x;
+  const uint32_t p0 = ptp[(uint64_t)b * TC], p1 = ptp[(uint64_t)b * TC + 1];
+  for (uint32_t j = threadIdx.x; j < p1 - p0; j += 256) hsrc[hbase[b] + j] = bsrc[p0 + j];
+}
+
 inline uint32_t grid256(uint64_t m) { return (uint32_t)ceil_div(m ? m : 1, 256); }
 
 __global__ __launch_bounds__(256) void residual_clear_kernel(unsigned long long* __restrict__ lanes, uint64_t words) {
@@ -839,6 +869,8 @@ void binned_build(hipStream_t s, const ActiveCsc& a, uint32_t n, uint64_t nnz, B
   const uint64_t n_tiles = (uint64_t)L.n_bins * TC;
   // etp / ptp: each tile's first entry and first pair in the padded layout
   std::vector<uint32_t> etp(n_tiles + 1, 0), ptp(n_tiles + 1, 0);
+  // hot pairs of bin b: pair p is hsrc[p - hdel[b]] (hot tiles stored back to back)
+  std::vector<uint32_t> hdel(L.n_bins, 0);
   uint64_t etotal = 0, ptotal = 0;
   if (nnz) {
     int ub = 1;  // bits of a source id
@@ -908,8 +940,27 @@ void binned_build(hipStream_t s, const ActiveCsc& a, uint32_t n, uint64_t nnz, B
     bin_pad_kernel<<<grid256(n_tiles), 256, 0, s>>>(n_tiles, d_tp.p, d_tpp.p, d_etp.p, d_ptp.p, L.brow.p, L.bsrc.p,
                                                     L.bpid.p);
     NWG_LAUNCH_CHECK();
+    // the hot tiles' sources as 32-bit words (the bin pass loads them like pair values)
+    uint64_t hp = 0;
+    std::vector<uint32_t> hbase(L.n_bins, 0);
+    if (hc)
+      for (uint32_t b = 0; b < L.n_bins; ++b) {
+        const uint32_t p0 = ptp[(uint64_t)b * TC], p1 = ptp[(uint64_t)b * TC + 1];
+        hbase[b] = (uint32_t)hp;
+        hdel[b] = p0 - (uint32_t)hp;
+        hp += p1 - p0;
+      }
+    L.hsrc.alloc(std::max<uint64_t>(hp, 1));
+    if (hp) {
+      DevBuf<uint32_t> d_hbase(L.n_bins);
+      NWG_HIP(hipMemcpyAsync(d_hbase.p, hbase.data(), L.n_bins * 4, hipMemcpyHostToDevice, s));
+      hot_src_kernel<<<L.n_bins, 256, 0, s>>>(d_ptp.p, TC, d_hbase.p, L.bsrc.p, L.hsrc.p);
+      NWG_LAUNCH_CHECK();
+      NWG_HIP(hipStreamSynchronize(s));  // before d_hbase is freed
+    }
     NWG_HIP(hipStreamSynchronize(s));
   } else {
+    L.hsrc.alloc(1);
     L.padded = kBinBlock;
     L.pairs = 8;
     L.brow.alloc(L.padded);
@@ -950,7 +1001,7 @@ void binned_build(hipStream_t s, const ActiveCsc& a, uint32_t n, uint64_t nnz, B
     cap2 = 1u << lg;
   }
   std::vector<uint4> it2, it3;
-  std::vector<uint32_t> ihe;
+  std::vector<uint32_t> ihe, ihd;
   uint32_t slots = 0;
   for (uint32_t b = 0; b < L.n_bins; ++b) {
     const uint32_t p0 = etp[(uint64_t)b * TC], p1 = etp[(uint64_t)(b + 1) * TC];
@@ -970,6 +1021,7 @@ void binned_build(hipStream_t s, const ActiveCsc& a, uint32_t n, uint64_t nnz, B
       const uint32_t c0 = cut(q), c1 = cut(q + 1);
       it2.push_back(make_uint4(b, c0, c1, cnt > 1 ? slots++ : kNoSlot));
       ihe.push_back(std::min(std::max(he[b], c0), c1));  // the item's part of the hot tile ends here
+      ihd.push_back(hdel[b]);
     }
   }
   // Items of bins without entries go last.  Their rows have no in-edges, so one sweep takes
@@ -980,17 +1032,20 @@ void binned_build(hipStream_t s, const ActiveCsc& a, uint32_t n, uint64_t nnz, B
   // that at RMAT-25 half of the bins are empty and the rest hold no tiny bins.
   {
     std::vector<uint4> o2;
-    std::vector<uint32_t> ohe;
+    std::vector<uint32_t> ohe, ohd;
     o2.reserve(it2.size());
     ohe.reserve(it2.size());
+    ohd.reserve(it2.size());
     for (int pass = 0; pass < 2; ++pass)
       for (uint32_t q = 0; q < (uint32_t)it2.size(); ++q)
         if ((it2[q].z > it2[q].y) == (pass == 0)) {
           o2.push_back(it2[q]);
           ohe.push_back(ihe[q]);
+          ohd.push_back(ihd[q]);
         }
     it2.swap(o2);
     ihe.swap(ohe);
+    ihd.swap(ohd);
     L.n_items2_live = 0;
     while (L.n_items2_live < it2.size() && it2[L.n_items2_live].z > it2[L.n_items2_live].y) ++L.n_items2_live;
   }
@@ -1052,6 +1107,8 @@ void binned_build(hipStream_t s, const ActiveCsc& a, uint32_t n, uint64_t nnz, B
   if (!it2.empty()) NWG_HIP(hipMemcpyAsync(L.items2.p, it2.data(), it2.size() * sizeof(uint4), hipMemcpyHostToDevice, s));
   L.item_he.alloc(std::max<size_t>(ihe.size(), 1));
   if (!ihe.empty()) NWG_HIP(hipMemcpyAsync(L.item_he.p, ihe.data(), ihe.size() * 4, hipMemcpyHostToDevice, s));
+  L.item_hd.alloc(std::max<size_t>(ihd.size(), 1));
+  if (!ihd.empty()) NWG_HIP(hipMemcpyAsync(L.item_hd.p, ihd.data(), ihd.size() * 4, hipMemcpyHostToDevice, s));
   if (!it1.empty()) NWG_HIP(hipMemcpyAsync(L.items1.p, it1.data(), it1.size() * sizeof(uint4), hipMemcpyHostToDevice, s));
   if (!sst.empty()) {
     NWG_HIP(hipMemcpyAsync(L.seg_start.p, sst.data(), sst.size() * 4, hipMemcpyHostToDevice, s));

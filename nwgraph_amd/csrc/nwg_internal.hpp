@@ -123,6 +123,8 @@ struct BinnedLayout {
   DevBuf<uint16_t> bsrc;         // per pair: source & (2^kChunkBits - 1), or the source in a hot tile
   DevBuf<uint32_t> bpid;         // per kBinBlock entries: pair of the block's first entry
   DevBuf<uint32_t> item_he;      // per bin-pass item: end of its part of the bin's hot tile (its first tile)
+  DevBuf<uint32_t> item_hd;      // per bin-pass item: its bin's hot pairs start at pair item_hd + their index in hsrc
+  DevBuf<uint32_t> hsrc;         // the hot tiles' pair sources (as in bsrc), 32-bit, bin after bin
   DevBuf<uint4> items2;          // bin pass: {bin, first entry, end entry, partial slot (or none)}
   DevBuf<uint4> items3;          // combine pass units: {split bin, first slot, slots, first row | log2 lanes << 16}
   DevBuf<uint4> items1;          // chunk pass: {chunk, first tile, end tile, groups of 8 pairs}
@@ -172,6 +174,8 @@ struct BinParams {
   const float* cin;   // contributions of the previous sweep (read by the chunk pass and the hot tiles)
   float* cout;        // contributions written by the row updates (may equal cin without hot tiles)
   const uint32_t* item_he;  // per bin-pass item: end of its hot entries
+  const uint32_t* item_hd;  // per bin-pass item: pair - item_hd indexes hsrc in its hot tile
+  const uint32_t* hsrc;     // hot tiles' pair sources, 32-bit
   uint32_t n_hot;
   const float* basev;
   float base;

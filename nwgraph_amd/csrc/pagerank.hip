@@ -754,6 +754,8 @@ static BinParams make_bin_params(nwg_graph* g, PrState& st, const SweepParams& P
   B.deg = P.deg;
   B.x = P.x;
   B.item_he = L.item_he.p;
+  B.item_hd = L.item_hd.p;
+  B.hsrc = L.hsrc.p;
   B.n_hot = L.n_hot;
   B.basev = P.basev;
   B.base = P.base;

@@ -578,6 +578,8 @@ static uint64_t fixed_point(nwg_ctx* ctx, WState& st, std::vector<std::unique_pt
       B.seg_flat = L->seg_flat.p;
       B.part = st.bpart.p;
       B.item_he = L->item_he.p;
+      B.item_hd = L->item_hd.p;
+      B.hsrc = L->hsrc.p;
       B.n_hot = L->n_hot;
       B.basev = base;
       B.damp = damping;
