@@ -390,14 +390,16 @@ __device__ __forceinline__ void bin_block_add(const BinBlock& b, const uint32_t 
 }
 
 // Pipeline depths of the bin pass: row offsets kDE blocks ahead, pair loads kDV blocks ahead,
-// in rings of kRE and kRV buffers.  Bins of 2^13 rows keep two workgroups per CU (64 VGPRs);
-// bins of 2^14 rows leave one, whose waves can hold twice the registers and go deeper.
-// The weighted sweep's blocks carry weights too: it goes less deep.
+// in rings of kRE and kRV buffers (the loop is unrolled by lcm(kRE, kRV)).  Measured on MI355X
+// at RMAT-25 (2^14-row bins), ms per sweep: depths 3/1 0.631, 2/1 0.632, 4/2 0.634, 5/2 0.636,
+// 7/3 0.643, 9/4 0.650, 11/5 0.657; rings whose lcm unrolls the loop 45 or 66 times 0.71-0.77
+// (code size).  A wave's loads of its next blocks overlap the other waves' adds: 16 waves per
+// workgroup keep enough in flight without deep rings.  Bins of 2^13 rows (two workgroups per
+// CU): 2/1 (3/1 and 3/2 were slower on the 4096^2 grid).
 template <bool W, int BB>
 struct BinDepth {
-  static constexpr bool kBig = BB >= 14;
-  static constexpr uint32_t DE = !kBig ? 2 : W ? 5 : 9, DV = !kBig ? 1 : W ? 2 : 4;
-  static constexpr uint32_t RE = !kBig ? 4 : W ? 6 : 10, RV = !kBig ? 2 : W ? 3 : 5;
+  static constexpr uint32_t DE = BB >= 14 ? 3 : 2, DV = 1;
+  static constexpr uint32_t RE = 4, RV = 2;
   static_assert(RE > DE && RV > DV && DE > DV, "pipeline rings must hold the blocks in flight");
 };
 constexpr uint32_t gcd_u32(uint32_t a, uint32_t b) { return b ? gcd_u32(b, a % b) : a; }
@@ -427,10 +429,10 @@ constexpr uint32_t bin_static_blocks() {
 // bounds, and never added.  Entries are ordered by source, so one row rarely repeats inside a
 // lane or an instruction: each entry is one LDS add, with no run tracking.  The sums are
 // exact, so which wave adds a block does not matter.  sync() is the workgroup's set-up
-// (clearing lacc, staging hot) and barrier: every wave calls it once, with 2^13-row bins after
-// issuing its first blocks' row offsets, which need neither.  (Their pair loads wait on the
-// row offsets and follow the barrier: waiting before it held every wave there until the
-// slowest had its row offsets.)
+// (clearing lacc, staging hot) and barrier: every wave calls it once, after issuing its first
+// blocks' row offsets, which need neither.  (Their pair loads wait on the row offsets and
+// follow the barrier: waiting before it held every wave there until the slowest had its row
+// offsets.)
 template <int NT, bool W, int BB, class Sync>
 __device__ __forceinline__ void bin_range(const BinParams& P, uint32_t e0, uint32_t he, uint32_t e1, uint32_t hd,
                                           unsigned long long* lacc, const float* hot, uint32_t* next, Sync&& sync) {
@@ -497,24 +499,15 @@ __device__ __forceinline__ void bin_range(const BinParams& P, uint32_t e0, uint3
     if (kq[K % kRE] < ke) add(kq[K % kRE], eb[K % kRE], rb[K % kRV], wb[K % kRV]);
     __builtin_amdgcn_sched_barrier(0);
   };
-  // bins of 2^13 rows: the first blocks' row offsets are issued before the workgroup's set-up
-  // and barrier (sync) and in flight over it; bins of 2^14 rows, whose one workgroup per CU
-  // waits for its slowest wave at the barrier: after it (measured: 2-5 % faster on the 4096^2
-  // grid and the weighted sweep, 1 % slower at RMAT-25)
-  constexpr bool kEarly = BB < 14;
-  if constexpr (!kEarly) {
-    sync();
-    if (kb >= ke) return;  // a wave past the range's last block
-  }
+  // the first blocks' row offsets are issued before the workgroup's set-up and barrier (sync)
+  // and are in flight over it
 #pragma unroll
   for (uint32_t d = 0; d < kDE; ++d) {
     kq[d] = kb + d * kWaves;
     eb[d] = bin_block_load<NT>(P, clamp(kq[d]), lane);
   }
-  if constexpr (kEarly) {
-    sync();
-    if (kb >= ke) return;  // a wave past the range's last block (its loads were the last block's)
-  }
+  sync();
+  if (kb >= ke) return;  // a wave past the range's last block (its loads were the last block's)
 #pragma unroll
   for (uint32_t d = 0; d < kDV; ++d) {
     if constexpr (W) bin_weights_load<NT>(P, clamp(kq[d]), lane, wb[d]);
@@ -578,9 +571,10 @@ __device__ __forceinline__ void finish_apply(const BinParams& P, uint32_t r0, ui
 // the hot contributions staged in LDS.  An item's hot end (P.item_he) is precomputed, so that
 // no load waits on another before the hot contributions are staged.
 // (8 waves per SIMD: 64 KiB of row sums allow two workgroups per CU, which 64 VGPRs keep; the
-// weighted pipeline needs more registers, and bins of 2^14 rows more LDS: one workgroup per CU)
+// weighted sweep's 2^13-row bins too, 3 % faster than one workgroup per CU with more
+// registers at RMAT-20.  Bins of 2^14 rows take 128 KiB of LDS: one workgroup per CU.)
 template <bool MULTI, int NT, bool W, int BB>
-__global__ __launch_bounds__(kBT, (W || BB >= 14) ? 4 : 8) void pr_bin_kernel(BinParams P) {
+__global__ __launch_bounds__(kBT, (BB >= 14) ? 4 : 8) void pr_bin_kernel(BinParams P) {
   constexpr uint32_t kBinRows = bin_rows<BB>();
   if (converged_before<W>(P)) return;
   __shared__ unsigned long long lacc[kBinRows];
