@@ -94,9 +94,12 @@ __device__ __forceinline__ unsigned long long quantize(float c) {
   const uint32_t l = (uint32_t)(r * 4294967296.f);  // floor(r * 2^32) < 2^32
   return ((unsigned long long)h << 32) | l;
 }
-// v * 2^-60 rounded to the nearest float (ties to even), with integer operations only
+// v * 2^-60 rounded to the nearest float (ties to even).  Below 2^53 (sums under 2^-7, nearly
+// every row) v is exact as a double, so one rounding to float and an exact scale by 2^-60 give
+// it; above, integer operations do.
 __device__ __forceinline__ float dequantize(unsigned long long v) {
-  if (v == 0ull) return 0.f;
+  const uint32_t hi = (uint32_t)(v >> 32);
+  if (hi < (1u << 21)) return (float)__builtin_fma((double)hi, 4294967296.0, (double)(uint32_t)v) * 0x1p-60f;
   const int lz = __clzll((long long)v);
   const unsigned long long nv = v << lz;  // bit 63 set
   uint32_t mant = (uint32_t)(nv >> 40);   // 24 significant bits
