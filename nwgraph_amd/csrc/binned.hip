@@ -45,7 +45,7 @@ constexpr int kBT = 1024;  // threads of a bin / chunk pass workgroup (16 waves,
 template <int BB>
 constexpr uint32_t bin_rows() { return 1u << BB; }
 constexpr uint32_t kChunkSrc = 1u << kChunkBits;
-constexpr uint32_t kItem2Log = 20;  // log2 of the most entries of a bin-pass item
+constexpr uint32_t kItem2Log = 21;  // log2 of the most entries of a bin-pass item
 constexpr uint32_t kItem1Pairs = 1u << 18;    // pairs per chunk-pass item
 constexpr uint32_t kPadSrc = 0xFFFFu;           // source offset of a pad pair (value 0)
 constexpr uint32_t kNoSlot = 0xFFFFFFFFu;     // bin-pass item of a bin that is one item
@@ -978,19 +978,20 @@ void binned_build(hipStream_t s, const ActiveCsc& a, uint32_t n, uint64_t nnz, B
   // bin pass items: every bin, cut into ranges of at most cap2 entries (inner cuts on
   // block boundaries, so no block is walked by two items of one bin); the items of a split bin
   // get consecutive partial slots, and the bin combine units
-  // Item size: the power of two nearest to entries / (2 x CUs) (4 x CUs with 2^13-row bins),
-  // between 2^15 and 2^kItem2Log.  Big graphs take big items (fewer split bins, so fewer
-  // partial rows to store and combine); small ones take small items, so that their heavy
+  // Item size: the power of two nearest to entries / CUs (entries / (4 x CUs) with 2^13-row
+  // bins), between 2^15 and 2^kItem2Log.  Big graphs take big items (fewer split bins, so
+  // fewer partial rows to store and combine); small ones take small items, so that their heavy
   // bins spread over all CUs.  (Measured on MI355X with 2^13-row bins: 2^19 best at RMAT-25,
-  // 2^18 for the 256-graph union, 2^17 at RMAT-23, 2^16 at RMAT-22 and below; 2^14-row bins
-  // store twice the partial rows per split item, and RMAT-25 takes 2^20: 0.675 against
-  // 0.69 ms per sweep at 2^19 and 2^21.)
+  // 2^18 for the 256-graph union, 2^17 at RMAT-23, 2^16 at RMAT-22 and below.  2^14-row bins
+  // store twice the partial rows per split item, and their waves take blocks from a counter:
+  // RMAT-25 takes 2^21, 0.622 ms per sweep against 0.627 at 2^20, 0.657 at 2^19, 0.71 at
+  // 3 x 2^20 and 0.78 at 2^22.)
   int dev = 0, cus = 0;
   NWG_HIP(hipGetDevice(&dev));
   NWG_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
   uint32_t cap2 = 1u << 15;
   {
-    const uint64_t q = etotal / ((L.bin_bits >= 14 ? 2ull : 4ull) * (uint64_t)std::max(cus, 1));
+    const uint64_t q = etotal / ((L.bin_bits >= 14 ? 1ull : 4ull) * (uint64_t)std::max(cus, 1));
     uint32_t lg = 0;
     while (lg < 63 && (2ull << lg) <= q) ++lg;  // floor(log2 q)
     if (q && (double)q >= 1.4142 * (double)(1ull << lg)) ++lg;
