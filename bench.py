@@ -31,11 +31,12 @@ HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
 
 # HBM bytes of one sweep measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE (each scaled by
 # what the counter reads for a stream copy of known size), for the configurations profiled in
-# profiles/: (workload, RMAT scale or grid width, binned, relabel) -> (bytes per sweep, profile file)
+# profiles/: (workload, RMAT scale or grid width, binned, relabel, bin_bits) -> (bytes per sweep,
+# profile file)
 MEASURED_TRAFFIC = {
-    ("rmat", 25, 1, 1): (3229212352, "profiles/pr_binned_sweep_rmat25.md"),
-    ("rmat", 25, 0, 1): (23300197580, "profiles/pr_sweep_rmat25_relabel.md"),
-    ("grid", 4096, 1, 0): (783533792, "profiles/pr_binned_sweep_grid4096.md"),
+    ("rmat", 25, 1, 1, 14): (3176601920, "profiles/pr_binned_sweep_rmat25.md"),
+    ("rmat", 25, 0, 1, 0): (23300197580, "profiles/pr_sweep_rmat25_relabel.md"),
+    ("grid", 4096, 1, 0, 14): (708700864, "profiles/pr_binned_sweep_grid4096.md"),
 }
 METRIC = "GTEPS per PR iteration + % HBM roofline; batched PPR/adjoint vs plain"
 
@@ -237,7 +238,8 @@ def run_plain(args, ctx, grid=False):
     traffic, traffic_src = args.traffic, "--traffic" if args.traffic else None
     if traffic is None and (grid or args.edge_factor == 16):
         key = ("grid", args.width) if grid else ("rmat", args.scale)
-        hit = MEASURED_TRAFFIC.get(key + (int(binned), int(plan.get("relabel", 0))))
+        hit = MEASURED_TRAFFIC.get(key + (int(binned), int(plan.get("relabel", 0)),
+                                          int(plan.get("bin_bits", 0)) if binned else 0))
         if hit:
             traffic, traffic_src = hit[0], f"rocprofv3 FETCH_SIZE + WRITE_SIZE, {hit[1]}"
     out = base_line(args, round(value, 3), "GTEPS", ms_per_step, "f32", {
@@ -383,7 +385,7 @@ def run_multi(args, ctx):
     out["roofline"] = roofline_entry(alg, float(np.mean(ev)), "all sweeps of the union in one captured HIP graph "
                                       "(device time)", "sum over graphs of sweeps x (4E + 4(N+1) + 20N)")
     if binned and args.ngraphs == 256:
-        out["traffic_per_sweep"] = {"bytes": 1355790336, "source": "rocprofv3 FETCH_SIZE + WRITE_SIZE, "
+        out["traffic_per_sweep"] = {"bytes": 1358957008, "source": "rocprofv3 FETCH_SIZE + WRITE_SIZE, "
                                     "profiles/pr_binned_multigraph_256.md"}
     mg.close()
     for g in graphs:
@@ -448,7 +450,7 @@ def run_wpr(args, ctx):
     out["roofline"] = roofline_entry(alg_f, fi["ms_total"], "weighted forward solve (binned weighted sweep: "
                                      "pr_chunk_kernel + pr_bin_kernel + pr_bin_combine_kernel + pr_wfold_kernel "
                                      "per sweep), device time", "sweeps x (8E + 4(N+1) + 12N)")
-    out["traffic_per_sweep"] = {"bytes": 263619584, "source": "rocprofv3 FETCH_SIZE + WRITE_SIZE, "
+    out["traffic_per_sweep"] = {"bytes": 285350032, "source": "rocprofv3 FETCH_SIZE + WRITE_SIZE, "
                                 "profiles/wpr_adjoint_rmat20.md"} if args.wpr_scale == 20 else None
     out["roofline_backward"] = roofline_entry(alg_b, bi["ms_total"], "adjoint solve over the CSR + edge and "
                                               "teleport gradients, device time",
