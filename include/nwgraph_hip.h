@@ -240,7 +240,11 @@ int nwg_graph_autotune(nwg_ctx* ctx, nwg_graph* g, nwg_plan_params* chosen /*nul
 /* k personalised PageRank vectors solved together as one SpMM per sweep.
  * Teleport j is uniform over seeds[seed_ptr[j] .. seed_ptr[j+1]) ([host] arrays).
  * scores: n x k row-major (score of vertex i for vector j at [i*k + j]).
- * Each vector stops updating after its own first sweep with L1 change < threshold.
+ * Each vector stops updating after its own first sweep with L1 change < threshold.  The solve
+ * stores no scores between sweeps: a row's old score is read back from its previous
+ * contribution as (x / deg) * deg, within 1 ulp of x, so a vector's L1 change can differ from a
+ * single-vector solve's by up to ~2^-23 * sum(x) (< 1.2e-7 for scores summing to 1), and a
+ * vector whose change lands that close to the threshold may stop one sweep apart from it.
  * iterations: [host] k counts, nullable. */
 int nwg_ppr_batched(nwg_ctx* ctx, nwg_graph* g, uint32_t k, const uint32_t* seed_ptr,
                     const uint32_t* seeds, float* scores, float damping, float threshold,
