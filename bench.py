@@ -482,7 +482,7 @@ def summary(out, keep=()):
     for k in ("n", "nnz", "sweeps_per_step", "ms_per_sweep_events", "plan") + tuple(keep):
         if k in cfg:
             r[k] = cfg[k]
-    for k in ("roofline", "roofline_backward", "validation", "cpu_baseline"):
+    for k in ("roofline", "roofline_backward", "traffic_per_sweep", "validation", "cpu_baseline"):
         if k in out:
             r[k] = out[k]
     return r
