@@ -191,7 +191,7 @@ __global__ __launch_bounds__(kThreads) void ppr_sweep_kernel(PprParams P) {
     // ids per coalesced load and kUnroll gathers in flight; each row's sum (in edge order)
     // goes to LDS, and the kR row updates are issued together at the end.  Every test
     // that steers the loops is uniform over the group, so all its lanes take part in the
-    // shuffles (converged vectors only skip their gathers and updates).
+    // shuffles (converged vectors gather with their group and only skip their updates).
     constexpr uint32_t kR = PprRows<T>::value;
     constexpr uint32_t kSlots = kThreads / T;
     const uint32_t slot = threadIdx.x / T;
@@ -209,12 +209,17 @@ __global__ __launch_bounds__(kThreads) void ppr_sweep_kernel(PprParams P) {
       for (uint32_t p = (uint32_t)__shfl((int)my_lrp, 0, T); p < pend; p += T) {
         const uint32_t m = min((uint32_t)T, pend - p);
         const uint32_t c = vl < m ? P.lcol[p + vl] : 0u;
+        // Every lane gathers: lanes of converged vectors share their group's line, and edges
+        // past m read source 0 (c is 0 there).  Loads that a lane condition skipped made the
+        // compiler drain every load in flight before each add (4 % of the solve at RMAT-22).
+        // (Issuing the next round's gathers before adding this one's took 84 VGPRs instead of
+        // 54 and was 17 % slower.)
         for (uint32_t q0 = 0; q0 < m; q0 += kUnroll) {  // q0 + t < T: q0 is a multiple of 8 below m <= T
           float v[kUnroll];
 #pragma unroll
           for (uint32_t t = 0; t < kUnroll; ++t) {
             const uint32_t u = (uint32_t)__shfl((int)c, (int)(q0 + t), T);
-            v[t] = (live && q0 + t < m) ? ctile[(uint64_t)u * T + vl] : 0.f;
+            v[t] = ctile[(uint64_t)u * T + vl];
           }
 #pragma unroll
           for (uint32_t t = 0; t < kUnroll; ++t) {
@@ -225,7 +230,7 @@ __global__ __launch_bounds__(kThreads) void ppr_sweep_kernel(PprParams P) {
               ++r;
               rend = (uint32_t)__shfl((int)my_lrp, (int)(r + 1), T);
             }
-            z += v[t];
+            z += v[t];  // a converged vector's sums are never used
           }
         }
       }
@@ -272,7 +277,7 @@ __global__ __launch_bounds__(kThreads) void ppr_sweep_kernel(PprParams P) {
 #pragma unroll
     for (uint32_t t = 0; t < kSegPerGroup; ++t) {
       const uint32_t u = (uint32_t)__shfl((int)c[t / T], (int)(t % T), T);
-      v[t] = (live && p0 + t < p1) ? ctile[(uint64_t)u * T + vl] : 0.f;
+      v[t] = ctile[(uint64_t)u * T + vl];  // every lane: past p1, c is 0 (source 0); see the light rows
     }
     float z = 0.f;
 #pragma unroll
