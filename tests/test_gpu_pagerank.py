@@ -273,7 +273,8 @@ def with_plan(g, plan):
     return g
 
 
-@pytest.mark.parametrize("plan", [{}, {"binned": 1}, {"binned": 1, "relabel": 1}])
+@pytest.mark.parametrize("plan", [{}, {"binned": 1}, {"binned": 1, "relabel": 1}, {"binned": 1, "bin_bits": 14},
+                                  {"binned": 1, "relabel": 1, "bin_bits": 14}])
 def test_edge_cases(ctx, plan):
     # no edges: every score becomes base after the first sweep
     g = with_plan(ctx.graph_from_edges(5, np.zeros(0, np.uint32), np.zeros(0, np.uint32)), plan)
